@@ -1,0 +1,170 @@
+"""Pins the oracle's VSH translation and aggregate T-matrix assembly.
+
+Reference: cpp/src/vsh_translation.cpp:12-237, 428-470;
+cpp/src/interactions.cpp:370-462.
+
+Pins:
+- the symmetry identities of the reference's tests/test_vsh_translation.py:15-48
+  (position reversal, (m,n)<->(u,v) swaps);
+- the VSH addition theorem: the field radiated by particle j's outgoing
+  expansion, evaluated near particle i with an independent numpy/scipy VSH
+  evaluator (src/miepy/vsh/expansion.py:13-42, vsh_functions.py:40-78
+  conventions), equals the incident-type expansion around i whose
+  coefficients are -X_ij p_j, where X_ij is the (i, j) block of the assembled
+  matrix with unit Mie coefficients.  This fixes every A/B coefficient, sign
+  and index of the assembly independently of the oracle's formulas;
+- assembled blocks vs the scalar vsh_translation entry point;
+- the general T-matrix assembly reduces to the sphere assembly for diagonal
+  (Mie) T-matrices.
+"""
+
+import numpy as np
+import pytest
+from math import factorial
+from scipy import special
+
+
+def _modes(lmax):
+    return [(n, m) for n in range(1, lmax + 1) for m in range(-n, n + 1)]
+
+
+# ---- reference symmetry identities -------------------------------------------------
+
+def test_reference_symmetries(oracle):
+    r_ji, theta_ji, phi_ji, lmax, k = 1, 0.7, 0.3, 2, 0.8
+    for n, m in _modes(lmax):
+        for v, u in _modes(lmax):
+            A, B = oracle.vsh_translation(m, n, u, v, r_ji, theta_ji, phi_ji, k)
+            A2, B2 = oracle.vsh_translation(m, n, u, v, r_ji, np.pi - theta_ji, np.pi + phi_ji, k)
+            assert np.allclose(A2, (-1) ** (n + v) * A) and np.allclose(B2, (-1) ** (n + v + 1) * B)
+            A2, B2 = oracle.vsh_translation(-u, v, -m, n, r_ji, theta_ji, phi_ji, k)
+            assert np.allclose(A2, (-1) ** (m + u) * A) and np.allclose(B2, (-1) ** (m + u + 1) * B)
+            A2, B2 = oracle.vsh_translation(-u, v, -m, n, r_ji, np.pi - theta_ji, np.pi + phi_ji, k)
+            assert np.allclose(A2, (-1) ** (m + u + n + v) * A) and np.allclose(B2, (-1) ** (m + u + n + v) * B)
+
+
+# ---- independent VSH evaluator (numpy/scipy) ----------------------------------------
+
+def _P(n, m, x):
+    """Associated Legendre without Condon-Shortley phase (GSL NONE)."""
+    if abs(m) > n:
+        return 0.0 * x
+    val = special.lpmv(abs(m), n, x) * (-1) ** abs(m)
+    if m < 0:
+        val = val * (-1) ** (-m) * factorial(n + m) / factorial(n - m)
+    return val
+
+
+def _dP_dtheta(n, m, theta, h=1e-3):
+    f = lambda t: _P(n, m, np.cos(t))
+    d1 = (f(theta + h) - f(theta - h)) / (2 * h)
+    d2 = (f(theta + 2 * h) - f(theta - 2 * h)) / (4 * h)
+    return (4 * d1 - d2) / 3  # Richardson: O(h^4)
+
+
+def _zn(n, x, mode, derivative=False):
+    j = special.spherical_jn(n, x, derivative)
+    if mode == "incident":
+        return j
+    return j + 1j * special.spherical_yn(n, x, derivative)
+
+
+def _vsh_cart(n, m, mode, pts, center, k):
+    d = pts - center
+    r = np.linalg.norm(d, axis=1)
+    theta = np.arccos(d[:, 2] / r)
+    phi = np.arctan2(d[:, 1], d[:, 0])
+    kr = k * r
+    H, Hp = _zn(n, kr, mode), _zn(n, kr, mode, True)
+    e = np.exp(1j * m * phi)
+    P = _P(n, m, np.cos(theta))
+    pi = m / np.sin(theta) * P
+    tau = _dP_dtheta(n, m, theta)
+    fac = (H + kr * Hp) * e / kr
+    N = np.array([n * (n + 1) * P * H / kr * e, tau * fac, 1j * pi * fac])
+    M = np.array([0 * kr, 1j * H * e * pi, -H * e * tau])
+    rh = np.array([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)])
+    th = np.array([np.cos(theta) * np.cos(phi), np.cos(theta) * np.sin(phi), -np.sin(theta)])
+    ph = np.array([-np.sin(phi), np.cos(phi), 0 * phi])
+    to_cart = lambda S: S[0] * rh + S[1] * th + S[2] * ph
+    return to_cart(N), to_cart(M)
+
+
+def _expand(p, lmax, mode, pts, center, k):
+    """E = factor * sum Emn (p_e N + p_m M); factor = i (outgoing), -i (incident)."""
+    factor = 1j if mode == "outgoing" else -1j
+    E = 0
+    for r, (n, m) in enumerate(_modes(lmax)):
+        Emn = 1j ** n * np.sqrt((2 * n + 1) * factorial(n - m) / (n * (n + 1) * factorial(n + m)))
+        N, M = _vsh_cart(n, m, mode, pts, center, k)
+        E = E + factor * Emn * (p[0, r] * N + p[1, r] * M)
+    return E
+
+
+@pytest.mark.parametrize("lmax,tol", [(2, 1e-5), (3, 3e-8), (4, 1e-10)])
+def test_addition_theorem(oracle, lmax, tol):
+    rng = np.random.default_rng(7 + lmax)
+    rmax = lmax * (lmax + 2)
+    k = 2 * np.pi / 0.9
+    pos = np.array([[0.11, -0.07, 0.21], [0.52, 0.33, -0.18]])  # d ~ 0.66, kd ~ 4.6
+    mie_one = np.ones((2, 2, lmax), dtype=complex)
+    X = oracle.sphere_aggregate_tmatrix(pos, mie_one, k).reshape(2, 2, rmax, 2, 2, rmax)
+    for (i, j) in [(0, 1), (1, 0)]:
+        p_j = rng.normal(size=(2, rmax)) + 1j * rng.normal(size=(2, rmax))
+        c_i = -np.einsum("arbs,bs->ar", X[i, :, :, j], p_j)
+        dirs = rng.normal(size=(6, 3))
+        dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+        rels = []
+        for f in (0.004, 0.001):
+            pts = pos[i] + f * np.linalg.norm(pos[0] - pos[1]) * dirs
+            E_out = _expand(p_j, lmax, "outgoing", pts, pos[j], k)
+            E_in = _expand(c_i, lmax, "incident", pts, pos[i], k)
+            rels.append(np.linalg.norm(E_out - E_in) / np.linalg.norm(E_out))
+        # truncating the regular expansion at lmax leaves an O(rho^lmax) error
+        assert rels[1] < tol, rels
+        assert rels[0] / rels[1] > 0.5 * 4 ** lmax, rels
+
+
+def test_blocks_match_scalar_translation(oracle):
+    lmax, k = 3, 2 * np.pi / 0.7
+    rmax = lmax * (lmax + 2)
+    pos = np.array([[0.0, 0.0, 0.0], [0.3, -0.4, 0.25], [-0.2, 0.5, -0.1]])
+    rng = np.random.default_rng(3)
+    mie = rng.normal(size=(3, 2, lmax)) + 1j * rng.normal(size=(3, 2, lmax))
+    T = oracle.sphere_aggregate_tmatrix(pos, mie, k).reshape(3, 2, rmax, 3, 2, rmax)
+    for (i, j) in [(0, 1), (2, 0), (1, 2)]:
+        d = pos[i] - pos[j]
+        rad = np.linalg.norm(d)
+        th, ph = np.arccos(d[2] / rad), np.arctan2(d[1], d[0])
+        for r, (n, m) in enumerate(_modes(lmax)):
+            for s, (v, u) in enumerate(_modes(lmax)):
+                A, B = oracle.vsh_translation(m, n, u, v, rad, th, ph, k)
+                for a in range(2):
+                    for b in range(2):
+                        want = (A, B)[(a + b) % 2] * mie[j, b, v - 1]
+                        got = T[i, a, r, j, b, s]
+                        assert abs(got - want) <= 1e-12 * max(1e-3, abs(want)), (i, j, n, m, v, u, a, b, got, want)
+    # diagonal blocks are zero (interactions.cpp:379)
+    for i in range(3):
+        assert np.all(T[i, :, :, i] == 0)
+
+
+def test_single_particle_is_zero(oracle):
+    T = oracle.sphere_aggregate_tmatrix(np.zeros((1, 3)), np.ones((1, 2, 2), dtype=complex), 1.0)
+    assert T.shape == (16, 16) and np.all(T == 0)
+
+
+def test_particle_tmatrix_reduces_to_sphere(oracle):
+    lmax, k = 2, 2 * np.pi / 0.8
+    rmax = lmax * (lmax + 2)
+    pos = np.array([[0.0, 0.0, 0.0], [0.35, 0.1, -0.2], [-0.1, 0.4, 0.3]])
+    rng = np.random.default_rng(11)
+    mie = rng.normal(size=(3, 2, lmax)) + 1j * rng.normal(size=(3, 2, lmax))
+    nidx = np.concatenate([[n - 1] * (2 * n + 1) for n in range(1, lmax + 1)])
+    tm = np.zeros((3, 2, rmax, 2, rmax), dtype=complex)
+    for i in range(3):
+        for a in range(2):
+            tm[i, a, np.arange(rmax), a, np.arange(rmax)] = mie[i, a, nidx]
+    T1 = oracle.sphere_aggregate_tmatrix(pos, mie, k)
+    T2 = oracle.particle_aggregate_tmatrix(pos, tm, k)
+    assert np.max(np.abs(T1 - T2)) <= 1e-13 * np.max(np.abs(T1))
