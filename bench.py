@@ -1,0 +1,264 @@
+#!/usr/bin/env python
+"""Cluster solves/s on one MI355X: complex64 + FP64 refinement vs FP64.
+
+A step is one complete cluster solve of the workload with its inputs
+(positions, Mie coefficients, source coefficients p_src) already resident in
+HBM: assemble the interaction matrix, solve (I + T) p_inc = p_src for every
+source to the workload tolerance, and reduce p_scat/p_int, cross-sections,
+forces and torques on the device.  Default workload: 1000 random metal
+spheres (eps -20+1.2j, r 100 nm, 10 um cube, gap 2.2 r, seed 2) at 600 nm,
+lmax 3 -> a 30000 x 30000 complex system, tol 1e-10.
+
+Each precision runs W warmup + K timed steps; the JSON line reports the
+FP32+refine throughput as `value`, the FP64 throughput beside it, p50/p99
+step latency, the relative error of the FP32+refine results against the FP64
+results, the roofline of the dominant kernel (timed live with HIP events on
+the launching stream) and the CPU oracle baseline (timed on this host).
+
+    python bench.py [--steps K] [--warmup W] [--workload rand1000|hex37|rand2500]
+"""
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="rand1000")
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-gemvs", type=int, default=3, help="timed oracle GEMVs in the CPU sample")
+    return ap.parse_args()
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_workload(name):
+    from miepy_amd import workloads as W
+    return {"rand1000": W.rand1000, "hex37": W.hex37, "rand2500": W.rand2500}[name]()
+
+
+class Problem:
+    """Host-side inputs of one workload, uploaded once."""
+
+    def __init__(self, w):
+        import miepy_amd.device as D
+        from miepy_amd.mie import mie_coefficients
+        self.w = w
+        k = w.k
+        self.mie_scat, self.mie_int = mie_coefficients(w.radius, w.eps, np.ones(w.N), w.medium_eps, 1.0, k, w.lmax)
+        self.p_src = np.stack([s.structure(w.position, k, w.lmax) for s in w.sources])
+        self.S = len(w.sources)
+        self.n = w.n
+        self.pos_d = D.to_device(w.position, None)
+        self.ms_d = D.to_device(self.mie_scat, D.C128)
+        self.mi_d = D.to_device(self.mie_int, D.C128)
+        self.src_d = D.to_device(self.p_src, D.C128)
+
+
+class Solver:
+    def __init__(self, prob, precision):
+        import torch
+        import miepy_amd.device as D
+        from miepy_amd import _lib as L
+        self.D, self.L, self.torch = D, L, torch
+        self.prob = prob
+        self.precision = precision
+        n, S = prob.n, prob.S
+        dev = prob.pos_d.device
+        self.A64 = torch.empty((n, n), dtype=D.C128, device=dev)
+        self.A32 = torch.empty((n, n), dtype=D.C64, device=dev) if precision == "mixed" else None
+        lib = L.load()
+        nb = lib.mpa_bicgstab_workspace_bytes(n, S, L.MPA_C128) if precision == "fp64" else \
+            lib.mpa_refine_workspace_bytes(n, S)
+        self.ws = torch.empty(nb, dtype=torch.uint8, device=dev)
+        self.x = torch.empty((S, n), dtype=D.C128, device=dev)
+        self.report = None
+        self.obs = None
+
+    def step(self):
+        D, p, w = self.D, self.prob, self.prob.w
+        b = p.src_d.reshape(p.S, p.n)
+        D.assemble(p.pos_d, p.ms_d, w.k, dtype=D.C128, add_identity=True, out=self.A64)
+        if self.precision == "fp64":
+            _, self.report = D.bicgstab(self.A64, b, tol=w.tol, maxiter=5000, x=self.x, workspace=self.ws)
+        else:
+            D.assemble(p.pos_d, p.ms_d, w.k, dtype=D.C64, add_identity=True, out=self.A32)
+            _, self.report = D.solve_refined(self.A32, self.A64, b, tol=w.tol, maxiter=5000, x=self.x,
+                                             workspace=self.ws)
+        self.obs = D.observables(self.x.reshape(p.S, w.N, 2, w.rmax), p.src_d, p.ms_d, p.mi_d, w.k, w.medium_eps)
+
+
+def run_timed(solver, steps, warmup, profile=False):
+    import torch
+    L = solver.L
+    lib = L.load()
+    for _ in range(warmup):
+        solver.step()
+    torch.cuda.synchronize()
+    if profile:
+        lib.mpa_profile_enable(1)
+    lat = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s = time.perf_counter()
+        solver.step()
+        torch.cuda.synchronize()
+        lat.append(time.perf_counter() - s)
+    torch.cuda.synchronize()
+    total = time.perf_counter() - t0
+    prof = {}
+    if profile:
+        import ctypes
+        for kind, name in [(0, "matvec_c128"), (1, "matvec_c64"), (2, "assemble_c128"), (3, "assemble_c64")]:
+            c = ctypes.c_int(0)
+            t = ctypes.c_double(0)
+            L.check(lib.mpa_profile_read(kind, ctypes.byref(c), ctypes.byref(t)), "profile_read")
+            if c.value:
+                prof[name] = dict(count=c.value, avg_ms=t.value / c.value, total_ms=t.value)
+        lib.mpa_profile_enable(0)
+    return total, np.array(lat), prof
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a), np.asarray(b)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def cpu_baseline(w, prob, iters, gemvs):
+    """Oracle (CPU FP64 C++/OpenMP restatement of the reference path) on this
+    host: the full assembly of the workload plus `gemvs` timed GEMVs; the solve
+    is projected with the FP64 GPU run's BiCGSTAB iteration count (2 GEMVs per
+    iteration + 1), the same algorithm the oracle runs."""
+    import oracle as O
+    O.build()
+    threads = O.max_threads()
+    T = np.empty((w.n, w.n), dtype=np.complex128)
+    T.fill(0)  # first touch of the pages outside the timed region
+    t = time.perf_counter()
+    O.sphere_aggregate_tmatrix(w.position, prob.mie_scat, w.k, out=T)
+    t_asm = time.perf_counter() - t
+    x = np.ascontiguousarray(prob.p_src[0].reshape(-1))
+    times = []
+    for _ in range(gemvs):
+        t = time.perf_counter()
+        O.gemv(T, x)
+        times.append(time.perf_counter() - t)
+    t_mv = float(np.median(times))
+    del T
+    t_solve = t_asm + (2 * iters + 1) * t_mv
+    return dict(value=1.0 / t_solve, unit="cluster solves/s", cores=threads, kind="port",
+                sample=(f"oracle FP64 on {threads} host threads: full {w.name} assembly ({t_asm:.2f} s) + "
+                        f"{gemvs} GEMVs (median {t_mv:.3f} s); solve projected with {iters} BiCGSTAB iterations "
+                        f"-> {t_solve:.2f} s per solve"))
+
+
+def main():
+    args = parse()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py runs on one GPU: the cluster solve here is built and measured on a single MI355X "
+              "(multi-GPU sharding is design-only)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    import miepy_amd.device as D
+    D.require_gpu()
+    log(f"workload {args.workload}")
+    w = make_workload(args.workload)
+    prob = Problem(w)
+    log(f"{w}: n={w.n}, S={prob.S}, tol={w.tol}")
+
+    results = {}
+    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+        s = Solver(prob, prec)
+        total, lat, prof = run_timed(s, args.steps, args.warmup, profile=True)
+        res = dict(total=total, lat=lat, prof=prof, report=s.report,
+                   obs={k: (v.cpu().numpy() if v is not None else None) for k, v in s.obs.items()})
+        results[prec] = res
+        log(f"{prec}: {args.steps} steps in {total:.3f} s ({total / args.steps * 1e3:.1f} ms/step), "
+            f"report {s.report}, kernels {prof}")
+        del s
+        torch.cuda.empty_cache()
+
+    head = "mixed" if "mixed" in results else "fp64"
+    r = results[head]
+    n, S = w.n, prob.S
+    value = args.steps / r["total"]
+    line = {
+        "metric": "cluster solves/s (FP32+refine vs FP64, <=1e-8 rel err); p50/p99 solve latency",
+        "value": value,
+        "unit": "cluster solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r["total"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c64+fp64-refine" if head == "mixed" else "c128",
+        "data": "synthetic (seeded random cluster)",
+        "config": {"workload": w.name, "N": w.N, "lmax": w.lmax, "n": n, "rhs": S, "tol": w.tol,
+                   "wavelength_nm": w.wavelength * 1e9, "precision": head},
+        "latency_ms": {"p50": float(np.percentile(r["lat"], 50) * 1e3),
+                       "p99": float(np.percentile(r["lat"], 99) * 1e3)},
+        "rhs_per_s": value * S,
+    }
+    if "fp64" in results:
+        f = results["fp64"]
+        line["fp64"] = {"solves_per_s": args.steps / f["total"], "ms_per_step": f["total"] / args.steps * 1e3,
+                        "p50_ms": float(np.percentile(f["lat"], 50) * 1e3),
+                        "p99_ms": float(np.percentile(f["lat"], 99) * 1e3),
+                        "iterations": f["report"].iterations}
+    if "fp64" in results and "mixed" in results:
+        a, b = results["mixed"]["obs"], results["fp64"]["obs"]
+        Fscale = np.abs(b["F"]).max()
+        line["mixed_vs_fp64"] = {
+            "speedup": line["fp64"]["ms_per_step"] / line["ms_per_step"],
+            "p_scat_rel_err": rel_err(a["p_scat"], b["p_scat"]),
+            "cross_section_max_rel_err": float(np.max(np.abs(a["C_total"] - b["C_total"]) / np.abs(b["C_total"]))),
+            "force_max_rel_err": float(np.max(np.abs(a["F"] - b["F"])) / Fscale),
+            "outer": results["mixed"]["report"].outer,
+            "inner_iterations": results["mixed"]["report"].iterations,
+        }
+    # roofline of the dominant kernel (largest summed time in the timed region)
+    prof = r["prof"]
+    dom = max(prof, key=lambda k: prof[k]["total_ms"]) if prof else None
+    if dom:
+        esize = 16 if dom.endswith("c128") else 8
+        if dom.startswith("matvec"):
+            nbytes = esize * n * n + 2 * esize * n * S
+        else:
+            nbytes = esize * n * n
+        avg_s = prof[dom]["avg_ms"] / 1e3
+        ach = nbytes / avg_s / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                            "avg_launch_ms": prof[dom]["avg_ms"], "launches": prof[dom]["count"],
+                            "bytes_per_launch": nbytes}
+        line["kernels"] = prof
+    if not args.no_cpu_baseline and "fp64" in results:
+        it = int(max(results["fp64"]["report"].iterations))
+        log("cpu baseline (oracle) ...")
+        line["cpu_baseline"] = cpu_baseline(w, prob, it, args.cpu_gemvs)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

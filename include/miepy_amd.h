@@ -1,0 +1,149 @@
+/* C ABI of the MI355X cluster-solve library (libmiepy_amd.so, gfx950).
+ *
+ * Every pointer argument named like a matrix or vector is a DEVICE pointer
+ * (HBM) unless documented as host; `stream` is a hipStream_t passed as void*
+ * (NULL = default stream).  Complex arrays are interleaved (re, im): c128 =
+ * two doubles, c64 = two floats, i.e. numpy complex128 / complex64.  Matrices
+ * are row-major [n, n] with n = 2 * rmax * N, rmax = lmax (lmax + 2), and the
+ * flat index particle * 2 rmax + pol * rmax + r(n, m), r(n, m) = n(n+2)-n+m-1
+ * (the reference's ComplexMatrix layout, cpp/src/vec.hpp:14,
+ * src/miepy/vsh/mode_indices.py:16).  Several right-hand sides are stored
+ * RHS-major: [nrhs, n].
+ *
+ * Every function returns an int status (MPA_OK = 0).  On failure
+ * mpa_last_error() describes the cause.  Nothing here allocates device
+ * memory except the per-lmax translation tables, built once per process and
+ * device on first use (a few hundred KB at most).
+ */
+#ifndef MIEPY_AMD_H
+#define MIEPY_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes (the reference returns silently at maxiter and never checks
+ * for breakdown or non-finite values, cpp/src/interactions.cpp:138-139,365) */
+#define MPA_OK 0
+#define MPA_ERR_ARG 1
+#define MPA_ERR_HIP 2
+#define MPA_NOT_CONVERGED 3
+#define MPA_BREAKDOWN 4
+#define MPA_NONFINITE 5
+#define MPA_ERR_OOM 6
+
+/* element types */
+#define MPA_C128 0
+#define MPA_C64 1
+
+const char* mpa_version(void);
+const char* mpa_last_error(void);
+
+/* Aggregate T-matrix of a sphere cluster.
+ * Replaces interactions.sphere_aggregate_tmatrix(positions, mie, k)
+ * (cpp/src/interactions_pyb.cpp:26-31 -> cpp/src/interactions.cpp:370-415).
+ *   positions  [N, 3] f64 (device)
+ *   mie        [N, 2, lmax] c128 (device): mie(j, b*lmax + v-1), as the
+ *              reference's mie argument
+ *   k          medium wavenumber
+ *   out_dtype  MPA_C128 or MPA_C64 (FP64 arithmetic, rounded on store)
+ *   add_identity  0: T exactly as the reference; 1: I + T (the system matrix
+ *              of cpp/src/interactions.cpp:351-353)
+ *   out        [n, n] of out_dtype (device); every entry is written. */
+int mpa_sphere_aggregate_tmatrix(int N, int lmax, const double* positions, const double* mie, double k, int out_dtype,
+                                 int add_identity, void* out, void* stream);
+
+/* y = (add_identity ? x : 0) + A x for nrhs vectors ([nrhs, n]).
+ * Replaces the OpenMP row-block GEMV inside Eigen's BiCGSTAB
+ * (cpp/src/interactions.cpp:62-95). */
+int mpa_matvec(int n, int nrhs, int dtype, const void* A, int add_identity, const void* x, void* y, void* stream);
+
+/* Bytes of device workspace mpa_bicgstab needs. */
+size_t mpa_bicgstab_workspace_bytes(int n, int nrhs, int dtype);
+
+/* BiCGSTAB for (add_identity ? I + A : A) x = b, nrhs systems sharing A.
+ * Replaces the bicgstab branch of solve_linear_system
+ * (cpp/src/interactions.cpp:347-368; Eigen::BiCGSTAB, identity
+ * preconditioner): zero initial guess, stop at ||r|| <= tol ||b|| or maxiter.
+ * The reference hard-codes tol = 1e-5, maxiter = 1000.
+ *   dtype       MPA_C128 (FP64) or MPA_C64 (FP32 vectors, FP64 scalars)
+ *   b, x        [nrhs, n] of dtype (device); x is overwritten
+ *   workspace   mpa_bicgstab_workspace_bytes(n, nrhs, dtype) bytes (device)
+ *   iters_out, resid_out, status_out  host arrays [nrhs] or NULL:
+ *               iterations, ||r||/||b||, per-RHS status
+ *   sync_every  iterations enqueued between host convergence checks (>= 1)
+ * Returns MPA_OK, or the first non-OK per-RHS status (MPA_NOT_CONVERGED,
+ * MPA_NONFINITE). */
+int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, const void* b, void* x, double tol,
+                 int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out, int sync_every,
+                 void* stream);
+
+/* Bytes of device workspace mpa_solve_refined needs. */
+size_t mpa_refine_workspace_bytes(int n, int nrhs);
+
+/* Mixed-precision solve of (I + T) x = b: complex64 BiCGSTAB corrections on
+ * the FP32 matrix A32, FP64 iterative refinement with the FP64 residual
+ * r = b - A64 x, until ||r|| <= tol ||b|| (per RHS) or max_refine
+ * corrections or stagnation.  Same system as mpa_bicgstab (the reference's
+ * solve_linear_system, cpp/src/interactions.cpp:347-368), answered to FP64
+ * accuracy; it has no counterpart in the reference, which solves in FP64 only.
+ *   A32  [n, n] c64, A64 [n, n] c128 (device); add_identity* as mpa_bicgstab
+ *   b, x [nrhs, n] c128 (device); inner_tol = relative tolerance of each
+ *   complex64 correction solve, maxiter its iteration cap
+ *   outer_out, inner_out, resid_out, status_out: host arrays [nrhs] or NULL. */
+int mpa_solve_refined(int n, int nrhs, const void* A32, int add_identity32, const void* A64, int add_identity64,
+                      const void* b, void* x, double tol, double inner_tol, int maxiter, int max_refine,
+                      void* workspace, int* outer_out, int* inner_out, double* resid_out, int* status_out,
+                      void* stream);
+
+/* Observables of a solved cluster for nsrc sources, all FP64 (device):
+ *   p_inc, p_src [nsrc, N, 2, rmax] c128; mie_scat, mie_int [N, 2, lmax] c128
+ *   p_scat (out) = p_inc * mie_scat[:, :, n_idx]        (sphere_cluster.py:716)
+ *   p_int  (out, may be NULL) = p_inc * mie_int[:, ::-1, n_idx] (:717)
+ *   C_part (out) [nsrc, N, 3, 2, lmax] scattering/absorption/extinction per
+ *          multipole: flux.particle_cross_sections (cpp/src/flux.cpp:5-30)
+ *   C_total (out, may be NULL) [nsrc, 3] cluster sums (sphere_cluster.py:442-454)
+ *   F (out) [nsrc, N, 3]: forces.force_all (cpp/src/forces.cpp:7-86,139-154)
+ *   T (out, may be NULL) [nsrc, N, 3]: forces.torque_all (:88-137,156-171)
+ * eps_b is the (real) medium permittivity; mu_b is unused, as in the reference
+ * (cpp/src/forces.cpp:10). */
+int mpa_cluster_observables(int N, int lmax, int nsrc, const double* p_inc, const double* p_src,
+                            const double* mie_scat, const double* mie_int, double k, double eps_b, double* p_scat,
+                            double* p_int, double* C_part, double* C_total, double* F, double* T, void* stream);
+
+/* Force and/or torque on M particles from explicit coefficients.
+ * Replaces forces.force_all / torque_all (cpp/src/forces_pyb.cpp:27-43 ->
+ * cpp/src/forces.cpp:139-171): p_scat, p_inc [M, 2 rmax] c128 (device);
+ * F, T [M, 3] f64 (device, either may be NULL). */
+int mpa_force_torque(int M, int lmax, const double* p_scat, const double* p_inc, double k, double eps_b, double* F,
+                     double* T, void* stream);
+
+/* Per-multipole cross-sections of M particles.
+ * Replaces flux.particle_cross_sections (cpp/src/flux_pyb.cpp:12 ->
+ * cpp/src/flux.cpp:5-30), batched: p_scat, p_inc, p_src [M, 2 rmax] c128;
+ * C_part [M, 3, 2, lmax] f64 (scattering, absorption, extinction). */
+int mpa_particle_cross_sections(int M, int lmax, const double* p_scat, const double* p_inc, const double* p_src,
+                                double k, double* C_part, void* stream);
+
+/* Launch timing with HIP events recorded on the launching stream (off by
+ * default).  kind: 0 matvec c128, 1 matvec c64, 2 assembly c128,
+ * 3 assembly c64, 4 matrix-free matvec c128, 5 matrix-free matvec c64.
+ * mpa_profile_enable(1) clears the records and starts recording;
+ * mpa_profile_read sums the recorded launch groups of one kind (host outputs). */
+int mpa_profile_enable(int on);
+int mpa_profile_read(int kind, int* count, double* total_ms);
+
+/* Host copy of the per-lmax translation coefficient table (tests):
+ * meta_out [rmax*rmax*4] int, coef_out [rmax*rmax*(2 lmax+1)] c128. */
+int mpa_translation_table(int lmax, int* meta_out, double* coef_out);
+
+/* Column blocks per assembly workgroup tile chosen for lmax. */
+int mpa_assembly_tile(int lmax);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MIEPY_AMD_H */

@@ -1,0 +1,92 @@
+"""ctypes binding of libmiepy_amd.so (the C ABI declared in include/miepy_amd.h).
+
+The library is loaded from miepy_amd/lib/ only; there is no fallback path: a
+missing library or a machine without a GPU raises immediately.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libmiepy_amd.so")
+
+MPA_OK = 0
+MPA_ERR_ARG = 1
+MPA_ERR_HIP = 2
+MPA_NOT_CONVERGED = 3
+MPA_BREAKDOWN = 4
+MPA_NONFINITE = 5
+MPA_ERR_OOM = 6
+
+MPA_C128 = 0
+MPA_C64 = 1
+
+STATUS_NAMES = {
+    MPA_OK: "ok",
+    MPA_ERR_ARG: "bad argument",
+    MPA_ERR_HIP: "HIP error",
+    MPA_NOT_CONVERGED: "not converged",
+    MPA_BREAKDOWN: "breakdown",
+    MPA_NONFINITE: "non-finite",
+    MPA_ERR_OOM: "out of memory",
+}
+
+
+class MpaError(RuntimeError):
+    def __init__(self, status, where, detail=""):
+        self.status = status
+        super().__init__(f"{where}: {STATUS_NAMES.get(status, status)}" + (f" ({detail})" if detail else ""))
+
+
+_i = ctypes.c_int
+_d = ctypes.c_double
+_v = ctypes.c_void_p
+_sz = ctypes.c_size_t
+_pi = ctypes.POINTER(ctypes.c_int)
+_pd = ctypes.POINTER(ctypes.c_double)
+
+# name -> (restype, argtypes); kept in the order of include/miepy_amd.h
+SIGNATURES = {
+    "mpa_version": (ctypes.c_char_p, []),
+    "mpa_last_error": (ctypes.c_char_p, []),
+    "mpa_sphere_aggregate_tmatrix": (_i, [_i, _i, _v, _v, _d, _i, _i, _v, _v]),
+    "mpa_matvec": (_i, [_i, _i, _i, _v, _i, _v, _v, _v]),
+    "mpa_bicgstab_workspace_bytes": (_sz, [_i, _i, _i]),
+    "mpa_bicgstab": (_i, [_i, _i, _i, _v, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
+    "mpa_refine_workspace_bytes": (_sz, [_i, _i]),
+    "mpa_solve_refined": (_i, [_i, _i, _v, _i, _v, _i, _v, _v, _d, _d, _i, _i, _v, _pi, _pi, _pd, _pi, _v]),
+    "mpa_cluster_observables": (_i, [_i, _i, _i, _v, _v, _v, _v, _d, _d, _v, _v, _v, _v, _v, _v, _v]),
+    "mpa_force_torque": (_i, [_i, _i, _v, _v, _d, _d, _v, _v, _v]),
+    "mpa_particle_cross_sections": (_i, [_i, _i, _v, _v, _v, _d, _v, _v]),
+    "mpa_profile_enable": (_i, [_i]),
+    "mpa_profile_read": (_i, [_i, _pi, _pd]),
+    "mpa_translation_table": (_i, [_i, _pi, _pd]),
+    "mpa_assembly_tile": (_i, [_i]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with `python -m miepy_amd.build`")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def last_error():
+    return load().mpa_last_error().decode(errors="replace")
+
+
+def check(status, where):
+    if status != MPA_OK:
+        raise MpaError(status, where, last_error())
+    return status

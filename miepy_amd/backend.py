@@ -1,0 +1,161 @@
+"""The MI355X backend: the reference's dispatch functions, same signatures.
+
+Each function here is what `miepy.interactions` / `flux` / `forces` would
+call when the 'mi355x' backend is selected (the slots the JAX 'gpu' backend
+takes in src/miepy/interactions.py:81-83,141-143 and src/miepy/flux.py:21-26):
+
+  sphere_aggregate_tmatrix(positions, mie, k)       interactions.py:132-150
+  solve_linear_system(tmatrix, p_src, method, ...)  interactions.py:71-92
+  particle_cross_sections(p_scat, p_inc, p_src, k)  flux.py:12-32
+  force_all / torque_all(p_scat, p_inc, k, eps_b, mu_b)  forces.py:44-81
+
+Like the JAX backend, the assembled matrix stays in HBM: it is returned as a
+`DeviceTMatrix`, which has the reference's [N,2,rmax,N,2,rmax] shape,
+supports `.reshape` and converts to numpy on demand (`np.asarray(T)`), and is
+consumed in place by `solve_linear_system`.
+"""
+
+import enum
+
+import numpy as np
+import torch
+
+from . import device as D
+
+
+class solver(enum.Enum):
+    """Mirror of the pybind enum miepy.solver (cpp/src/interactions_pyb.cpp:33-37)."""
+
+    bicgstab = 0
+    exact = 1
+
+
+# Solver knobs the reference hard-codes (cpp/src/interactions.cpp:362-363).
+options = dict(tol=1e-5, maxiter=1000, exact_tol=1e-14)
+
+
+class DeviceTMatrix:
+    """Aggregate T-matrix resident in HBM (torch tensor [n, n])."""
+
+    def __init__(self, data, shape):
+        self.data = data
+        self._shape = tuple(shape)
+
+    @property
+    def shape(self):
+        return self._shape
+
+    @property
+    def dtype(self):
+        return np.complex128 if self.data.dtype == torch.complex128 else np.complex64
+
+    def reshape(self, *shape):
+        if len(shape) == 1 and isinstance(shape[0], (tuple, list)):
+            shape = tuple(shape[0])
+        if int(np.prod(shape)) != self.data.numel():
+            raise ValueError(f"cannot reshape {self._shape} to {shape}")
+        return DeviceTMatrix(self.data, shape)
+
+    def __array__(self, dtype=None, copy=None):
+        a = self.data.cpu().numpy().reshape(self._shape)
+        return a.astype(dtype) if dtype is not None else a
+
+
+def sphere_aggregate_tmatrix(positions, mie, k):
+    """positions[N,3], mie[N,2,lmax], k -> T[N,2,rmax,N,2,rmax] (device-resident)."""
+    positions = np.asarray(positions, dtype=float).reshape(-1, 3)
+    N = positions.shape[0]
+    lmax = np.shape(mie)[-1]
+    rmax = lmax * (lmax + 2)
+    T = D.assemble(positions, np.asarray(mie, dtype=complex).reshape(N, 2, lmax), float(np.real(k)))
+    return DeviceTMatrix(T, (N, 2, rmax, N, 2, rmax))
+
+
+def _device_matrix(tmatrix, n):
+    if isinstance(tmatrix, DeviceTMatrix):
+        return tmatrix.data.reshape(n, n)
+    return D.to_device(np.asarray(tmatrix, dtype=complex).reshape(n, n), D.C128)
+
+
+def solve_linear_system(tmatrix, p_src, method=solver.bicgstab, M_inv_blocks=None, block_size=0, tol=None,
+                        maxiter=None):
+    """Solve (I + T) p_inc = p_src.  p_src[N,2,rmax] -> p_inc[N,2,rmax] (host numpy).
+
+    method=bicgstab runs BiCGSTAB (default tolerance 1e-5 as the reference);
+    method=exact runs the same solver to a tolerance of 1e-14."""
+    if M_inv_blocks is not None:
+        raise NotImplementedError("block-diagonal preconditioning is not available on the mi355x backend")
+    p_src = np.asarray(p_src)
+    shape = p_src.shape
+    n = p_src.size
+    A = _device_matrix(tmatrix, n)
+    exact = (method == solver.exact) or (getattr(method, "name", None) == "exact")
+    t = options["exact_tol"] if exact else (options["tol"] if tol is None else tol)
+    mi = max(options["maxiter"], 10 * n) if exact else (options["maxiter"] if maxiter is None else maxiter)
+    b = D.to_device(p_src.reshape(1, n), A.dtype)
+    x, report = D.bicgstab(A, b, tol=t, maxiter=mi, add_identity=True)
+    solve_linear_system.last_report = report
+    return x.cpu().numpy().reshape(shape)
+
+
+solve_linear_system.last_report = None
+
+
+def _rows(a):
+    a = np.asarray(a, dtype=complex)
+    return a.reshape(a.shape[0], -1) if a.ndim > 1 else a.reshape(1, -1)
+
+
+def particle_cross_sections(p_scat, p_inc, p_src, k):
+    """Per-multipole (scattering, absorption, extinction) of one particle, each [2, lmax]."""
+    from .flux import cross_sections
+    rows = [_rows(a) for a in (p_scat, p_inc, p_src)]
+    lmax = int(-1 + (1 + rows[0].shape[1] // 2) ** 0.5)
+    C = _xs_rows(*rows, k, lmax)[0]
+    return cross_sections(C[0], C[1], C[2])
+
+
+def _xs_rows(ps, pi, pr, k, lmax):
+    import ctypes
+    from . import _lib as L
+    M = ps.shape[0]
+    d = [D.to_device(a, D.C128) for a in (ps, pi, pr)]
+    C = torch.empty((M, 3, 2, lmax), dtype=torch.float64, device=d[0].device)
+    L.check(L.load().mpa_particle_cross_sections(M, lmax, *[ctypes.c_void_p(t.data_ptr()) for t in d], float(np.real(k)),
+                                                 ctypes.c_void_p(C.data_ptr()), D._stream()),
+            "particle_cross_sections")
+    return C.cpu().numpy()
+
+
+def _force_torque(p_scat, p_inc, k, eps_b, want_force):
+    import ctypes
+    from . import _lib as L
+    ps, pi = _rows(p_scat), _rows(p_inc)
+    M = ps.shape[0]
+    lmax = int(-1 + (1 + ps.shape[1] // 2) ** 0.5)
+    d = [D.to_device(a, D.C128) for a in (ps, pi)]
+    out = torch.empty((M, 3), dtype=torch.float64, device=d[0].device)
+    null = ctypes.c_void_p(0)
+    ptr = ctypes.c_void_p(out.data_ptr())
+    L.check(L.load().mpa_force_torque(M, lmax, ctypes.c_void_p(d[0].data_ptr()), ctypes.c_void_p(d[1].data_ptr()),
+                                      float(np.real(k)), float(np.real(eps_b)), ptr if want_force else null,
+                                      null if want_force else ptr, D._stream()), "force_torque")
+    return out.cpu().numpy()
+
+
+def force_all(p_scat, p_inc, k, eps_b, mu_b):
+    """F[N,3] on every particle (mu_b unused, as cpp/src/forces.cpp:10)."""
+    return _force_torque(p_scat, p_inc, k, eps_b, True)
+
+
+def torque_all(p_scat, p_inc, k, eps_b, mu_b):
+    """T[N,3] on every particle."""
+    return _force_torque(p_scat, p_inc, k, eps_b, False)
+
+
+def force(p_scat, p_inc, k, eps_b, mu_b):
+    return force_all(np.asarray(p_scat).reshape(1, -1), np.asarray(p_inc).reshape(1, -1), k, eps_b, mu_b)[0]
+
+
+def torque(p_scat, p_inc, k, eps_b, mu_b):
+    return torque_all(np.asarray(p_scat).reshape(1, -1), np.asarray(p_inc).reshape(1, -1), k, eps_b, mu_b)[0]

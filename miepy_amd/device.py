@@ -1,0 +1,199 @@
+"""Device-level operations: torch tensors in HBM -> libmiepy_amd C ABI.
+
+torch provides device memory and the current HIP stream; every computation is
+a hand-written gfx950 kernel behind include/miepy_amd.h.  There is no CPU
+path: without a visible GPU or without the built library these raise.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+C128 = torch.complex128
+C64 = torch.complex64
+DTYPE_CODE = {C128: L.MPA_C128, C64: L.MPA_C64}
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("miepy_amd needs a HIP GPU (MI355X); none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def to_device(a, dtype=None):
+    dev = require_gpu()
+    if isinstance(a, torch.Tensor):
+        t = a.to(dev)
+    else:
+        t = torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    if dtype is not None:
+        t = t.to(dtype)
+    return t.contiguous()
+
+
+def lmax_to_rmax(lmax):
+    return lmax * (lmax + 2)
+
+
+def assemble(positions, mie, k, dtype=C128, add_identity=False, out=None):
+    """Aggregate T-matrix [n, n] (or I + T) on the device.
+
+    positions [N, 3] f64, mie [N, 2, lmax] c128 (device tensors or arrays)."""
+    lib = L.load()
+    pos = to_device(positions, torch.float64).reshape(-1, 3)
+    N = pos.shape[0]
+    mie_d = to_device(mie, C128).reshape(N, 2, -1)
+    lmax = mie_d.shape[-1]
+    n = 2 * lmax_to_rmax(lmax) * N
+    if out is None:
+        out = torch.empty((n, n), dtype=dtype, device=pos.device)
+    elif out.shape != (n, n) or out.dtype != dtype or not out.is_contiguous():
+        raise ValueError("out must be a contiguous [n, n] tensor of the requested dtype")
+    L.check(lib.mpa_sphere_aggregate_tmatrix(N, lmax, _ptr(pos), _ptr(mie_d), float(k), DTYPE_CODE[dtype],
+                                             1 if add_identity else 0, _ptr(out), _stream()),
+            "sphere_aggregate_tmatrix")
+    return out
+
+
+def matvec(A, x, add_identity=False, out=None):
+    """y = (I +) A x; x is [n] or [S, n] on the device, same dtype as A."""
+    lib = L.load()
+    n = A.shape[0]
+    x2 = x.reshape(-1, n)
+    if x2.dtype != A.dtype:
+        raise ValueError("x must have the matrix dtype")
+    y = torch.empty_like(x2) if out is None else out
+    L.check(lib.mpa_matvec(n, x2.shape[0], DTYPE_CODE[A.dtype], _ptr(A), 1 if add_identity else 0, _ptr(x2.contiguous()),
+                           _ptr(y), _stream()), "matvec")
+    return y.reshape(x.shape)
+
+
+class SolveReport:
+    def __init__(self, iters, resid, status):
+        self.iterations = iters
+        self.residual = resid
+        self.status = status
+
+    @property
+    def converged(self):
+        return all(s == L.MPA_OK for s in self.status)
+
+    def __repr__(self):
+        return f"SolveReport(iterations={self.iterations}, residual={self.residual}, status={self.status})"
+
+
+def bicgstab(A, b, tol=1e-5, maxiter=1000, add_identity=False, x=None, workspace=None, sync_every=1, check=True):
+    """BiCGSTAB (Eigen's algorithm) for (I +) A x = b on the device.
+
+    b: [n] or [S, n] with A's dtype.  Returns (x, SolveReport)."""
+    lib = L.load()
+    n = A.shape[0]
+    b2 = b.reshape(-1, n).contiguous()
+    S = b2.shape[0]
+    code = DTYPE_CODE[A.dtype]
+    if b2.dtype != A.dtype:
+        raise ValueError("b must have the matrix dtype")
+    nbytes = lib.mpa_bicgstab_workspace_bytes(n, S, code)
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = torch.empty(nbytes, dtype=torch.uint8, device=A.device)
+    if x is None:
+        x = torch.empty_like(b2)
+    it = (ctypes.c_int * S)()
+    res = (ctypes.c_double * S)()
+    st = (ctypes.c_int * S)()
+    rc = lib.mpa_bicgstab(n, S, code, _ptr(A), 1 if add_identity else 0, _ptr(b2), _ptr(x), float(tol), int(maxiter),
+                          _ptr(workspace), it, res, st, int(sync_every), _stream())
+    report = SolveReport(list(it), list(res), list(st))
+    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED, L.MPA_NONFINITE) or (check and rc == L.MPA_NONFINITE):
+        L.check(rc, "bicgstab")
+    return x.reshape(b.shape), report
+
+
+def observables(p_inc, p_src, mie_scat, mie_int, k, eps_b, want_int=True, want_torque=True):
+    """Per-source observables of a solved cluster (all FP64, on the device).
+
+    p_inc, p_src: [S, N, 2, rmax] c128.  Returns a dict of device tensors:
+    p_scat, p_int, C_part [S, N, 3, 2, lmax], C_total [S, 3], F [S, N, 3], T [S, N, 3]."""
+    lib = L.load()
+    S, N, _, rmax = p_inc.shape
+    lmax = mie_scat.shape[-1]
+    dev = p_inc.device
+    p_scat = torch.empty_like(p_inc)
+    p_int = torch.empty_like(p_inc) if want_int else None
+    C_part = torch.empty((S, N, 3, 2, lmax), dtype=torch.float64, device=dev)
+    C_total = torch.empty((S, 3), dtype=torch.float64, device=dev)
+    F = torch.empty((S, N, 3), dtype=torch.float64, device=dev)
+    T = torch.empty((S, N, 3), dtype=torch.float64, device=dev) if want_torque else None
+    nullp = ctypes.c_void_p(0)
+    L.check(lib.mpa_cluster_observables(N, lmax, S, _ptr(p_inc), _ptr(p_src), _ptr(mie_scat),
+                                        _ptr(mie_int) if want_int else nullp, float(k), float(np.real(eps_b)),
+                                        _ptr(p_scat), _ptr(p_int) if want_int else nullp, _ptr(C_part), _ptr(C_total),
+                                        _ptr(F), _ptr(T) if want_torque else nullp, _stream()),
+            "cluster_observables")
+    return dict(p_scat=p_scat, p_int=p_int, C_part=C_part, C_total=C_total, F=F, T=T)
+
+
+def translation_table(lmax):
+    """Host copy of the device translation table: (meta [rmax^2, 4], coef [rmax^2, 2 lmax + 1])."""
+    lib = L.load()
+    rmax = lmax_to_rmax(lmax)
+    meta = np.zeros((rmax * rmax, 4), dtype=np.int32)
+    coef = np.zeros((rmax * rmax, 2 * lmax + 1), dtype=np.complex128)
+    L.check(lib.mpa_translation_table(lmax, meta.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+                                      coef.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "translation_table")
+    return meta, coef
+
+
+class RefineReport:
+    def __init__(self, outer, inner, resid, status):
+        self.outer = outer
+        self.iterations = inner
+        self.residual = resid
+        self.status = status
+
+    @property
+    def converged(self):
+        return all(s == L.MPA_OK for s in self.status)
+
+    def __repr__(self):
+        return (f"RefineReport(outer={self.outer}, inner_iterations={self.iterations}, residual={self.residual}, "
+                f"status={self.status})")
+
+
+def solve_refined(A32, A64, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refine=8, add_identity=False,
+                  workspace=None, x=None):
+    """Mixed-precision solve: c64 BiCGSTAB corrections, FP64 refinement.
+
+    A32 [n, n] c64, A64 [n, n] c128 (device), b [S, n] c128.  Returns (x c128, RefineReport)."""
+    lib = L.load()
+    n = A32.shape[0]
+    b2 = b.reshape(-1, n).contiguous()
+    S = b2.shape[0]
+    if A32.dtype != C64 or A64.dtype != C128 or b2.dtype != C128:
+        raise ValueError("solve_refined needs A32 complex64, A64 complex128, b complex128")
+    nbytes = lib.mpa_refine_workspace_bytes(n, S)
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = torch.empty(nbytes, dtype=torch.uint8, device=A32.device)
+    if x is None:
+        x = torch.empty_like(b2)
+    outer = (ctypes.c_int * S)()
+    inner = (ctypes.c_int * S)()
+    res = (ctypes.c_double * S)()
+    st = (ctypes.c_int * S)()
+    ai = 1 if add_identity else 0
+    rc = lib.mpa_solve_refined(n, S, _ptr(A32), ai, _ptr(A64), ai, _ptr(b2), _ptr(x), float(tol), float(inner_tol),
+                               int(maxiter), int(max_refine), _ptr(workspace), outer, inner, res, st, _stream())
+    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED):
+        L.check(rc, "solve_refined")
+    return x.reshape(b.shape), RefineReport(list(outer), list(inner), list(res), list(st))

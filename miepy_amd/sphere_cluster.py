@@ -1,0 +1,199 @@
+"""Generalized Mie theory for a sphere cluster, solved on one MI355X.
+
+Same constructor and result attributes as the reference's
+src/miepy/sphere_cluster.py:27-137 (position, radius, material, source,
+wavelength, lmax, medium, origin, interactions, method -> mie_scat, mie_int,
+p_src, p_inc, p_scat, p_int; cross_sections() :442, force() :521,
+torque() :541, per-particle variants :456-519, update_position() :637), with
+the solve of :664-717 running as HIP kernels on the device:
+
+  host:   material data, Mie coefficients (vectorized scipy), p_src
+  device: assemble (I + T) -> BiCGSTAB (FP64) or complex64 BiCGSTAB + FP64
+          refinement ('mixed') -> p_scat / p_int, cross-sections, force,
+          torque in one fused kernel.
+
+Additional keyword arguments (the reference hard-codes these,
+cpp/src/interactions.cpp:362-363): precision ('fp64' | 'mixed'), tol,
+maxiter.  `source` may also be a list of sources, solved together as
+several right-hand sides sharing one matrix; results then carry a leading
+source axis.
+"""
+
+import numpy as np
+
+from . import device as D
+from .backend import solver
+from .flux import cross_sections as _cs
+from .materials import constant_material, material_struct
+from .mie import mie_coefficients
+
+
+def _atleast(array, dim, length, dtype=None):
+    """src/miepy/utils.py:6-21"""
+    ret = np.asarray(np.atleast_1d(array), dtype=dtype)
+    if dim not in (ret.ndim, ret.ndim + 1):
+        raise ValueError(f"dim = {dim} is invalid with input dim = {ret.ndim}")
+    if len(ret.shape) == dim and ret.shape[0] != length:
+        ret = np.repeat(ret, length)
+    elif len(ret.shape) != dim:
+        ret = np.array([ret] * length)
+    return ret
+
+
+def n_index(lmax):
+    """r -> n-1 (sphere_cluster.py:11-18)."""
+    return np.concatenate([[n - 1] * (2 * n + 1) for n in range(1, lmax + 1)]).astype(int)
+
+
+class sphere_cluster:
+    def __init__(self, *, position, radius, material, source, wavelength, lmax, medium=None, origin=None,
+                 symmetry=None, interface=None, interactions=True, method=None, precision="fp64", tol=None,
+                 maxiter=1000, refine_tol=None):
+        if symmetry is not None or interface is not None:
+            raise NotImplementedError("periodic symmetry and interfaces are not part of the mi355x cluster solve")
+        self.position = np.asarray(np.atleast_2d(position), dtype=float)
+        self.radius = _atleast(radius, dim=1, length=self.position.shape[0], dtype=float)
+        self.material = _atleast(material, dim=1, length=self.position.shape[0], dtype=object)
+        if not (self.position.shape[0] == self.radius.shape[0] == self.material.shape[0]):
+            raise ValueError("The shapes of position, radius, and material do not match")
+        self.Nparticles = self.radius.shape[0]
+        self.symmetry = symmetry
+        self.interface = interface
+        self.source = source
+        self.wavelength = wavelength
+        self.lmax = lmax
+        self.rmax = lmax * (lmax + 2)
+        self.interactions = interactions
+        self.method = method
+        self.precision = precision
+        self.tol = tol
+        self.maxiter = maxiter
+        self.refine_tol = refine_tol
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+
+        self.auto_origin = False
+        if origin is None:
+            self.origin = np.zeros(3)
+        elif isinstance(origin, str) and origin == "auto":
+            self.auto_origin = True
+            self.origin = np.average(self.position, axis=0)
+        else:
+            self.origin = np.asarray(origin)
+
+        if medium is None:
+            self.medium = constant_material(eps=1.0, mu=1.0)
+        else:
+            self.medium = medium
+            if (np.imag(self.medium.eps(self.wavelength)) != 0) or (np.imag(self.medium.mu(self.wavelength)) != 0):
+                raise ValueError("medium must be non-absorbing")
+
+        self.material_data = material_struct(self.material, self.medium, wavelength=self.wavelength)
+        md = self.material_data
+        conducting = np.array([getattr(m, "name", None) == "metal" for m in self.material])
+        self.mie_scat, self.mie_int = mie_coefficients(self.radius, md.eps, md.mu, md.eps_b, md.mu_b,
+                                                       float(np.real(md.k_b)), lmax, conducting)
+        self._multi = isinstance(source, (list, tuple))
+        self.p_cluster = None
+        self.report = None
+        self.solve()
+
+    # ---- solve ---------------------------------------------------------------------
+
+    def _sources(self):
+        return list(self.source) if self._multi else [self.source]
+
+    def solve(self, wavelength=None, source=None):
+        md = self.material_data
+        k = float(np.real(md.k_b))
+        N, rmax, lmax = self.Nparticles, self.rmax, self.lmax
+        srcs = self._sources()
+        S = len(srcs)
+        p_src = np.stack([s.structure(self.position, k, lmax) for s in srcs])  # [S, N, 2, rmax]
+        dev = D.require_gpu()
+        p_src_d = D.to_device(p_src, D.C128)
+        mie_scat_d = D.to_device(self.mie_scat, D.C128)
+        mie_int_d = D.to_device(self.mie_int, D.C128)
+        n = 2 * rmax * N
+        if self.interactions and N > 1:
+            exact = self.method == solver.exact or getattr(self.method, "name", None) == "exact"
+            tol = 1e-14 if exact else (1e-5 if self.tol is None else self.tol)
+            maxiter = max(self.maxiter, 10 * n) if exact else self.maxiter
+            b = p_src_d.reshape(S, n)
+            if self.precision == "fp64":
+                A = D.assemble(self.position, mie_scat_d, k, dtype=D.C128, add_identity=True)
+                x, self.report = D.bicgstab(A, b, tol=tol, maxiter=maxiter)
+                del A
+            else:
+                A32 = D.assemble(self.position, mie_scat_d, k, dtype=D.C64, add_identity=True)
+                A64 = D.assemble(self.position, mie_scat_d, k, dtype=D.C128, add_identity=True)
+                rt = tol if self.refine_tol is None else self.refine_tol
+                x, self.report = D.solve_refined(A32, A64, b, tol=rt, maxiter=maxiter)
+                del A32, A64
+            p_inc_d = x.reshape(S, N, 2, rmax)
+        else:
+            p_inc_d = p_src_d.clone()
+        obs = D.observables(p_inc_d, p_src_d, mie_scat_d, mie_int_d, k, md.eps_b)
+        host = {key: (v.cpu().numpy() if v is not None else None) for key, v in obs.items()}
+        p_inc = p_inc_d.cpu().numpy()
+        sq = (lambda a: a) if self._multi else (lambda a: a[0])
+        self.p_src = sq(p_src)
+        self.p_inc = sq(p_inc)
+        self.p_scat = sq(host["p_scat"])
+        self.p_int = sq(host["p_int"])
+        self._C_part = sq(host["C_part"])      # [(S,) N, 3, 2, lmax]
+        self._C_total = sq(host["C_total"])    # [(S,) 3]
+        self._F = sq(host["F"])                # [(S,) N, 3]
+        self._T = sq(host["T"])
+
+    def update_position(self, position):
+        self.position = np.asarray(np.atleast_2d(position), dtype=float)
+        self.p_cluster = None
+        if self.auto_origin:
+            self.origin = np.average(self.position, axis=0)
+        self.solve()
+
+    # ---- observables -----------------------------------------------------------------
+
+    def cross_sections(self):
+        """Cluster (scattering, absorption, extinction)."""
+        C = self._C_total
+        return _cs(C[..., 0], C[..., 1], C[..., 2])
+
+    def cross_sections_per_multipole_of_particle(self, i):
+        C = self._C_part[..., i, :, :, :]
+        return _cs(C[..., 0, :, :], C[..., 1, :, :], C[..., 2, :, :])
+
+    def cross_sections_of_particle(self, i):
+        C = self.cross_sections_per_multipole_of_particle(i)
+        return _cs(*[np.sum(A, axis=(-2, -1)) for A in C])
+
+    def _forces(self, torque, source):
+        if source:
+            return self._T if torque else self._F
+        from . import backend as B
+        p_inc = self.p_inc - self.p_src
+        md = self.material_data
+        fn = B.torque_all if torque else B.force_all
+        if self._multi:
+            return np.stack([fn(self.p_scat[s], p_inc[s], md.k_b, md.eps_b, md.mu_b) for s in range(len(self.source))])
+        return fn(self.p_scat, p_inc, md.k_b, md.eps_b, md.mu_b)
+
+    def force(self, source=True):
+        """F[N, 3] on every particle."""
+        return self._forces(False, source)
+
+    def torque(self, source=True):
+        """T[N, 3] on every particle."""
+        return self._forces(True, source)
+
+    def force_on_particle(self, i, source=True):
+        return self.force(source)[..., i, :]
+
+    def torque_on_particle(self, i, source=True):
+        return self.torque(source)[..., i, :]
+
+    def __repr__(self):
+        return (f"{self.__class__.__name__}:\n    Nparticles = {self.Nparticles}\n    source = {self.source}\n"
+                f"    wavelength = {self.wavelength:.2e}\n    medium = {self.medium}\n    lmax = {self.lmax}\n"
+                f"    origin = {self.origin}\n    precision = {self.precision}\n")

@@ -180,15 +180,19 @@ def vsh_translation(m, n, u, v, rad, theta, phi, k, mode="outgoing"):
 
 # ---- aggregate T-matrix and solve ------------------------------------------------
 
-def sphere_aggregate_tmatrix(positions, mie, k):
-    """positions[N,3], mie[N,2,lmax] -> T[n,n] complex128 (row-major)."""
+def sphere_aggregate_tmatrix(positions, mie, k, out=None):
+    """positions[N,3], mie[N,2,lmax] -> T[n,n] complex128 (row-major).
+
+    `out` may be a preallocated C-contiguous [n, n] complex128 array."""
     positions = _f64(positions).reshape(-1, 3)
     N = positions.shape[0]
     lmax = mie.shape[-1]
     rmax = lmax_to_rmax(lmax)
     n = 2 * rmax * N
     mie2 = _c128(mie).reshape(N, 2 * lmax)
-    out = np.empty((n, n), dtype=np.complex128)
+    if out is None:
+        out = np.empty((n, n), dtype=np.complex128)
+    assert out.shape == (n, n) and out.dtype == np.complex128 and out.flags.c_contiguous
     lib().orc_sphere_aggregate_tmatrix(_p(positions), _p(mie2), N, lmax, float(k), _p(out))
     return out
 

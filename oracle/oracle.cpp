@@ -50,12 +50,14 @@ int rmax_to_lmax(int rmax) { return -1 + int(std::sqrt(1.0 + rmax)); }  // cpp/s
 // Spherical Bessel / Hankel by upward recurrence
 // ---------------------------------------------------------------------------
 
+constexpr int kMaxDegH = 128;
+
 // cpp/src/special.cpp:90-112 (spherical_hn_recursion): j_0 = sin z / z,
 // y_0 = -cos z / z, j_1 = j_0/z + y_0, y_1 = y_0/z - j_0, then the upward
 // three-term recurrence f_i = (2i-1)/z f_{i-1} - f_{i-2} for both.
 void hn_recursion(int nmax, double z, cd* out) {
     double s = std::sin(z), c = std::cos(z);
-    std::vector<double> jn(nmax + 2), yn(nmax + 2);
+    double jn[kMaxDegH], yn[kMaxDegH];
     jn[0] = s / z;
     yn[0] = -c / z;
     if (nmax >= 1) {
@@ -71,8 +73,8 @@ void hn_recursion(int nmax, double z, cd* out) {
 
 // cpp/src/special.cpp:54-88 (spherical_hn, single order, same recurrence)
 cd spherical_hn(int n, double z) {
-    std::vector<cd> v(n + 2);
-    hn_recursion(n, z, v.data());
+    cd v[kMaxDegH];
+    hn_recursion(n, z, v);
     return v[n];
 }
 
@@ -102,28 +104,30 @@ cd spherical_jn(int n, cd z) {
 //   P_m^m = (2m-1)!! (1-x^2)^{m/2},  P_{m+1}^m = x (2m+1) P_m^m,
 //   (l-m) P_l^m = (2l-1) x P_{l-1}^m - (l+m-1) P_{l-2}^m
 // (the textbook recurrences; GSL's NONE normalization carries no (-1)^m).
-// Output `out[l][m]` for 0 <= m <= l <= lmax.
-void legendre_nonneg(int lmax, double x, std::vector<std::vector<double>>& out) {
-    out.assign(lmax + 1, std::vector<double>(lmax + 1, 0.0));
+// Output P[l * (lmax+1) + m] for 0 <= m <= l <= lmax (caller-owned buffer).
+void legendre_nonneg(int lmax, double x, double* P) {
+    const int w = lmax + 1;
     double sx = std::sqrt(std::max(0.0, (1 - x) * (1 + x)));
     double pmm = 1.0;
     for (int m = 0; m <= lmax; m++) {
         if (m > 0) pmm *= (2 * m - 1) * sx;
-        out[m][m] = pmm;
-        if (m + 1 <= lmax) out[m + 1][m] = x * (2 * m + 1) * pmm;
+        P[m * w + m] = pmm;
+        if (m + 1 <= lmax) P[(m + 1) * w + m] = x * (2 * m + 1) * pmm;
         for (int l = m + 2; l <= lmax; l++)
-            out[l][m] = ((2 * l - 1) * x * out[l - 1][m] - (l + m - 1) * out[l - 2][m]) / (l - m);
+            P[l * w + m] = ((2 * l - 1) * x * P[(l - 1) * w + m] - (l + m - 1) * P[(l - 2) * w + m]) / (l - m);
     }
 }
+
+constexpr int kMaxDeg = 64;
 
 // cpp/src/special.cpp:180-204: index n(n+2)-n+m, negative m through
 // P_n^{-|m|} = (-1)^{|m|} (n-|m|)!/(n+|m|)! P_n^{|m|} (special.cpp:193).
 void legendre_recursion(int nmax, double z, double* out) {
-    std::vector<std::vector<double>> P;
-    legendre_nonneg(nmax, z, P);
+    std::vector<double> P((nmax + 1) * (nmax + 1));
+    legendre_nonneg(nmax, z, P.data());
     for (int n = 0; n <= nmax; n++) {
         for (int m = -n; m <= n; m++) {
-            double leg = P[n][std::abs(m)];
+            double leg = P[n * (nmax + 1) + std::abs(m)];
             if (m < 0) leg *= neg1pow(m) * factorial(n + m) / factorial(n - m);
             out[n * (n + 2) - n + m] = leg;
         }
@@ -139,9 +143,9 @@ double associated_legendre(int n, int m, double z) {
         if (m == 1) return std::sqrt(1 - z * z);
         if (m == -1) return -std::sqrt(1 - z * z) / 2;
     }
-    std::vector<std::vector<double>> P;
-    legendre_nonneg(n, z, P);
-    double leg = P[n][std::abs(m)];
+    std::vector<double> P((n + 1) * (n + 1));
+    legendre_nonneg(n, z, P.data());
+    double leg = P[n * (n + 1) + std::abs(m)];
     if (m < 0) leg *= neg1pow(m) * factorial(n + m) / factorial(n - m);
     return leg;
 }
@@ -150,10 +154,10 @@ double associated_legendre(int n, int m, double z) {
 // the relation gsl_sf_legendre_deriv_array evaluates (linear in P at fixed m,
 // so it holds with or without the Condon-Shortley phase).
 double legendre_deriv_nonneg(int n, int m, double x) {
-    std::vector<std::vector<double>> P;
-    legendre_nonneg(n, x, P);
-    double pn = P[n][m];
-    double pn1 = (n - 1 >= m) ? P[n - 1][m] : 0.0;
+    std::vector<double> P((n + 1) * (n + 1));
+    legendre_nonneg(n, x, P.data());
+    double pn = P[n * (n + 1) + m];
+    double pn1 = (n - 1 >= m) ? P[(n - 1) * (n + 1) + m] : 0.0;
     return (n * x * pn - (n + m) * pn1) / (x * x - 1);
 }
 
@@ -367,10 +371,10 @@ void insert_pair_mie(cd* T, int size, const cd* mie, int lmax, int i, int j, dou
                      double phi, double k, const cache_table& cache) {
     int rmax = lmax_to_rmax(lmax);
     int p_max = 2 * lmax + 1;
-    std::vector<cd> zn(p_max + 1);
-    hn_recursion(p_max, k * rad, zn.data());
-    std::vector<double> Pnm((p_max + 1) * (p_max + 1));
-    legendre_recursion(p_max, std::cos(theta), Pnm.data());
+    cd zn[kMaxDeg];
+    hn_recursion(p_max, k * rad, zn);
+    double Pnm[kMaxDeg * kMaxDeg];
+    legendre_recursion(p_max, std::cos(theta), Pnm);
     auto MIE = [&](int part, int idx) { return mie[part * 2 * lmax + idx]; };
     auto AT = [&](int r, int c) -> cd& { return T[(size_t)r * size + c]; };
 
