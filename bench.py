@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-gemvs", type=int, default=3, help="timed oracle GEMVs in the CPU sample")
+    ap.add_argument("--batch-count", type=int, default=256, help="clusters in the batch256 workload")
     return ap.parse_args()
 
 
@@ -52,6 +53,58 @@ def log(msg):
 def make_workload(name):
     from miepy_amd import workloads as W
     return {"rand1000": W.rand1000, "hex37": W.hex37, "rand2500": W.rand2500}[name]()
+
+
+def bench_batch(args):
+    """256 uneven clusters x 8 sources in one batched solve per step."""
+    import torch
+    from miepy_amd import workloads as W
+    from miepy_amd.batch import ClusterBatch
+    ws = W.batch256(count=args.batch_count)
+    log(f"batch of {len(ws)} clusters: sum n^2 = {sum(w.n ** 2 for w in ws):.3e}, "
+        f"largest n = {max(w.n for w in ws)}")
+    out = {}
+    res = {}
+    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+        B = ClusterBatch(ws, precision=prec)
+        lib_prof = {}
+        for _ in range(args.warmup):
+            B.solve()
+        torch.cuda.synchronize()
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            s = time.perf_counter()
+            B.solve()
+            torch.cuda.synchronize()
+            lat.append(time.perf_counter() - s)
+        total = time.perf_counter() - t0
+        its = [max(r.iterations) for r in B.reports]
+        out[prec] = {"clusters_per_s": len(ws) * args.steps / total, "ms_per_step": total / args.steps * 1e3,
+                     "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+                     "converged": B.converged(), "max_iterations": int(max(its)),
+                     "mean_iterations": float(np.mean(its))}
+        res[prec] = B.host_results()
+        log(f"{prec}: {out[prec]}")
+        del B
+        torch.cuda.empty_cache()
+    head = "mixed" if "mixed" in out else "fp64"
+    line = {
+        "metric": "cluster solves/s (FP32+refine vs FP64, <=1e-8 rel err); p50/p99 solve latency",
+        "value": out[head]["clusters_per_s"], "unit": "cluster solves/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out[head]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64+fp64-refine" if head == "mixed" else "c128",
+        "data": "synthetic (seeded random clusters)",
+        "config": {"workload": "batch256", "clusters": len(ws), "rhs_per_cluster": 8, "tol": 1e-8,
+                   "precision": head},
+        "rhs_per_s": out[head]["clusters_per_s"] * 8, "runs": out,
+    }
+    if "fp64" in res and "mixed" in res:
+        errs = [float(np.max(np.abs(a["C_total"] - b["C_total"]) / np.abs(b["C_total"]).max()))
+                for a, b in zip(res["mixed"], res["fp64"])]
+        line["mixed_vs_fp64"] = {"speedup": out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"],
+                                 "cross_section_max_rel_err": max(errs)}
+    print(json.dumps(line), flush=True)
 
 
 class Problem:
@@ -181,6 +234,8 @@ def main():
     import miepy_amd.device as D
     D.require_gpu()
     log(f"workload {args.workload}")
+    if args.workload == "batch256":
+        return bench_batch(args)
     w = make_workload(args.workload)
     prob = Problem(w)
     log(f"{w}: n={w.n}, S={prob.S}, tol={w.tol}")

@@ -10,6 +10,9 @@
  * src/miepy/vsh/mode_indices.py:16).  Several right-hand sides are stored
  * RHS-major: [nrhs, n].
  *
+ * System sizes n are even (n = 2 rmax N for every cluster system); the
+ * 16-byte vector loads of the matvec rely on it.
+ *
  * Every function returns an int status (MPA_OK = 0).  On failure
  * mpa_last_error() describes the cause.  Nothing here allocates device
  * memory except the per-lmax translation tables, built once per process and
@@ -79,6 +82,63 @@ size_t mpa_bicgstab_workspace_bytes(int n, int nrhs, int dtype);
 int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, const void* b, void* x, double tol,
                  int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out, int sync_every,
                  void* stream);
+
+/* ---- batched solves: many independent systems, one set of launches -------- */
+
+/* One system (I + T) x = b with nrhs right-hand sides (host struct; all
+ * pointers are device pointers).  `state` holds the system's Krylov vectors
+ * and scalars: mpa_system_state_bytes(n, nrhs, dtype) bytes. */
+typedef struct {
+    int n, nrhs, add_identity, pad;
+    const void* A; /* [n, n] of dtype (T, or I + T with add_identity = 0) */
+    const void* b; /* [nrhs, n] */
+    void* x;       /* [nrhs, n], overwritten */
+    void* state;
+    double tol;    /* relative: ||r|| <= tol ||b|| */
+    int maxiter, pad2;
+} mpa_system;
+
+size_t mpa_system_state_bytes(int n, int nrhs, int dtype);
+/* Device scratch for the descriptors and work lists of one batch. */
+size_t mpa_batch_scratch_bytes(int nsys, const mpa_system* systems, int dtype);
+
+/* BiCGSTAB (as mpa_bicgstab) for nsys systems at once: every kernel launch
+ * covers all systems; converged systems cost nothing further.  The outputs
+ * are host arrays of sum(nrhs) entries in (system, RHS) order. */
+int mpa_bicgstab_batched(int nsys, const mpa_system* systems, int dtype, void* scratch, int* iters_out,
+                         double* resid_out, int* status_out, int sync_every, void* stream);
+
+/* Iteration-level control of the same engine, for continuous batching:
+ * mpa_systems_start initializes the given systems (x = 0, r = b, scalars);
+ * mpa_systems_advance runs `iterations` BiCGSTAB iterations over all given
+ * systems (converged ones are no-ops; a pending restart is served first),
+ * then synchronizes and reports per RHS, in (system, RHS) order: done flag,
+ * iterations, ||r||/||b||, status (host arrays of sum(nrhs), any may be NULL).
+ * The set of systems may change between calls; each system's state lives in
+ * its own `state` buffer. */
+int mpa_systems_start(int nsys, const mpa_system* systems, int dtype, void* scratch, void* stream);
+int mpa_systems_advance(int nsys, const mpa_system* systems, int dtype, void* scratch, int iterations, int* done_out,
+                        int* iters_out, double* resid_out, int* status_out, void* stream);
+
+/* One system for the mixed-precision batched solve (host struct, device
+ * pointers); `state`: mpa_refine_state_bytes(n, nrhs) bytes. */
+typedef struct {
+    int n, nrhs, add_identity, pad;
+    const void* A32; /* [n, n] c64 */
+    const void* A64; /* [n, n] c128, the FP64 residual operator */
+    const void* b;   /* [nrhs, n] c128 */
+    void* x;         /* [nrhs, n] c128 */
+    void* state;
+    double tol, inner_tol;
+    int maxiter, max_refine;
+} mpa_refine_system;
+
+size_t mpa_refine_state_bytes(int n, int nrhs);
+size_t mpa_refine_scratch_bytes(int nsys, const mpa_refine_system* systems);
+/* mpa_solve_refined for nsys systems at once (one batched complex64 engine
+ * for the corrections, one batched c128 matvec for the residuals). */
+int mpa_solve_refined_batched(int nsys, const mpa_refine_system* systems, void* scratch, int* outer_out,
+                              int* inner_out, double* resid_out, int* status_out, void* stream);
 
 /* Bytes of device workspace mpa_solve_refined needs. */
 size_t mpa_refine_workspace_bytes(int n, int nrhs);

@@ -64,6 +64,34 @@ SIGNATURES = {
     "mpa_assembly_tile": (_i, [_i]),
 }
 
+class MpaSystem(ctypes.Structure):
+    """mpa_system (include/miepy_amd.h)"""
+    _fields_ = [("n", ctypes.c_int), ("nrhs", ctypes.c_int), ("add_identity", ctypes.c_int), ("pad", ctypes.c_int),
+                ("A", ctypes.c_void_p), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p), ("state", ctypes.c_void_p),
+                ("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("pad2", ctypes.c_int)]
+
+
+class MpaRefineSystem(ctypes.Structure):
+    """mpa_refine_system (include/miepy_amd.h)"""
+    _fields_ = [("n", ctypes.c_int), ("nrhs", ctypes.c_int), ("add_identity", ctypes.c_int), ("pad", ctypes.c_int),
+                ("A32", ctypes.c_void_p), ("A64", ctypes.c_void_p), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("state", ctypes.c_void_p), ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
+                ("maxiter", ctypes.c_int), ("max_refine", ctypes.c_int)]
+
+
+_ps = ctypes.POINTER(MpaSystem)
+_prs = ctypes.POINTER(MpaRefineSystem)
+SIGNATURES.update({
+    "mpa_system_state_bytes": (_sz, [_i, _i, _i]),
+    "mpa_batch_scratch_bytes": (_sz, [_i, _ps, _i]),
+    "mpa_bicgstab_batched": (_i, [_i, _ps, _i, _v, _pi, _pd, _pi, _i, _v]),
+    "mpa_systems_start": (_i, [_i, _ps, _i, _v, _v]),
+    "mpa_systems_advance": (_i, [_i, _ps, _i, _v, _i, _pi, _pi, _pd, _pi, _v]),
+    "mpa_refine_state_bytes": (_sz, [_i, _i]),
+    "mpa_refine_scratch_bytes": (_sz, [_i, _prs]),
+    "mpa_solve_refined_batched": (_i, [_i, _prs, _v, _pi, _pi, _pd, _pi, _v]),
+})
+
 _lib = None
 
 

@@ -109,7 +109,7 @@ def _rows(a):
 def particle_cross_sections(p_scat, p_inc, p_src, k):
     """Per-multipole (scattering, absorption, extinction) of one particle, each [2, lmax]."""
     from .flux import cross_sections
-    rows = [_rows(a) for a in (p_scat, p_inc, p_src)]
+    rows = [np.asarray(a, dtype=complex).reshape(1, -1) for a in (p_scat, p_inc, p_src)]
     lmax = int(-1 + (1 + rows[0].shape[1] // 2) ** 0.5)
     C = _xs_rows(*rows, k, lmax)[0]
     return cross_sections(C[0], C[1], C[2])

@@ -1,0 +1,86 @@
+"""Many independent cluster solves in one batched launch sequence.
+
+Each cluster (positions, radii, permittivities, wavelength, lmax, sources)
+becomes one system (I + T_c) p_inc = p_src with one right-hand side per
+source.  Host work (Mie coefficients, p_src) is done once in `prepare`; the
+device work (assembly of every matrix, one batched BiCGSTAB or one batched
+complex64 + FP64-refinement solve over all clusters, observables) is
+`ClusterBatch.solve`.  The batched engine covers all clusters with one launch
+per kernel and iteration, and a cluster that converged costs nothing further.
+"""
+
+import numpy as np
+import torch
+
+from . import device as D
+from .mie import mie_coefficients
+
+
+class PreparedCluster:
+    """Device-resident inputs of one cluster workload."""
+
+    def __init__(self, w):
+        self.w = w
+        k = w.k
+        conducting = np.zeros(w.N, dtype=bool) if w.conducting is None else np.asarray(w.conducting)
+        ms, mi = mie_coefficients(w.radius, w.eps, np.ones(w.N), w.medium_eps, 1.0, k, w.lmax, conducting)
+        p_src = np.stack([s.structure(w.position, k, w.lmax) for s in w.sources])
+        self.S = p_src.shape[0]
+        self.n = w.n
+        self.k = k
+        self.eps_b = float(np.real(w.medium_eps))
+        self.pos = D.to_device(w.position, torch.float64)
+        self.mie_scat = D.to_device(ms, D.C128)
+        self.mie_int = D.to_device(mi, D.C128)
+        self.p_src = D.to_device(p_src, D.C128)
+        self.A64 = None
+        self.A32 = None
+
+    def matrix_bytes(self, precision):
+        return 16 * self.n * self.n + (8 * self.n * self.n if precision == "mixed" else 0)
+
+
+class ClusterBatch:
+    def __init__(self, workloads, precision="mixed", inner_tol=1e-5, maxiter=2000, max_refine=8):
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.precision = precision
+        self.inner_tol = inner_tol
+        self.maxiter = maxiter
+        self.max_refine = max_refine
+        self.clusters = [PreparedCluster(w) for w in workloads]
+        for c in self.clusters:
+            c.A64 = torch.empty((c.n, c.n), dtype=D.C128, device=c.pos.device)
+            if precision == "mixed":
+                c.A32 = torch.empty((c.n, c.n), dtype=D.C64, device=c.pos.device)
+        self.reports = None
+        self.results = None
+
+    def solve(self):
+        """Assemble, solve and reduce every cluster; results stay on the device."""
+        cs = self.clusters
+        for c in cs:
+            D.assemble(c.pos, c.mie_scat, c.k, dtype=D.C128, add_identity=True, out=c.A64)
+            if self.precision == "mixed":
+                D.assemble(c.pos, c.mie_scat, c.k, dtype=D.C64, add_identity=True, out=c.A32)
+        rhs = [c.p_src.reshape(c.S, c.n) for c in cs]
+        tols = [c.w.tol for c in cs]
+        if self.precision == "fp64":
+            xs, self.reports = D.bicgstab_batched([c.A64 for c in cs], rhs, tol=tols, maxiter=self.maxiter)
+        else:
+            xs, self.reports = D.solve_refined_batched([c.A32 for c in cs], [c.A64 for c in cs], rhs, tol=tols,
+                                                       inner_tol=self.inner_tol, maxiter=self.maxiter,
+                                                       max_refine=self.max_refine)
+        self.results = []
+        for c, x in zip(cs, xs):
+            p_inc = x.reshape(c.S, c.w.N, 2, c.w.rmax)
+            obs = D.observables(p_inc, c.p_src, c.mie_scat, c.mie_int, c.k, c.eps_b)
+            obs["p_inc"] = p_inc
+            self.results.append(obs)
+        return self.results
+
+    def converged(self):
+        return all(r.converged for r in self.reports)
+
+    def host_results(self):
+        return [{k: (v.cpu().numpy() if v is not None else None) for k, v in r.items()} for r in self.results]

@@ -1,0 +1,571 @@
+// Batched BiCGSTAB over many independent systems (I + T_c) x_c = b_c.
+//
+// Replaces the solve of cpp/src/interactions.cpp:347-368 (Eigen::BiCGSTAB,
+// identity preconditioner, row-block GEMV :80-91) for any number of systems
+// at once.  The iteration is Eigen's (IterativeLinearSolvers/BiCGSTAB.h):
+// zero initial guess, stop at ||r||^2 <= tol^2 ||b||^2 or after maxiter
+// iterations, restart when |rho| < eps^2 ||r0||^2.  The reference hard-codes
+// tol 1e-5 and maxiter 1000 (:362-363); here both are per-system inputs.
+//
+// Every system keeps its matrix, Krylov vectors and per-RHS scalars in HBM.
+// One launch of each kernel covers all systems of the batch through a work
+// list (system, RHS, workgroup), so a batch of 256 small clusters costs the
+// launches of one solve, and a system that converged turns every kernel that
+// touches it into a no-op: systems of uneven size and iteration count share
+// the launches, and new systems can join between iterations (the serving
+// loop's continuous batching).  Dot products are two-stage and deterministic.
+// A single solve is a batch of one.
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "gemv.hpp"
+#include "krylov.hpp"
+
+using namespace mpa;
+
+namespace {
+
+// ---- vector kernels (one workgroup per (system, RHS, group)) -------------------
+
+#define VEC_PROLOGUE                                     \
+    const int3 wk = work[blockIdx.x];                    \
+    const SysDev& d = sys[wk.x];                         \
+    const int s = wk.y, g = wk.z;                        \
+    const int n = d.n;                                   \
+    const size_t base = (size_t)s * n;                   \
+    const int stride = d.G * kThreads;
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_init(const SysDev* __restrict__ sys, const int3* __restrict__ work) {
+    using C = cplx_t<T>;
+    VEC_PROLOGUE
+    const C* b = (const C*)d.b;
+    C *x = (C*)d.x, *r = (C*)d.r, *r0 = (C*)d.r0, *p = (C*)d.p, *v = (C*)d.v;
+    double2 acc[1] = {make_double2(0, 0)};
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const size_t k = base + i;
+        const C bv = b[k];
+        r[k] = bv;
+        r0[k] = bv;
+        const C z = cmake<T>(0, 0);
+        p[k] = z;
+        v[k] = z;
+        x[k] = z;
+        const double2 bd = to_d(bv);
+        acc[0].x += bd.x * bd.x + bd.y * bd.y;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) d.part[((size_t)s * d.G + g) * kSlots + 3] = acc[0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_init_scalars(const SysDev* __restrict__ sys, const int2* __restrict__ cw) {
+    const int2 c2 = cw[blockIdx.x];
+    const SysDev& d = sys[c2.x];
+    const int s = c2.y;
+    const double2 bb = reduce_partials(d.part, d.G, s, 3);
+    if (threadIdx.x == 0) {
+        Scal& c = d.scal[s];
+        c.rho = c.alpha = c.w = make_double2(1, 0);
+        c.alpha_next = c.w_next = make_double2(1, 0);
+        c.rhs_sq = bb.x;
+        c.r0_sq = bb.x;
+        c.rnorm2 = bb.x;
+        c.tol2 = d.tol * d.tol * bb.x;
+        c.it = 0;
+        c.restarts = 0;
+        c.maxiter = d.maxiter;
+        c.status = MPA_OK;
+        c.restart_pending = 0;
+        c.rho_next = make_double2(bb.x, 0);  // <r0, r> = ||b||^2 with x0 = 0
+        c.done = (bb.x == 0.0 || !(bb.x > c.tol2) || d.maxiter <= 0) ? 1 : 0;
+        if (bb.x != bb.x) {
+            c.done = 1;
+            c.status = MPA_NONFINITE;
+        }
+    }
+}
+
+// p = r + beta (p - w v), beta = (rho_new / rho)(alpha / w)
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_update_p(const SysDev* __restrict__ sys, const int3* __restrict__ work) {
+    using C = cplx_t<T>;
+    VEC_PROLOGUE
+    const Scal& c = d.scal[s];
+    if (c.done || c.restart_pending) return;
+    const double2 beta = cmul(cdiv(c.rho_next, c.rho), cdiv(c.alpha, c.w));
+    const C bt = from_d<T>(beta), wt = from_d<T>(c.w);
+    C *p = (C*)d.p, *r = (C*)d.r, *v = (C*)d.v;
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const size_t k = base + i;
+        p[k] = cadd(r[k], cmul(bt, csub(p[k], cmul(wt, v[k]))));
+    }
+}
+
+// partial <r0, v>
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_dot_r0v(const SysDev* __restrict__ sys, const int3* __restrict__ work) {
+    using C = cplx_t<T>;
+    VEC_PROLOGUE
+    if (d.scal[s].done || d.scal[s].restart_pending) return;
+    const C *r0 = (const C*)d.r0, *v = (const C*)d.v;
+    double2 acc[1] = {make_double2(0, 0)};
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) acc[0] = cadd(acc[0], cmulc(to_d(r0[base + i]), to_d(v[base + i])));
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) d.part[((size_t)s * d.G + g) * kSlots + 0] = acc[0];
+}
+
+// alpha = rho / <r0, v>; s = r - alpha v
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_update_s(const SysDev* __restrict__ sys, const int3* __restrict__ work) {
+    using C = cplx_t<T>;
+    VEC_PROLOGUE
+    if (d.scal[s].done || d.scal[s].restart_pending) return;
+    const double2 dv = reduce_partials(d.part, d.G, s, 0);
+    const double2 alpha = cdiv(d.scal[s].rho_next, dv);
+    const C at = from_d<T>(alpha);
+    C *sv = (C*)d.s, *r = (C*)d.r, *v = (C*)d.v;
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) sv[base + i] = csub(r[base + i], cmul(at, v[base + i]));
+    if (g == 0 && threadIdx.x == 0) d.scal[s].alpha_next = alpha;
+}
+
+// partials <t, s>, <t, t>
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_dot_ts(const SysDev* __restrict__ sys, const int3* __restrict__ work) {
+    using C = cplx_t<T>;
+    VEC_PROLOGUE
+    if (d.scal[s].done || d.scal[s].restart_pending) return;
+    const C *tv = (const C*)d.t, *sv = (const C*)d.s;
+    double2 acc[2] = {make_double2(0, 0), make_double2(0, 0)};
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const double2 t = to_d(tv[base + i]);
+        acc[0] = cadd(acc[0], cmulc(t, to_d(sv[base + i])));
+        acc[1].x += t.x * t.x + t.y * t.y;
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        d.part[((size_t)s * d.G + g) * kSlots + 1] = acc[0];
+        d.part[((size_t)s * d.G + g) * kSlots + 2] = acc[1];
+    }
+}
+
+// w = <t,s>/<t,t>; x += alpha p + w s; r = s - w t; partials ||r||^2, <r0, r>
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_update_xr(const SysDev* __restrict__ sys, const int3* __restrict__ work) {
+    using C = cplx_t<T>;
+    VEC_PROLOGUE
+    if (d.scal[s].done || d.scal[s].restart_pending) return;
+    const double2 ts = reduce_partials(d.part, d.G, s, 1);
+    const double2 tt = reduce_partials(d.part, d.G, s, 2);
+    const double2 om = tt.x > 0 ? cscale(ts, 1.0 / tt.x) : make_double2(0, 0);
+    const C at = from_d<T>(d.scal[s].alpha_next), wt = from_d<T>(om);
+    C *x = (C*)d.x, *p = (C*)d.p, *sv = (C*)d.s, *tv = (C*)d.t, *r = (C*)d.r, *r0 = (C*)d.r0;
+    double2 acc[2] = {make_double2(0, 0), make_double2(0, 0)};
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const size_t k = base + i;
+        const C s_ = sv[k];
+        x[k] = cadd(x[k], cadd(cmul(at, p[k]), cmul(wt, s_)));
+        const C rv = csub(s_, cmul(wt, tv[k]));
+        r[k] = rv;
+        const double2 rd = to_d(rv);
+        acc[0].x += rd.x * rd.x + rd.y * rd.y;
+        acc[1] = cadd(acc[1], cmulc(to_d(r0[k]), rd));
+    }
+    block_sum<2>(acc);
+    if (threadIdx.x == 0) {
+        d.part[((size_t)s * d.G + g) * kSlots + 3] = acc[0];
+        d.part[((size_t)s * d.G + g) * kSlots + 4] = acc[1];
+    }
+    if (g == 0 && threadIdx.x == 0) d.scal[s].w_next = om;
+}
+
+// commit the iteration's scalars; convergence, maxiter and restart checks
+__global__ __launch_bounds__(kThreads) void k_commit(const SysDev* __restrict__ sys, const int2* __restrict__ cw) {
+    const int2 c2 = cw[blockIdx.x];
+    const SysDev& d = sys[c2.x];
+    const int s = c2.y;
+    Scal& c = d.scal[s];
+    if (c.done || c.restart_pending) return;
+    const double2 rr = reduce_partials(d.part, d.G, s, 3);
+    const double2 r0r = reduce_partials(d.part, d.G, s, 4);
+    if (threadIdx.x == 0) {
+        c.rho = c.rho_next;
+        c.alpha = c.alpha_next;
+        c.w = c.w_next;
+        c.rnorm2 = rr.x;
+        c.it += 1;
+        c.rho_next = r0r;
+        const double eps = 2.220446049250313e-16;  // NumTraits<double>::epsilon()
+        if (!(rr.x == rr.x) || !(r0r.x == r0r.x) || !(r0r.y == r0r.y)) {
+            c.done = 1;
+            c.status = MPA_NONFINITE;
+        } else if (!(rr.x > c.tol2)) {
+            c.done = 1;
+        } else if (c.it >= c.maxiter) {
+            c.done = 1;
+            c.status = MPA_NOT_CONVERGED;
+        } else if (sqrt(r0r.x * r0r.x + r0r.y * r0r.y) < eps * eps * c.r0_sq) {
+            c.restart_pending = 1;
+        }
+    }
+}
+
+// restart: r = b - t (t = A x computed just before), r0 = r, partial ||r||^2
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_restart(const SysDev* __restrict__ sys, const int3* __restrict__ work) {
+    using C = cplx_t<T>;
+    VEC_PROLOGUE
+    if (!d.scal[s].restart_pending) return;
+    const C *b = (const C*)d.b, *tv = (const C*)d.t;
+    C *r = (C*)d.r, *r0 = (C*)d.r0;
+    double2 acc[1] = {make_double2(0, 0)};
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const size_t k = base + i;
+        const C rv = csub(b[k], tv[k]);
+        r[k] = rv;
+        r0[k] = rv;
+        const double2 rd = to_d(rv);
+        acc[0].x += rd.x * rd.x + rd.y * rd.y;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) d.part[((size_t)s * d.G + g) * kSlots + 3] = acc[0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_restart_commit(const SysDev* __restrict__ sys, const int2* __restrict__ cw) {
+    const int2 c2 = cw[blockIdx.x];
+    const SysDev& d = sys[c2.x];
+    const int s = c2.y;
+    if (!d.scal[s].restart_pending) return;
+    const double2 rr = reduce_partials(d.part, d.G, s, 3);
+    if (threadIdx.x == 0) {
+        Scal& c = d.scal[s];
+        // Eigen: rho = r0_sqnorm = ||r||^2 replaces this iteration's rho; the
+        // committed rho stays as rho_old for beta = (rho/rho_old)(alpha/w).
+        c.r0_sq = rr.x;
+        c.rho_next = make_double2(rr.x, 0);
+        if (c.restarts++ == 0) c.it = 0;
+        c.restart_pending = 0;
+    }
+}
+
+// ---- host side ---------------------------------------------------------------------
+
+size_t vec_bytes(int n, int S, int esize) { return align256((size_t)esize * 2 * n * S); }
+
+size_t state_bytes(int n, int S, int dtype) {
+    const int es = dtype == MPA_C128 ? 8 : 4;
+    return 6 * vec_bytes(n, S, es) + align256(sizeof(Scal) * S) +
+           align256(sizeof(double2) * (size_t)S * groups_for(n) * kSlots);
+}
+
+// Scratch for descriptors and work lists of a batch (device).
+struct Plan {
+    std::vector<SysDev> sys;
+    std::vector<int3> vec_work;
+    std::vector<int2> scal_work;
+    std::vector<MvDev> mv_ps, mv_st, mv_xt;
+    std::vector<int2> mv_work[kTileClasses];  // (job, row block) per tile class; same for every job set
+    // device copies
+    SysDev* d_sys = nullptr;
+    int3* d_vec = nullptr;
+    int2* d_scal = nullptr;
+    MvDev *d_mv_ps = nullptr, *d_mv_st = nullptr, *d_mv_xt = nullptr;
+    int2* d_mv_work[kTileClasses] = {nullptr, nullptr, nullptr};
+    int n_mv_work[kTileClasses] = {0, 0, 0};
+};
+
+size_t plan_bytes(int nsys, const mpa_system* s, int dtype) {
+    size_t vec = 0, sc = 0, mvw = 0;
+    for (int i = 0; i < nsys; i++) {
+        vec += (size_t)s[i].nrhs * groups_for(s[i].n);
+        sc += s[i].nrhs;
+        mvw += gemv_row_blocks(s[i].n, s[i].nrhs);
+    }
+    (void)dtype;
+    return align256(sizeof(SysDev) * nsys) + align256(sizeof(int3) * vec) + align256(sizeof(int2) * sc) +
+           3 * align256(sizeof(MvDev) * nsys) + kTileClasses * 256 + align256(sizeof(int2) * mvw);
+}
+
+int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch, hipStream_t st) {
+    P.sys.resize(nsys);
+    P.vec_work.clear();
+    P.scal_work.clear();
+    P.mv_ps.resize(nsys);
+    P.mv_st.resize(nsys);
+    P.mv_xt.resize(nsys);
+    for (auto& w : P.mv_work) w.clear();
+    const int es = dtype == MPA_C128 ? 8 : 4;
+    for (int i = 0; i < nsys; i++) {
+        const mpa_system& q = S_[i];
+        SysDev& d = P.sys[i];
+        const int n = q.n, S = q.nrhs;
+        char* p = (char*)q.state;
+        const size_t vb = vec_bytes(n, S, es);
+        d.A = q.A;
+        d.b = q.b;
+        d.x = q.x;
+        d.r = p; p += vb;
+        d.r0 = p; p += vb;
+        d.p = p; p += vb;
+        d.v = p; p += vb;
+        d.s = p; p += vb;
+        d.t = p; p += vb;
+        d.scal = (Scal*)p; p += align256(sizeof(Scal) * S);
+        d.part = (double2*)p;
+        d.tol = q.tol;
+        d.n = n;
+        d.S = S;
+        d.G = groups_for(n);
+        d.add_identity = q.add_identity;
+        d.maxiter = q.maxiter;
+        d.pad = 0;
+        for (int s = 0; s < S; s++) {
+            P.scal_work.push_back(make_int2(i, s));
+            for (int g = 0; g < d.G; g++) P.vec_work.push_back(make_int3(i, s, g));
+        }
+        P.mv_ps[i] = MvDev{q.A, d.p, d.v, n, S, q.add_identity, i};
+        P.mv_st[i] = MvDev{q.A, d.s, d.t, n, S, q.add_identity, i};
+        P.mv_xt[i] = MvDev{q.A, q.x, d.t, n, S, q.add_identity, -1};
+        const int rb = gemv_row_blocks(n, S);
+        std::vector<int2>& mw = P.mv_work[gemv_class(S)];
+        for (int b = 0; b < rb; b++) mw.push_back(make_int2(i, b));
+    }
+    char* p = (char*)scratch;
+    P.d_sys = (SysDev*)p; p += align256(sizeof(SysDev) * nsys);
+    P.d_vec = (int3*)p; p += align256(sizeof(int3) * P.vec_work.size());
+    P.d_scal = (int2*)p; p += align256(sizeof(int2) * P.scal_work.size());
+    P.d_mv_ps = (MvDev*)p; p += align256(sizeof(MvDev) * nsys);
+    P.d_mv_st = (MvDev*)p; p += align256(sizeof(MvDev) * nsys);
+    P.d_mv_xt = (MvDev*)p; p += align256(sizeof(MvDev) * nsys);
+    for (int c = 0; c < kTileClasses; c++) {
+        P.d_mv_work[c] = (int2*)p;
+        P.n_mv_work[c] = (int)P.mv_work[c].size();
+        p += align256(sizeof(int2) * P.mv_work[c].size());
+    }
+    MPA_HIP_CHECK(hipMemcpyAsync(P.d_sys, P.sys.data(), sizeof(SysDev) * nsys, hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(P.d_vec, P.vec_work.data(), sizeof(int3) * P.vec_work.size(), hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(P.d_scal, P.scal_work.data(), sizeof(int2) * P.scal_work.size(), hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(P.d_mv_ps, P.mv_ps.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(P.d_mv_st, P.mv_st.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(P.d_mv_xt, P.mv_xt.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
+    for (int c = 0; c < kTileClasses; c++)
+        if (P.n_mv_work[c])
+            MPA_HIP_CHECK(hipMemcpyAsync(P.d_mv_work[c], P.mv_work[c].data(), sizeof(int2) * P.n_mv_work[c],
+                                         hipMemcpyHostToDevice, st));
+    // the host vectors are staged by the runtime before the calls return
+    return MPA_OK;
+}
+
+template <typename T>
+int launch_start(const Plan& P, hipStream_t st) {
+    const int nv = (int)P.vec_work.size(), ns = (int)P.scal_work.size();
+    hipLaunchKernelGGL(k_init<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    hipLaunchKernelGGL(k_init_scalars, dim3(ns), dim3(kThreads), 0, st, P.d_sys, P.d_scal);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+template <typename T>
+int launch_iteration(const Plan& P, hipStream_t st) {
+    const int nv = (int)P.vec_work.size(), ns = (int)P.scal_work.size();
+    hipLaunchKernelGGL(k_update_p<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    int rc = gemv_launch<T>(P.d_mv_ps, P.d_sys, P.d_mv_work, P.n_mv_work, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dot_r0v<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    hipLaunchKernelGGL(k_update_s<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    rc = gemv_launch<T>(P.d_mv_st, P.d_sys, P.d_mv_work, P.n_mv_work, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_dot_ts<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    hipLaunchKernelGGL(k_update_xr<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    hipLaunchKernelGGL(k_commit, dim3(ns), dim3(kThreads), 0, st, P.d_sys, P.d_scal);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+template <typename T>
+int launch_restart(const Plan& P, hipStream_t st) {
+    const int nv = (int)P.vec_work.size(), ns = (int)P.scal_work.size();
+    int rc = gemv_launch<T>(P.d_mv_xt, P.d_sys, P.d_mv_work, P.n_mv_work, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_restart<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    hipLaunchKernelGGL(k_restart_commit, dim3(ns), dim3(kThreads), 0, st, P.d_sys, P.d_scal);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+// Read back the scalars of every RHS (synchronizes the stream).
+int read_scalars(const Plan& P, std::vector<Scal>& out, hipStream_t st) {
+    size_t total = P.scal_work.size();
+    out.resize(total);
+    size_t k = 0;
+    for (const SysDev& d : P.sys) {
+        MPA_HIP_CHECK(hipMemcpyAsync(&out[k], d.scal, sizeof(Scal) * d.S, hipMemcpyDeviceToHost, st));
+        k += d.S;
+    }
+    MPA_HIP_CHECK(hipStreamSynchronize(st));
+    return MPA_OK;
+}
+
+template <typename T>
+int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, double* resid, int* status, int sync_every,
+                hipStream_t st) {
+    Plan P;
+    const int dtype = sizeof(T) == 8 ? MPA_C128 : MPA_C64;
+    int rc = build_plan(P, nsys, S_, dtype, scratch, st);
+    if (rc) return rc;
+    rc = launch_start<T>(P, st);
+    if (rc) return rc;
+    int maxit = 0;
+    for (int i = 0; i < nsys; i++) maxit = std::max(maxit, S_[i].maxiter);
+    const int k_sync = sync_every > 0 ? sync_every : 1;
+    std::vector<Scal> host;
+    long launched = 0;
+    for (;;) {
+        rc = read_scalars(P, host, st);
+        if (rc) return rc;
+        bool all_done = true, any_restart = false;
+        for (const Scal& c : host) {
+            all_done &= c.done != 0;
+            any_restart |= c.restart_pending != 0;
+        }
+        if (any_restart) {
+            rc = launch_restart<T>(P, st);
+            if (rc) return rc;
+            continue;
+        }
+        if (all_done) break;
+        if (launched > 4L * (maxit + 8)) {
+            mpa_set_error("bicgstab: iteration budget exhausted");
+            return MPA_NOT_CONVERGED;
+        }
+        for (int k = 0; k < k_sync; k++) {
+            rc = launch_iteration<T>(P, st);
+            if (rc) return rc;
+        }
+        launched += k_sync;
+    }
+    int worst = MPA_OK;
+    for (size_t k = 0; k < host.size(); k++) {
+        if (iters) iters[k] = host[k].it;
+        if (resid) resid[k] = host[k].rhs_sq > 0 ? std::sqrt(host[k].rnorm2 / host[k].rhs_sq) : 0.0;
+        if (status) status[k] = host[k].status;
+        if (host[k].status != MPA_OK && worst == MPA_OK) worst = host[k].status;
+    }
+    return worst;
+}
+
+bool valid_systems(int nsys, const mpa_system* s) {
+    if (nsys < 1 || !s) return false;
+    for (int i = 0; i < nsys; i++)
+        if (s[i].n < 2 || (s[i].n & 1) || s[i].nrhs < 1 || !s[i].A || !s[i].b || !s[i].x || !s[i].state ||
+            !(s[i].tol >= 0) || s[i].maxiter < 0)
+            return false;
+    return true;
+}
+
+}  // namespace
+
+// ---- batched engine, exposed for the refinement driver and the serving loop ------
+
+int mpa_internal_solve_batch(int nsys, const mpa_system* systems, int dtype, void* scratch, int* iters, double* resid,
+                             int* status, int sync_every, hipStream_t st) {
+    if (dtype == MPA_C128) return solve_batch<double>(nsys, systems, scratch, iters, resid, status, sync_every, st);
+    return solve_batch<float>(nsys, systems, scratch, iters, resid, status, sync_every, st);
+}
+
+extern "C" {
+
+size_t mpa_system_state_bytes(int n, int nrhs, int dtype) { return state_bytes(n, nrhs, dtype); }
+
+size_t mpa_batch_scratch_bytes(int nsys, const mpa_system* systems, int dtype) {
+    return plan_bytes(nsys, systems, dtype);
+}
+
+int mpa_bicgstab_batched(int nsys, const mpa_system* systems, int dtype, void* scratch, int* iters_out,
+                         double* resid_out, int* status_out, int sync_every, void* stream) {
+    if (!valid_systems(nsys, systems) || !scratch || (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("bicgstab_batched: bad argument");
+        return MPA_ERR_ARG;
+    }
+    return mpa_internal_solve_batch(nsys, systems, dtype, scratch, iters_out, resid_out, status_out, sync_every,
+                                    (hipStream_t)stream);
+}
+
+int mpa_systems_start(int nsys, const mpa_system* systems, int dtype, void* scratch, void* stream) {
+    if (!valid_systems(nsys, systems) || !scratch || (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("systems_start: bad argument");
+        return MPA_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    Plan P;
+    int rc = build_plan(P, nsys, systems, dtype, scratch, st);
+    if (rc) return rc;
+    return dtype == MPA_C128 ? launch_start<double>(P, st) : launch_start<float>(P, st);
+}
+
+int mpa_systems_advance(int nsys, const mpa_system* systems, int dtype, void* scratch, int iterations, int* done_out,
+                        int* iters_out, double* resid_out, int* status_out, void* stream) {
+    if (!valid_systems(nsys, systems) || !scratch || iterations < 0 || (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("systems_advance: bad argument");
+        return MPA_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    Plan P;
+    int rc = build_plan(P, nsys, systems, dtype, scratch, st);
+    if (rc) return rc;
+    std::vector<Scal> host;
+    rc = read_scalars(P, host, st);
+    if (rc) return rc;
+    bool restart = false;
+    for (const Scal& c : host) restart |= c.restart_pending != 0;
+    if (restart) {
+        rc = dtype == MPA_C128 ? launch_restart<double>(P, st) : launch_restart<float>(P, st);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < iterations; k++) {
+        rc = dtype == MPA_C128 ? launch_iteration<double>(P, st) : launch_iteration<float>(P, st);
+        if (rc) return rc;
+    }
+    rc = read_scalars(P, host, st);
+    if (rc) return rc;
+    for (size_t k = 0; k < host.size(); k++) {
+        if (done_out) done_out[k] = host[k].done;
+        if (iters_out) iters_out[k] = host[k].it;
+        if (resid_out) resid_out[k] = host[k].rhs_sq > 0 ? std::sqrt(host[k].rnorm2 / host[k].rhs_sq) : 0.0;
+        if (status_out) status_out[k] = host[k].status;
+    }
+    return MPA_OK;
+}
+
+size_t mpa_bicgstab_workspace_bytes(int n, int nrhs, int dtype) {
+    mpa_system s{};
+    s.n = n;
+    s.nrhs = nrhs;
+    return state_bytes(n, nrhs, dtype) + plan_bytes(1, &s, dtype);
+}
+
+int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, const void* b, void* x, double tol,
+                 int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out, int sync_every,
+                 void* stream) {
+    if (n < 2 || (n & 1) || nrhs < 1 || !A || !b || !x || !workspace || !(tol >= 0) || maxiter < 0 ||
+        (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("bicgstab: bad argument (n must be even and positive)");
+        return MPA_ERR_ARG;
+    }
+    mpa_system s{};
+    s.n = n;
+    s.nrhs = nrhs;
+    s.add_identity = add_identity;
+    s.A = A;
+    s.b = b;
+    s.x = x;
+    s.state = workspace;
+    s.tol = tol;
+    s.maxiter = maxiter;
+    void* scratch = (char*)workspace + state_bytes(n, nrhs, dtype);
+    return mpa_internal_solve_batch(1, &s, dtype, scratch, iters_out, resid_out, status_out, sync_every,
+                                    (hipStream_t)stream);
+}
+
+}  // extern "C"

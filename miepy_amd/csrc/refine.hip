@@ -1,0 +1,350 @@
+// Mixed precision for batches of cluster systems: complex64 BiCGSTAB
+// corrections, FP64 iterative refinement.
+//
+// Solves the reference's system (I + T) x = b (cpp/src/interactions.cpp:347-368)
+// to an FP64 tolerance.  Per outer step and per RHS still above tolerance:
+//   r = b - A64 x (FP64), d = BiCGSTAB_c64(A32, r / ||r||) to inner_tol,
+//   x += ||r|| d (FP64).
+// Outer steps stop per RHS at ||r|| <= tol ||b||, after max_refine
+// corrections, or when a correction reduces ||r|| by less than 4x.  The
+// inner solves of all systems run as one batch of the complex64 engine;
+// the FP64 residual matvecs of all systems run as one batched c128 matvec.
+
+#include <cmath>
+#include <vector>
+
+#include "gemv.hpp"
+#include "krylov.hpp"
+
+using namespace mpa;
+
+int mpa_internal_solve_batch(int nsys, const mpa_system* systems, int dtype, void* scratch, int* iters, double* resid,
+                             int* status, int sync_every, hipStream_t st);
+
+namespace {
+
+struct RefDev {
+    const double2* b;
+    double2* x;
+    double2* r;
+    double2* y;
+    float2* r32;
+    float2* d32;
+    double2* part;  // [S][G]
+    double* norms;  // [S]
+    double* scale;  // [S]
+    double* unscale;  // [S]
+    int n, S, G, pad;
+};
+
+#define REF_PROLOGUE                       \
+    const int3 wk = work[blockIdx.x];      \
+    const RefDev& d = sys[wk.x];           \
+    const int s = wk.y, g = wk.z;          \
+    const int n = d.n;                     \
+    const size_t base = (size_t)s * n;     \
+    const int stride = d.G * kThreads;
+
+// x = 0, r = b, partial ||b||^2
+__global__ __launch_bounds__(kThreads) void k_ref_start(const RefDev* __restrict__ sys, const int3* __restrict__ work) {
+    REF_PROLOGUE
+    double2 acc[1] = {make_double2(0, 0)};
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const double2 bv = d.b[base + i];
+        d.r[base + i] = bv;
+        d.x[base + i] = make_double2(0, 0);
+        acc[0].x += bv.x * bv.x + bv.y * bv.y;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) d.part[(size_t)s * d.G + g] = acc[0];
+}
+
+// norms[s] = sum of the partials (fixed order), one block per (system, RHS)
+__global__ __launch_bounds__(kThreads) void k_ref_norms(const RefDev* __restrict__ sys, const int2* __restrict__ cw) {
+    const int2 c2 = cw[blockIdx.x];
+    const RefDev& d = sys[c2.x];
+    const int s = c2.y;
+    __shared__ double red[kThreads];
+    double a = 0;
+    for (int g = threadIdx.x; g < d.G; g += kThreads) a += d.part[(size_t)s * d.G + g].x;
+    red[threadIdx.x] = a;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) d.norms[s] = red[0];
+}
+
+// r32 = r * scale, rounded to complex64
+__global__ __launch_bounds__(kThreads) void k_ref_to32(const RefDev* __restrict__ sys, const int3* __restrict__ work) {
+    REF_PROLOGUE
+    const double f = d.scale[s];
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const double2 z = d.r[base + i];
+        d.r32[base + i] = make_float2((float)(z.x * f), (float)(z.y * f));
+    }
+}
+
+// x += d32 * unscale (FP64)
+__global__ __launch_bounds__(kThreads) void k_ref_accum(const RefDev* __restrict__ sys, const int3* __restrict__ work) {
+    REF_PROLOGUE
+    const double f = d.unscale[s];
+    if (f == 0.0) return;
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const float2 z = d.d32[base + i];
+        double2 xv = d.x[base + i];
+        xv.x += (double)z.x * f;
+        xv.y += (double)z.y * f;
+        d.x[base + i] = xv;
+    }
+}
+
+// r = b - y, partial ||r||^2
+__global__ __launch_bounds__(kThreads) void k_ref_residual(const RefDev* __restrict__ sys, const int3* __restrict__ work) {
+    REF_PROLOGUE
+    double2 acc[1] = {make_double2(0, 0)};
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const double2 rv = csub(d.b[base + i], d.y[base + i]);
+        d.r[base + i] = rv;
+        acc[0].x += rv.x * rv.x + rv.y * rv.y;
+    }
+    block_sum<1>(acc);
+    if (threadIdx.x == 0) d.part[(size_t)s * d.G + g] = acc[0];
+}
+
+size_t v64(int n, int S) { return align256(sizeof(double2) * (size_t)n * S); }
+size_t v32(int n, int S) { return align256(sizeof(float2) * (size_t)n * S); }
+
+size_t own_bytes(int n, int S) {
+    return 2 * v64(n, S) + 2 * v32(n, S) + align256(sizeof(double2) * (size_t)S * groups_for(n)) +
+           3 * align256(sizeof(double) * S);
+}
+
+bool valid(int nsys, const mpa_refine_system* s) {
+    if (nsys < 1 || !s) return false;
+    for (int i = 0; i < nsys; i++)
+        if (s[i].n < 2 || (s[i].n & 1) || s[i].nrhs < 1 || !s[i].A32 || !s[i].A64 || !s[i].b || !s[i].x || !s[i].state ||
+            !(s[i].tol > 0) || !(s[i].inner_tol > 0) || s[i].maxiter < 1 || s[i].max_refine < 1)
+            return false;
+    return true;
+}
+
+int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer_out, int* inner_out,
+                 double* resid_out, int* status_out, hipStream_t st) {
+    // descriptors
+    std::vector<RefDev> ref(nsys);
+    std::vector<mpa_system> inner(nsys);
+    std::vector<MvDev> mv(nsys);
+    std::vector<int3> vw;
+    std::vector<int2> cw;
+    std::vector<int2> mw[kTileClasses];
+    int total = 0;
+    for (int i = 0; i < nsys; i++) {
+        const mpa_refine_system& q = R[i];
+        const int n = q.n, S = q.nrhs;
+        RefDev& d = ref[i];
+        char* p = (char*)q.state;
+        d.b = (const double2*)q.b;
+        d.x = (double2*)q.x;
+        d.r = (double2*)p; p += v64(n, S);
+        d.y = (double2*)p; p += v64(n, S);
+        d.r32 = (float2*)p; p += v32(n, S);
+        d.d32 = (float2*)p; p += v32(n, S);
+        d.part = (double2*)p; p += align256(sizeof(double2) * (size_t)S * groups_for(n));
+        d.norms = (double*)p; p += align256(sizeof(double) * S);
+        d.scale = (double*)p; p += align256(sizeof(double) * S);
+        d.unscale = (double*)p; p += align256(sizeof(double) * S);
+        d.n = n;
+        d.S = S;
+        d.G = groups_for(n);
+        d.pad = 0;
+        mpa_system& in = inner[i];
+        in.n = n;
+        in.nrhs = S;
+        in.add_identity = q.add_identity;
+        in.A = q.A32;
+        in.b = d.r32;
+        in.x = d.d32;
+        in.state = p;
+        in.tol = q.inner_tol;
+        in.maxiter = q.maxiter;
+        mv[i] = MvDev{q.A64, q.x, d.y, n, S, q.add_identity, -1};
+        for (int s = 0; s < S; s++) {
+            cw.push_back(make_int2(i, s));
+            for (int g = 0; g < d.G; g++) vw.push_back(make_int3(i, s, g));
+        }
+        const int rb = gemv_row_blocks(n, S);
+        for (int b = 0; b < rb; b++) mw[gemv_class(S)].push_back(make_int2(i, b));
+        total += S;
+    }
+    char* p = (char*)scratch;
+    RefDev* d_ref = (RefDev*)p; p += align256(sizeof(RefDev) * nsys);
+    MvDev* d_mv = (MvDev*)p; p += align256(sizeof(MvDev) * nsys);
+    int3* d_vw = (int3*)p; p += align256(sizeof(int3) * vw.size());
+    int2* d_cw = (int2*)p; p += align256(sizeof(int2) * cw.size());
+    int2* d_mw[kTileClasses];
+    int n_mw[kTileClasses];
+    for (int c = 0; c < kTileClasses; c++) {
+        d_mw[c] = (int2*)p;
+        n_mw[c] = (int)mw[c].size();
+        p += align256(sizeof(int2) * mw[c].size());
+    }
+    void* inner_scratch = p;
+    MPA_HIP_CHECK(hipMemcpyAsync(d_ref, ref.data(), sizeof(RefDev) * nsys, hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(d_mv, mv.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(d_vw, vw.data(), sizeof(int3) * vw.size(), hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(d_cw, cw.data(), sizeof(int2) * cw.size(), hipMemcpyHostToDevice, st));
+    for (int c = 0; c < kTileClasses; c++)
+        if (n_mw[c]) MPA_HIP_CHECK(hipMemcpyAsync(d_mw[c], mw[c].data(), sizeof(int2) * n_mw[c], hipMemcpyHostToDevice, st));
+
+    const int nv = (int)vw.size(), nc = (int)cw.size();
+    hipLaunchKernelGGL(k_ref_start, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
+    hipLaunchKernelGGL(k_ref_norms, dim3(nc), dim3(kThreads), 0, st, d_ref, d_cw);
+    MPA_LAUNCH_CHECK();
+    // host copies of per-RHS state, flattened in (system, RHS) order
+    std::vector<double> bn2(total), rn2(total), prev(total, INFINITY), sc(total), usc(total);
+    std::vector<int> outer(total, 0), inner_it(total, 0), status(total, MPA_NOT_CONVERGED);
+    std::vector<int> it(total), ist(total);
+    std::vector<double> ires(total);
+    auto read_norms = [&](std::vector<double>& out) -> int {
+        int k = 0;
+        for (int i = 0; i < nsys; i++) {
+            MPA_HIP_CHECK(hipMemcpyAsync(&out[k], ref[i].norms, sizeof(double) * ref[i].S, hipMemcpyDeviceToHost, st));
+            k += ref[i].S;
+        }
+        MPA_HIP_CHECK(hipStreamSynchronize(st));
+        return MPA_OK;
+    };
+    int rc = read_norms(bn2);
+    if (rc) return rc;
+    rn2 = bn2;
+    for (int step = 0;; step++) {
+        bool any = false;
+        int k = 0;
+        for (int i = 0; i < nsys; i++) {
+            for (int s = 0; s < ref[i].S; s++, k++) {
+                const double tol = R[i].tol;
+                int stt;
+                if (!std::isfinite(rn2[k])) stt = MPA_NONFINITE;
+                else if (rn2[k] <= tol * tol * bn2[k]) stt = MPA_OK;
+                else if (step > 0 && rn2[k] > 0.0625 * prev[k]) stt = MPA_NOT_CONVERGED;  // < 4x per step
+                else if (step >= R[i].max_refine) stt = MPA_NOT_CONVERGED;
+                else stt = -1;
+                status[k] = stt < 0 ? MPA_NOT_CONVERGED : stt;
+                const bool active = stt < 0;
+                sc[k] = active ? 1.0 / std::sqrt(rn2[k]) : 0.0;
+                usc[k] = active ? std::sqrt(rn2[k]) : 0.0;
+                any |= active;
+            }
+        }
+        if (!any) break;
+        k = 0;
+        for (int i = 0; i < nsys; i++) {
+            MPA_HIP_CHECK(hipMemcpyAsync(ref[i].scale, &sc[k], sizeof(double) * ref[i].S, hipMemcpyHostToDevice, st));
+            MPA_HIP_CHECK(hipMemcpyAsync(ref[i].unscale, &usc[k], sizeof(double) * ref[i].S, hipMemcpyHostToDevice, st));
+            k += ref[i].S;
+        }
+        hipLaunchKernelGGL(k_ref_to32, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
+        MPA_LAUNCH_CHECK();
+        rc = mpa_internal_solve_batch(nsys, inner.data(), MPA_C64, inner_scratch, it.data(), ires.data(), ist.data(), 1,
+                                      st);
+        if (rc != MPA_OK && rc != MPA_NOT_CONVERGED && rc != MPA_NONFINITE) return rc;
+        for (int j = 0; j < total; j++)
+            if (usc[j] != 0.0) {
+                inner_it[j] += it[j];
+                outer[j] = step + 1;
+            }
+        hipLaunchKernelGGL(k_ref_accum, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
+        MPA_LAUNCH_CHECK();
+        rc = gemv_launch<double>(d_mv, nullptr, d_mw, n_mw, st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_ref_residual, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
+        hipLaunchKernelGGL(k_ref_norms, dim3(nc), dim3(kThreads), 0, st, d_ref, d_cw);
+        MPA_LAUNCH_CHECK();
+        prev = rn2;
+        rc = read_norms(rn2);
+        if (rc) return rc;
+    }
+    int worst = MPA_OK;
+    for (int j = 0; j < total; j++) {
+        if (outer_out) outer_out[j] = outer[j];
+        if (inner_out) inner_out[j] = inner_it[j];
+        if (resid_out) resid_out[j] = bn2[j] > 0 ? std::sqrt(rn2[j] / bn2[j]) : 0.0;
+        if (status_out) status_out[j] = status[j];
+        if (status[j] != MPA_OK && worst == MPA_OK) worst = status[j];
+    }
+    return worst;
+}
+
+size_t refine_scratch_bytes(int nsys, const mpa_refine_system* R) {
+    size_t vw = 0, cw = 0, mw = 0;
+    std::vector<mpa_system> inner(nsys);
+    for (int i = 0; i < nsys; i++) {
+        vw += (size_t)R[i].nrhs * groups_for(R[i].n);
+        cw += R[i].nrhs;
+        mw += gemv_row_blocks(R[i].n, R[i].nrhs);
+        inner[i].n = R[i].n;
+        inner[i].nrhs = R[i].nrhs;
+    }
+    return align256(sizeof(RefDev) * nsys) + align256(sizeof(MvDev) * nsys) + align256(sizeof(int3) * vw) +
+           align256(sizeof(int2) * cw) + kTileClasses * 256 + align256(sizeof(int2) * mw) +
+           mpa_batch_scratch_bytes(nsys, inner.data(), MPA_C64);
+}
+
+}  // namespace
+
+extern "C" {
+
+size_t mpa_refine_state_bytes(int n, int nrhs) { return own_bytes(n, nrhs) + mpa_system_state_bytes(n, nrhs, MPA_C64); }
+
+size_t mpa_refine_scratch_bytes(int nsys, const mpa_refine_system* systems) {
+    return refine_scratch_bytes(nsys, systems);
+}
+
+int mpa_solve_refined_batched(int nsys, const mpa_refine_system* systems, void* scratch, int* outer_out,
+                              int* inner_out, double* resid_out, int* status_out, void* stream) {
+    if (!valid(nsys, systems) || !scratch) {
+        mpa_set_error("solve_refined_batched: bad argument");
+        return MPA_ERR_ARG;
+    }
+    return refine_batch(nsys, systems, scratch, outer_out, inner_out, resid_out, status_out, (hipStream_t)stream);
+}
+
+size_t mpa_refine_workspace_bytes(int n, int nrhs) {
+    mpa_refine_system s{};
+    s.n = n;
+    s.nrhs = nrhs;
+    return mpa_refine_state_bytes(n, nrhs) + refine_scratch_bytes(1, &s);
+}
+
+int mpa_solve_refined(int n, int nrhs, const void* A32, int add_identity32, const void* A64, int add_identity64,
+                      const void* b, void* x, double tol, double inner_tol, int maxiter, int max_refine,
+                      void* workspace, int* outer_out, int* inner_out, double* resid_out, int* status_out,
+                      void* stream) {
+    if (add_identity32 != add_identity64) {
+        mpa_set_error("solve_refined: A32 and A64 must describe the same system (add_identity differs)");
+        return MPA_ERR_ARG;
+    }
+    mpa_refine_system s{};
+    s.n = n;
+    s.nrhs = nrhs;
+    s.add_identity = add_identity32;
+    s.A32 = A32;
+    s.A64 = A64;
+    s.b = b;
+    s.x = x;
+    s.state = workspace;
+    s.tol = tol;
+    s.inner_tol = inner_tol;
+    s.maxiter = maxiter;
+    s.max_refine = max_refine;
+    if (!valid(1, &s)) {
+        mpa_set_error("solve_refined: bad argument");
+        return MPA_ERR_ARG;
+    }
+    void* scratch = (char*)workspace + mpa_refine_state_bytes(n, nrhs);
+    return refine_batch(1, &s, scratch, outer_out, inner_out, resid_out, status_out, (hipStream_t)stream);
+}
+
+}  // extern "C"

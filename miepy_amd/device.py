@@ -197,3 +197,98 @@ def solve_refined(A32, A64, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refi
     if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED):
         L.check(rc, "solve_refined")
     return x.reshape(b.shape), RefineReport(list(outer), list(inner), list(res), list(st))
+
+
+def _sys_struct(A, b, x, state, tol, maxiter, add_identity):
+    n = A.shape[0]
+    s = L.MpaSystem()
+    s.n = n
+    s.nrhs = b.numel() // n
+    s.add_identity = 1 if add_identity else 0
+    s.A = A.data_ptr()
+    s.b = b.data_ptr()
+    s.x = x.data_ptr()
+    s.state = state.data_ptr()
+    s.tol = float(tol)
+    s.maxiter = int(maxiter)
+    return s
+
+
+def bicgstab_batched(matrices, rhs, tol=1e-5, maxiter=1000, add_identity=False, sync_every=1):
+    """Solve several independent systems in one batched BiCGSTAB.
+
+    matrices: list of [n_i, n_i] tensors (one dtype); rhs: list of [S_i, n_i].
+    tol / maxiter: scalars or per-system lists.  Returns (list of x, list of SolveReport)."""
+    lib = L.load()
+    k = len(matrices)
+    dtype = matrices[0].dtype
+    code = DTYPE_CODE[dtype]
+    tols = tol if isinstance(tol, (list, tuple)) else [tol] * k
+    maxits = maxiter if isinstance(maxiter, (list, tuple)) else [maxiter] * k
+    xs, states, structs = [], [], (L.MpaSystem * k)()
+    for i, (A, b) in enumerate(zip(matrices, rhs)):
+        n = A.shape[0]
+        b2 = b.reshape(-1, n)
+        if A.dtype != dtype or b2.dtype != dtype or not b2.is_contiguous():
+            raise ValueError("all matrices and right-hand sides need one dtype and contiguous storage")
+        x = torch.empty_like(b2)
+        st = torch.empty(lib.mpa_system_state_bytes(n, b2.shape[0], code), dtype=torch.uint8, device=A.device)
+        xs.append(x)
+        states.append(st)
+        structs[i] = _sys_struct(A, b2, x, st, tols[i], maxits[i], add_identity)
+    scratch = torch.empty(lib.mpa_batch_scratch_bytes(k, structs, code), dtype=torch.uint8, device=matrices[0].device)
+    total = sum(x.shape[0] for x in xs)
+    it = (ctypes.c_int * total)()
+    res = (ctypes.c_double * total)()
+    stt = (ctypes.c_int * total)()
+    rc = lib.mpa_bicgstab_batched(k, structs, code, _ptr(scratch), it, res, stt, int(sync_every), _stream())
+    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED, L.MPA_NONFINITE):
+        L.check(rc, "bicgstab_batched")
+    reports, o = [], 0
+    for x in xs:
+        S = x.shape[0]
+        reports.append(SolveReport(list(it[o:o + S]), list(res[o:o + S]), list(stt[o:o + S])))
+        o += S
+    return xs, reports
+
+
+def solve_refined_batched(A32s, A64s, rhs, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refine=8,
+                          add_identity=False):
+    """Mixed-precision solve of several systems at once (c64 corrections, FP64 refinement).
+
+    A32s: list of [n_i, n_i] c64, A64s: list of [n_i, n_i] c128, rhs: list of [S_i, n_i] c128."""
+    lib = L.load()
+    k = len(A32s)
+    tols = tol if isinstance(tol, (list, tuple)) else [tol] * k
+    structs = (L.MpaRefineSystem * k)()
+    xs, states = [], []
+    for i, (A32, A64, b) in enumerate(zip(A32s, A64s, rhs)):
+        n = A32.shape[0]
+        b2 = b.reshape(-1, n)
+        if A32.dtype != C64 or A64.dtype != C128 or b2.dtype != C128 or not b2.is_contiguous():
+            raise ValueError("solve_refined_batched needs c64 / c128 matrices and contiguous c128 right-hand sides")
+        x = torch.empty_like(b2)
+        st = torch.empty(lib.mpa_refine_state_bytes(n, b2.shape[0]), dtype=torch.uint8, device=A32.device)
+        xs.append(x)
+        states.append(st)
+        s = L.MpaRefineSystem()
+        s.n, s.nrhs, s.add_identity = n, b2.shape[0], 1 if add_identity else 0
+        s.A32, s.A64, s.b, s.x, s.state = A32.data_ptr(), A64.data_ptr(), b2.data_ptr(), x.data_ptr(), st.data_ptr()
+        s.tol, s.inner_tol, s.maxiter, s.max_refine = float(tols[i]), float(inner_tol), int(maxiter), int(max_refine)
+        structs[i] = s
+    scratch = torch.empty(lib.mpa_refine_scratch_bytes(k, structs), dtype=torch.uint8, device=A32s[0].device)
+    total = sum(x.shape[0] for x in xs)
+    outer = (ctypes.c_int * total)()
+    inner = (ctypes.c_int * total)()
+    res = (ctypes.c_double * total)()
+    stt = (ctypes.c_int * total)()
+    rc = lib.mpa_solve_refined_batched(k, structs, _ptr(scratch), outer, inner, res, stt, _stream())
+    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED):
+        L.check(rc, "solve_refined_batched")
+    reports, o = [], 0
+    for x in xs:
+        S = x.shape[0]
+        reports.append(RefineReport(list(outer[o:o + S]), list(inner[o:o + S]), list(res[o:o + S]),
+                                    list(stt[o:o + S])))
+        o += S
+    return xs, reports

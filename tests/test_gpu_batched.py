@@ -1,0 +1,116 @@
+"""Batched solves (many independent cluster systems per launch) through the
+C ABI: every system's result equals its own single-system solve bit for bit
+(the per-system arithmetic and reduction order do not depend on the batch),
+and agrees with the CPU oracle's Eigen-style BiCGSTAB iteration counts."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+nm = 1e-9
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+    import miepy_amd.device as D
+    D.require_gpu()
+    return D
+
+
+def _systems(dev, seed=0, count=6):
+    from miepy_amd.workloads import random_cluster, random_plane_waves
+    from miepy_amd.mie import mie_coefficients
+    rng = np.random.default_rng(seed)
+    out = []
+    for c in range(count):
+        N = int(rng.integers(3, 40))
+        lmax = int(rng.choice([1, 2, 3, 4]))
+        S = int(rng.choice([1, 2, 3, 5]))
+        a = 80 * nm
+        pos = random_cluster(N, 600 * nm * N ** (1 / 3), a, rng)
+        k = 2 * np.pi / (rng.uniform(450, 900) * nm)
+        eps = complex(rng.uniform(2, 6), rng.uniform(0, 0.2)) if c % 2 else -15 + 1j
+        ms, _ = mie_coefficients(np.full(N, a), np.full(N, eps), np.ones(N), 1.0, 1.0, k, lmax)
+        srcs = random_plane_waves(S, rng, random_pol=True)
+        b = np.stack([s.structure(pos, k, lmax).reshape(-1) for s in srcs])
+        out.append((pos, ms, k, b))
+    return out
+
+
+def test_batched_bicgstab_equals_single(dev, oracle):
+    systems = _systems(dev)
+    As, bs, As32, bs32 = [], [], [], []
+    for pos, ms, k, b in systems:
+        As.append(dev.assemble(pos, ms, k, add_identity=True))
+        bs.append(dev.to_device(b, dev.C128))
+    xs, reps = dev.bicgstab_batched(As, bs, tol=1e-10, maxiter=2000)
+    for i, (A, b) in enumerate(zip(As, bs)):
+        x1, rep1 = dev.bicgstab(A, b, tol=1e-10, maxiter=2000)
+        assert reps[i].iterations == rep1.iterations, (i, reps[i], rep1)
+        assert reps[i].converged
+        assert np.array_equal(xs[i].cpu().numpy(), x1.cpu().numpy())
+        An = A.cpu().numpy()
+        for s in range(b.shape[0]):
+            bn = b[s].cpu().numpy()
+            _, it_o, _ = oracle.bicgstab_eigen(An, bn, tol=1e-10, maxiter=2000)
+            assert abs(reps[i].iterations[s] - it_o) <= 1
+            r = np.linalg.norm(An @ xs[i][s].cpu().numpy() - bn) / np.linalg.norm(bn)
+            assert r <= 1.01e-10
+
+
+def test_batched_refined_equals_single(dev):
+    systems = _systems(dev, seed=1, count=5)
+    A32s, A64s, bs = [], [], []
+    for pos, ms, k, b in systems:
+        A64s.append(dev.assemble(pos, ms, k, add_identity=True))
+        A32s.append(dev.assemble(pos, ms, k, dtype=dev.C64, add_identity=True))
+        bs.append(dev.to_device(b, dev.C128))
+    xs, reps = dev.solve_refined_batched(A32s, A64s, bs, tol=1e-10)
+    for i in range(len(bs)):
+        assert reps[i].converged, reps[i]
+        x1, rep1 = dev.solve_refined(A32s[i], A64s[i], bs[i], tol=1e-10)
+        assert np.array_equal(xs[i].cpu().numpy(), x1.cpu().numpy())
+        xf, _ = dev.bicgstab(A64s[i], bs[i], tol=1e-12, maxiter=5000)
+        err = np.linalg.norm((xs[i] - xf).cpu().numpy()) / np.linalg.norm(xf.cpu().numpy())
+        assert err < 1e-8, err
+
+
+def test_batched_mixed_status(dev):
+    """A system forced to stop at maxiter reports it; the others converge."""
+    systems = _systems(dev, seed=2, count=3)
+    As = [dev.assemble(p, m, k, add_identity=True) for p, m, k, _ in systems]
+    bs = [dev.to_device(b, dev.C128) for *_, b in systems]
+    xs, reps = dev.bicgstab_batched(As, bs, tol=[1e-10, 1e-16, 1e-10], maxiter=[2000, 3, 2000])
+    assert reps[0].converged and reps[2].converged
+    assert all(s == 3 for s in reps[1].status) and all(i == 3 for i in reps[1].iterations)
+
+
+def test_cluster_batch_matches_single_clusters(dev):
+    """ClusterBatch (all clusters in one batched solve) vs one sphere_cluster
+    per cluster: cross-sections and forces per source within 1e-9 (FP64) and
+    1e-8 (complex64 + FP64 refinement)."""
+    import miepy_amd as M
+    from miepy_amd.batch import ClusterBatch
+    from miepy_amd.workloads import batch256
+    ws = batch256(seed=7, count=5)
+    for w in ws:
+        w.position = w.position[: min(w.N, 60)]
+        w.N = w.position.shape[0]
+        w.radius, w.eps = w.radius[: w.N], w.eps[: w.N]
+        w.tol = 1e-11
+    for precision, rtol in (("fp64", 1e-9), ("mixed", 1e-8)):
+        B = ClusterBatch(ws, precision=precision)
+        B.solve()
+        assert B.converged()
+        res = B.host_results()
+        for w, r in zip(ws, res):
+            one = M.sphere_cluster(position=w.position, radius=w.radius, material=w.materials(), source=w.sources,
+                                   wavelength=w.wavelength, lmax=w.lmax, medium=w.medium(), tol=1e-12)
+            C = np.array(one.cross_sections()).T  # [S, 3]
+            np.testing.assert_allclose(r["C_total"], C, rtol=rtol, atol=rtol * np.abs(C).max())
+            F = one.force()
+            assert np.max(np.abs(r["F"] - F)) <= rtol * 10 * np.abs(F).max()

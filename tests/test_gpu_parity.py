@@ -208,7 +208,11 @@ def test_end_to_end_hex37(dev, H):
     c = _gpu_cluster(w, sources=w.sources[0])
     err = np.linalg.norm(c.p_scat - ref["p_scat"]) / np.linalg.norm(ref["p_scat"])
     assert err < 1e-9, err
-    np.testing.assert_allclose(c.cross_sections(), ref["cross_sections"], rtol=1e-9)
+    C, Cr = np.array(c.cross_sections()), ref["cross_sections"]
+    # scattering and extinction to 1e-9; absorption of these lossless spheres is
+    # roundoff (~1e-30), compared on the extinction scale
+    np.testing.assert_allclose(C[[0, 2]], Cr[[0, 2]], rtol=1e-9)
+    assert abs(C[1] - Cr[1]) < 1e-9 * Cr[2]
     Fs = np.abs(ref["force"]).max()
     assert np.max(np.abs(c.force() - ref["force"])) < 1e-8 * Fs
     Ts = np.abs(ref["torque"]).max()
