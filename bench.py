@@ -39,10 +39,18 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="rand1000")
-    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--precision", default="all", choices=["all", "both", "fp64", "mixed", "matfree"],
+                    help="all = fp64, mixed and matfree; both = fp64 and mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-gemvs", type=int, default=3, help="timed oracle GEMVs in the CPU sample")
     ap.add_argument("--batch-count", type=int, default=256, help="clusters in the batch256 workload")
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second (Poisson)")
+    ap.add_argument("--serve-seconds", type=float, default=5.0, help="arrival window of the serving stream")
+    ap.add_argument("--serve-nmax", type=int, default=1000)
+    ap.add_argument("--serve-budget-gb", type=float, default=200.0)
+    ap.add_argument("--serve-wall", type=float, default=120.0, help="wall-clock cap of the serving run")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 4 GiB device copy (reference for HBM byte counters) and exit")
     return ap.parse_args()
 
 
@@ -53,6 +61,40 @@ def log(msg):
 def make_workload(name):
     from miepy_amd import workloads as W
     return {"rand1000": W.rand1000, "hex37": W.hex37, "rand2500": W.rand2500}[name]()
+
+
+def bench_serving(args):
+    """Poisson stream of cluster-solve requests served with continuous batching."""
+    import torch
+    from miepy_amd import workloads as W
+    from miepy_amd.serve import ServingEngine, latency_summary
+    arrivals = [(t, ws) for t, ws in W.serving_requests(rate=args.serve_rate, duration=args.serve_seconds,
+                                                        Nmax=args.serve_nmax)]
+    log(f"serving: {len(arrivals)} requests over {args.serve_seconds} s at {args.serve_rate}/s, "
+        f"{sum(len(ws) for _, ws in arrivals)} (cluster, wavelength) systems")
+    eng = ServingEngine(hbm_budget_bytes=args.serve_budget_gb * 1e9, iterations_per_step=2)
+    t0 = time.perf_counter()
+    reqs = eng.run_stream(arrivals, max_wall=args.serve_wall,
+                          progress=lambda t, e: log(f"serving t={t:.0f}s: {e.solves_done} systems done, "
+                                                    f"{len(e.active)} active, {len(e.queue)} queued"))
+    wall = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    lat = latency_summary(reqs)
+    done_sys = eng.solves_done
+    line = {
+        "metric": "cluster solves/s (FP32+refine vs FP64, <=1e-8 rel err); p50/p99 solve latency",
+        "value": done_sys / wall, "unit": "cluster solves/s", "n_gpus": 1, "steps": eng.steps, "warmup": 0,
+        "ms_per_step": wall / max(eng.steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c128", "data": "synthetic (seeded Poisson request stream)",
+        "config": {"workload": "serving", "rate_per_s": args.serve_rate, "arrival_seconds": args.serve_seconds,
+                   "Nmax": args.serve_nmax, "hbm_budget_gb": args.serve_budget_gb, "wall_cap_s": args.serve_wall},
+        "latency_ms": {k: v for k, v in lat.items() if k.endswith("_ms")},
+        "requests": {"arrived": len(reqs), "completed": lat.get("completed", 0),
+                     "rejected_too_large": sum(r.status == "too_large" for r in reqs),
+                     "unfinished": sum(r.status in ("queued", "running") for r in reqs)},
+        "wall_s": wall,
+    }
+    print(json.dumps(line), flush=True)
 
 
 def bench_batch(args):
@@ -135,11 +177,12 @@ class Solver:
         self.precision = precision
         n, S = prob.n, prob.S
         dev = prob.pos_d.device
-        self.A64 = torch.empty((n, n), dtype=D.C128, device=dev)
+        self.op = D.ClusterOperator(prob.pos_d, prob.ms_d, prob.w.k, max_rhs=S) if precision == "matfree" else None
+        self.A64 = torch.empty((n, n), dtype=D.C128, device=dev) if precision != "matfree" else None
         self.A32 = torch.empty((n, n), dtype=D.C64, device=dev) if precision == "mixed" else None
         lib = L.load()
-        nb = lib.mpa_bicgstab_workspace_bytes(n, S, L.MPA_C128) if precision == "fp64" else \
-            lib.mpa_refine_workspace_bytes(n, S)
+        nb = lib.mpa_refine_workspace_bytes(n, S) if precision == "mixed" else \
+            lib.mpa_bicgstab_workspace_bytes(n, S, L.MPA_C128)
         self.ws = torch.empty(nb, dtype=torch.uint8, device=dev)
         self.x = torch.empty((S, n), dtype=D.C128, device=dev)
         self.report = None
@@ -148,11 +191,14 @@ class Solver:
     def step(self):
         D, p, w = self.D, self.prob, self.prob.w
         b = p.src_d.reshape(p.S, p.n)
-        D.assemble(p.pos_d, p.ms_d, w.k, dtype=D.C128, add_identity=True, out=self.A64)
-        if self.precision == "fp64":
+        if self.precision == "matfree":
+            _, self.report = D.bicgstab(self.op, b, tol=w.tol, maxiter=5000, add_identity=True, x=self.x,
+                                        workspace=self.ws)
+        elif self.precision == "fp64":
+            D.assemble(p.pos_d, p.ms_d, w.k, dtype=D.C128, add_identity=True, out=self.A64)
             _, self.report = D.bicgstab(self.A64, b, tol=w.tol, maxiter=5000, x=self.x, workspace=self.ws)
         else:
-            D.assemble(p.pos_d, p.ms_d, w.k, dtype=D.C64, add_identity=True, out=self.A32)
+            D.assemble_dual(p.pos_d, p.ms_d, w.k, add_identity=True, out64=self.A64, out32=self.A32)
             _, self.report = D.solve_refined(self.A32, self.A64, b, tol=w.tol, maxiter=5000, x=self.x,
                                              workspace=self.ws)
         self.obs = D.observables(self.x.reshape(p.S, w.N, 2, w.rmax), p.src_d, p.ms_d, p.mi_d, w.k, w.medium_eps)
@@ -180,7 +226,8 @@ def run_timed(solver, steps, warmup, profile=False):
     prof = {}
     if profile:
         import ctypes
-        for kind, name in [(0, "matvec_c128"), (1, "matvec_c64"), (2, "assemble_c128"), (3, "assemble_c64")]:
+        for kind, name in [(0, "matvec_c128"), (1, "matvec_c64"), (2, "assemble_c128"), (3, "assemble_c64"),
+                           (4, "matfree_c128")]:
             c = ctypes.c_int(0)
             t = ctypes.c_double(0)
             L.check(lib.mpa_profile_read(kind, ctypes.byref(c), ctypes.byref(t)), "profile_read")
@@ -233,15 +280,30 @@ def main():
     import torch
     import miepy_amd.device as D
     D.require_gpu()
+    if args.copy_calibration:
+        import ctypes
+        from miepy_amd import _lib as L
+        nbytes = 1 << 32
+        a = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(2):
+            L.check(L.load().mpa_stream_copy(ctypes.c_void_p(a.data_ptr()), ctypes.c_void_p(b.data_ptr()), nbytes,
+                                             D._stream()), "stream_copy")
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": nbytes, "copy_bytes_written": nbytes, "launches": 2}), flush=True)
+        return
     log(f"workload {args.workload}")
     if args.workload == "batch256":
         return bench_batch(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     w = make_workload(args.workload)
     prob = Problem(w)
     log(f"{w}: n={w.n}, S={prob.S}, tol={w.tol}")
 
     results = {}
-    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+    precs = {"all": ["fp64", "mixed", "matfree"], "both": ["fp64", "mixed"]}.get(args.precision, [args.precision])
+    for prec in precs:
         s = Solver(prob, prec)
         total, lat, prof = run_timed(s, args.steps, args.warmup, profile=True)
         res = dict(total=total, lat=lat, prof=prof, report=s.report,

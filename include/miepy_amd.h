@@ -58,6 +58,12 @@ const char* mpa_last_error(void);
 int mpa_sphere_aggregate_tmatrix(int N, int lmax, const double* positions, const double* mie, double k, int out_dtype,
                                  int add_identity, void* out, void* stream);
 
+/* The same matrix written twice in one pass: out64 [n, n] c128 and its
+ * complex64 rounding out32 [n, n] (device), for the mixed-precision solve,
+ * which needs both; the FP64 arithmetic is done once. */
+int mpa_sphere_aggregate_tmatrix_dual(int N, int lmax, const double* positions, const double* mie, double k,
+                                      int add_identity, void* out64, void* out32, void* stream);
+
 /* y = (add_identity ? x : 0) + A x for nrhs vectors ([nrhs, n]).
  * Replaces the OpenMP row-block GEMV inside Eigen's BiCGSTAB
  * (cpp/src/interactions.cpp:62-95). */
@@ -83,11 +89,46 @@ int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, co
                  int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out, int sync_every,
                  void* stream);
 
+/* ---- matrix-free operator ----------------------------------------------------
+ * The aggregate interaction operator of a sphere cluster applied without
+ * storing it: the translation blocks of interactions.cpp:370-415 /
+ * vsh_translation.cpp:161-237 are recomputed inside every product.  Memory
+ * is O(n) (positions, Mie coefficients, a workspace of partial rows), so
+ * clusters whose n x n matrix does not fit in HBM can be solved; no
+ * counterpart exists in the reference, which always assembles.  Host
+ * struct, device pointers. */
+typedef struct {
+    int N, lmax;
+    const double* positions; /* [N, 3] */
+    const double* mie;       /* [N, 2, lmax] c128 (mie_scat) */
+    double k;                /* wavenumber in the medium */
+    void* workspace;         /* mpa_matfree_workspace_bytes(N, lmax, nrhs) bytes for the largest nrhs used */
+} mpa_cluster_operator;
+
+size_t mpa_matfree_workspace_bytes(int N, int lmax, int nrhs);
+
+/* y = (add_identity ? x : 0) + T x, x and y [nrhs, n] c128 (device), FP64.
+ * Same operator as mpa_sphere_aggregate_tmatrix followed by mpa_matvec. */
+int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,
+                       void* stream);
+
+/* BiCGSTAB (as mpa_bicgstab, FP64) on (add_identity ? I + T : T) x = b with
+ * the matrix-free operator; workspace: mpa_bicgstab_workspace_bytes(n,
+ * nrhs, MPA_C128) bytes, n = 2 lmax (lmax + 2) N; op->workspace sized for
+ * nrhs.  Same role as the bicgstab branch of solve_linear_system
+ * (cpp/src/interactions.cpp:347-368) for a cluster whose matrix is not
+ * stored. */
+int mpa_bicgstab_matfree(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* b, void* x,
+                         double tol, int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out,
+                         int sync_every, void* stream);
+
 /* ---- batched solves: many independent systems, one set of launches -------- */
 
 /* One system (I + T) x = b with nrhs right-hand sides (host struct; all
  * pointers are device pointers).  `state` holds the system's Krylov vectors
- * and scalars: mpa_system_state_bytes(n, nrhs, dtype) bytes. */
+ * and scalars: mpa_system_state_bytes(n, nrhs, dtype) bytes.  With `op`
+ * set, A must be NULL and the operator is applied matrix-free (dtype
+ * MPA_C128 only; op->workspace sized for nrhs). */
 typedef struct {
     int n, nrhs, add_identity, pad;
     const void* A; /* [n, n] of dtype (T, or I + T with add_identity = 0) */
@@ -96,6 +137,7 @@ typedef struct {
     void* state;
     double tol;    /* relative: ||r|| <= tol ||b|| */
     int maxiter, pad2;
+    const mpa_cluster_operator* op; /* host pointer or NULL */
 } mpa_system;
 
 size_t mpa_system_state_bytes(int n, int nrhs, int dtype);
@@ -194,6 +236,11 @@ int mpa_particle_cross_sections(int M, int lmax, const double* p_scat, const dou
  * mpa_profile_read sums the recorded launch groups of one kind (host outputs). */
 int mpa_profile_enable(int on);
 int mpa_profile_read(int kind, int* count, double* total_ms);
+
+/* Plain device-to-device copy of `bytes` (multiple of 16) with 16-byte
+ * loads and stores: the known-size reference kernel used to scale hardware
+ * byte counters to HBM bytes in measurements. */
+int mpa_stream_copy(const void* src, void* dst, size_t bytes, void* stream);
 
 /* Host copy of the per-lmax translation coefficient table (tests):
  * meta_out [rmax*rmax*4] int, coef_out [rmax*rmax*(2 lmax+1)] c128. */

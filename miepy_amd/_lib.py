@@ -50,6 +50,7 @@ SIGNATURES = {
     "mpa_version": (ctypes.c_char_p, []),
     "mpa_last_error": (ctypes.c_char_p, []),
     "mpa_sphere_aggregate_tmatrix": (_i, [_i, _i, _v, _v, _d, _i, _i, _v, _v]),
+    "mpa_sphere_aggregate_tmatrix_dual": (_i, [_i, _i, _v, _v, _d, _i, _v, _v, _v]),
     "mpa_matvec": (_i, [_i, _i, _i, _v, _i, _v, _v, _v]),
     "mpa_bicgstab_workspace_bytes": (_sz, [_i, _i, _i]),
     "mpa_bicgstab": (_i, [_i, _i, _i, _v, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
@@ -58,6 +59,7 @@ SIGNATURES = {
     "mpa_cluster_observables": (_i, [_i, _i, _i, _v, _v, _v, _v, _d, _d, _v, _v, _v, _v, _v, _v, _v]),
     "mpa_force_torque": (_i, [_i, _i, _v, _v, _d, _d, _v, _v, _v]),
     "mpa_particle_cross_sections": (_i, [_i, _i, _v, _v, _v, _d, _v, _v]),
+    "mpa_stream_copy": (_i, [_v, _v, _sz, _v]),
     "mpa_profile_enable": (_i, [_i]),
     "mpa_profile_read": (_i, [_i, _pi, _pd]),
     "mpa_translation_table": (_i, [_i, _pi, _pd]),
@@ -68,7 +70,14 @@ class MpaSystem(ctypes.Structure):
     """mpa_system (include/miepy_amd.h)"""
     _fields_ = [("n", ctypes.c_int), ("nrhs", ctypes.c_int), ("add_identity", ctypes.c_int), ("pad", ctypes.c_int),
                 ("A", ctypes.c_void_p), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p), ("state", ctypes.c_void_p),
-                ("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("pad2", ctypes.c_int)]
+                ("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("pad2", ctypes.c_int),
+                ("op", ctypes.c_void_p)]
+
+
+class MpaClusterOperator(ctypes.Structure):
+    """mpa_cluster_operator (include/miepy_amd.h)"""
+    _fields_ = [("N", ctypes.c_int), ("lmax", ctypes.c_int), ("positions", ctypes.c_void_p),
+                ("mie", ctypes.c_void_p), ("k", ctypes.c_double), ("workspace", ctypes.c_void_p)]
 
 
 class MpaRefineSystem(ctypes.Structure):
@@ -80,8 +89,12 @@ class MpaRefineSystem(ctypes.Structure):
 
 
 _ps = ctypes.POINTER(MpaSystem)
+_pop = ctypes.POINTER(MpaClusterOperator)
 _prs = ctypes.POINTER(MpaRefineSystem)
 SIGNATURES.update({
+    "mpa_matfree_workspace_bytes": (_sz, [_i, _i, _i]),
+    "mpa_matfree_matvec": (_i, [_pop, _i, _i, _v, _v, _v]),
+    "mpa_bicgstab_matfree": (_i, [_pop, _i, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_system_state_bytes": (_sz, [_i, _i, _i]),
     "mpa_batch_scratch_bytes": (_sz, [_i, _ps, _i]),
     "mpa_bicgstab_batched": (_i, [_i, _ps, _i, _v, _pi, _pd, _pi, _i, _v]),

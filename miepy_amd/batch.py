@@ -60,9 +60,10 @@ class ClusterBatch:
         """Assemble, solve and reduce every cluster; results stay on the device."""
         cs = self.clusters
         for c in cs:
-            D.assemble(c.pos, c.mie_scat, c.k, dtype=D.C128, add_identity=True, out=c.A64)
             if self.precision == "mixed":
-                D.assemble(c.pos, c.mie_scat, c.k, dtype=D.C64, add_identity=True, out=c.A32)
+                D.assemble_dual(c.pos, c.mie_scat, c.k, add_identity=True, out64=c.A64, out32=c.A32)
+            else:
+                D.assemble(c.pos, c.mie_scat, c.k, dtype=D.C128, add_identity=True, out=c.A64)
         rhs = [c.p_src.reshape(c.S, c.n) for c in cs]
         tols = [c.w.tol for c in cs]
         if self.precision == "fp64":

@@ -95,13 +95,16 @@ __device__ inline void store<float2>(float2* p, double2 v) {
     *p = make_float2((float)v.x, (float)v.y);
 }
 
-template <typename OutT>
+// OutT: element type of `out`; DUAL: also write the complex64 rounding of
+// every entry to `out32` in the same pass (the mixed-precision solve needs
+// both matrices; the FP64 arithmetic is done once).
+template <typename OutT, bool DUAL>
 __global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int J, int ntiles,
                                                              const double* __restrict__ pos,
                                                              const double2* __restrict__ mie, double k,
                                                              const int4* __restrict__ tmeta,
                                                              const double2* __restrict__ tcoef, int add_identity,
-                                                             OutT* __restrict__ out) {
+                                                             OutT* __restrict__ out, float2* __restrict__ out32) {
     __shared__ PairLDS L;
     __shared__ double2 mieJ[kJMax][2][MPA_LMAX_MAX];
 
@@ -145,10 +148,12 @@ __global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int
         const int s = cb - b * rmax;
         const int j = j0 + jj;
         OutT* col = out + row0 * n + col0 + c;
+        float2* col32 = DUAL ? out32 + row0 * n + col0 + c : nullptr;
         if (j == i) {
             for (int l = 0; l < bs; l++) {
                 double2 v = make_double2((add_identity && l == cb) ? 1.0 : 0.0, 0.0);
                 store<OutT>(col + (size_t)l * n, v);
+                if (DUAL) store<float2>(col32 + (size_t)l * n, v);
             }
             continue;
         }
@@ -179,6 +184,7 @@ __global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int
             if (meta.z < 0) e.y = -e.y;
             const double2 val = cmul(cmul(e, acc), mj);
             store<OutT>(col + (size_t)l * n, val);
+            if (DUAL) store<float2>(col32 + (size_t)l * n, val);
         }
     }
 }
@@ -225,13 +231,39 @@ extern "C" int mpa_sphere_aggregate_tmatrix(int N, int lmax, const double* posit
     hipStream_t s = (hipStream_t)stream;
     ProfScope prof(out_dtype == MPA_C128 ? 2 : 3, s);
     if (out_dtype == MPA_C128)
-        hipLaunchKernelGGL(assemble_kernel<double2>, dim3((unsigned)nblocks), dim3(kThreads), 0, s, N, lmax, J, ntiles,
-                           positions, (const double2*)mie, k, T->meta, T->coef, add_identity, (double2*)out);
+        hipLaunchKernelGGL((assemble_kernel<double2, false>), dim3((unsigned)nblocks), dim3(kThreads), 0, s, N, lmax, J,
+                           ntiles, positions, (const double2*)mie, k, T->meta, T->coef, add_identity, (double2*)out,
+                           (float2*)nullptr);
     else
-        hipLaunchKernelGGL(assemble_kernel<float2>, dim3((unsigned)nblocks), dim3(kThreads), 0, s, N, lmax, J, ntiles,
-                           positions, (const double2*)mie, k, T->meta, T->coef, add_identity, (float2*)out);
+        hipLaunchKernelGGL((assemble_kernel<float2, false>), dim3((unsigned)nblocks), dim3(kThreads), 0, s, N, lmax, J,
+                           ntiles, positions, (const double2*)mie, k, T->meta, T->coef, add_identity, (float2*)out,
+                           (float2*)nullptr);
     MPA_LAUNCH_CHECK();
     return MPA_OK;
 }
 
 extern "C" int mpa_assembly_tile(int lmax) { return pick_tile(lmax); }
+
+extern "C" int mpa_sphere_aggregate_tmatrix_dual(int N, int lmax, const double* positions, const double* mie, double k,
+                                                 int add_identity, void* out64, void* out32, void* stream) {
+    if (N < 1 || !positions || !mie || !out64 || !out32 || !(k > 0)) {
+        mpa_set_error("sphere_aggregate_tmatrix_dual: bad argument (N=%d, k=%g)", N, k);
+        return MPA_ERR_ARG;
+    }
+    const TransTable* T = mpa_get_table(lmax);
+    if (!T) return MPA_ERR_ARG;
+    const int J = pick_tile(lmax);
+    const int ntiles = (N + J - 1) / J;
+    const long long nblocks = (long long)N * ntiles;
+    if (nblocks > 0x7fffffffLL) {
+        mpa_set_error("sphere_aggregate_tmatrix_dual: N=%d too large for one launch", N);
+        return MPA_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    ProfScope prof(2, s);
+    hipLaunchKernelGGL((assemble_kernel<double2, true>), dim3((unsigned)nblocks), dim3(kThreads), 0, s, N, lmax, J,
+                       ntiles, positions, (const double2*)mie, k, T->meta, T->coef, add_identity, (double2*)out64,
+                       (float2*)out32);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}

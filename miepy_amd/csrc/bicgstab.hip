@@ -22,6 +22,7 @@
 
 #include "gemv.hpp"
 #include "krylov.hpp"
+#include "matfree.hpp"
 
 using namespace mpa;
 
@@ -266,13 +267,14 @@ struct Plan {
     std::vector<int2> scal_work;
     std::vector<MvDev> mv_ps, mv_st, mv_xt;
     std::vector<int2> mv_work[kTileClasses];  // (job, row block) per tile class; same for every job set
+    std::vector<MfJob> mf_ps, mf_st, mf_xt;   // matrix-free systems (launched by value)
     // device copies
     SysDev* d_sys = nullptr;
     int3* d_vec = nullptr;
     int2* d_scal = nullptr;
     MvDev *d_mv_ps = nullptr, *d_mv_st = nullptr, *d_mv_xt = nullptr;
-    int2* d_mv_work[kTileClasses] = {nullptr, nullptr, nullptr};
-    int n_mv_work[kTileClasses] = {0, 0, 0};
+    int2* d_mv_work[kTileClasses] = {};
+    int n_mv_work[kTileClasses] = {};
 };
 
 size_t plan_bytes(int nsys, const mpa_system* s, int dtype) {
@@ -295,6 +297,9 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
     P.mv_st.resize(nsys);
     P.mv_xt.resize(nsys);
     for (auto& w : P.mv_work) w.clear();
+    P.mf_ps.clear();
+    P.mf_st.clear();
+    P.mf_xt.clear();
     const int es = dtype == MPA_C128 ? 8 : 4;
     for (int i = 0; i < nsys; i++) {
         const mpa_system& q = S_[i];
@@ -327,6 +332,23 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
         P.mv_ps[i] = MvDev{q.A, d.p, d.v, n, S, q.add_identity, i};
         P.mv_st[i] = MvDev{q.A, d.s, d.t, n, S, q.add_identity, i};
         P.mv_xt[i] = MvDev{q.A, q.x, d.t, n, S, q.add_identity, -1};
+        if (q.op) {
+            const mpa_cluster_operator& o = *q.op;
+            const MfJob base{o.N, o.lmax, S, q.add_identity ? 1 : 0, o.positions, (const double2*)o.mie, o.k,
+                             nullptr, nullptr, (double2*)o.workspace, i, 0};
+            MfJob j = base;
+            j.x = (const double2*)d.p;
+            j.y = (double2*)d.v;
+            P.mf_ps.push_back(j);
+            j.x = (const double2*)d.s;
+            j.y = (double2*)d.t;
+            P.mf_st.push_back(j);
+            j.x = (const double2*)q.x;
+            j.y = (double2*)d.t;
+            j.skip = -1;
+            P.mf_xt.push_back(j);
+            continue;
+        }
         const int rb = gemv_row_blocks(n, S);
         std::vector<int2>& mw = P.mv_work[gemv_class(S)];
         for (int b = 0; b < rb; b++) mw.push_back(make_int2(i, b));
@@ -357,6 +379,23 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
     return MPA_OK;
 }
 
+// All matvec jobs of one phase: the batched GEMV over stored matrices and
+// the matrix-free systems.
+template <typename T>
+int matvec_phase(const Plan& P, const MvDev* d_jobs, const std::vector<MfJob>& mf, hipStream_t st) {
+    bool any = false;
+    for (int c = 0; c < kTileClasses; c++) any |= P.n_mv_work[c] > 0;
+    if (any) {
+        const int rc = gemv_launch<T>(d_jobs, P.d_sys, P.d_mv_work, P.n_mv_work, st);
+        if (rc) return rc;
+    }
+    for (const MfJob& j : mf) {
+        const int rc = matfree_launch(j, P.d_sys, st);
+        if (rc) return rc;
+    }
+    return MPA_OK;
+}
+
 template <typename T>
 int launch_start(const Plan& P, hipStream_t st) {
     const int nv = (int)P.vec_work.size(), ns = (int)P.scal_work.size();
@@ -370,11 +409,11 @@ template <typename T>
 int launch_iteration(const Plan& P, hipStream_t st) {
     const int nv = (int)P.vec_work.size(), ns = (int)P.scal_work.size();
     hipLaunchKernelGGL(k_update_p<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
-    int rc = gemv_launch<T>(P.d_mv_ps, P.d_sys, P.d_mv_work, P.n_mv_work, st);
+    int rc = matvec_phase<T>(P, P.d_mv_ps, P.mf_ps, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_dot_r0v<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
     hipLaunchKernelGGL(k_update_s<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
-    rc = gemv_launch<T>(P.d_mv_st, P.d_sys, P.d_mv_work, P.n_mv_work, st);
+    rc = matvec_phase<T>(P, P.d_mv_st, P.mf_st, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_dot_ts<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
     hipLaunchKernelGGL(k_update_xr<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
@@ -386,7 +425,7 @@ int launch_iteration(const Plan& P, hipStream_t st) {
 template <typename T>
 int launch_restart(const Plan& P, hipStream_t st) {
     const int nv = (int)P.vec_work.size(), ns = (int)P.scal_work.size();
-    int rc = gemv_launch<T>(P.d_mv_xt, P.d_sys, P.d_mv_work, P.n_mv_work, st);
+    int rc = matvec_phase<T>(P, P.d_mv_xt, P.mf_xt, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_restart<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
     hipLaunchKernelGGL(k_restart_commit, dim3(ns), dim3(kThreads), 0, st, P.d_sys, P.d_scal);
@@ -455,12 +494,21 @@ int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, doubl
     return worst;
 }
 
-bool valid_systems(int nsys, const mpa_system* s) {
-    if (nsys < 1 || !s) return false;
-    for (int i = 0; i < nsys; i++)
-        if (s[i].n < 2 || (s[i].n & 1) || s[i].nrhs < 1 || !s[i].A || !s[i].b || !s[i].x || !s[i].state ||
-            !(s[i].tol >= 0) || s[i].maxiter < 0)
+bool valid_systems(int nsys, const mpa_system* s, int dtype) {
+    if (nsys < 1 || !s || (dtype != MPA_C128 && dtype != MPA_C64)) return false;
+    for (int i = 0; i < nsys; i++) {
+        if (s[i].n < 2 || (s[i].n & 1) || s[i].nrhs < 1 || !s[i].b || !s[i].x || !s[i].state || !(s[i].tol >= 0) ||
+            s[i].maxiter < 0)
             return false;
+        if (s[i].op) {
+            const mpa_cluster_operator& o = *s[i].op;
+            if (s[i].A || dtype != MPA_C128 || o.N < 1 || o.lmax < 1 || o.lmax > MPA_LMAX_MAX || !o.positions ||
+                !o.mie || !o.workspace || !(o.k > 0) || (long long)2 * lmax_to_rmax(o.lmax) * o.N != s[i].n)
+                return false;
+        } else if (!s[i].A) {
+            return false;
+        }
+    }
     return true;
 }
 
@@ -484,7 +532,7 @@ size_t mpa_batch_scratch_bytes(int nsys, const mpa_system* systems, int dtype) {
 
 int mpa_bicgstab_batched(int nsys, const mpa_system* systems, int dtype, void* scratch, int* iters_out,
                          double* resid_out, int* status_out, int sync_every, void* stream) {
-    if (!valid_systems(nsys, systems) || !scratch || (dtype != MPA_C128 && dtype != MPA_C64)) {
+    if (!valid_systems(nsys, systems, dtype) || !scratch) {
         mpa_set_error("bicgstab_batched: bad argument");
         return MPA_ERR_ARG;
     }
@@ -493,7 +541,7 @@ int mpa_bicgstab_batched(int nsys, const mpa_system* systems, int dtype, void* s
 }
 
 int mpa_systems_start(int nsys, const mpa_system* systems, int dtype, void* scratch, void* stream) {
-    if (!valid_systems(nsys, systems) || !scratch || (dtype != MPA_C128 && dtype != MPA_C64)) {
+    if (!valid_systems(nsys, systems, dtype) || !scratch) {
         mpa_set_error("systems_start: bad argument");
         return MPA_ERR_ARG;
     }
@@ -506,7 +554,7 @@ int mpa_systems_start(int nsys, const mpa_system* systems, int dtype, void* scra
 
 int mpa_systems_advance(int nsys, const mpa_system* systems, int dtype, void* scratch, int iterations, int* done_out,
                         int* iters_out, double* resid_out, int* status_out, void* stream) {
-    if (!valid_systems(nsys, systems) || !scratch || iterations < 0 || (dtype != MPA_C128 && dtype != MPA_C64)) {
+    if (!valid_systems(nsys, systems, dtype) || !scratch || iterations < 0) {
         mpa_set_error("systems_advance: bad argument");
         return MPA_ERR_ARG;
     }
@@ -565,6 +613,32 @@ int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, co
     s.maxiter = maxiter;
     void* scratch = (char*)workspace + state_bytes(n, nrhs, dtype);
     return mpa_internal_solve_batch(1, &s, dtype, scratch, iters_out, resid_out, status_out, sync_every,
+                                    (hipStream_t)stream);
+}
+
+int mpa_bicgstab_matfree(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* b, void* x,
+                         double tol, int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out,
+                         int sync_every, void* stream) {
+    if (!op || op->N < 1 || op->lmax < 1 || op->lmax > MPA_LMAX_MAX) {
+        mpa_set_error("bicgstab_matfree: bad operator");
+        return MPA_ERR_ARG;
+    }
+    mpa_system s{};
+    s.n = 2 * lmax_to_rmax(op->lmax) * op->N;
+    s.nrhs = nrhs;
+    s.add_identity = add_identity;
+    s.b = b;
+    s.x = x;
+    s.state = workspace;
+    s.tol = tol;
+    s.maxiter = maxiter;
+    s.op = op;
+    if (!valid_systems(1, &s, MPA_C128) || !workspace) {
+        mpa_set_error("bicgstab_matfree: bad argument");
+        return MPA_ERR_ARG;
+    }
+    void* scratch = (char*)workspace + state_bytes(s.n, nrhs, MPA_C128);
+    return mpa_internal_solve_batch(1, &s, MPA_C128, scratch, iters_out, resid_out, status_out, sync_every,
                                     (hipStream_t)stream);
 }
 

@@ -17,3 +17,22 @@ extern "C" int mpa_matvec(int n, int nrhs, int dtype, const void* A, int add_ide
     mpa_set_error("matvec: unknown dtype %d", dtype);
     return MPA_ERR_ARG;
 }
+
+// Plain 16-byte copy of `bytes` (a multiple of 16): the reference point for
+// scaling hardware byte counters (FETCH_SIZE / WRITE_SIZE) to HBM bytes.
+__global__ __launch_bounds__(kThreads) void k_copy16(size_t nchunks, const dvec2* __restrict__ src,
+                                                     dvec2* __restrict__ dst) {
+    for (size_t i = (size_t)blockIdx.x * kThreads + threadIdx.x; i < nchunks; i += (size_t)gridDim.x * kThreads)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(src + i), dst + i);
+}
+
+extern "C" int mpa_stream_copy(const void* src, void* dst, size_t bytes, void* stream) {
+    if (!src || !dst || bytes % 16) {
+        mpa_set_error("stream_copy: bad argument");
+        return MPA_ERR_ARG;
+    }
+    hipLaunchKernelGGL(k_copy16, dim3(8192), dim3(kThreads), 0, (hipStream_t)stream, bytes / 16, (const dvec2*)src,
+                       (dvec2*)dst);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}

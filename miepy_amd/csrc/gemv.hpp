@@ -143,11 +143,12 @@ __global__ __launch_bounds__(kThreads) void k_gemv1(MvDev J) {
 
 // Tile classes: systems are grouped by RHS count so that every launch uses
 // one template instance.  Rows per wave x RHS per pass: class 0 (S = 1)
-// 8 x 1, class 1 (S = 2) 4 x 2, class 2 (S >= 3) 2 x 4; a workgroup is four
+// 8 x 1, class 1 (S = 2) 4 x 2, class 2 (S = 3..4) 2 x 4, class 3 (S >= 5)
+// 1 x 8 (one pass over the matrix for up to 8 RHS); a workgroup is four
 // waves, so it covers 4 R rows.
-constexpr int kTileClasses = 3;
-inline int gemv_class(int S) { return S == 1 ? 0 : (S == 2 ? 1 : 2); }
-inline int gemv_rows_of_class(int c) { return 4 * (c == 0 ? 8 : (c == 1 ? 4 : 2)); }
+constexpr int kTileClasses = 4;
+inline int gemv_class(int S) { return S == 1 ? 0 : (S == 2 ? 1 : (S <= 4 ? 2 : 3)); }
+inline int gemv_rows_of_class(int c) { return 4 * (c == 0 ? 8 : (c == 1 ? 4 : (c == 2 ? 2 : 1))); }
 inline int gemv_row_blocks(int n, int S) {
     const int rb = gemv_rows_of_class(gemv_class(S));
     return (n + rb - 1) / rb;
@@ -160,6 +161,7 @@ int gemv_launch(const MvDev* jobs, const SysDev* sys, int2* const* work, const i
     if (nwork[0]) hipLaunchKernelGGL((k_gemv<T, 8, 1, 2>), dim3(nwork[0]), dim3(kThreads), 0, st, jobs, sys, work[0]);
     if (nwork[1]) hipLaunchKernelGGL((k_gemv<T, 4, 2, 2>), dim3(nwork[1]), dim3(kThreads), 0, st, jobs, sys, work[1]);
     if (nwork[2]) hipLaunchKernelGGL((k_gemv<T, 2, 4, 2>), dim3(nwork[2]), dim3(kThreads), 0, st, jobs, sys, work[2]);
+    if (nwork[3]) hipLaunchKernelGGL((k_gemv<T, 1, 8, 2>), dim3(nwork[3]), dim3(kThreads), 0, st, jobs, sys, work[3]);
     MPA_LAUNCH_CHECK();
     return MPA_OK;
 }
@@ -172,7 +174,8 @@ int gemv_launch1(const MvDev& J, hipStream_t st) {
     const int c = gemv_class(J.S);
     if (c == 0) hipLaunchKernelGGL((k_gemv1<T, 8, 1, 2>), dim3(nb), dim3(kThreads), 0, st, J);
     else if (c == 1) hipLaunchKernelGGL((k_gemv1<T, 4, 2, 2>), dim3(nb), dim3(kThreads), 0, st, J);
-    else hipLaunchKernelGGL((k_gemv1<T, 2, 4, 2>), dim3(nb), dim3(kThreads), 0, st, J);
+    else if (c == 2) hipLaunchKernelGGL((k_gemv1<T, 2, 4, 2>), dim3(nb), dim3(kThreads), 0, st, J);
+    else hipLaunchKernelGGL((k_gemv1<T, 1, 8, 2>), dim3(nb), dim3(kThreads), 0, st, J);
     MPA_LAUNCH_CHECK();
     return MPA_OK;
 }

@@ -1,0 +1,28 @@
+// Matrix-free sphere-cluster operator shared by the standalone matvec and the
+// batched Krylov engine.
+#pragma once
+
+#include "krylov.hpp"
+
+namespace mpa {
+
+// One y = (I +) T x job, T applied from positions and Mie coefficients.
+struct MfJob {
+    int N, lmax, S, add_identity;
+    const double* pos;   // [N, 3]
+    const double2* mie;  // [N, 2, lmax]
+    double k;
+    const double2* x;    // [S, n]
+    double2* y;          // [S, n]
+    double2* part;       // [chunks, S, n]
+    int skip;            // SysDev index whose all-done flag skips the job, or -1
+    int pad;
+};
+
+// Workspace bytes of a job (the per-chunk partial rows).
+size_t matfree_workspace_bytes(int N, int lmax, int S);
+
+// Launches the two kernels of one job; `sys` may be null when skip < 0.
+int matfree_launch(const MfJob& job, const SysDev* sys, hipStream_t st);
+
+}  // namespace mpa

@@ -239,6 +239,16 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             }
         }
         if (!any) break;
+        // each correction only needs to reach the remaining factor to tol
+        // (with a 4x margin), never tighter than the configured inner_tol
+        k = 0;
+        for (int i = 0; i < nsys; i++) {
+            double need = INFINITY;
+            for (int s = 0; s < ref[i].S; s++)
+                if (usc[k + s] != 0.0) need = std::min(need, 0.25 * R[i].tol * std::sqrt(bn2[k + s] / rn2[k + s]));
+            inner[i].tol = std::isfinite(need) ? std::max(R[i].inner_tol, need) : R[i].inner_tol;
+            k += ref[i].S;
+        }
         k = 0;
         for (int i = 0; i < nsys; i++) {
             MPA_HIP_CHECK(hipMemcpyAsync(ref[i].scale, &sc[k], sizeof(double) * ref[i].S, hipMemcpyHostToDevice, st));

@@ -66,6 +66,25 @@ def assemble(positions, mie, k, dtype=C128, add_identity=False, out=None):
     return out
 
 
+def assemble_dual(positions, mie, k, add_identity=False, out64=None, out32=None):
+    """One assembly pass writing the c128 matrix and its complex64 rounding."""
+    lib = L.load()
+    pos = to_device(positions, torch.float64).reshape(-1, 3)
+    N = pos.shape[0]
+    mie_d = to_device(mie, C128).reshape(N, 2, -1)
+    n = 2 * lmax_to_rmax(mie_d.shape[-1]) * N
+    if out64 is None:
+        out64 = torch.empty((n, n), dtype=C128, device=pos.device)
+    if out32 is None:
+        out32 = torch.empty((n, n), dtype=C64, device=pos.device)
+    if out64.shape != (n, n) or out32.shape != (n, n) or out64.dtype != C128 or out32.dtype != C64:
+        raise ValueError("out64 / out32 must be [n, n] complex128 / complex64")
+    L.check(lib.mpa_sphere_aggregate_tmatrix_dual(N, mie_d.shape[-1], _ptr(pos), _ptr(mie_d), float(k),
+                                                  1 if add_identity else 0, _ptr(out64), _ptr(out32), _stream()),
+            "sphere_aggregate_tmatrix_dual")
+    return out64, out32
+
+
 def matvec(A, x, add_identity=False, out=None):
     """y = (I +) A x; x is [n] or [S, n] on the device, same dtype as A."""
     lib = L.load()
@@ -101,6 +120,8 @@ def bicgstab(A, b, tol=1e-5, maxiter=1000, add_identity=False, x=None, workspace
     n = A.shape[0]
     b2 = b.reshape(-1, n).contiguous()
     S = b2.shape[0]
+    if isinstance(A, ClusterOperator):
+        return _bicgstab_matfree(lib, A, b, b2, tol, maxiter, add_identity, x, workspace, sync_every, check)
     code = DTYPE_CODE[A.dtype]
     if b2.dtype != A.dtype:
         raise ValueError("b must have the matrix dtype")
@@ -199,12 +220,94 @@ def solve_refined(A32, A64, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refi
     return x.reshape(b.shape), RefineReport(list(outer), list(inner), list(res), list(st))
 
 
+class ClusterOperator:
+    """Matrix-free (I +) T of a sphere cluster (mpa_cluster_operator).
+
+    Holds the device positions [N, 3], Mie coefficients [N, 2, lmax] and the
+    workspace for up to `max_rhs` right-hand sides; T itself is never
+    stored, so memory is O(n)."""
+
+    def __init__(self, positions, mie, k, max_rhs=1):
+        lib = L.load()
+        self.pos = to_device(positions, torch.float64).reshape(-1, 3)
+        self.N = self.pos.shape[0]
+        self.mie = to_device(mie, C128).reshape(self.N, 2, -1)
+        self.lmax = self.mie.shape[-1]
+        self.k = float(k)
+        self.n = 2 * lmax_to_rmax(self.lmax) * self.N
+        self.shape = (self.n, self.n)
+        self.dtype = C128
+        self.device = self.pos.device
+        self.max_rhs = int(max_rhs)
+        nb = lib.mpa_matfree_workspace_bytes(self.N, self.lmax, self.max_rhs)
+        if nb == 0:
+            raise ValueError(f"matrix-free operator: unsupported N={self.N}, lmax={self.lmax}")
+        self.workspace = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        self.struct = L.MpaClusterOperator(self.N, self.lmax, self.pos.data_ptr(), self.mie.data_ptr(), self.k,
+                                           self.workspace.data_ptr())
+
+    def reserve(self, nrhs):
+        if nrhs > self.max_rhs:
+            self.__init__(self.pos, self.mie, self.k, nrhs)
+
+
+def matfree_matvec(op, x, add_identity=False, out=None):
+    """y = (I +) T x with T recomputed inside the product; x [nrhs, n] c128."""
+    lib = L.load()
+    x2 = x.reshape(-1, op.n)
+    if x2.dtype != C128 or not x2.is_contiguous():
+        raise ValueError("x must be contiguous complex128 [nrhs, n]")
+    op.reserve(x2.shape[0])
+    if out is None:
+        out = torch.empty_like(x2)
+    L.check(lib.mpa_matfree_matvec(ctypes.byref(op.struct), x2.shape[0], 1 if add_identity else 0, _ptr(x2),
+                                   _ptr(out), _stream()), "matfree_matvec")
+    return out.reshape(x.shape)
+
+
+def bicgstab_matfree(op, b, tol=1e-5, maxiter=1000, add_identity=True, x=None, workspace=None, sync_every=1):
+    """BiCGSTAB on (I +) T x = b with the matrix-free operator (FP64)."""
+    return bicgstab(op, b, tol=tol, maxiter=maxiter, add_identity=add_identity, x=x, workspace=workspace,
+                    sync_every=sync_every)
+
+
+def _bicgstab_matfree(lib, op, b, b2, tol, maxiter, add_identity, x, workspace, sync_every, check):
+    S, n = b2.shape
+    if b2.dtype != C128:
+        raise ValueError("the matrix-free operator solves complex128 systems")
+    op.reserve(S)
+    nbytes = lib.mpa_bicgstab_workspace_bytes(n, S, L.MPA_C128)
+    if workspace is None or workspace.numel() < nbytes:
+        workspace = torch.empty(nbytes, dtype=torch.uint8, device=op.device)
+    if x is None:
+        x = torch.empty_like(b2)
+    it = (ctypes.c_int * S)()
+    res = (ctypes.c_double * S)()
+    st = (ctypes.c_int * S)()
+    rc = lib.mpa_bicgstab_matfree(ctypes.byref(op.struct), S, 1 if add_identity else 0, _ptr(b2), _ptr(x),
+                                  float(tol), int(maxiter), _ptr(workspace), it, res, st, int(sync_every), _stream())
+    report = SolveReport(list(it), list(res), list(st))
+    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED, L.MPA_NONFINITE) or (check and rc == L.MPA_NONFINITE):
+        L.check(rc, "bicgstab_matfree")
+    return x.reshape(b.shape), report
+
+
 def _sys_struct(A, b, x, state, tol, maxiter, add_identity):
     n = A.shape[0]
     s = L.MpaSystem()
     s.n = n
     s.nrhs = b.numel() // n
     s.add_identity = 1 if add_identity else 0
+    if isinstance(A, ClusterOperator):
+        A.reserve(s.nrhs)
+        s.A = None
+        s.op = ctypes.cast(ctypes.pointer(A.struct), ctypes.c_void_p)
+        s.b = b.data_ptr()
+        s.x = x.data_ptr()
+        s.state = state.data_ptr()
+        s.tol = float(tol)
+        s.maxiter = int(maxiter)
+        return s
     s.A = A.data_ptr()
     s.b = b.data_ptr()
     s.x = x.data_ptr()
@@ -217,7 +320,8 @@ def _sys_struct(A, b, x, state, tol, maxiter, add_identity):
 def bicgstab_batched(matrices, rhs, tol=1e-5, maxiter=1000, add_identity=False, sync_every=1):
     """Solve several independent systems in one batched BiCGSTAB.
 
-    matrices: list of [n_i, n_i] tensors (one dtype); rhs: list of [S_i, n_i].
+    matrices: list of [n_i, n_i] tensors (one dtype) or ClusterOperator
+    (matrix-free, complex128); rhs: list of [S_i, n_i].
     tol / maxiter: scalars or per-system lists.  Returns (list of x, list of SolveReport)."""
     lib = L.load()
     k = len(matrices)

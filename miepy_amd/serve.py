@@ -1,0 +1,230 @@
+"""Serving cluster-solve requests on one GPU with continuous batching.
+
+A request is one cluster (geometry, materials, sources) at one or more
+wavelengths; each (request, wavelength) is one system (I + T) p_inc = p_src
+with one right-hand side per source, and the response carries, per
+wavelength and source, the cross-sections and the per-particle forces.
+
+The engine admits systems first-come first-served while their HBM footprint
+(matrix + Krylov state + inputs) fits the budget, assembles them and joins
+them to the running batch; every engine step advances all active systems by
+a few BiCGSTAB iterations in one batched launch sequence
+(`mpa_systems_advance`), retires the systems that converged (observables,
+memory freed) and admits waiting ones into the freed memory.  Requests that
+arrive while others are on the GPU therefore start as soon as memory frees
+up, and their iterations overlap with everyone else's.  Matrices, Krylov
+vectors and per-RHS scalars stay on the device for the life of a system;
+each system's arithmetic does not depend on what else is in the batch, so a
+request's result is the same alone and in the mix.
+
+A system whose footprint alone exceeds the budget is rejected with status
+"too_large" instead of blocking the queue.
+"""
+
+import collections
+import ctypes
+import time
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from . import device as D
+from .mie import mie_coefficients
+
+
+class _System:
+    __slots__ = ("req", "index", "w", "n", "S", "k", "eps_b", "pos", "mie_scat", "mie_int", "p_src", "A", "x",
+                 "state", "struct", "done", "iters", "footprint")
+
+
+class Request:
+    def __init__(self, rid, workloads, arrival):
+        self.id = rid
+        self.workloads = list(workloads)
+        self.arrival = arrival
+        self.admitted = None
+        self.finished = None
+        self.status = "queued"
+        self.results = [None] * len(self.workloads)
+        self.pending = len(self.workloads)
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+def system_footprint(w):
+    """Device bytes of one system: c128 matrix, Krylov state, x / b / inputs."""
+    n, S = w.n, len(w.sources)
+    state = L.load().mpa_system_state_bytes(n, S, L.MPA_C128)
+    return 16 * n * n + state + 2 * 16 * S * n + 64 * w.N * (w.lmax + 2)
+
+
+class ServingEngine:
+    def __init__(self, hbm_budget_bytes=200e9, iterations_per_step=2, maxiter=2000, clock=time.perf_counter):
+        D.require_gpu()
+        self.budget = float(hbm_budget_bytes)
+        self.used = 0.0
+        self.k = int(iterations_per_step)
+        self.maxiter = int(maxiter)
+        self.clock = clock
+        self.queue = collections.deque()  # systems waiting for memory, FIFO
+        self.active = []
+        self.finished = []
+        self.rejected = []
+        self.steps = 0
+        self.solves_done = 0
+        self._scratch = torch.empty(0, dtype=torch.uint8, device="cuda")
+        self.lib = L.load()
+
+    # ---- intake --------------------------------------------------------------------
+
+    def submit(self, request):
+        for i, w in enumerate(request.workloads):
+            s = _System()
+            s.req, s.index, s.w = request, i, w
+            s.footprint = system_footprint(w)
+            if s.footprint > self.budget:
+                request.status = "too_large"
+                request.results[i] = {"status": "too_large"}
+                request.pending -= 1
+                continue
+            self.queue.append(s)
+        if request.pending == 0:
+            request.finished = self.clock()
+            self.rejected.append(request)
+
+    def _prepare(self, s):
+        w = s.w
+        s.n, s.S = w.n, len(w.sources)
+        s.k = w.k
+        s.eps_b = float(np.real(w.medium_eps))
+        conducting = np.zeros(w.N, dtype=bool) if w.conducting is None else np.asarray(w.conducting)
+        ms, mi = mie_coefficients(w.radius, w.eps, np.ones(w.N), w.medium_eps, 1.0, s.k, w.lmax, conducting)
+        p_src = np.stack([src.structure(w.position, s.k, w.lmax) for src in w.sources])
+        s.pos = D.to_device(w.position, torch.float64)
+        s.mie_scat = D.to_device(ms, D.C128)
+        s.mie_int = D.to_device(mi, D.C128)
+        s.p_src = D.to_device(p_src, D.C128)
+        s.A = D.assemble(s.pos, s.mie_scat, s.k, dtype=D.C128, add_identity=True)
+        s.x = torch.empty((s.S, s.n), dtype=D.C128, device="cuda")
+        s.state = torch.empty(self.lib.mpa_system_state_bytes(s.n, s.S, L.MPA_C128), dtype=torch.uint8,
+                              device="cuda")
+        s.struct = D._sys_struct(s.A, s.p_src.reshape(s.S, s.n), s.x, s.state, w.tol, self.maxiter, False)
+        s.done = False
+        s.iters = 0
+
+    def _admit(self):
+        new = []
+        while self.queue and self.used + self.queue[0].footprint <= self.budget:
+            s = self.queue.popleft()
+            self._prepare(s)
+            self.used += s.footprint
+            if s.req.admitted is None:
+                s.req.admitted = self.clock()
+                s.req.status = "running"
+            new.append(s)
+        if new:
+            arr = (L.MpaSystem * len(new))(*[s.struct for s in new])
+            scratch = self._scratch_for(arr, len(new))
+            L.check(self.lib.mpa_systems_start(len(new), arr, L.MPA_C128, D._ptr(scratch), D._stream()),
+                    "systems_start")
+            self.active.extend(new)
+        return len(new)
+
+    def _scratch_for(self, arr, count):
+        need = self.lib.mpa_batch_scratch_bytes(count, arr, L.MPA_C128)
+        if self._scratch.numel() < need:
+            self._scratch = torch.empty(int(need * 1.5) + 4096, dtype=torch.uint8, device="cuda")
+        return self._scratch
+
+    # ---- one engine step ---------------------------------------------------------------
+
+    def step(self):
+        """Admit what fits, advance every active system, retire the converged."""
+        self._admit()
+        if not self.active:
+            return False
+        count = len(self.active)
+        arr = (L.MpaSystem * count)(*[s.struct for s in self.active])
+        total = sum(s.S for s in self.active)
+        done = (ctypes.c_int * total)()
+        its = (ctypes.c_int * total)()
+        res = (ctypes.c_double * total)()
+        stt = (ctypes.c_int * total)()
+        scratch = self._scratch_for(arr, count)
+        L.check(self.lib.mpa_systems_advance(count, arr, L.MPA_C128, D._ptr(scratch), self.k, done, its, res, stt,
+                                             D._stream()), "systems_advance")
+        self.steps += 1
+        keep, o = [], 0
+        for s in self.active:
+            d = all(done[o + j] for j in range(s.S))
+            if d:
+                self._retire(s, list(its[o:o + s.S]), list(res[o:o + s.S]), list(stt[o:o + s.S]))
+            else:
+                keep.append(s)
+            o += s.S
+        self.active = keep
+        return True
+
+    def _retire(self, s, iters, resid, status):
+        p_inc = s.x.reshape(s.S, s.w.N, 2, s.w.rmax)
+        obs = D.observables(p_inc, s.p_src, s.mie_scat, s.mie_int, s.k, s.eps_b, want_int=False,
+                            want_torque=False)
+        s.req.results[s.index] = {
+            "wavelength": s.w.wavelength,
+            "cross_sections": obs["C_total"].cpu().numpy(),  # [S, 3]
+            "force": obs["F"].cpu().numpy(),                  # [S, N, 3]
+            "iterations": iters, "residual": resid, "status": status,
+        }
+        self.solves_done += 1
+        self.used -= s.footprint
+        s.A = s.x = s.state = s.pos = s.mie_scat = s.mie_int = s.p_src = None
+        s.req.pending -= 1
+        if s.req.pending == 0:
+            s.req.finished = self.clock()
+            s.req.status = "done" if s.req.status == "running" else s.req.status
+            self.finished.append(s.req)
+
+    # ---- drivers ---------------------------------------------------------------------------
+
+    def drain(self):
+        while self.queue or self.active:
+            if not self.step() and self.queue:
+                raise RuntimeError("queue is stuck: the head system does not fit the budget")
+
+    def run_stream(self, arrivals, max_wall=None, progress=None, progress_every=10.0):
+        """Serve (arrival_offset_s, workloads) pairs in real time; returns the requests.
+
+        progress(elapsed_s, engine) is called every `progress_every` seconds."""
+        arrivals = list(arrivals)
+        t0 = self.clock()
+        next_report = progress_every
+        reqs = []
+        i = 0
+        while i < len(arrivals) or self.queue or self.active:
+            now = self.clock() - t0
+            if progress is not None and now >= next_report:
+                progress(now, self)
+                next_report += progress_every
+            while i < len(arrivals) and arrivals[i][0] <= now:
+                r = Request(i, arrivals[i][1], t0 + arrivals[i][0])
+                reqs.append(r)
+                self.submit(r)
+                i += 1
+            if max_wall is not None and now > max_wall:
+                break
+            if not self.step() and i < len(arrivals):
+                wait = arrivals[i][0] - (self.clock() - t0)
+                if wait > 0:
+                    time.sleep(min(wait, 0.01))
+        return reqs
+
+
+def latency_summary(requests):
+    lat = np.array([r.latency for r in requests if r.latency is not None and r.status == "done"])
+    if lat.size == 0:
+        return {"completed": 0}
+    return {"completed": int(lat.size), "p50_ms": float(np.percentile(lat, 50) * 1e3),
+            "p99_ms": float(np.percentile(lat, 99) * 1e3), "mean_ms": float(lat.mean() * 1e3)}

@@ -125,8 +125,7 @@ class sphere_cluster:
                 x, self.report = D.bicgstab(A, b, tol=tol, maxiter=maxiter)
                 del A
             else:
-                A32 = D.assemble(self.position, mie_scat_d, k, dtype=D.C64, add_identity=True)
-                A64 = D.assemble(self.position, mie_scat_d, k, dtype=D.C128, add_identity=True)
+                A64, A32 = D.assemble_dual(self.position, mie_scat_d, k, add_identity=True)
                 rt = tol if self.refine_tol is None else self.refine_tol
                 x, self.report = D.solve_refined(A32, A64, b, tol=rt, maxiter=maxiter)
                 del A32, A64

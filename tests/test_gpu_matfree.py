@@ -1,0 +1,162 @@
+"""Matrix-free operator: (I + T) x recomputed inside every product.
+
+Parity targets: the oracle's assembled T times x (FP64, 1e-12 relative to
+|T||x| per row block: the kernel sums the same products in another order),
+the device's own assembled matvec, and the oracle's solution of the cluster
+system (BiCGSTAB to 1e-10 and LU), for the shapes the dense path covers
+(lmax 1-6, several right-hand sides, aligned pairs, single particles).
+"""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+nm = 1e-9
+
+
+@pytest.fixture(scope="module")
+def dev():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+    import miepy_amd.device as D
+    D.require_gpu()
+    return D
+
+
+@pytest.fixture(scope="module")
+def H(oracle):
+    import oracle.host as H
+    return H
+
+
+def _cluster(N, lmax, seed, spacing=350 * nm):
+    rng = np.random.default_rng(seed)
+    from miepy_amd.workloads import random_cluster
+    pos = random_cluster(N, spacing * N ** (1 / 3) * 1.5, 75 * nm, rng)
+    mie = rng.normal(size=(N, 2, lmax)) * 0.3 + 1j * rng.normal(size=(N, 2, lmax)) * 0.3
+    return pos, mie
+
+
+def _rel_err(got, ref):
+    return np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("N,lmax,S", [(2, 1, 1), (5, 2, 2), (9, 3, 1), (7, 4, 3), (5, 5, 1), (4, 6, 2),
+                                      (40, 2, 5), (33, 3, 4)])
+def test_matfree_matches_oracle_gemv(dev, oracle, N, lmax, S):
+    pos, mie = _cluster(N, lmax, seed=100 + N * 10 + lmax)
+    k = 2 * np.pi / (600 * nm)
+    T = oracle.sphere_aggregate_tmatrix(pos, mie, k)
+    n = T.shape[0]
+    rng = np.random.default_rng(N + S)
+    X = rng.normal(size=(S, n)) + 1j * rng.normal(size=(S, n))
+    op = dev.ClusterOperator(pos, mie, k, max_rhs=S)
+    Xd = dev.to_device(X, dev.C128)
+    got = dev.matfree_matvec(op, Xd).cpu().numpy()
+    got_i = dev.matfree_matvec(op, Xd, add_identity=True).cpu().numpy()
+    for s in range(S):
+        ref = oracle.gemv(T, X[s])
+        assert _rel_err(got[s], ref) <= 1e-12
+        assert _rel_err(got_i[s], ref + X[s]) <= 1e-12
+    # against the device's assembled matrix and GEMV
+    A = dev.assemble(pos, mie, k, add_identity=True)
+    dense = dev.matvec(A, Xd).cpu().numpy()
+    assert _rel_err(got_i, dense) <= 1e-12
+
+
+def test_matfree_edge_geometries(dev, oracle):
+    k = 2 * np.pi / (700 * nm)
+    # single particle: T = 0
+    op = dev.ClusterOperator(np.zeros((1, 3)), np.ones((1, 2, 3)), k)
+    x = dev.to_device(np.arange(30, dtype=complex).reshape(1, 30), dev.C128)
+    assert np.all(dev.matfree_matvec(op, x).cpu().numpy() == 0)
+    assert np.all(dev.matfree_matvec(op, x, add_identity=True).cpu().numpy() == x.cpu().numpy())
+    # pairs along z (theta = 0, pi) and in the xy plane
+    pos = np.array([[0, 0, 0], [0, 0, 300 * nm], [0, 0, -300 * nm], [300 * nm, 0, 0], [-300 * nm, 0, 0],
+                    [0, 300 * nm, 0]], float)
+    mie = np.full((6, 2, 3), 0.2 + 0.1j)
+    T = oracle.sphere_aggregate_tmatrix(pos, mie, k)
+    X = np.random.default_rng(1).normal(size=T.shape[0]) + 0j
+    got = dev.matfree_matvec(dev.ClusterOperator(pos, mie, k), dev.to_device(X.reshape(1, -1), dev.C128))
+    assert _rel_err(got.cpu().numpy()[0], oracle.gemv(T, X)) <= 1e-12
+
+
+def test_matfree_large_cluster_matches_dense(dev):
+    """rand1000-sized chunking (several column chunks per row)."""
+    from miepy_amd.workloads import rand1000
+    from miepy_amd.mie import mie_coefficients
+    w = rand1000()
+    ms, _ = mie_coefficients(w.radius, w.eps, np.ones(w.N), w.medium_eps, 1.0, w.k, w.lmax)
+    import torch
+    g = torch.Generator(device="cuda").manual_seed(0)
+    x = torch.randn((2, w.n), dtype=dev.C128, device="cuda", generator=g)
+    op = dev.ClusterOperator(w.position, ms, w.k, max_rhs=2)
+    got = dev.matfree_matvec(op, x, add_identity=True)
+    A = dev.assemble(w.position, ms, w.k, add_identity=True)
+    ref = dev.matvec(A, x)
+    del A
+    err = (got - ref).abs().max().item() / ref.abs().max().item()
+    assert err <= 1e-12, err
+
+
+def test_matfree_bicgstab_matches_oracle(dev, oracle, H):
+    """Solve with the matrix-free operator: same iterations as the oracle's
+    Eigen BiCGSTAB on the assembled matrix, solution equal to LU."""
+    N, lmax = 12, 2
+    pos, _ = _cluster(N, lmax, seed=5)
+    k = 2 * np.pi / (600 * nm)
+    ms, _ = H.mie_coefficients(np.full(N, 75 * nm), np.full(N, -15 + 1j), np.ones(N), 1.0, 1.0, k, lmax)
+    T = oracle.sphere_aggregate_tmatrix(pos, ms, k)
+    n = T.shape[0]
+    rng = np.random.default_rng(9)
+    b = rng.normal(size=n) + 1j * rng.normal(size=n)
+    A = np.eye(n) + T
+    x_lu = oracle.lu_solve(A, b)
+    op = dev.ClusterOperator(pos, ms, k)
+    for tol in (1e-5, 1e-10):
+        _, ito, _ = oracle.bicgstab_eigen(A, b, tol=tol)
+        x, rep = dev.bicgstab_matfree(op, dev.to_device(b.reshape(1, n), dev.C128), tol=tol)
+        x = x.cpu().numpy()[0]
+        assert abs(rep.iterations[0] - ito) <= 1, (rep.iterations, ito)
+        assert np.linalg.norm(A @ x - b) / np.linalg.norm(b) <= 1.01 * tol
+        if tol <= 1e-10:
+            assert np.linalg.norm(x - x_lu) / np.linalg.norm(x_lu) < 100 * tol
+
+
+def test_matfree_and_dense_in_one_batch(dev, H):
+    """A batch mixing stored matrices and matrix-free systems gives each
+    system the result it gets alone."""
+    k = 2 * np.pi / (600 * nm)
+    systems = []
+    for seed, (N, lmax, S) in enumerate([(10, 2, 1), (20, 3, 2), (8, 1, 3)]):
+        pos, _ = _cluster(N, lmax, seed=seed + 50)
+        ms, _ = H.mie_coefficients(np.full(N, 75 * nm), np.full(N, -15 + 1j), np.ones(N), 1.0, 1.0, k, lmax)
+        rng = np.random.default_rng(seed)
+        n = 2 * lmax * (lmax + 2) * N
+        b = dev.to_device(rng.normal(size=(S, n)) + 1j * rng.normal(size=(S, n)), dev.C128)
+        systems.append((pos, ms, b))
+    ops = [dev.ClusterOperator(p, m, k, max_rhs=b.shape[0]) for p, m, b in systems]
+    dense = [dev.assemble(p, m, k) for p, m, _ in systems]  # T only: add_identity adds I
+    rhs = [b for _, _, b in systems]
+    xs, reps = dev.bicgstab_batched([ops[0], dense[1], ops[2]], rhs, tol=1e-10, add_identity=True)
+    assert all(r.converged for r in reps)
+    x_dense, _ = dev.bicgstab(dense[1], rhs[1], tol=1e-10, add_identity=True)
+    assert np.array_equal(xs[1].cpu().numpy(), x_dense.cpu().numpy())
+    for i in (0, 2):
+        x_alone, _ = dev.bicgstab_matfree(ops[i], rhs[i], tol=1e-10)
+        assert np.array_equal(xs[i].cpu().numpy(), x_alone.reshape(xs[i].shape).cpu().numpy())
+        ref = dev.bicgstab(dense[i], rhs[i], tol=1e-10, add_identity=True)[0].cpu().numpy()
+        got = xs[i].cpu().numpy()
+        assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < 1e-8
+
+
+def test_matfree_bad_arguments(dev):
+    import ctypes
+    import miepy_amd._lib as L
+    lib = L.load()
+    assert lib.mpa_matfree_matvec(None, 1, 0, None, None, None) == L.MPA_ERR_ARG
+    assert lib.mpa_matfree_workspace_bytes(10, 7, 1) == 0
+    op = L.MpaClusterOperator(4, 9, None, None, 1.0, None)
+    assert lib.mpa_matfree_matvec(ctypes.byref(op), 1, 0, None, None, None) == L.MPA_ERR_ARG

@@ -273,3 +273,14 @@ def test_backend_dropin_functions(dev, oracle, H):
     np.testing.assert_allclose(F, oracle.force_all(ps, p_inc, k), rtol=1e-12, atol=1e-14 * np.abs(F).max())
     Tq = B.torque_all(ps, p_inc, k, 1.0, 1.0)
     np.testing.assert_allclose(Tq, oracle.torque_all(ps, p_inc, k), rtol=1e-12, atol=1e-14 * np.abs(Tq).max())
+
+
+def test_dual_assembly_equals_single(dev, oracle):
+    """One pass writing c128 and c64: the c128 output equals the c128-only
+    assembly bit for bit, the c64 output is its rounding."""
+    pos, mie = _cluster(11, 3, seed=21)
+    k = 2 * np.pi / (640 * nm)
+    a64 = dev.assemble(pos, mie, k, add_identity=True).cpu().numpy()
+    d64, d32 = dev.assemble_dual(pos, mie, k, add_identity=True)
+    assert np.array_equal(d64.cpu().numpy(), a64)
+    assert np.array_equal(d32.cpu().numpy(), a64.astype(np.complex64))
