@@ -107,7 +107,8 @@ def bench_batch(args):
         f"largest n = {max(w.n for w in ws)}")
     out = {}
     res = {}
-    for prec in (["fp64", "mixed"] if args.precision == "both" else [args.precision]):
+    precs = {"all": ["fp64", "mixed", "matfree"], "both": ["fp64", "mixed"]}.get(args.precision, [args.precision])
+    for prec in precs:
         B = ClusterBatch(ws, precision=prec)
         lib_prof = {}
         for _ in range(args.warmup):
@@ -141,11 +142,12 @@ def bench_batch(args):
                    "precision": head},
         "rhs_per_s": out[head]["clusters_per_s"] * 8, "runs": out,
     }
-    if "fp64" in res and "mixed" in res:
-        errs = [float(np.max(np.abs(a["C_total"] - b["C_total"]) / np.abs(b["C_total"]).max()))
-                for a, b in zip(res["mixed"], res["fp64"])]
-        line["mixed_vs_fp64"] = {"speedup": out["fp64"]["ms_per_step"] / out["mixed"]["ms_per_step"],
-                                 "cross_section_max_rel_err": max(errs)}
+    for other in ("mixed", "matfree"):
+        if "fp64" in res and other in res:
+            errs = [float(np.max(np.abs(a["C_total"] - b["C_total"]) / np.abs(b["C_total"]).max()))
+                    for a, b in zip(res[other], res["fp64"])]
+            line[f"{other}_vs_fp64"] = {"speedup": out["fp64"]["ms_per_step"] / out[other]["ms_per_step"],
+                                        "cross_section_max_rel_err": max(errs)}
     print(json.dumps(line), flush=True)
 
 
@@ -235,6 +237,45 @@ def run_timed(solver, steps, warmup, profile=False):
                 prof[name] = dict(count=c.value, avg_ms=t.value / c.value, total_ms=t.value)
         lib.mpa_profile_enable(0)
     return total, np.array(lat), prof
+
+
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
+
+
+def pair_flops(lmax):
+    """Algorithmic FP64 FLOP of one unordered pair's translation blocks:
+    10 (qA + 1) + 10 qB + 16 per mode tuple that the reference evaluates
+    (v <= n, skipping n == v with u < -m; the other blocks follow by
+    symmetry, cpp/src/vsh_translation.cpp:171-176), plus 6 per Mie scaling
+    of the 8 rmax^2 entries; qA, qB as in cpp/src/vsh_translation.cpp:15-16
+    with m' = -m."""
+    f = 0
+    for n in range(1, lmax + 1):
+        for m in range(-n, n + 1):
+            for v in range(1, n + 1):
+                for u in range(-v, v + 1):
+                    if n == v and u < -m:
+                        continue
+                    mu = u - m
+                    qa = min(n, v, (n + v - abs(mu)) // 2)
+                    qb = min(n, v, (n + v + 1 - abs(mu)) // 2)
+                    f += 10 * (qa + 1) + 10 * qb + 16
+    rmax = lmax * (lmax + 2)
+    return f + 48 * rmax * rmax
+
+
+def measured_traffic(workload, kernel):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE counters (profiles/traffic.json),
+    scaled by the copy-kernel calibration; None when not profiled."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        k = t["kernels"][f"{workload}/{kernel}"]
+    except (OSError, KeyError, ValueError):
+        return None
+    return k["fetch_kib"] * 1024 * t["fetch_scale"] + k["write_kib"] * 1024 * t["write_scale"]
 
 
 def rel_err(a, b):
@@ -349,7 +390,9 @@ def main():
         line["mixed_vs_fp64"] = {
             "speedup": line["fp64"]["ms_per_step"] / line["ms_per_step"],
             "p_scat_rel_err": rel_err(a["p_scat"], b["p_scat"]),
-            "cross_section_max_rel_err": float(np.max(np.abs(a["C_total"] - b["C_total"]) / np.abs(b["C_total"]))),
+            # relative to the largest cross-section of each source (a lossless cluster has C_abs = 0)
+            "cross_section_max_rel_err": float(np.max(np.abs(a["C_total"] - b["C_total"]) /
+                                                      np.abs(b["C_total"]).max(axis=-1, keepdims=True))),
             "force_max_rel_err": float(np.max(np.abs(a["F"] - b["F"])) / Fscale),
             "outer": results["mixed"]["report"].outer,
             "inner_iterations": results["mixed"]["report"].iterations,
@@ -366,10 +409,39 @@ def main():
         avg_s = prof[dom]["avg_ms"] / 1e3
         ach = nbytes / avg_s / 1e9
         line["roofline"] = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                            "frac": ach / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+                            "frac": ach / HBM_PEAK_GBS, "traffic": measured_traffic(w.name, dom), "kernel": dom,
                             "avg_launch_ms": prof[dom]["avg_ms"], "launches": prof[dom]["count"],
                             "bytes_per_launch": nbytes}
         line["kernels"] = prof
+    pairs = w.N * (w.N - 1) // 2
+    rmax = w.rmax
+    if "fp64" in results and "assemble_c128" in results["fp64"]["prof"]:
+        a = results["fp64"]["prof"]["assemble_c128"]
+        flop = pairs * pair_flops(w.lmax)
+        line["assembly"] = {"kernel": "assemble_c128", "avg_launch_ms": a["avg_ms"], "flop_per_launch": flop,
+                            "achieved_tflops": flop / (a["avg_ms"] / 1e3) / 1e12,
+                            "fp64_peak_tflops": FP64_PEAK_TFLOPS,
+                            "frac_fp64": flop / (a["avg_ms"] / 1e3) / 1e12 / FP64_PEAK_TFLOPS,
+                            "write_gbs": 16 * n * n / (a["avg_ms"] / 1e3) / 1e9,
+                            "frac_hbm": 16 * n * n / (a["avg_ms"] / 1e3) / 1e9 / HBM_PEAK_GBS}
+    if "matfree" in results:
+        m = results["matfree"]
+        mp = m["prof"].get("matfree_c128")
+        block = {"solves_per_s": args.steps / m["total"], "ms_per_step": m["total"] / args.steps * 1e3,
+                 "p50_ms": float(np.percentile(m["lat"], 50) * 1e3),
+                 "p99_ms": float(np.percentile(m["lat"], 99) * 1e3), "iterations": m["report"].iterations}
+        if mp:
+            flop = pairs * (pair_flops(w.lmax) - 48 * rmax * rmax + 2 * 8 * (2 * rmax) ** 2 * S)
+            block["kernel"] = {"name": "matfree_c128", "avg_launch_ms": mp["avg_ms"], "launches": mp["count"],
+                               "flop_per_launch": flop, "achieved_tflops": flop / (mp["avg_ms"] / 1e3) / 1e12,
+                               "frac_fp64": flop / (mp["avg_ms"] / 1e3) / 1e12 / FP64_PEAK_TFLOPS}
+        if "fp64" in results:
+            a, b = m["obs"], results["fp64"]["obs"]
+            block["speedup_vs_fp64"] = line["fp64"]["ms_per_step"] / block["ms_per_step"]
+            block["p_scat_rel_err_vs_fp64"] = rel_err(a["p_scat"], b["p_scat"])
+            block["cross_section_max_rel_err_vs_fp64"] = float(np.max(
+                np.abs(a["C_total"] - b["C_total"]) / np.abs(b["C_total"]).max(axis=-1, keepdims=True)))
+        line["matfree"] = block
     if not args.no_cpu_baseline and "fp64" in results:
         it = int(max(results["fp64"]["report"].iterations))
         log("cpu baseline (oracle) ...")

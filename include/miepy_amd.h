@@ -41,6 +41,13 @@ extern "C" {
 #define MPA_C128 0
 #define MPA_C64 1
 
+/* runtime options (mpa_set_option / mpa_get_option) */
+#define MPA_OPT_GRAPHS 0 /* 1 (default): the solver loops replay their iterations from a captured
+                            HIP graph (one launch per iteration instead of one per kernel); 0: eager
+                            launches.  Results are identical either way.  Graphs are not used while
+                            launch timing (mpa_profile_enable) is on. */
+#define MPA_OPT_COUNT 1
+
 const char* mpa_version(void);
 const char* mpa_last_error(void);
 
@@ -228,6 +235,10 @@ int mpa_force_torque(int M, int lmax, const double* p_scat, const double* p_inc,
  * C_part [M, 3, 2, lmax] f64 (scattering, absorption, extinction). */
 int mpa_particle_cross_sections(int M, int lmax, const double* p_scat, const double* p_inc, const double* p_src,
                                 double k, double* C_part, void* stream);
+
+/* Process-wide runtime options (no counterpart in the reference). */
+int mpa_set_option(int key, int value);
+int mpa_get_option(int key);
 
 /* Launch timing with HIP events recorded on the launching stream (off by
  * default).  kind: 0 matvec c128, 1 matvec c64, 2 assembly c128,

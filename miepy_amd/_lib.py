@@ -20,6 +20,7 @@ MPA_ERR_OOM = 6
 
 MPA_C128 = 0
 MPA_C64 = 1
+MPA_OPT_GRAPHS = 0
 
 STATUS_NAMES = {
     MPA_OK: "ok",
@@ -60,6 +61,8 @@ SIGNATURES = {
     "mpa_force_torque": (_i, [_i, _i, _v, _v, _d, _d, _v, _v, _v]),
     "mpa_particle_cross_sections": (_i, [_i, _i, _v, _v, _v, _d, _v, _v]),
     "mpa_stream_copy": (_i, [_v, _v, _sz, _v]),
+    "mpa_set_option": (_i, [_i, _i]),
+    "mpa_get_option": (_i, [_i]),
     "mpa_profile_enable": (_i, [_i]),
     "mpa_profile_read": (_i, [_i, _pi, _pd]),
     "mpa_translation_table": (_i, [_i, _pi, _pd]),

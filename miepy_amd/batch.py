@@ -4,7 +4,8 @@ Each cluster (positions, radii, permittivities, wavelength, lmax, sources)
 becomes one system (I + T_c) p_inc = p_src with one right-hand side per
 source.  Host work (Mie coefficients, p_src) is done once in `prepare`; the
 device work (assembly of every matrix, one batched BiCGSTAB or one batched
-complex64 + FP64-refinement solve over all clusters, observables) is
+complex64 + FP64-refinement solve over all clusters, or one batched
+matrix-free FP64 BiCGSTAB with no stored matrices; observables) is
 `ClusterBatch.solve`.  The batched engine covers all clusters with one launch
 per kernel and iteration, and a cluster that converged costs nothing further.
 """
@@ -35,21 +36,27 @@ class PreparedCluster:
         self.p_src = D.to_device(p_src, D.C128)
         self.A64 = None
         self.A32 = None
+        self.op = None
 
     def matrix_bytes(self, precision):
+        if precision == "matfree":
+            return 0
         return 16 * self.n * self.n + (8 * self.n * self.n if precision == "mixed" else 0)
 
 
 class ClusterBatch:
     def __init__(self, workloads, precision="mixed", inner_tol=1e-5, maxiter=2000, max_refine=8):
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in ("fp64", "mixed", "matfree"):
+            raise ValueError("precision must be 'fp64', 'mixed' or 'matfree'")
         self.precision = precision
         self.inner_tol = inner_tol
         self.maxiter = maxiter
         self.max_refine = max_refine
         self.clusters = [PreparedCluster(w) for w in workloads]
         for c in self.clusters:
+            if precision == "matfree":
+                c.op = D.ClusterOperator(c.pos, c.mie_scat, c.k, max_rhs=c.S)
+                continue
             c.A64 = torch.empty((c.n, c.n), dtype=D.C128, device=c.pos.device)
             if precision == "mixed":
                 c.A32 = torch.empty((c.n, c.n), dtype=D.C64, device=c.pos.device)
@@ -59,6 +66,10 @@ class ClusterBatch:
     def solve(self):
         """Assemble, solve and reduce every cluster; results stay on the device."""
         cs = self.clusters
+        if self.precision == "matfree":
+            xs, self.reports = D.bicgstab_batched([c.op for c in cs], [c.p_src.reshape(c.S, c.n) for c in cs],
+                                                  tol=[c.w.tol for c in cs], maxiter=self.maxiter, add_identity=True)
+            return self._reduce(xs)
         for c in cs:
             if self.precision == "mixed":
                 D.assemble_dual(c.pos, c.mie_scat, c.k, add_identity=True, out64=c.A64, out32=c.A32)
@@ -72,6 +83,10 @@ class ClusterBatch:
             xs, self.reports = D.solve_refined_batched([c.A32 for c in cs], [c.A64 for c in cs], rhs, tol=tols,
                                                        inner_tol=self.inner_tol, maxiter=self.maxiter,
                                                        max_refine=self.max_refine)
+        return self._reduce(xs)
+
+    def _reduce(self, xs):
+        cs = self.clusters
         self.results = []
         for c, x in zip(cs, xs):
             p_inc = x.reshape(c.S, c.w.N, 2, c.w.rmax)

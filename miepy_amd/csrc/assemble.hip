@@ -9,80 +9,29 @@
 // add_identity is set, giving the system matrix I + T of interactions.cpp:351-353).
 //
 // Design: one workgroup owns block row i and a tile of J consecutive column
-// blocks j0..j0+J-1.  Phase 1 computes, per pair, h_p(k d) (FP64 upward
-// recurrence, special.cpp:90-112), P_p^mu(cos theta) without Condon-Shortley
-// phase (special.cpp:180-204) for mu >= 0, and e^{i mu phi} into LDS.  Phase 2
-// gives each lane a fixed output column and walks the 2 rmax rows of the
-// block row, so every row store of the wave is one contiguous run of
-// J * 2 rmax entries: the matrix is written exactly once, fully coalesced,
-// with no memset and no scattered mirror writes.  Both (i,j) and (j,i) blocks
-// are produced by their own row owners; the translation work that the
-// reference shares through its symmetry is recomputed (the kernel is bound by
-// HBM write bandwidth, not by these FP64 flops).
+// blocks j0..j0+J-1.  Phase 1 forms, per pair, the products
+// HL[p][mu] = h_p(k d) P_p^mu(cos theta) and e^{i mu phi} in LDS
+// (translation.hpp), with the compact coefficient table staged in LDS next
+// to them (lmax <= 3; larger tables are read through the caches).  Phase 2
+// gives each lane one (pair, column mode s) and walks the row modes r: the
+// lane forms X_A[r,s] and X_B[r,s] once and writes the four entries they
+// make, (a,b) = (0,0) X_A mie_b0, (0,1) X_B mie_b1, (1,0) X_B mie_b0,
+// (1,1) X_A mie_b1.  For a fixed r the lanes of a wave cover contiguous runs
+// of the row, so every entry of the matrix is written exactly once,
+// coalesced, with no memset and no mirrored writes.  Both (i,j) and (j,i)
+// blocks are produced by their own row owners; the translation work that
+// the reference shares through its symmetry is recomputed (the kernel is
+// bound by HBM write bandwidth, not by these FP64 flops).
 
-#include "common.hpp"
+#include "translation.hpp"
+
+using namespace mpa;
 
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kJMax = 16;
-constexpr int kLegMax = (MPA_PMAX_MAX + 1) * (MPA_PMAX_MAX + 2) / 2;
-constexpr int kMuMax = 2 * MPA_LMAX_MAX + 1;  // mu = 0 .. 2 lmax
-
-struct PairLDS {
-    double2 hp[kJMax][MPA_PMAX_MAX + 2];
-    double leg[kJMax][kLegMax];
-    double2 eph[kJMax][kMuMax];
-    int valid[kJMax];
-};
-
-// Per-pair special functions into LDS; one thread per (pair, mu column).
-__device__ void pair_setup(PairLDS& L, int jj, int mu, int lmax, double dx, double dy, double dz, double k) {
-    const int pmax = 2 * lmax + 1;
-    const double rad = sqrt(dx * dx + dy * dy + dz * dz);
-    const double x = dz / rad;
-    const double s = sqrt(fmax(0.0, (1.0 - x) * (1.0 + x)));
-    // Legendre column mu: P_mu^mu = (2mu-1)!! s^mu; P_{mu+1}^mu = x (2mu+1) P_mu^mu;
-    // (l-mu) P_l^mu = (2l-1) x P_{l-1}^mu - (l+mu-1) P_{l-2}^mu
-    double pmm = 1.0;
-    for (int i = 1; i <= mu; i++) pmm *= (2 * i - 1) * s;
-    L.leg[jj][leg_index(mu, mu)] = pmm;
-    if (mu + 1 <= pmax) {
-        double p1 = x * (2 * mu + 1) * pmm;
-        L.leg[jj][leg_index(mu + 1, mu)] = p1;
-        double p2 = pmm;
-        for (int l = mu + 2; l <= pmax; l++) {
-            double pl = ((2 * l - 1) * x * p1 - (l + mu - 1) * p2) / (l - mu);
-            L.leg[jj][leg_index(l, mu)] = pl;
-            p2 = p1;
-            p1 = pl;
-        }
-    }
-    if (mu <= 2 * lmax) {
-        const double phi = atan2(dy, dx);
-        double sn, cs;
-        sincos(mu * phi, &sn, &cs);
-        L.eph[jj][mu] = make_double2(cs, sn);
-    }
-    if (mu == 0) {
-        const double z = k * rad;
-        double sz, cz;
-        sincos(z, &sz, &cz);
-        double j0 = sz / z, y0 = -cz / z;
-        double j1 = j0 / z + y0, y1 = y0 / z - j0;
-        L.hp[jj][0] = make_double2(j0, y0);
-        L.hp[jj][1] = make_double2(j1, y1);
-        for (int i = 2; i <= pmax; i++) {
-            double j2 = (2 * i - 1) / z * j1 - j0;
-            double y2 = (2 * i - 1) / z * y1 - y0;
-            L.hp[jj][i] = make_double2(j2, y2);
-            j0 = j1;
-            j1 = j2;
-            y0 = y1;
-            y1 = y2;
-        }
-    }
-}
+constexpr int kJMax = 32;
+constexpr int kTabLdsLmax = 3;  // largest lmax whose table is staged in LDS
 
 template <typename OutT>
 __device__ inline void store(OutT* p, double2 v);
@@ -95,108 +44,100 @@ __device__ inline void store<float2>(float2* p, double2 v) {
     *p = make_float2((float)v.x, (float)v.y);
 }
 
+template <typename OutT, bool DUAL>
+__device__ inline void put(OutT* out, float2* out32, size_t off, double2 v) {
+    store<OutT>(out + off, v);
+    if (DUAL) store<float2>(out32 + off, v);
+}
+
+// LDS bytes: HL, e^{i mu phi}, Mie coefficients of the J column particles,
+// then (TAB_LDS) the compact table.
+size_t lds_bytes(int lmax, int J, int nterms, bool tab_lds) {
+    const size_t b = sizeof(double2) * ((size_t)J * hl_count(lmax) + (size_t)J * (2 * lmax + 1) + (size_t)J * 2 * lmax);
+    return b + (tab_lds ? tab_lds_bytes(lmax_to_rmax(lmax), nterms) : 0);
+}
+
 // OutT: element type of `out`; DUAL: also write the complex64 rounding of
 // every entry to `out32` in the same pass (the mixed-precision solve needs
-// both matrices; the FP64 arithmetic is done once).
-template <typename OutT, bool DUAL>
+// both matrices; the FP64 arithmetic is done once); TAB_LDS: stage the
+// coefficient table in LDS.
+template <typename OutT, bool DUAL, bool TAB_LDS>
 __global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int J, int ntiles,
                                                              const double* __restrict__ pos,
-                                                             const double2* __restrict__ mie, double k,
-                                                             const int4* __restrict__ tmeta,
-                                                             const double2* __restrict__ tcoef, int add_identity,
-                                                             OutT* __restrict__ out, float2* __restrict__ out32) {
-    __shared__ PairLDS L;
-    __shared__ double2 mieJ[kJMax][2][MPA_LMAX_MAX];
-
+                                                             const double2* __restrict__ mie, double k, TabDev gtab,
+                                                             int nterms, int add_identity, OutT* __restrict__ out,
+                                                             float2* __restrict__ out32) {
+    extern __shared__ double2 lds[];
     const int rmax = lmax_to_rmax(lmax);
     const int bs = 2 * rmax;
     const size_t n = (size_t)bs * N;
     const int pmax = 2 * lmax + 1;
-    const int qa_slots = lmax + 1;
-    const int coef_slots = 2 * lmax + 1;
+    const int Lc = hl_count(lmax);
+    const int nmu = 2 * lmax + 1;
+    double2* HL = lds;
+    double2* eph = HL + J * Lc;
+    double2* mieJ = eph + J * nmu;
 
     const int i = blockIdx.x / ntiles;
     const int j0 = (blockIdx.x % ntiles) * J;
     const int jcount = min(J, N - j0);
 
+    TabDev tab = gtab;
+    if (TAB_LDS) tab = stage_table(gtab, rmax, nterms, (char*)(mieJ + J * 2 * lmax));
     // phase 1: per-pair special functions
-    const int ncol = pmax + 1;
-    for (int t = threadIdx.x; t < jcount * ncol; t += kThreads) {
-        const int jj = t / ncol, mu = t % ncol;
+    for (int t = threadIdx.x; t < jcount * (pmax + 1); t += kThreads) {
+        const int jj = t / (pmax + 1), mu = t - jj * (pmax + 1);
         const int j = j0 + jj;
-        if (j != i) {
-            const double dx = pos[3 * i] - pos[3 * j];
-            const double dy = pos[3 * i + 1] - pos[3 * j + 1];
-            const double dz = pos[3 * i + 2] - pos[3 * j + 2];
-            pair_setup(L, jj, mu, lmax, dx, dy, dz, k);
-        }
+        if (j != i)
+            pair_hl(HL + jj * Lc, eph + jj * nmu, mu, lmax, pos[3 * i] - pos[3 * j], pos[3 * i + 1] - pos[3 * j + 1],
+                    pos[3 * i + 2] - pos[3 * j + 2], k);
     }
-    for (int t = threadIdx.x; t < jcount * 2 * lmax; t += kThreads) {
-        const int jj = t / (2 * lmax), rest = t % (2 * lmax);
-        mieJ[jj][rest / lmax][rest % lmax] = mie[(size_t)(j0 + jj) * 2 * lmax + rest];
-    }
+    for (int t = threadIdx.x; t < jcount * 2 * lmax; t += kThreads) mieJ[t] = mie[(size_t)j0 * 2 * lmax + t];
     __syncthreads();
 
-    // phase 2: each lane owns one output column of the tile, walks the rows
-    const int W = jcount * bs;
+    // phase 2: lane (pair jj, column mode s) walks the row modes r
     const size_t row0 = (size_t)i * bs;
     const size_t col0 = (size_t)j0 * bs;
-    for (int c = threadIdx.x; c < W; c += kThreads) {
-        const int jj = c / bs;
-        const int cb = c - jj * bs;
-        const int b = cb >= rmax ? 1 : 0;
-        const int s = cb - b * rmax;
+    for (int c = threadIdx.x; c < jcount * rmax; c += kThreads) {
+        const int jj = c / rmax;
+        const int s = c - jj * rmax;
         const int j = j0 + jj;
-        OutT* col = out + row0 * n + col0 + c;
-        float2* col32 = DUAL ? out32 + row0 * n + col0 + c : nullptr;
+        const size_t base = row0 * n + col0 + (size_t)jj * bs + s;  // row (0, 0), column (b = 0, s)
         if (j == i) {
-            for (int l = 0; l < bs; l++) {
-                double2 v = make_double2((add_identity && l == cb) ? 1.0 : 0.0, 0.0);
-                store<OutT>(col + (size_t)l * n, v);
-                if (DUAL) store<float2>(col32 + (size_t)l * n, v);
+            for (int r = 0; r < rmax; r++) {
+                const double2 d = make_double2((add_identity && r == s) ? 1.0 : 0.0, 0.0);
+                const double2 z = make_double2(0.0, 0.0);
+                put<OutT, DUAL>(out, out32, base + (size_t)r * n, d);
+                put<OutT, DUAL>(out, out32, base + (size_t)r * n + rmax, z);
+                put<OutT, DUAL>(out, out32, base + (size_t)(rmax + r) * n, z);
+                put<OutT, DUAL>(out, out32, base + (size_t)(rmax + r) * n + rmax, d);
             }
             continue;
         }
         const int v = (int)sqrtf((float)(s + 1) + 0.5f);
-        const double2 mj = mieJ[jj][b][v - 1];
-        for (int l = 0; l < bs; l++) {
-            const int a = l >= rmax ? 1 : 0;
-            const int r = l - a * rmax;
-            const int t = r * rmax + s;
-            const int4 meta = tmeta[t];
-            const int amu = meta.z < 0 ? -meta.z : meta.z;
-            const double2* cf = tcoef + (size_t)t * coef_slots;
-            double2 acc = make_double2(0.0, 0.0);
-            if (((a + b) & 1) == 0) {
-                for (int q = 0; q < meta.x; q++) {
-                    const int p = meta.w - 2 * q;
-                    const double2 hl = cscale(L.hp[jj][p], L.leg[jj][leg_index(p, amu)]);
-                    cfma(acc, cf[q], hl);
-                }
-            } else {
-                for (int q = 1; q <= meta.y; q++) {
-                    const int p = meta.w - 2 * q + 1;
-                    const double2 hl = cscale(L.hp[jj][p], L.leg[jj][leg_index(p, amu)]);
-                    cfma(acc, cf[qa_slots + q - 1], hl);
-                }
-            }
-            double2 e = L.eph[jj][amu];
-            if (meta.z < 0) e.y = -e.y;
-            const double2 val = cmul(cmul(e, acc), mj);
-            store<OutT>(col + (size_t)l * n, val);
-            if (DUAL) store<float2>(col32 + (size_t)l * n, val);
+        const double2 m0 = mieJ[(jj * 2 + 0) * lmax + v - 1];
+        const double2 m1 = mieJ[(jj * 2 + 1) * lmax + v - 1];
+        const double2* hl = HL + jj * Lc;
+        const double2* ep = eph + jj * nmu;
+        for (int r = 0; r < rmax; r++) {
+            double2 xa, xb;
+            tuple_xab(tab, r * rmax + s, hl, ep, xa, xb);
+            put<OutT, DUAL>(out, out32, base + (size_t)r * n, cmul(xa, m0));
+            put<OutT, DUAL>(out, out32, base + (size_t)r * n + rmax, cmul(xb, m1));
+            put<OutT, DUAL>(out, out32, base + (size_t)(rmax + r) * n, cmul(xb, m0));
+            put<OutT, DUAL>(out, out32, base + (size_t)(rmax + r) * n + rmax, cmul(xa, m1));
         }
     }
 }
 
-// Column blocks per tile: the J <= 16 that keeps the most lanes busy in
-// phase 2 (W = J * 2 rmax columns over 256 lanes); ties go to the larger J.
+// Column blocks per tile: the J <= 32 that keeps the most lanes busy in
+// phase 2 (J * rmax lanes over 256); ties go to the larger J.
 int pick_tile(int lmax) {
-    const int bs = 2 * lmax_to_rmax(lmax);
+    const int rmax = lmax_to_rmax(lmax);
     int best = 1;
     double best_eff = -1;
     for (int J = 1; J <= kJMax; J++) {
-        const int W = J * bs;
+        const int W = J * rmax;
         const int iters = (W + kThreads - 1) / kThreads;
         const double eff = (double)W / (iters * kThreads);
         if (eff >= best_eff - 1e-12) {
@@ -205,6 +146,33 @@ int pick_tile(int lmax) {
         }
     }
     return best;
+}
+
+template <typename OutT, bool DUAL>
+int launch_assembly(int N, int lmax, const double* positions, const double* mie, double k, int add_identity,
+                    OutT* out, float2* out32, hipStream_t s, const char* who) {
+    const TransTable* T = mpa_get_table(lmax);
+    if (!T) return MPA_ERR_ARG;
+    const int J = pick_tile(lmax);
+    const int ntiles = (N + J - 1) / J;
+    const long long nblocks = (long long)N * ntiles;
+    if (nblocks > 0x7fffffffLL) {
+        mpa_set_error("%s: N=%d too large for one launch", who, N);
+        return MPA_ERR_ARG;
+    }
+    const TabDev tab{T->cmeta, T->ccoef, T->clidx};
+    const bool tab_lds = lmax <= kTabLdsLmax;
+    const size_t lds = lds_bytes(lmax, J, T->nterms, tab_lds);
+    if (tab_lds)
+        hipLaunchKernelGGL((assemble_kernel<OutT, DUAL, true>), dim3((unsigned)nblocks), dim3(kThreads), lds, s, N,
+                           lmax, J, ntiles, positions, (const double2*)mie, k, tab, T->nterms, add_identity, out,
+                           out32);
+    else
+        hipLaunchKernelGGL((assemble_kernel<OutT, DUAL, false>), dim3((unsigned)nblocks), dim3(kThreads), lds, s, N,
+                           lmax, J, ntiles, positions, (const double2*)mie, k, tab, T->nterms, add_identity, out,
+                           out32);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
 }
 
 }  // namespace
@@ -219,27 +187,13 @@ extern "C" int mpa_sphere_aggregate_tmatrix(int N, int lmax, const double* posit
         mpa_set_error("sphere_aggregate_tmatrix: unknown dtype %d", out_dtype);
         return MPA_ERR_ARG;
     }
-    const TransTable* T = mpa_get_table(lmax);
-    if (!T) return MPA_ERR_ARG;
-    const int J = pick_tile(lmax);
-    const int ntiles = (N + J - 1) / J;
-    const long long nblocks = (long long)N * ntiles;
-    if (nblocks > 0x7fffffffLL) {
-        mpa_set_error("sphere_aggregate_tmatrix: N=%d too large for one launch", N);
-        return MPA_ERR_ARG;
-    }
     hipStream_t s = (hipStream_t)stream;
     ProfScope prof(out_dtype == MPA_C128 ? 2 : 3, s);
     if (out_dtype == MPA_C128)
-        hipLaunchKernelGGL((assemble_kernel<double2, false>), dim3((unsigned)nblocks), dim3(kThreads), 0, s, N, lmax, J,
-                           ntiles, positions, (const double2*)mie, k, T->meta, T->coef, add_identity, (double2*)out,
-                           (float2*)nullptr);
-    else
-        hipLaunchKernelGGL((assemble_kernel<float2, false>), dim3((unsigned)nblocks), dim3(kThreads), 0, s, N, lmax, J,
-                           ntiles, positions, (const double2*)mie, k, T->meta, T->coef, add_identity, (float2*)out,
-                           (float2*)nullptr);
-    MPA_LAUNCH_CHECK();
-    return MPA_OK;
+        return launch_assembly<double2, false>(N, lmax, positions, mie, k, add_identity, (double2*)out, nullptr, s,
+                                               "sphere_aggregate_tmatrix");
+    return launch_assembly<float2, false>(N, lmax, positions, mie, k, add_identity, (float2*)out, nullptr, s,
+                                          "sphere_aggregate_tmatrix");
 }
 
 extern "C" int mpa_assembly_tile(int lmax) { return pick_tile(lmax); }
@@ -250,20 +204,8 @@ extern "C" int mpa_sphere_aggregate_tmatrix_dual(int N, int lmax, const double* 
         mpa_set_error("sphere_aggregate_tmatrix_dual: bad argument (N=%d, k=%g)", N, k);
         return MPA_ERR_ARG;
     }
-    const TransTable* T = mpa_get_table(lmax);
-    if (!T) return MPA_ERR_ARG;
-    const int J = pick_tile(lmax);
-    const int ntiles = (N + J - 1) / J;
-    const long long nblocks = (long long)N * ntiles;
-    if (nblocks > 0x7fffffffLL) {
-        mpa_set_error("sphere_aggregate_tmatrix_dual: N=%d too large for one launch", N);
-        return MPA_ERR_ARG;
-    }
     hipStream_t s = (hipStream_t)stream;
     ProfScope prof(2, s);
-    hipLaunchKernelGGL((assemble_kernel<double2, true>), dim3((unsigned)nblocks), dim3(kThreads), 0, s, N, lmax, J,
-                       ntiles, positions, (const double2*)mie, k, T->meta, T->coef, add_identity, (double2*)out64,
-                       (float2*)out32);
-    MPA_LAUNCH_CHECK();
-    return MPA_OK;
+    return launch_assembly<double2, true>(N, lmax, positions, mie, k, add_identity, (double2*)out64, (float2*)out32,
+                                          s, "sphere_aggregate_tmatrix_dual");
 }

@@ -250,6 +250,13 @@ __global__ __launch_bounds__(kThreads) void k_restart_commit(const SysDev* __res
     }
 }
 
+// copy every RHS's scalars into one contiguous array for a single readback
+__global__ __launch_bounds__(64) void k_gather_scalars(const SysDev* __restrict__ sys, const int2* __restrict__ cw,
+                                                        int count, Scal* __restrict__ out) {
+    const int k = blockIdx.x * 64 + threadIdx.x;
+    if (k < count) out[k] = sys[cw[k].x].scal[cw[k].y];
+}
+
 // ---- host side ---------------------------------------------------------------------
 
 size_t vec_bytes(int n, int S, int esize) { return align256((size_t)esize * 2 * n * S); }
@@ -275,6 +282,7 @@ struct Plan {
     MvDev *d_mv_ps = nullptr, *d_mv_st = nullptr, *d_mv_xt = nullptr;
     int2* d_mv_work[kTileClasses] = {};
     int n_mv_work[kTileClasses] = {};
+    Scal* d_scal_out = nullptr;  // gathered scalars, [sum(nrhs)]
 };
 
 size_t plan_bytes(int nsys, const mpa_system* s, int dtype) {
@@ -286,7 +294,8 @@ size_t plan_bytes(int nsys, const mpa_system* s, int dtype) {
     }
     (void)dtype;
     return align256(sizeof(SysDev) * nsys) + align256(sizeof(int3) * vec) + align256(sizeof(int2) * sc) +
-           3 * align256(sizeof(MvDev) * nsys) + kTileClasses * 256 + align256(sizeof(int2) * mvw);
+           3 * align256(sizeof(MvDev) * nsys) + kTileClasses * 256 + align256(sizeof(int2) * mvw) +
+           align256(sizeof(Scal) * sc);
 }
 
 int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch, hipStream_t st) {
@@ -334,6 +343,7 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
         P.mv_xt[i] = MvDev{q.A, q.x, d.t, n, S, q.add_identity, -1};
         if (q.op) {
             const mpa_cluster_operator& o = *q.op;
+            if (!mpa_get_table(o.lmax)) return MPA_ERR_ARG;  // built here, never inside a graph capture
             const MfJob base{o.N, o.lmax, S, q.add_identity ? 1 : 0, o.positions, (const double2*)o.mie, o.k,
                              nullptr, nullptr, (double2*)o.workspace, i, 0};
             MfJob j = base;
@@ -365,6 +375,7 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
         P.n_mv_work[c] = (int)P.mv_work[c].size();
         p += align256(sizeof(int2) * P.mv_work[c].size());
     }
+    P.d_scal_out = (Scal*)p;
     MPA_HIP_CHECK(hipMemcpyAsync(P.d_sys, P.sys.data(), sizeof(SysDev) * nsys, hipMemcpyHostToDevice, st));
     MPA_HIP_CHECK(hipMemcpyAsync(P.d_vec, P.vec_work.data(), sizeof(int3) * P.vec_work.size(), hipMemcpyHostToDevice, st));
     MPA_HIP_CHECK(hipMemcpyAsync(P.d_scal, P.scal_work.data(), sizeof(int2) * P.scal_work.size(), hipMemcpyHostToDevice, st));
@@ -433,16 +444,58 @@ int launch_restart(const Plan& P, hipStream_t st) {
     return MPA_OK;
 }
 
-// Read back the scalars of every RHS (synchronizes the stream).
+// Read back the scalars of every RHS (synchronizes the stream): one gather
+// kernel and one copy through pinned host memory, however many systems.
 int read_scalars(const Plan& P, std::vector<Scal>& out, hipStream_t st) {
-    size_t total = P.scal_work.size();
+    const int total = (int)P.scal_work.size();
     out.resize(total);
-    size_t k = 0;
-    for (const SysDev& d : P.sys) {
-        MPA_HIP_CHECK(hipMemcpyAsync(&out[k], d.scal, sizeof(Scal) * d.S, hipMemcpyDeviceToHost, st));
-        k += d.S;
-    }
+    hipLaunchKernelGGL(k_gather_scalars, dim3((total + 63) / 64), dim3(64), 0, st, P.d_sys, P.d_scal, total,
+                       P.d_scal_out);
+    MPA_LAUNCH_CHECK();
+    const size_t bytes = sizeof(Scal) * total;
+    PinnedLease pin(bytes);
+    void* dst = pin.ptr() ? pin.ptr() : (void*)out.data();
+    MPA_HIP_CHECK(hipMemcpyAsync(dst, P.d_scal_out, bytes, hipMemcpyDeviceToHost, st));
     MPA_HIP_CHECK(hipStreamSynchronize(st));
+    if (dst != out.data()) std::memcpy(out.data(), dst, bytes);
+    return MPA_OK;
+}
+
+// k BiCGSTAB iterations of a plan recorded once as a HIP graph and replayed.
+struct IterGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    IterGraph() = default;
+    IterGraph(const IterGraph&) = delete;
+    IterGraph& operator=(const IterGraph&) = delete;
+    ~IterGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+template <typename T>
+int launch_iteration(const Plan& P, hipStream_t st);
+
+template <typename T>
+int capture_iterations(const Plan& P, int k, IterGraph& g) {
+    hipStream_t cap = mpa_capture_stream();
+    if (!cap) {
+        mpa_set_error("bicgstab: no capture stream");
+        return MPA_ERR_HIP;
+    }
+    MPA_HIP_CHECK(hipStreamBeginCapture(cap, hipStreamCaptureModeRelaxed));
+    int rc = MPA_OK;
+    for (int i = 0; i < k && rc == MPA_OK; i++) rc = launch_iteration<T>(P, cap);
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(cap, &graph);
+    if (rc != MPA_OK) {
+        if (e == hipSuccess && graph) (void)hipGraphDestroy(graph);
+        return rc;
+    }
+    MPA_HIP_CHECK(e);
+    g.graph = graph;
+    MPA_HIP_CHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
     return MPA_OK;
 }
 
@@ -460,6 +513,12 @@ int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, doubl
     const int k_sync = sync_every > 0 ? sync_every : 1;
     std::vector<Scal> host;
     long launched = 0;
+    IterGraph graph;
+    const bool use_graph = mpa_option(MPA_OPT_GRAPHS) != 0 && !mpa_profile_active();
+    if (use_graph) {
+        rc = capture_iterations<T>(P, k_sync, graph);
+        if (rc) return rc;
+    }
     for (;;) {
         rc = read_scalars(P, host, st);
         if (rc) return rc;
@@ -478,9 +537,13 @@ int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, doubl
             mpa_set_error("bicgstab: iteration budget exhausted");
             return MPA_NOT_CONVERGED;
         }
-        for (int k = 0; k < k_sync; k++) {
-            rc = launch_iteration<T>(P, st);
-            if (rc) return rc;
+        if (use_graph) {
+            MPA_HIP_CHECK(hipGraphLaunch(graph.exec, st));
+        } else {
+            for (int k = 0; k < k_sync; k++) {
+                rc = launch_iteration<T>(P, st);
+                if (rc) return rc;
+            }
         }
         launched += k_sync;
     }

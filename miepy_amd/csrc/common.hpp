@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 
 #include "../../include/miepy_amd.h"
 
@@ -88,6 +89,11 @@ struct TransTable {
     int coef_slots = 0;  // 2 lmax + 1
     int4* meta = nullptr;      // device, [rmax*rmax]
     double2* coef = nullptr;   // device, [rmax*rmax*coef_slots]
+    // compact form (only the nonzero terms; translation.hpp TabDev):
+    int nterms = 0;
+    int2* cmeta = nullptr;            // device, [rmax*rmax]
+    double2* ccoef = nullptr;         // device, [nterms]
+    unsigned short* clidx = nullptr;  // device, [nterms]
 };
 
 // Returns the (cached, per device) table for lmax; nullptr on failure.
@@ -105,6 +111,23 @@ class ProfScope {
     int kind_;
     hipStream_t stream_;
     void* start_;
+};
+
+// ---- runtime settings and resources (runtime.cpp) ------------------------------
+int mpa_option(int key);
+bool mpa_profile_active();
+// Non-blocking stream of the current device on which iterations are captured.
+hipStream_t mpa_capture_stream();
+// Exclusive use of the process's pinned host staging buffer (>= bytes) for
+// the lifetime of the lease; ptr() is null if the allocation failed.
+class PinnedLease {
+   public:
+    explicit PinnedLease(size_t bytes);
+    void* ptr() const { return ptr_; }
+
+   private:
+    std::lock_guard<std::mutex> lock_;
+    void* ptr_;
 };
 
 // ---- error plumbing ------------------------------------------------------------

@@ -12,40 +12,56 @@
 // FP64 VALU and LDS issue, not by HBM.
 //
 // Work decomposition: a workgroup owns block row i and a chunk of column
-// particles and walks the chunk in batches of JB pairs.  Per batch, phase 1
-// gives one thread per (pair, Legendre order mu) the products
-// HL[p][mu] = h_p(k d) P_p^mu(cos theta) (same recurrences and operation
-// order as the assembly) and e^{i mu phi}, and forms z_j for the batch, all
-// in LDS.  Phase 2 gives each thread one (pair, row mode r); it walks the
-// column modes s, forms X_A[r,s] and X_B[r,s] from the table's q-sums and
-// accumulates both polarizations of y_i for up to four right-hand sides in
-// registers.  At the end the JB partial rows meet in LDS in a fixed order and
-// the chunk's partial row is written; a second kernel adds the chunks in a
-// fixed order plus the identity term, so the result is deterministic.
+// particles and walks the chunk in batches of JB pairs.  Per batch, one
+// thread per pair forms the pair geometry (cos theta, sin theta, k|d|,
+// sin/cos k|d|, e^{i phi}); then one thread per (pair, Legendre order mu)
+// forms the products HL[p][mu] = h_p(k|d|) P_p^mu(cos theta) and
+// e^{i mu phi}, and the batch's z_j, all in LDS.  Phase 2 gives each lane
+// one (row mode r, pair): lanes that share r sit in the same wave, so the
+// coefficient table is read as a broadcast and the q-loops of a wave
+// diverge over only a few tuples; the per-pair LDS arrays have odd strides,
+// so the lanes' reads hit distinct banks.  A lane walks the column modes s,
+// forms X_A[r,s] and X_B[r,s] and accumulates both polarizations of y_i for
+// SG right-hand sides in registers.  At the end the JB partial rows meet in
+// LDS in a fixed order and the chunk's partial row is written; a second
+// kernel adds the chunks in a fixed order plus the identity term, so the
+// result is deterministic.
 
 #include "matfree.hpp"
+#include "translation.hpp"
 
 namespace mpa {
 namespace {
 
 constexpr int kJB = 16;   // most pairs per batch
-constexpr int kSMax = 4;  // right-hand sides per pass
+constexpr int kSMax = 4;  // right-hand sides per pass (largest SG)
 
 __host__ __device__ inline int pairs_per_batch(int rmax) {
     const int jb = kThreads / rmax;
     return jb > kJB ? kJB : jb;
 }
-__host__ __device__ inline int leg_count(int lmax) {
-    const int pmax = 2 * lmax + 1;
-    return (pmax + 1) * (pmax + 2) / 2;
-}
+constexpr int kTabLdsLmax = 3;  // largest lmax whose table is staged in LDS
 
-// LDS bytes of one workgroup: HL, e^{i mu phi}, z / reduction buffer.
-inline size_t lds_bytes(int lmax, int S) {
+// Odd per-pair stride of the HL array (conflict-free 16-byte LDS reads).
+__host__ __device__ inline int hl_stride(int lmax) { return hl_count(lmax) | 1; }
+
+// Right-hand sides per pass for S right-hand sides.
+inline int sg_for(int S) { return S == 1 ? 1 : (S == 2 ? 2 : kSMax); }
+
+// Per-pair geometry staged for phase 1.
+struct PairGeo {
+    double x, s, z, sz, cz, ex, ey, pad;
+};
+
+// LDS bytes of one workgroup: pair geometry, HL, e^{i mu phi}, z /
+// reduction buffer, then (lmax <= kTabLdsLmax) the compact coefficient table.
+inline size_t lds_bytes(int lmax, int SG, int nterms) {
     const int rmax = lmax_to_rmax(lmax);
     const int JB = pairs_per_batch(rmax);
-    const int sg = S < kSMax ? S : kSMax;
-    return sizeof(double2) * ((size_t)JB * leg_count(lmax) + (size_t)JB * (2 * lmax + 1) + (size_t)sg * 2 * JB * rmax);
+    const size_t b = sizeof(PairGeo) * JB +
+                     sizeof(double2) * ((size_t)JB * hl_stride(lmax) + (size_t)JB * (2 * lmax + 1) +
+                                        (size_t)SG * 2 * JB * rmax);
+    return b + (lmax <= kTabLdsLmax ? tab_lds_bytes(rmax, nterms) : 0);
 }
 
 // Chunks of column particles per block row: enough workgroups to fill the
@@ -57,22 +73,38 @@ inline int chunks_for(int N) {
     return c < 1 ? 1 : c;
 }
 
-// Thread (pair, mu): HL[l][mu] = h_l(k r) P_l^mu(cos theta) for l = mu..pmax
-// and e^{i mu phi}.  Spherical Hankel h_l = j_l + i y_l by upward recurrence
-// (cpp/src/special.cpp:90-112); P_l^mu without Condon-Shortley phase
-// (cpp/src/special.cpp:180-204).
-__device__ void pair_hl(double2* HL, double2* eph, int mu, int lmax, double dx, double dy, double dz, double k) {
-    const int pmax = 2 * lmax + 1;
+__device__ inline bool job_done(const SysDev& d) {
+    for (int s = 0; s < d.S; s++)
+        if (!d.scal[s].done) return false;
+    return true;
+}
+
+// Pair geometry for d = r_i - r_j: cos/sin theta, k|d| with its sine and
+// cosine, and e^{i phi} = (dx + i dy) / rho (1 on the z axis, where every
+// P_p^mu with mu > 0 vanishes).
+__device__ inline PairGeo pair_geo(double dx, double dy, double dz, double k) {
+    PairGeo g;
     const double rad = sqrt(dx * dx + dy * dy + dz * dz);
-    const double x = dz / rad;
-    const double s = sqrt(fmax(0.0, (1.0 - x) * (1.0 + x)));
+    g.x = dz / rad;
+    g.s = sqrt(fmax(0.0, (1.0 - g.x) * (1.0 + g.x)));
+    g.z = k * rad;
+    sincos(g.z, &g.sz, &g.cz);
+    const double rho = sqrt(dx * dx + dy * dy);
+    g.ex = rho > 0 ? dx / rho : 1.0;
+    g.ey = rho > 0 ? dy / rho : 0.0;
+    g.pad = 0;
+    return g;
+}
+
+// Thread (pair, mu): HL[leg_index(l, mu)] = h_l(z) P_l^mu(x), l = mu..pmax,
+// with the recurrences of pair_hl, and eph[mu] = (e^{i phi})^mu.
+__device__ inline void pair_hl_geo(double2* HL, double2* eph, int mu, int lmax, const PairGeo& g) {
+    const int pmax = 2 * lmax + 1;
     double pmm = 1.0;
-    for (int i = 1; i <= mu; i++) pmm *= (2 * i - 1) * s;
-    const double z = k * rad;
-    double sz, cz;
-    sincos(z, &sz, &cz);
-    double ja = sz / z, ya = -cz / z;        // h_{l-2} after the first two steps
-    double jb = ja / z + ya, yb = ya / z - ja;  // h_{l-1}
+    for (int i = 1; i <= mu; i++) pmm *= (2 * i - 1) * g.s;
+    const double rz = 1.0 / g.z;
+    double ja = g.sz * rz, ya = -g.cz * rz;
+    double jb = ja * rz + ya, yb = ya * rz - ja;
     double p2 = 0.0, p1 = 0.0;
     for (int l = 0; l <= pmax; l++) {
         double jl, yl;
@@ -83,8 +115,8 @@ __device__ void pair_hl(double2* HL, double2* eph, int mu, int lmax, double dx, 
             jl = jb;
             yl = yb;
         } else {
-            jl = (2 * l - 1) / z * jb - ja;
-            yl = (2 * l - 1) / z * yb - ya;
+            jl = (2 * l - 1) * rz * jb - ja;
+            yl = (2 * l - 1) * rz * yb - ya;
             ja = jb;
             jb = jl;
             ya = yb;
@@ -95,30 +127,26 @@ __device__ void pair_hl(double2* HL, double2* eph, int mu, int lmax, double dx, 
         if (l == mu)
             P = pmm;
         else if (l == mu + 1)
-            P = x * (2 * mu + 1) * pmm;
+            P = g.x * (2 * mu + 1) * pmm;
         else
-            P = ((2 * l - 1) * x * p1 - (l + mu - 1) * p2) / (l - mu);
+            P = ((2 * l - 1) * g.x * p1 - (l + mu - 1) * p2) / (l - mu);
         p2 = p1;
         p1 = P;
         HL[leg_index(l, mu)] = cscale(make_double2(jl, yl), P);
     }
     if (mu <= 2 * lmax) {
-        double sn, cs;
-        sincos(mu * atan2(dy, dx), &sn, &cs);
-        eph[mu] = make_double2(cs, sn);
+        double2 e = make_double2(1.0, 0.0);
+        const double2 e1 = make_double2(g.ex, g.ey);
+        for (int i = 0; i < mu; i++) e = cmul(e, e1);
+        eph[mu] = e;
     }
 }
 
-__device__ inline bool job_done(const SysDev& d) {
-    for (int s = 0; s < d.S; s++)
-        if (!d.scal[s].done) return false;
-    return true;
-}
-
-// Partial rows: part[chunk][rhs][row].
+// Partial rows: part[chunk][rhs][row].  SG right-hand sides per pass;
+// TAB_LDS: stage the table in LDS.
+template <int SG, bool TAB_LDS>
 __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __restrict__ sys, int nchunks, int chunk,
-                                                      const int4* __restrict__ tmeta,
-                                                      const double2* __restrict__ tcoef) {
+                                                      TabDev gtab, int nterms) {
     extern __shared__ double2 lds[];
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const int lmax = J.lmax;
@@ -126,38 +154,36 @@ __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __r
     const int bs = 2 * rmax;
     const int n = bs * J.N;
     const int pmax = 2 * lmax + 1;
-    const int Lc = leg_count(lmax);
+    const int Lp = hl_stride(lmax);
     const int nmu = 2 * lmax + 1;
-    const int qa_slots = lmax + 1;
-    const int coef_slots = 2 * lmax + 1;
     const int JB = pairs_per_batch(rmax);
     const int zs = JB * rmax;  // z / reduction stride per (rhs, polarization)
-    double2* HL = lds;
-    double2* eph = HL + JB * Lc;
+    PairGeo* geo = (PairGeo*)lds;
+    double2* HL = (double2*)(geo + JB);
+    double2* eph = HL + JB * Lp;
     double2* zr = eph + JB * nmu;
+    TabDev tab = gtab;
+    if (TAB_LDS) tab = stage_table(gtab, rmax, nterms, (char*)(zr + SG * 2 * zs));
 
     const int i = blockIdx.x / nchunks;
     const int c = blockIdx.x % nchunks;
     const int jbeg = c * chunk, jend = min(J.N, jbeg + chunk);
     const int t = threadIdx.x;
-    const int jme = t / rmax, r = t - jme * rmax;
-    const bool worker = jme < JB;
+    const int r = t / JB, jme = t - r * JB;  // lanes sharing r are adjacent
+    const bool worker = r < rmax;
     const double xi = J.pos[3 * i], yi = J.pos[3 * i + 1], zi = J.pos[3 * i + 2];
 
-    for (int s0 = 0; s0 < J.S; s0 += kSMax) {
-        const int sg = min(kSMax, J.S - s0);
-        double2 acc[kSMax][2];
+    for (int s0 = 0; s0 < J.S; s0 += SG) {
+        const int sg = min(SG, J.S - s0);
+        double2 acc[SG][2];
 #pragma unroll
-        for (int q = 0; q < kSMax; q++) acc[q][0] = acc[q][1] = make_double2(0, 0);
+        for (int q = 0; q < SG; q++) acc[q][0] = acc[q][1] = make_double2(0, 0);
         for (int j0 = jbeg; j0 < jend; j0 += JB) {
             const int jc = min(JB, jend - j0);
             __syncthreads();
-            for (int u = t; u < jc * (pmax + 1); u += kThreads) {
-                const int jj = u / (pmax + 1), mu = u - jj * (pmax + 1);
-                const int j = j0 + jj;
-                if (j != i)
-                    pair_hl(HL + jj * Lc, eph + jj * nmu, mu, lmax, xi - J.pos[3 * j], yi - J.pos[3 * j + 1],
-                            zi - J.pos[3 * j + 2], J.k);
+            if (t < jc) {
+                const int j = j0 + t;
+                geo[t] = pair_geo(xi - J.pos[3 * j], yi - J.pos[3 * j + 1], zi - J.pos[3 * j + 2], J.k);
             }
             for (int u = t; u < jc * sg * bs; u += kThreads) {
                 const int jj = u / (sg * bs), rest = u - jj * (sg * bs);
@@ -169,25 +195,20 @@ __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __r
                     cmul(J.mie[((size_t)j * 2 + b) * lmax + v - 1], J.x[(size_t)(s0 + q) * n + (size_t)j * bs + e]);
             }
             __syncthreads();
+            for (int u = t; u < jc * (pmax + 1); u += kThreads) {
+                const int jj = u / (pmax + 1), mu = u - jj * (pmax + 1);
+                if (j0 + jj != i) pair_hl_geo(HL + jj * Lp, eph + jj * nmu, mu, lmax, geo[jj]);
+            }
+            __syncthreads();
             if (worker && jme < jc && j0 + jme != i) {
-                const double2* hl = HL + jme * Lc;
+                const double2* hl = HL + jme * Lp;
                 const double2* ep = eph + jme * nmu;
                 const double2* zj = zr + jme * rmax;
                 for (int s = 0; s < rmax; s++) {
-                    const int tt = r * rmax + s;
-                    const int4 meta = tmeta[tt];
-                    const int amu = meta.z < 0 ? -meta.z : meta.z;
-                    const double2* cf = tcoef + (size_t)tt * coef_slots;
-                    double2 xa = make_double2(0, 0), xb = make_double2(0, 0);
-                    for (int q = 0; q < meta.x; q++) cfma(xa, cf[q], hl[leg_index(meta.w - 2 * q, amu)]);
-                    for (int q = 1; q <= meta.y; q++)
-                        cfma(xb, cf[qa_slots + q - 1], hl[leg_index(meta.w - 2 * q + 1, amu)]);
-                    double2 e = ep[amu];
-                    if (meta.z < 0) e.y = -e.y;
-                    xa = cmul(e, xa);
-                    xb = cmul(e, xb);
+                    double2 xa, xb;
+                    tuple_xab(tab, r * rmax + s, hl, ep, xa, xb);
 #pragma unroll
-                    for (int q = 0; q < kSMax; q++) {
+                    for (int q = 0; q < SG; q++) {
                         if (q < sg) {
                             const double2 z0 = zj[(q * 2) * zs + s], z1 = zj[(q * 2 + 1) * zs + s];
                             cfma(acc[q][0], xa, z0);
@@ -203,7 +224,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __r
         __syncthreads();
         if (worker) {
 #pragma unroll
-            for (int q = 0; q < kSMax; q++)
+            for (int q = 0; q < SG; q++)
                 if (q < sg) {
                     zr[(q * 2) * zs + jme * rmax + r] = acc[q][0];
                     zr[(q * 2 + 1) * zs + jme * rmax + r] = acc[q][1];
@@ -245,9 +266,18 @@ int matfree_launch(const MfJob& J, const SysDev* sys, hipStream_t st) {
     const long long nblocks = (long long)J.N * nchunks;
     const size_t total = (size_t)J.S * 2 * lmax_to_rmax(J.lmax) * J.N;
     const unsigned sum_blocks = (unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads);
+    const TabDev tab{T->cmeta, T->ccoef, T->clidx};
+    const int SG = sg_for(J.S);
+    const size_t lds = lds_bytes(J.lmax, SG, T->nterms);
+    const bool tl = J.lmax <= kTabLdsLmax;
+    const dim3 grid((unsigned)nblocks), block(kThreads);
     ProfScope prof(4, st);
-    hipLaunchKernelGGL(k_matfree, dim3((unsigned)nblocks), dim3(kThreads), lds_bytes(J.lmax, J.S), st, J, sys, nchunks,
-                       chunk, T->meta, T->coef);
+    if (SG == 1 && tl) hipLaunchKernelGGL((k_matfree<1, true>), grid, block, lds, st, J, sys, nchunks, chunk, tab, T->nterms);
+    else if (SG == 1) hipLaunchKernelGGL((k_matfree<1, false>), grid, block, lds, st, J, sys, nchunks, chunk, tab, T->nterms);
+    else if (SG == 2 && tl) hipLaunchKernelGGL((k_matfree<2, true>), grid, block, lds, st, J, sys, nchunks, chunk, tab, T->nterms);
+    else if (SG == 2) hipLaunchKernelGGL((k_matfree<2, false>), grid, block, lds, st, J, sys, nchunks, chunk, tab, T->nterms);
+    else if (tl) hipLaunchKernelGGL((k_matfree<kSMax, true>), grid, block, lds, st, J, sys, nchunks, chunk, tab, T->nterms);
+    else hipLaunchKernelGGL((k_matfree<kSMax, false>), grid, block, lds, st, J, sys, nchunks, chunk, tab, T->nterms);
     hipLaunchKernelGGL(k_matfree_sum, dim3(sum_blocks), dim3(kThreads), 0, st, J, sys, nchunks);
     MPA_LAUNCH_CHECK();
     return MPA_OK;

@@ -51,6 +51,11 @@ ProfScope::~ProfScope() {
     g_recs.push_back({kind_, (hipEvent_t)start_, stop});
 }
 
+bool mpa_profile_active() {
+    std::lock_guard<std::mutex> lock(g_mu);
+    return g_enabled;
+}
+
 extern "C" {
 
 int mpa_profile_enable(int on) {

@@ -138,6 +138,23 @@ TransTable* build_table(int lmax) {
                     }
                     meta[t] = make_int4(qA + 1, qB, mu, n + v);
                 }
+    // compact form: only the nonzero-range terms, with their HL indices
+    std::vector<int2> cmeta(rmax * rmax);
+    std::vector<double2> ccoef;
+    std::vector<unsigned short> clidx;
+    for (int t = 0; t < rmax * rmax; t++) {
+        const int4 mt = meta[t];
+        const int amu = std::abs(mt.z);
+        cmeta[t] = make_int2((int)ccoef.size(), mt.x | (mt.y << 8) | ((mt.z + 64) << 16));
+        for (int q = 0; q < mt.x; q++) {
+            ccoef.push_back(coef[(size_t)t * coef_slots + q]);
+            clidx.push_back((unsigned short)leg_index(mt.w - 2 * q, amu));
+        }
+        for (int q = 1; q <= mt.y; q++) {
+            ccoef.push_back(coef[(size_t)t * coef_slots + qa_slots + q - 1]);
+            clidx.push_back((unsigned short)leg_index(mt.w - 2 * q + 1, amu));
+        }
+    }
     TransTable* T = new TransTable();
     T->lmax = lmax;
     T->rmax = rmax;
@@ -149,8 +166,20 @@ TransTable* build_table(int lmax) {
         delete T;
         return nullptr;
     }
+    T->nterms = (int)ccoef.size();
+    if (hipMalloc(&T->cmeta, cmeta.size() * sizeof(int2)) != hipSuccess ||
+        hipMalloc(&T->ccoef, ccoef.size() * sizeof(double2)) != hipSuccess ||
+        hipMalloc(&T->clidx, clidx.size() * sizeof(unsigned short)) != hipSuccess) {
+        mpa_set_error("hipMalloc for the lmax=%d translation table failed", lmax);
+        delete T;
+        return nullptr;
+    }
     if (hipMemcpy(T->meta, meta.data(), meta.size() * sizeof(int4), hipMemcpyHostToDevice) != hipSuccess ||
-        hipMemcpy(T->coef, coef.data(), coef.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+        hipMemcpy(T->coef, coef.data(), coef.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(T->cmeta, cmeta.data(), cmeta.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(T->ccoef, ccoef.data(), ccoef.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(T->clidx, clidx.data(), clidx.size() * sizeof(unsigned short), hipMemcpyHostToDevice) !=
+            hipSuccess) {
         mpa_set_error("upload of the lmax=%d translation table failed", lmax);
         delete T;
         return nullptr;

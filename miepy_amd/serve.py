@@ -6,7 +6,8 @@ with one right-hand side per source, and the response carries, per
 wavelength and source, the cross-sections and the per-particle forces.
 
 The engine admits systems first-come first-served while their HBM footprint
-(matrix + Krylov state + inputs) fits the budget, assembles them and joins
+(matrix + Krylov state + solution) fits in a free block of the engine's
+arena (one device allocation of the HBM budget), assembles them and joins
 them to the running batch; every engine step advances all active systems by
 a few BiCGSTAB iterations in one batched launch sequence
 (`mpa_systems_advance`), retires the systems that converged (observables,
@@ -35,7 +36,57 @@ from .mie import mie_coefficients
 
 class _System:
     __slots__ = ("req", "index", "w", "n", "S", "k", "eps_b", "pos", "mie_scat", "mie_int", "p_src", "A", "x",
-                 "state", "struct", "done", "iters", "footprint")
+                 "state", "struct", "done", "iters", "footprint", "block")
+
+
+class Arena:
+    """First-fit allocator over one device buffer: the engine's HBM budget.
+
+    Matrices, Krylov state and solutions of admitted systems are carved out
+    of a single allocation, so admission succeeds exactly when a contiguous
+    block is free (no fragmentation inside the framework's caching
+    allocator), and freed blocks coalesce with their neighbours."""
+
+    ALIGN = 256
+
+    def __init__(self, nbytes, device="cuda"):
+        self.size = int(nbytes) // self.ALIGN * self.ALIGN
+        self.buf = torch.empty(self.size, dtype=torch.uint8, device=device)
+        self.free_list = [(0, self.size)]  # sorted (offset, size)
+        self.live = {}
+
+    def alloc(self, nbytes):
+        need = (int(nbytes) + self.ALIGN - 1) // self.ALIGN * self.ALIGN
+        for k, (off, size) in enumerate(self.free_list):
+            if size >= need:
+                if size == need:
+                    del self.free_list[k]
+                else:
+                    self.free_list[k] = (off + need, size - need)
+                self.live[off] = need
+                return off
+        return None
+
+    def free(self, off):
+        size = self.live.pop(off)
+        fl = self.free_list
+        k = 0
+        while k < len(fl) and fl[k][0] < off:
+            k += 1
+        fl.insert(k, (off, size))
+        if k + 1 < len(fl) and fl[k][0] + fl[k][1] == fl[k + 1][0]:  # merge with the next block
+            fl[k] = (fl[k][0], fl[k][1] + fl[k + 1][1])
+            del fl[k + 1]
+        if k > 0 and fl[k - 1][0] + fl[k - 1][1] == fl[k][0]:  # and with the previous one
+            fl[k - 1] = (fl[k - 1][0], fl[k - 1][1] + fl[k][1])
+            del fl[k]
+
+    def view(self, off, nbytes, dtype, shape):
+        return self.buf[off:off + nbytes].view(dtype).reshape(shape)
+
+    @property
+    def used(self):
+        return sum(self.live.values())
 
 
 class Request:
@@ -54,18 +105,22 @@ class Request:
         return None if self.finished is None else self.finished - self.arrival
 
 
+def _aligned(nbytes):
+    return (int(nbytes) + Arena.ALIGN - 1) // Arena.ALIGN * Arena.ALIGN
+
+
 def system_footprint(w):
-    """Device bytes of one system: c128 matrix, Krylov state, x / b / inputs."""
+    """Arena bytes of one system: c128 matrix, Krylov state and solution."""
     n, S = w.n, len(w.sources)
     state = L.load().mpa_system_state_bytes(n, S, L.MPA_C128)
-    return 16 * n * n + state + 2 * 16 * S * n + 64 * w.N * (w.lmax + 2)
+    return _aligned(16 * n * n) + _aligned(state) + _aligned(16 * S * n)
 
 
 class ServingEngine:
     def __init__(self, hbm_budget_bytes=200e9, iterations_per_step=2, maxiter=2000, clock=time.perf_counter):
         D.require_gpu()
-        self.budget = float(hbm_budget_bytes)
-        self.used = 0.0
+        self.arena = Arena(hbm_budget_bytes)
+        self.budget = float(self.arena.size)
         self.k = int(iterations_per_step)
         self.maxiter = int(maxiter)
         self.clock = clock
@@ -107,20 +162,29 @@ class ServingEngine:
         s.mie_scat = D.to_device(ms, D.C128)
         s.mie_int = D.to_device(mi, D.C128)
         s.p_src = D.to_device(p_src, D.C128)
-        s.A = D.assemble(s.pos, s.mie_scat, s.k, dtype=D.C128, add_identity=True)
-        s.x = torch.empty((s.S, s.n), dtype=D.C128, device="cuda")
-        s.state = torch.empty(self.lib.mpa_system_state_bytes(s.n, s.S, L.MPA_C128), dtype=torch.uint8,
-                              device="cuda")
+        off = s.block
+        nA, nst = _aligned(16 * s.n * s.n), _aligned(self.lib.mpa_system_state_bytes(s.n, s.S, L.MPA_C128))
+        A = self.arena.view(off, 16 * s.n * s.n, D.C128, (s.n, s.n))
+        s.A = D.assemble(s.pos, s.mie_scat, s.k, dtype=D.C128, add_identity=True, out=A)
+        s.state = self.arena.view(off + nA, nst, torch.uint8, (nst,))
+        s.x = self.arena.view(off + nA + nst, 16 * s.S * s.n, D.C128, (s.S, s.n))
         s.struct = D._sys_struct(s.A, s.p_src.reshape(s.S, s.n), s.x, s.state, w.tol, self.maxiter, False)
         s.done = False
         s.iters = 0
 
+    @property
+    def used(self):
+        return self.arena.used
+
     def _admit(self):
         new = []
-        while self.queue and self.used + self.queue[0].footprint <= self.budget:
+        while self.queue:
+            off = self.arena.alloc(self.queue[0].footprint)
+            if off is None:
+                break  # FIFO: the head waits for memory, later systems wait behind it
             s = self.queue.popleft()
+            s.block = off
             self._prepare(s)
-            self.used += s.footprint
             if s.req.admitted is None:
                 s.req.admitted = self.clock()
                 s.req.status = "running"
@@ -179,8 +243,8 @@ class ServingEngine:
             "iterations": iters, "residual": resid, "status": status,
         }
         self.solves_done += 1
-        self.used -= s.footprint
         s.A = s.x = s.state = s.pos = s.mie_scat = s.mie_int = s.p_src = None
+        self.arena.free(s.block)
         s.req.pending -= 1
         if s.req.pending == 0:
             s.req.finished = self.clock()

@@ -91,8 +91,8 @@ def test_batched_mixed_status(dev):
 
 def test_cluster_batch_matches_single_clusters(dev):
     """ClusterBatch (all clusters in one batched solve) vs one sphere_cluster
-    per cluster: cross-sections and forces per source within 1e-9 (FP64) and
-    1e-8 (complex64 + FP64 refinement)."""
+    per cluster: cross-sections and forces per source within 1e-9 (FP64 and
+    matrix-free FP64) and 1e-8 (complex64 + FP64 refinement)."""
     import miepy_amd as M
     from miepy_amd.batch import ClusterBatch
     from miepy_amd.workloads import batch256
@@ -102,7 +102,7 @@ def test_cluster_batch_matches_single_clusters(dev):
         w.N = w.position.shape[0]
         w.radius, w.eps = w.radius[: w.N], w.eps[: w.N]
         w.tol = 1e-11
-    for precision, rtol in (("fp64", 1e-9), ("mixed", 1e-8)):
+    for precision, rtol in (("fp64", 1e-9), ("mixed", 1e-8), ("matfree", 1e-9)):
         B = ClusterBatch(ws, precision=precision)
         B.solve()
         assert B.converged()
@@ -114,3 +114,44 @@ def test_cluster_batch_matches_single_clusters(dev):
             np.testing.assert_allclose(r["C_total"], C, rtol=rtol, atol=rtol * np.abs(C).max())
             F = one.force()
             assert np.max(np.abs(r["F"] - F)) <= rtol * 10 * np.abs(F).max()
+
+
+def test_graph_replay_equals_eager(dev):
+    """Iterations replayed from captured HIP graphs (the default) give the
+    eager launches' results bit for bit, for single and batched solves,
+    stored and matrix-free operators, c128 and c64, one or several
+    iterations per graph; each solve replays its graph many times with host
+    synchronization between replays."""
+    import miepy_amd._lib as L
+    lib = L.load()
+    systems = _systems(dev, seed=3, count=4)
+    As = [dev.assemble(pos, ms, k, add_identity=True) for pos, ms, k, _ in systems]
+    bs = [dev.to_device(b, dev.C128) for *_, b in systems]
+    ops = [dev.ClusterOperator(pos, ms, k, max_rhs=b.shape[0]) for pos, ms, k, b in systems]
+    assert lib.mpa_get_option(L.MPA_OPT_GRAPHS) == 1
+
+    def run():
+        out = []
+        for sync in (1, 3):
+            xs, reps = dev.bicgstab_batched(As, bs, tol=1e-10, maxiter=2000, sync_every=sync)
+            out += [(x.cpu().numpy(), r.iterations) for x, r in zip(xs, reps)]
+            xs, reps = dev.bicgstab_batched(ops, bs, tol=1e-10, maxiter=2000, add_identity=True,
+                                            sync_every=sync)
+            out += [(x.cpu().numpy(), r.iterations) for x, r in zip(xs, reps)]
+        A32 = [A.to(dev.C64) for A in As]
+        xs, reps = dev.bicgstab_batched(A32, [b.to(dev.C64) for b in bs], tol=1e-5, maxiter=2000)
+        out += [(x.cpu().numpy(), r.iterations) for x, r in zip(xs, reps)]
+        x, rep = dev.bicgstab(As[0], bs[0], tol=1e-10)
+        out.append((x.cpu().numpy(), rep.iterations))
+        return out
+
+    graphed = run()
+    try:
+        assert lib.mpa_set_option(L.MPA_OPT_GRAPHS, 0) == 0
+        eager = run()
+    finally:
+        lib.mpa_set_option(L.MPA_OPT_GRAPHS, 1)
+    assert len(graphed) == len(eager)
+    for (xg, ig), (xe, ie) in zip(graphed, eager):
+        assert ig == ie
+        assert np.array_equal(xg, xe)

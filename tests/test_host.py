@@ -110,3 +110,29 @@ def test_materials():
     assert d.eps(500e-9) == pytest.approx(2 + 1.5j)
     ms = M.material_struct([M.constant_material(4.0)], M.materials.water(), wavelength=500e-9)
     assert ms.k_b == pytest.approx(2 * np.pi * 1.33 / 500e-9)
+
+
+def test_serving_arena_first_fit_and_coalescing():
+    """The serving engine's HBM arena (run here on a host buffer): first-fit,
+    256-byte alignment, typed views, and freed neighbours merge back."""
+    import torch
+    from miepy_amd.serve import Arena
+    a = Arena(4096 + 100, device="cpu")
+    assert a.size == 4096
+    o1 = a.alloc(1000)
+    o2 = a.alloc(1000)
+    o3 = a.alloc(1000)
+    assert (o1, o2, o3) == (0, 1024, 2048) and a.used == 3072
+    assert a.alloc(2000) is None  # 1024 bytes left
+    a.free(o2)
+    assert a.alloc(1100) is None  # the hole is 1024: first fit does not split across blocks
+    o4 = a.alloc(512)
+    assert o4 == 1024  # first fit reuses the hole
+    a.free(o1)
+    a.free(o4)
+    assert a.free_list[0] == (0, 2048)  # o1 + the hole merged
+    a.free(o3)
+    assert a.free_list == [(0, 4096)] and a.used == 0
+    v = a.view(a.alloc(256), 256, torch.complex128, (4, 4))
+    v[:] = 1 + 2j
+    assert v.dtype == torch.complex128 and complex(v.sum()) == 16 + 32j
