@@ -71,10 +71,37 @@ int mpa_sphere_aggregate_tmatrix(int N, int lmax, const double* positions, const
 int mpa_sphere_aggregate_tmatrix_dual(int N, int lmax, const double* positions, const double* mie, double k,
                                       int add_identity, void* out64, void* out32, void* stream);
 
+/* Aggregate T-matrix of a cluster of particles with general T-matrices.
+ * Replaces interactions.particle_aggregate_tmatrix(positions, tmatrix, k)
+ * (cpp/src/interactions_pyb.cpp:12-24 -> cpp/src/interactions.cpp:417-462,
+ * cpp/src/vsh_translation.cpp:53-159): block (i, j) = X^{ij} Tm_j.
+ *   tmatrix  [N, 2 rmax, 2 rmax] c128 (device), row-major per particle
+ * Other arguments as mpa_sphere_aggregate_tmatrix. */
+int mpa_particle_aggregate_tmatrix(int N, int lmax, const double* positions, const double* tmatrix, double k,
+                                   int out_dtype, int add_identity, void* out, void* stream);
+
+/* y_j = Tm_j x_j for every particle j and right-hand side: x, y [nrhs, n]
+ * c128 (device, distinct), tmatrix as above.  The einsum
+ * "naibj,nbj->nai" of src/miepy/cluster.py:650,657 (p_scat = T p_inc). */
+int mpa_tmatrix_apply(int N, int lmax, int nrhs, const double* tmatrix, const void* x, void* y, void* stream);
+
 /* y = (add_identity ? x : 0) + A x for nrhs vectors ([nrhs, n]).
  * Replaces the OpenMP row-block GEMV inside Eigen's BiCGSTAB
  * (cpp/src/interactions.cpp:62-95). */
 int mpa_matvec(int n, int nrhs, int dtype, const void* A, int add_identity, const void* x, void* y, void* stream);
+
+/* Autotuner for the matvec tiles (no counterpart in the reference).  Times
+ * every tile variant of the RHS-count class of `nrhs` (vector-ALU row tiles
+ * of several shapes; the matrix-core GEMM for 3+ RHS) on the caller's
+ * operands, `reps` launches each (y is overwritten), and makes the fastest
+ * the variant used for (dtype, class) by every later matvec and solve in
+ * the process.  Outputs (host, may be NULL): the chosen variant and its
+ * average launch time in ms. */
+int mpa_autotune_matvec(int n, int nrhs, int dtype, const void* A, const void* x, void* y, int reps,
+                        int* variant_out, double* ms_out, void* stream);
+/* Variant in use for (dtype, class of nrhs); set_to >= 0 selects one. */
+int mpa_matvec_variant(int dtype, int nrhs, int set_to);
+const char* mpa_matvec_variant_name(int dtype, int nrhs, int variant);
 
 /* Bytes of device workspace mpa_bicgstab needs. */
 size_t mpa_bicgstab_workspace_bytes(int n, int nrhs, int dtype);
@@ -107,9 +134,11 @@ int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, co
 typedef struct {
     int N, lmax;
     const double* positions; /* [N, 3] */
-    const double* mie;       /* [N, 2, lmax] c128 (mie_scat) */
+    const double* mie;       /* [N, 2, lmax] c128 (mie_scat) for spheres, or NULL */
     double k;                /* wavenumber in the medium */
     void* workspace;         /* mpa_matfree_workspace_bytes(N, lmax, nrhs) bytes for the largest nrhs used */
+    const double* tmatrix;   /* [N, 2 rmax, 2 rmax] c128 per-particle T-matrices, or NULL; exactly one of
+                                mie / tmatrix is set (the operator of particle_aggregate_tmatrix) */
 } mpa_cluster_operator;
 
 size_t mpa_matfree_workspace_bytes(int N, int lmax, int nrhs);
@@ -257,8 +286,19 @@ int mpa_stream_copy(const void* src, void* dst, size_t bytes, void* stream);
  * meta_out [rmax*rmax*4] int, coef_out [rmax*rmax*(2 lmax+1)] c128. */
 int mpa_translation_table(int lmax, int* meta_out, double* coef_out);
 
-/* Column blocks per assembly workgroup tile chosen for lmax. */
+/* Column blocks per assembly workgroup tile in use for lmax. */
 int mpa_assembly_tile(int lmax);
+
+/* Autotuner for the assembly tiles (no counterpart in the reference): times
+ * every tile variant for lmax (column blocks per workgroup; coefficient
+ * table in LDS or read through the caches) on the caller's cluster, `reps`
+ * launches each (`out`, [n, n] of out_dtype, is overwritten), and keeps the
+ * fastest for the process.  Outputs as mpa_autotune_matvec. */
+int mpa_autotune_assembly(int N, int lmax, const double* positions, const double* mie, double k, int out_dtype,
+                          void* out, int reps, int* variant_out, double* ms_out, void* stream);
+/* Assembly variant in use for lmax; set_to >= 0 selects one; *count_out
+ * (may be NULL) receives the number of variants; -1 on a bad argument. */
+int mpa_assembly_variant(int lmax, int set_to, int* count_out);
 
 #ifdef __cplusplus
 }

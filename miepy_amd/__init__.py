@@ -16,6 +16,7 @@ from .materials import (constant_material, data_material, dielectric, function_m
                         materials)
 from .sources import plane_wave, combined_source
 from .sphere_cluster import sphere_cluster
+from .cluster import cluster, sphere, tmatrix_particle
 
 backends.register_backend("mi355x", _backend)
 
@@ -37,6 +38,6 @@ def mode_indices(lmax):
             r += 1
 
 
-__all__ = ["sphere_cluster", "plane_wave", "combined_source", "constant_material", "data_material", "dielectric",
+__all__ = ["sphere_cluster", "cluster", "sphere", "tmatrix_particle", "plane_wave", "combined_source", "constant_material", "data_material", "dielectric",
            "function_material", "material_struct", "materials", "cross_sections", "solver", "DeviceTMatrix",
            "backends", "lmax_to_rmax", "rmax_to_lmax", "mode_indices"]

@@ -61,6 +61,11 @@ SIGNATURES = {
     "mpa_force_torque": (_i, [_i, _i, _v, _v, _d, _d, _v, _v, _v]),
     "mpa_particle_cross_sections": (_i, [_i, _i, _v, _v, _v, _d, _v, _v]),
     "mpa_stream_copy": (_i, [_v, _v, _sz, _v]),
+    "mpa_autotune_matvec": (_i, [_i, _i, _i, _v, _v, _v, _i, _pi, _pd, _v]),
+    "mpa_autotune_assembly": (_i, [_i, _i, _v, _v, _d, _i, _v, _i, _pi, _pd, _v]),
+    "mpa_assembly_variant": (_i, [_i, _i, _pi]),
+    "mpa_matvec_variant": (_i, [_i, _i, _i]),
+    "mpa_matvec_variant_name": (ctypes.c_char_p, [_i, _i, _i]),
     "mpa_set_option": (_i, [_i, _i]),
     "mpa_get_option": (_i, [_i]),
     "mpa_profile_enable": (_i, [_i]),
@@ -80,7 +85,8 @@ class MpaSystem(ctypes.Structure):
 class MpaClusterOperator(ctypes.Structure):
     """mpa_cluster_operator (include/miepy_amd.h)"""
     _fields_ = [("N", ctypes.c_int), ("lmax", ctypes.c_int), ("positions", ctypes.c_void_p),
-                ("mie", ctypes.c_void_p), ("k", ctypes.c_double), ("workspace", ctypes.c_void_p)]
+                ("mie", ctypes.c_void_p), ("k", ctypes.c_double), ("workspace", ctypes.c_void_p),
+                ("tmatrix", ctypes.c_void_p)]
 
 
 class MpaRefineSystem(ctypes.Structure):
@@ -95,6 +101,8 @@ _ps = ctypes.POINTER(MpaSystem)
 _pop = ctypes.POINTER(MpaClusterOperator)
 _prs = ctypes.POINTER(MpaRefineSystem)
 SIGNATURES.update({
+    "mpa_particle_aggregate_tmatrix": (_i, [_i, _i, _v, _v, _d, _i, _i, _v, _v]),
+    "mpa_tmatrix_apply": (_i, [_i, _i, _i, _v, _v, _v, _v]),
     "mpa_matfree_workspace_bytes": (_sz, [_i, _i, _i]),
     "mpa_matfree_matvec": (_i, [_pop, _i, _i, _v, _v, _v]),
     "mpa_bicgstab_matfree": (_i, [_pop, _i, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),

@@ -71,6 +71,17 @@ def sphere_aggregate_tmatrix(positions, mie, k):
     return DeviceTMatrix(T, (N, 2, rmax, N, 2, rmax))
 
 
+def particle_aggregate_tmatrix(positions, tmatrix, k):
+    """positions[N,3], tmatrix[N,2,rmax,2,rmax], k -> T[N,2,rmax,N,2,rmax]
+    (device-resident).  src/miepy/interactions.py:190-205."""
+    positions = np.asarray(positions, dtype=float).reshape(-1, 3)
+    N = positions.shape[0]
+    rmax = np.shape(tmatrix)[-1]
+    T = D.assemble_tmatrix(positions, np.asarray(tmatrix, dtype=complex).reshape(N, 2, rmax, 2, rmax),
+                           float(np.real(k)))
+    return DeviceTMatrix(T, (N, 2, rmax, N, 2, rmax))
+
+
 def _device_matrix(tmatrix, n):
     if isinstance(tmatrix, DeviceTMatrix):
         return tmatrix.data.reshape(n, n)

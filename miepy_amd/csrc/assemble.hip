@@ -23,6 +23,9 @@
 // the reference shares through its symmetry is recomputed (the kernel is
 // bound by HBM write bandwidth, not by these FP64 flops).
 
+#include <algorithm>
+#include <vector>
+
 #include "translation.hpp"
 
 using namespace mpa;
@@ -92,7 +95,8 @@ __global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int
             pair_hl(HL + jj * Lc, eph + jj * nmu, mu, lmax, pos[3 * i] - pos[3 * j], pos[3 * i + 1] - pos[3 * j + 1],
                     pos[3 * i + 2] - pos[3 * j + 2], k);
     }
-    for (int t = threadIdx.x; t < jcount * 2 * lmax; t += kThreads) mieJ[t] = mie[(size_t)j0 * 2 * lmax + t];
+    for (int t = threadIdx.x; t < jcount * 2 * lmax; t += kThreads)
+        mieJ[t] = mie ? mie[(size_t)j0 * 2 * lmax + t] : make_double2(1.0, 0.0);  // null: the bare X blocks
     __syncthreads();
 
     // phase 2: lane (pair jj, column mode s) walks the row modes r
@@ -148,12 +152,36 @@ int pick_tile(int lmax) {
     return best;
 }
 
+// Tile variants per lmax (chosen by mpa_autotune_assembly): column blocks
+// per tile J and whether the coefficient table is staged in LDS.
+struct AsmVariant {
+    int J;
+    bool tab_lds;
+};
+constexpr size_t kLdsLimit = 65536;
+int g_asm_variant[MPA_LMAX_MAX + 1] = {};
+
+std::vector<AsmVariant> asm_variants(int lmax, int nterms) {
+    const int J0 = pick_tile(lmax);
+    std::vector<AsmVariant> v;
+    for (int J : {J0, (J0 + 1) / 2})
+        for (bool tl : {lmax <= kTabLdsLmax, !(lmax <= kTabLdsLmax)}) {
+            if (lds_bytes(lmax, J, nterms, tl) > kLdsLimit) continue;
+            bool dup = false;
+            for (const AsmVariant& w : v) dup |= (w.J == J && w.tab_lds == tl);
+            if (!dup) v.push_back({J, tl});
+        }
+    return v;
+}
+
 template <typename OutT, bool DUAL>
 int launch_assembly(int N, int lmax, const double* positions, const double* mie, double k, int add_identity,
                     OutT* out, float2* out32, hipStream_t s, const char* who) {
     const TransTable* T = mpa_get_table(lmax);
     if (!T) return MPA_ERR_ARG;
-    const int J = pick_tile(lmax);
+    const std::vector<AsmVariant> vars = asm_variants(lmax, T->nterms);
+    const AsmVariant var = vars[std::min<size_t>(g_asm_variant[lmax], vars.size() - 1)];
+    const int J = var.J;
     const int ntiles = (N + J - 1) / J;
     const long long nblocks = (long long)N * ntiles;
     if (nblocks > 0x7fffffffLL) {
@@ -161,7 +189,7 @@ int launch_assembly(int N, int lmax, const double* positions, const double* mie,
         return MPA_ERR_ARG;
     }
     const TabDev tab{T->cmeta, T->ccoef, T->clidx};
-    const bool tab_lds = lmax <= kTabLdsLmax;
+    const bool tab_lds = var.tab_lds;
     const size_t lds = lds_bytes(lmax, J, T->nterms, tab_lds);
     if (tab_lds)
         hipLaunchKernelGGL((assemble_kernel<OutT, DUAL, true>), dim3((unsigned)nblocks), dim3(kThreads), lds, s, N,
@@ -175,7 +203,107 @@ int launch_assembly(int N, int lmax, const double* positions, const double* mie,
     return MPA_OK;
 }
 
+// ---- particles with general T-matrices --------------------------------------
+//
+// T_ij = X^{ij} Tm_j (cpp/src/vsh_translation.cpp:53-159: the pair's bare
+// translation blocks times the column particle's T-matrix; the reference
+// forms the product entry by entry for every pair inside an n x n scratch
+// matrix).  Here the bare X blocks are assembled in place (unit Mie
+// coefficients) and every column block is then multiplied on the right by
+// its particle's T-matrix: one workgroup per (column block j, 32 rows)
+// stages the old 32 x bs values and (bs <= 48) Tm_j in LDS, forms the
+// products in FP64 and writes the block back; the identity is added on the
+// diagonal blocks (which are zero before the product).
+
+constexpr int kApplyRows = 32;
+constexpr int kApplyTmLdsMax = 48;  // bs up to which Tm_j is staged in LDS (lmax <= 4)
+
+template <typename OutT>
+__device__ inline double2 load_c(const OutT* p);
+template <>
+__device__ inline double2 load_c<double2>(const double2* p) {
+    return *p;
+}
+template <>
+__device__ inline double2 load_c<float2>(const float2* p) {
+    const float2 v = *p;
+    return make_double2(v.x, v.y);
+}
+
+template <typename OutT>
+__global__ __launch_bounds__(kThreads) void k_apply_tmatrix(int N, int bs, int nrowblocks,
+                                                             const double2* __restrict__ tm, int add_identity,
+                                                             OutT* __restrict__ out) {
+    extern __shared__ double2 lds[];
+    const int j = blockIdx.x / nrowblocks;
+    const int r0 = (blockIdx.x % nrowblocks) * kApplyRows;
+    const size_t n = (size_t)bs * N;
+    const int nr = (int)min((size_t)kApplyRows, n - r0);
+    double2* old = lds;                         // [kApplyRows][bs]
+    double2* tml = old + kApplyRows * bs;       // [bs][bs] when staged
+    const double2* tmj = tm + (size_t)j * bs * bs;
+    const bool staged = bs <= kApplyTmLdsMax;
+    for (int u = threadIdx.x; u < nr * bs; u += kThreads) {
+        const int rr = u / bs, c = u - rr * bs;
+        old[u] = load_c<OutT>(out + (size_t)(r0 + rr) * n + (size_t)j * bs + c);
+    }
+    if (staged)
+        for (int u = threadIdx.x; u < bs * bs; u += kThreads) tml[u] = tmj[u];
+    __syncthreads();
+    const double2* T = staged ? tml : tmj;
+    for (int u = threadIdx.x; u < nr * bs; u += kThreads) {
+        const int rr = u / bs, c = u - rr * bs;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int q = 0; q < bs; q++) cfma(acc, old[rr * bs + q], T[q * bs + c]);
+        const size_t row = (size_t)r0 + rr;
+        if (add_identity && row == (size_t)j * bs + c) acc.x += 1.0;
+        store<OutT>(out + row * n + (size_t)j * bs + c, acc);
+    }
+}
+
+template <typename OutT>
+int launch_apply_tmatrix(int N, int lmax, const double* tmatrix, int add_identity, OutT* out, hipStream_t s) {
+    const int bs = 2 * lmax_to_rmax(lmax);
+    const long long n = (long long)bs * N;
+    const int nrb = (int)((n + kApplyRows - 1) / kApplyRows);
+    const long long nblocks = (long long)N * nrb;
+    if (nblocks > 0x7fffffffLL) {
+        mpa_set_error("particle_aggregate_tmatrix: N=%d too large for one launch", N);
+        return MPA_ERR_ARG;
+    }
+    const size_t lds = sizeof(double2) * ((size_t)kApplyRows * bs + (bs <= kApplyTmLdsMax ? (size_t)bs * bs : 0));
+    hipLaunchKernelGGL(k_apply_tmatrix<OutT>, dim3((unsigned)nblocks), dim3(kThreads), lds, s, N, bs, nrb,
+                       (const double2*)tmatrix, add_identity, out);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
 }  // namespace
+
+extern "C" int mpa_particle_aggregate_tmatrix(int N, int lmax, const double* positions, const double* tmatrix,
+                                              double k, int out_dtype, int add_identity, void* out, void* stream) {
+    if (N < 1 || !positions || !tmatrix || !out || !(k > 0)) {
+        mpa_set_error("particle_aggregate_tmatrix: bad argument (N=%d, k=%g)", N, k);
+        return MPA_ERR_ARG;
+    }
+    if (out_dtype != MPA_C128 && out_dtype != MPA_C64) {
+        mpa_set_error("particle_aggregate_tmatrix: unknown dtype %d", out_dtype);
+        return MPA_ERR_ARG;
+    }
+    hipStream_t s = (hipStream_t)stream;
+    ProfScope prof(out_dtype == MPA_C128 ? 2 : 3, s);
+    int rc;
+    if (out_dtype == MPA_C128) {
+        rc = launch_assembly<double2, false>(N, lmax, positions, nullptr, k, 0, (double2*)out, nullptr, s,
+                                             "particle_aggregate_tmatrix");
+        if (rc == MPA_OK) rc = launch_apply_tmatrix<double2>(N, lmax, tmatrix, add_identity, (double2*)out, s);
+    } else {
+        rc = launch_assembly<float2, false>(N, lmax, positions, nullptr, k, 0, (float2*)out, nullptr, s,
+                                            "particle_aggregate_tmatrix");
+        if (rc == MPA_OK) rc = launch_apply_tmatrix<float2>(N, lmax, tmatrix, add_identity, (float2*)out, s);
+    }
+    return rc;
+}
 
 extern "C" int mpa_sphere_aggregate_tmatrix(int N, int lmax, const double* positions, const double* mie, double k,
                                             int out_dtype, int add_identity, void* out, void* stream) {
@@ -196,7 +324,68 @@ extern "C" int mpa_sphere_aggregate_tmatrix(int N, int lmax, const double* posit
                                           "sphere_aggregate_tmatrix");
 }
 
-extern "C" int mpa_assembly_tile(int lmax) { return pick_tile(lmax); }
+extern "C" int mpa_assembly_tile(int lmax) {
+    const TransTable* T = mpa_get_table(lmax);
+    if (!T) return -1;
+    const std::vector<AsmVariant> vars = asm_variants(lmax, T->nterms);
+    return vars[std::min<size_t>(g_asm_variant[lmax], vars.size() - 1)].J;
+}
+
+// Variant in use for lmax (set_to >= 0 selects one); -1 for a bad lmax or
+// variant.  *count_out (may be NULL) receives the number of variants.
+extern "C" int mpa_assembly_variant(int lmax, int set_to, int* count_out) {
+    const TransTable* T = mpa_get_table(lmax);
+    if (!T) return -1;
+    const int nv = (int)asm_variants(lmax, T->nterms).size();
+    if (count_out) *count_out = nv;
+    if (set_to >= nv) return -1;
+    if (set_to >= 0) g_asm_variant[lmax] = set_to;
+    return g_asm_variant[lmax];
+}
+
+// Autotuner: time every tile variant for this lmax on the caller's cluster
+// (out is overwritten) and keep the fastest for the process.
+extern "C" int mpa_autotune_assembly(int N, int lmax, const double* positions, const double* mie, double k,
+                                     int out_dtype, void* out, int reps, int* variant_out, double* ms_out,
+                                     void* stream) {
+    if (N < 1 || !positions || !mie || !out || !(k > 0) || reps < 1 || (out_dtype != MPA_C128 && out_dtype != MPA_C64)) {
+        mpa_set_error("autotune_assembly: bad argument");
+        return MPA_ERR_ARG;
+    }
+    const TransTable* T = mpa_get_table(lmax);
+    if (!T) return MPA_ERR_ARG;
+    const int nv = (int)asm_variants(lmax, T->nterms).size();
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    MPA_HIP_CHECK(hipEventCreate(&e0));
+    MPA_HIP_CHECK(hipEventCreate(&e1));
+    const int saved = g_asm_variant[lmax];
+    int best = 0, rc = MPA_OK;
+    double best_ms = 1e30;
+    for (int v = 0; v < nv && rc == MPA_OK; v++) {
+        g_asm_variant[lmax] = v;
+        rc = mpa_sphere_aggregate_tmatrix(N, lmax, positions, mie, k, out_dtype, 0, out, stream);  // warm-up
+        if (rc) break;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && rc == MPA_OK; r++)
+            rc = mpa_sphere_aggregate_tmatrix(N, lmax, positions, mie, k, out_dtype, 0, out, stream);
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = MPA_ERR_HIP;
+        float ms = 0;
+        if (rc == MPA_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = MPA_ERR_HIP;
+        if (rc == MPA_OK && ms / reps < best_ms) {
+            best_ms = ms / reps;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_asm_variant[lmax] = rc == MPA_OK ? best : saved;
+    if (rc) return rc;
+    if (variant_out) *variant_out = best;
+    if (ms_out) *ms_out = best_ms;
+    return MPA_OK;
+}
 
 extern "C" int mpa_sphere_aggregate_tmatrix_dual(int N, int lmax, const double* positions, const double* mie, double k,
                                                  int add_identity, void* out64, void* out32, void* stream) {

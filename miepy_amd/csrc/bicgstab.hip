@@ -290,9 +290,8 @@ size_t plan_bytes(int nsys, const mpa_system* s, int dtype) {
     for (int i = 0; i < nsys; i++) {
         vec += (size_t)s[i].nrhs * groups_for(s[i].n);
         sc += s[i].nrhs;
-        mvw += gemv_row_blocks(s[i].n, s[i].nrhs);
+        mvw += gemv_row_blocks(s[i].n, s[i].nrhs, dtype == MPA_C128 ? 8 : 4);
     }
-    (void)dtype;
     return align256(sizeof(SysDev) * nsys) + align256(sizeof(int3) * vec) + align256(sizeof(int2) * sc) +
            3 * align256(sizeof(MvDev) * nsys) + kTileClasses * 256 + align256(sizeof(int2) * mvw) +
            align256(sizeof(Scal) * sc);
@@ -345,7 +344,7 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
             const mpa_cluster_operator& o = *q.op;
             if (!mpa_get_table(o.lmax)) return MPA_ERR_ARG;  // built here, never inside a graph capture
             const MfJob base{o.N, o.lmax, S, q.add_identity ? 1 : 0, o.positions, (const double2*)o.mie, o.k,
-                             nullptr, nullptr, (double2*)o.workspace, i, 0};
+                             nullptr, nullptr, (double2*)o.workspace, i, 0, (const double2*)o.tmatrix};
             MfJob j = base;
             j.x = (const double2*)d.p;
             j.y = (double2*)d.v;
@@ -359,7 +358,7 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
             P.mf_xt.push_back(j);
             continue;
         }
-        const int rb = gemv_row_blocks(n, S);
+        const int rb = gemv_row_blocks(n, S, es);
         std::vector<int2>& mw = P.mv_work[gemv_class(S)];
         for (int b = 0; b < rb; b++) mw.push_back(make_int2(i, b));
     }
@@ -397,7 +396,7 @@ int matvec_phase(const Plan& P, const MvDev* d_jobs, const std::vector<MfJob>& m
     bool any = false;
     for (int c = 0; c < kTileClasses; c++) any |= P.n_mv_work[c] > 0;
     if (any) {
-        const int rc = gemv_launch<T>(d_jobs, P.d_sys, P.d_mv_work, P.n_mv_work, st);
+        const int rc = gemv_launch(sizeof(T), d_jobs, P.d_sys, P.d_mv_work, P.n_mv_work, st);
         if (rc) return rc;
     }
     for (const MfJob& j : mf) {
@@ -566,7 +565,8 @@ bool valid_systems(int nsys, const mpa_system* s, int dtype) {
         if (s[i].op) {
             const mpa_cluster_operator& o = *s[i].op;
             if (s[i].A || dtype != MPA_C128 || o.N < 1 || o.lmax < 1 || o.lmax > MPA_LMAX_MAX || !o.positions ||
-                !o.mie || !o.workspace || !(o.k > 0) || (long long)2 * lmax_to_rmax(o.lmax) * o.N != s[i].n)
+                !o.mie == !o.tmatrix || !o.workspace || !(o.k > 0) ||
+                (long long)2 * lmax_to_rmax(o.lmax) * o.N != s[i].n)
                 return false;
         } else if (!s[i].A) {
             return false;

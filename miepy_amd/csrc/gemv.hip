@@ -1,7 +1,466 @@
-// C ABI of the dense complex matvec (see gemv.hpp for the kernel design).
-// Replaces the OpenMP row-block GEMV of cpp/src/interactions.cpp:62-95.
+// Dense complex matvec / multi-RHS product: kernels, tile variants, the
+// autotuner and the C ABI.  Replaces the OpenMP row-block GEMV of
+// cpp/src/interactions.cpp:62-95 (and the products inside Eigen's BiCGSTAB).
+//
+// The system matrix is streamed from HBM exactly once per pass with 16-byte
+// nontemporal loads (the matrix is not reused within a call, so it is kept
+// out of the caches that hold x).  Vector-ALU tiles: each wave of a
+// workgroup owns R consecutive rows of one system and runs along the full
+// rows with U chunks per lane in flight; the x chunk loaded for a column is
+// reused for the R rows and for up to SG right-hand sides held in registers.
+// Jobs whose system has converged for all right-hand sides return before
+// touching the matrix.
+
+#include <vector>
 
 #include "gemv.hpp"
+
+namespace mpa {
+namespace {
+
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+typedef float fvec4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct Chunk;
+template <> struct Chunk<double> {
+    using v = dvec2;
+    static constexpr int C = 1;  // complex entries per 16-byte chunk
+};
+template <> struct Chunk<float> {
+    using v = fvec4;
+    static constexpr int C = 2;
+};
+
+__device__ inline void chunk_fma(double& ar, double& ai, dvec2 a, dvec2 x) {
+    ar = fma(a.x, x.x, ar);
+    ar = fma(-a.y, x.y, ar);
+    ai = fma(a.x, x.y, ai);
+    ai = fma(a.y, x.x, ai);
+}
+__device__ inline void chunk_fma(float& ar, float& ai, fvec4 a, fvec4 x) {
+    ar = fmaf(a.x, x.x, ar);
+    ar = fmaf(-a.y, x.y, ar);
+    ai = fmaf(a.x, x.y, ai);
+    ai = fmaf(a.y, x.x, ai);
+    ar = fmaf(a.z, x.z, ar);
+    ar = fmaf(-a.w, x.w, ar);
+    ai = fmaf(a.z, x.w, ai);
+    ai = fmaf(a.w, x.z, ai);
+}
+
+__device__ inline bool all_done(const SysDev& d) {
+    for (int s = 0; s < d.S; s++)
+        if (!d.scal[s].done) return false;
+    return true;
+}
+
+// One workgroup's share of a matvec job: its four waves own R rows each,
+// starting at row 4 R rb; SG right-hand sides per pass, U chunks in flight.
+template <typename T, int R, int SG, int U>
+__device__ inline void gemv_rows(const MvDev& J, int rb) {
+    using V = typename Chunk<T>::v;
+    constexpr int C = Chunk<T>::C;
+    const int n = J.n;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int row0 = (rb * 4 + w) * R;
+    if (row0 >= n) return;
+    const int nv = n / C;  // 16-byte chunks per row (n is even)
+    const T* A = (const T*)J.A;
+    const V* Ar[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) Ar[r] = (const V*)(A + (size_t)2 * n * min(row0 + r, n - 1));
+    for (int s0 = 0; s0 < J.S; s0 += SG) {
+        T ar[R][SG], ai[R][SG];
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int s = 0; s < SG; s++) ar[r][s] = ai[r][s] = 0;
+        const V* xs[SG];
+#pragma unroll
+        for (int s = 0; s < SG; s++) xs[s] = (const V*)((const T*)J.x + (size_t)2 * n * min(s0 + s, J.S - 1));
+        int c = lane;
+        for (; c + 64 * (U - 1) < nv; c += 64 * U) {
+            V xv[SG][U];
+#pragma unroll
+            for (int s = 0; s < SG; s++)
+#pragma unroll
+                for (int u = 0; u < U; u++) xv[s][u] = xs[s][c + 64 * u];
+            V av[R][U];
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int u = 0; u < U; u++) av[r][u] = __builtin_nontemporal_load(Ar[r] + c + 64 * u);
+#pragma unroll
+            for (int r = 0; r < R; r++)
+#pragma unroll
+                for (int s = 0; s < SG; s++)
+#pragma unroll
+                    for (int u = 0; u < U; u++) chunk_fma(ar[r][s], ai[r][s], av[r][u], xv[s][u]);
+        }
+        for (; c < nv; c += 64) {
+            V xv[SG];
+#pragma unroll
+            for (int s = 0; s < SG; s++) xv[s] = xs[s][c];
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const V a = __builtin_nontemporal_load(Ar[r] + c);
+#pragma unroll
+                for (int s = 0; s < SG; s++) chunk_fma(ar[r][s], ai[r][s], a, xv[s]);
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; r++)
+#pragma unroll
+            for (int s = 0; s < SG; s++) {
+                T re = ar[r][s], im = ai[r][s];
+                for (int off = 32; off > 0; off >>= 1) {
+                    re += __shfl_xor(re, off);
+                    im += __shfl_xor(im, off);
+                }
+                const int row = row0 + r;
+                if (lane == 0 && row < n && s0 + s < J.S) {
+                    const size_t o = (size_t)(s0 + s) * n + row;
+                    cplx_t<T> yv = cmake<T>(re, im);
+                    if (J.add_identity) yv = cadd(yv, ((const cplx_t<T>*)J.x)[o]);
+                    ((cplx_t<T>*)J.y)[o] = yv;
+                }
+            }
+    }
+}
+
+// ---- multi-RHS products on the matrix cores ------------------------------------
+//
+// Y = (I +) A X for up to kGemmS right-hand sides per pass as a real GEMM:
+// a complex row of A is the real row (Re a_0, Im a_0, Re a_1, ...), and
+// right-hand side s contributes two real columns (one per output part),
+//   X'[2j][(s,re)] = Re x_j,  X'[2j][(s,im)] = Im x_j,
+//   X'[2j+1][(s,re)] = -Im x_j, X'[2j+1][(s,im)] = Re x_j,
+// so that Y_re = sum Re a Re x - Im a Im x and Y_im = sum Re a Im x + Im a Re x.
+// A is streamed once per pass with 16-byte nontemporal loads straight into
+// the MFMA operand registers (each lane reads a contiguous run of its row);
+// the X' block of a K step is staged in LDS once per workgroup and shared by
+// its four waves, whose lanes form their B operands from it.
+//   c64:  v_mfma_f32_32x32x2f32, FP32 accumulate; a wave owns 32 rows x
+//         32 real columns (16 RHS); K step 32 complex columns.
+//   c128: v_mfma_f64_16x16x4f64; a wave owns 16 rows x 2 tiles of 16 real
+//         columns (16 RHS); K step 32 complex columns.
+// Lane layouts (checked on the device): 32x32x2 A lane l -> (row l%32,
+// k l/32), B -> (col l%32, k l/32), D[i] -> (row 8(i/4) + 4(l/32) + i%4,
+// col l%32); f64 16x16x4 A lane l -> (row l%16, k l/16), B -> (col l%16,
+// k l/16), D[i] -> (row 4 i + l/16, col l%16).
+
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+constexpr int kGemmS = 16;   // right-hand sides per pass
+constexpr int kGemmKB = 32;  // complex columns per K step
+template <typename T> struct GemmRows;
+template <> struct GemmRows<float> { static constexpr int value = 128; };   // 4 waves x 32
+template <> struct GemmRows<double> { static constexpr int value = 64; };   // 4 waves x 16
+
+// element of X' for K parity `odd` and output part `part` from x = (re, im)
+template <typename R>
+__device__ inline R xprime(R re, R im, int odd, int part) {
+    return odd ? (part ? re : -im) : (part ? im : re);
+}
+
+__device__ inline void gemm_rows_c64(const MvDev& J, int rb) {
+    __shared__ float2 xs[2][kGemmS][kGemmKB + 1];
+    const int n = J.n;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int h = lane >> 5, col = lane & 31, s_of = col >> 1, part = col & 1;
+    const int row = rb * 128 + w * 32 + (lane & 31);
+    const fvec4* Ar = (const fvec4*)((const float2*)J.A + (size_t)min(row, n - 1) * n);
+    const float2* X = (const float2*)J.x;
+    const int nkb = (n + kGemmKB - 1) / kGemmKB;
+    for (int s0 = 0; s0 < J.S; s0 += kGemmS) {
+        auto stage = [&](int kb, int buf) {
+            const int j0 = kb * kGemmKB;
+            for (int u = threadIdx.x; u < kGemmS * kGemmKB; u += kThreads) {
+                const int s = u / kGemmKB, jj = u % kGemmKB;
+                float2 v = make_float2(0.f, 0.f);
+                if (s0 + s < J.S && j0 + jj < n) v = X[(size_t)(s0 + s) * n + j0 + jj];
+                xs[buf][s][jj] = v;
+            }
+        };
+        auto load_a = [&](int kb, fvec4 (&a)[8]) {
+            const int j0 = kb * kGemmKB + 16 * h;  // this lane's first complex column (even)
+            if (kb * kGemmKB + kGemmKB <= n) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[q] = __builtin_nontemporal_load(Ar + j0 / 2 + q);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    fvec4 z = {0.f, 0.f, 0.f, 0.f};
+                    a[q] = (j0 + 2 * q < n) ? __builtin_nontemporal_load(Ar + j0 / 2 + q) : z;
+                }
+            }
+        };
+        f16v acc;
+#pragma unroll
+        for (int i = 0; i < 16; i++) acc[i] = 0.f;
+        fvec4 a[8];
+        load_a(0, a);
+        stage(0, 0);
+        __syncthreads();
+        for (int kb = 0; kb < nkb; kb++) {
+            const int buf = kb & 1;
+            fvec4 an[8];
+            if (kb + 1 < nkb) {
+                load_a(kb + 1, an);
+                stage(kb + 1, buf ^ 1);
+            }
+#pragma unroll
+            for (int t = 0; t < 32; t++) {
+                const float2 xv = xs[buf][s_of][16 * h + (t >> 1)];
+                const float bf = xprime<float>(xv.x, xv.y, t & 1, part);
+                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t >> 2][t & 3], bf, acc, 0, 0, 0);
+            }
+            if (kb + 1 < nkb) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[q] = an[q];
+            }
+            __syncthreads();
+        }
+        const int s = s0 + s_of;
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const int r = rb * 128 + w * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+            if (r < n && s < J.S) {
+                const size_t o = 2 * ((size_t)s * n + r) + part;
+                float v = acc[i];
+                if (J.add_identity) v += ((const float*)J.x)[o];
+                ((float*)J.y)[o] = v;
+            }
+        }
+    }
+}
+
+__device__ inline void gemm_rows_c128(const MvDev& J, int rb) {
+    __shared__ double2 xs[2][kGemmS][kGemmKB + 1];
+    const int n = J.n;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int g = lane >> 4, col = lane & 15, s_of = col >> 1, part = col & 1;
+    const int row = rb * 64 + w * 16 + (lane & 15);
+    const dvec2* Ar = (const dvec2*)((const double2*)J.A + (size_t)min(row, n - 1) * n);
+    const double2* X = (const double2*)J.x;
+    const int nkb = (n + kGemmKB - 1) / kGemmKB;
+    for (int s0 = 0; s0 < J.S; s0 += kGemmS) {
+        auto stage = [&](int kb, int buf) {
+            const int j0 = kb * kGemmKB;
+            for (int u = threadIdx.x; u < kGemmS * kGemmKB; u += kThreads) {
+                const int s = u / kGemmKB, jj = u % kGemmKB;
+                double2 v = make_double2(0.0, 0.0);
+                if (s0 + s < J.S && j0 + jj < n) v = X[(size_t)(s0 + s) * n + j0 + jj];
+                xs[buf][s][jj] = v;
+            }
+        };
+        auto load_a = [&](int kb, dvec2 (&a)[8]) {
+            const int j0 = kb * kGemmKB + 8 * g;  // this lane's first complex column
+            if (kb * kGemmKB + kGemmKB <= n) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[q] = __builtin_nontemporal_load(Ar + j0 + q);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 8; q++) {
+                    dvec2 z = {0.0, 0.0};
+                    a[q] = (j0 + q < n) ? __builtin_nontemporal_load(Ar + j0 + q) : z;
+                }
+            }
+        };
+        d4v acc0, acc1;
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc0[i] = acc1[i] = 0.0;
+        dvec2 a[8];
+        load_a(0, a);
+        stage(0, 0);
+        __syncthreads();
+        for (int kb = 0; kb < nkb; kb++) {
+            const int buf = kb & 1;
+            dvec2 an[8];
+            if (kb + 1 < nkb) {
+                load_a(kb + 1, an);
+                stage(kb + 1, buf ^ 1);
+            }
+#pragma unroll
+            for (int t = 0; t < 16; t++) {
+                const double af = a[t >> 1][t & 1];
+                const double2 x0 = xs[buf][s_of][8 * g + (t >> 1)];
+                const double2 x1 = xs[buf][8 + s_of][8 * g + (t >> 1)];
+                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(af, xprime<double>(x0.x, x0.y, t & 1, part), acc0, 0, 0, 0);
+                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(af, xprime<double>(x1.x, x1.y, t & 1, part), acc1, 0, 0, 0);
+            }
+            if (kb + 1 < nkb) {
+#pragma unroll
+                for (int q = 0; q < 8; q++) a[q] = an[q];
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = rb * 64 + w * 16 + 4 * i + g;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int s = s0 + 8 * c + s_of;
+                if (r < n && s < J.S) {
+                    const size_t o = 2 * ((size_t)s * n + r) + part;
+                    double v = c ? acc1[i] : acc0[i];
+                    if (J.add_identity) v += ((const double*)J.x)[o];
+                    ((double*)J.y)[o] = v;
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+__device__ inline void gemm_rows(const MvDev& J, int rb);
+template <>
+__device__ inline void gemm_rows<float>(const MvDev& J, int rb) {
+    gemm_rows_c64(J, rb);
+}
+template <>
+__device__ inline void gemm_rows<double>(const MvDev& J, int rb) {
+    gemm_rows_c128(J, rb);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gemm(const MvDev* __restrict__ jobs, const SysDev* __restrict__ sys,
+                                                   const int2* __restrict__ work) {
+    const int2 wk = work[blockIdx.x];
+    const MvDev J = jobs[wk.x];
+    if (J.skip >= 0 && all_done(sys[J.skip])) return;
+    gemm_rows<T>(J, wk.y);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_gemm1(MvDev J) {
+    gemm_rows<T>(J, blockIdx.x);
+}
+
+// batched: one workgroup per (job, row block) of the work list
+template <typename T, int RB, int SG, int U>
+__global__ __launch_bounds__(kThreads) void k_gemv(const MvDev* __restrict__ jobs, const SysDev* __restrict__ sys,
+                                                   const int2* __restrict__ work) {
+    const int2 wk = work[blockIdx.x];
+    const MvDev J = jobs[wk.x];
+    if (J.skip >= 0 && all_done(sys[J.skip])) return;
+    gemv_rows<T, RB, SG, U>(J, wk.y);
+}
+
+// one job passed by value: workgroup b owns row block b
+template <typename T, int RB, int SG, int U>
+__global__ __launch_bounds__(kThreads) void k_gemv1(MvDev J) {
+    gemv_rows<T, RB, SG, U>(J, blockIdx.x);
+}
+
+
+// ---- tile variants ------------------------------------------------------------------
+
+struct Variant {
+    int R, SG, U;  // vector-ALU tile; R = 0 marks the matrix-core GEMM
+    const char* name;
+};
+constexpr int kMaxVariants = 5;
+const Variant kVariants[kTileClasses][kMaxVariants] = {
+    {{8, 1, 2, "r8u2"}, {4, 1, 2, "r4u2"}, {8, 1, 4, "r8u4"}, {16, 1, 1, "r16u1"}, {4, 1, 4, "r4u4"}},
+    {{4, 2, 2, "r4u2"}, {2, 2, 2, "r2u2"}, {4, 2, 4, "r4u4"}, {8, 2, 1, "r8u1"}, {0, 0, 0, nullptr}},
+    {{2, 4, 2, "r2u2"}, {1, 4, 2, "r1u2"}, {2, 4, 4, "r2u4"}, {0, 16, 0, "mfma"}, {0, 0, 0, nullptr}},
+    {{0, 16, 0, "mfma"}, {1, 8, 2, "r1s8u2"}, {2, 8, 1, "r2s8u1"}, {0, 0, 0, nullptr}, {0, 0, 0, nullptr}},
+};
+int g_variant[2][kTileClasses] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [es == 8][class]
+
+inline int nvariants(int c) {
+    int k = 0;
+    while (k < kMaxVariants && kVariants[c][k].name) k++;
+    return k;
+}
+inline const Variant& variant_of(int c, int es) { return kVariants[c][g_variant[es == 8][c]]; }
+
+template <typename T>
+int launch_variant(int c, int v, const MvDev* jobs, const SysDev* sys, const int2* work, int nwork, const MvDev* one,
+                   hipStream_t st) {
+    const dim3 grid(nwork), block(kThreads);
+#define MPA_VAR(R_, SG_, U_)                                                                          \
+    do {                                                                                              \
+        if (one) hipLaunchKernelGGL((k_gemv1<T, R_, SG_, U_>), grid, block, 0, st, *one);             \
+        else hipLaunchKernelGGL((k_gemv<T, R_, SG_, U_>), grid, block, 0, st, jobs, sys, work);      \
+    } while (0)
+    const Variant& V = kVariants[c][v];
+    if (V.R == 0) {
+        if (one) hipLaunchKernelGGL(k_gemm1<T>, grid, block, 0, st, *one);
+        else hipLaunchKernelGGL(k_gemm<T>, grid, block, 0, st, jobs, sys, work);
+    } else if (c == 0) {
+        switch (v) {
+            case 0: MPA_VAR(8, 1, 2); break;
+            case 1: MPA_VAR(4, 1, 2); break;
+            case 2: MPA_VAR(8, 1, 4); break;
+            case 3: MPA_VAR(16, 1, 1); break;
+            default: MPA_VAR(4, 1, 4); break;
+        }
+    } else if (c == 1) {
+        switch (v) {
+            case 0: MPA_VAR(4, 2, 2); break;
+            case 1: MPA_VAR(2, 2, 2); break;
+            case 2: MPA_VAR(4, 2, 4); break;
+            default: MPA_VAR(8, 2, 1); break;
+        }
+    } else if (c == 2) {
+        switch (v) {
+            case 0: MPA_VAR(2, 4, 2); break;
+            case 1: MPA_VAR(1, 4, 2); break;
+            default: MPA_VAR(2, 4, 4); break;
+        }
+    } else {
+        switch (v) {
+            case 1: MPA_VAR(1, 8, 2); break;
+            default: MPA_VAR(2, 8, 1); break;
+        }
+    }
+#undef MPA_VAR
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+int rows_of_variant(int c, int v, int es) {
+    const Variant& V = kVariants[c][v];
+    if (V.R == 0) return es == 8 ? GemmRows<double>::value : GemmRows<float>::value;
+    return 4 * V.R;
+}
+
+}  // namespace
+
+int gemv_class(int S) { return S == 1 ? 0 : (S == 2 ? 1 : (S <= 4 ? 2 : 3)); }
+
+int gemv_rows_of_class(int c, int es) { return rows_of_variant(c, g_variant[es == 8][c], es); }
+
+int gemv_row_blocks(int n, int S, int es) {
+    const int rb = gemv_rows_of_class(gemv_class(S), es);
+    return (n + rb - 1) / rb;
+}
+
+int gemv_launch(int es, const MvDev* jobs, const SysDev* sys, int2* const* work, const int* nwork, hipStream_t st) {
+    ProfScope prof(es == 8 ? 0 : 1, st);
+    for (int c = 0; c < kTileClasses; c++) {
+        if (!nwork[c]) continue;
+        const int v = g_variant[es == 8][c];
+        const int rc = es == 8 ? launch_variant<double>(c, v, jobs, sys, work[c], nwork[c], nullptr, st)
+                               : launch_variant<float>(c, v, jobs, sys, work[c], nwork[c], nullptr, st);
+        if (rc) return rc;
+    }
+    return MPA_OK;
+}
+
+int gemv_launch1(int es, const MvDev& J, hipStream_t st) {
+    ProfScope prof(es == 8 ? 0 : 1, st);
+    const int c = gemv_class(J.S);
+    const int v = g_variant[es == 8][c];
+    const int nb = (J.n + rows_of_variant(c, v, es) - 1) / rows_of_variant(c, v, es);
+    return es == 8 ? launch_variant<double>(c, v, nullptr, nullptr, nullptr, nb, &J, st)
+                   : launch_variant<float>(c, v, nullptr, nullptr, nullptr, nb, &J, st);
+}
+
+}  // namespace mpa
 
 using namespace mpa;
 
@@ -11,11 +470,69 @@ extern "C" int mpa_matvec(int n, int nrhs, int dtype, const void* A, int add_ide
         mpa_set_error("matvec: bad argument (n must be even and positive)");
         return MPA_ERR_ARG;
     }
+    if (dtype != MPA_C128 && dtype != MPA_C64) {
+        mpa_set_error("matvec: unknown dtype %d", dtype);
+        return MPA_ERR_ARG;
+    }
     MvDev J{A, x, y, n, nrhs, add_identity, -1};
-    if (dtype == MPA_C128) return gemv_launch1<double>(J, (hipStream_t)stream);
-    if (dtype == MPA_C64) return gemv_launch1<float>(J, (hipStream_t)stream);
-    mpa_set_error("matvec: unknown dtype %d", dtype);
-    return MPA_ERR_ARG;
+    return gemv_launch1(dtype == MPA_C128 ? 8 : 4, J, (hipStream_t)stream);
+}
+
+// Autotuner: time every tile variant of the class of `nrhs` on the caller's
+// operands (y is overwritten), keep the fastest for (dtype, class).
+extern "C" int mpa_autotune_matvec(int n, int nrhs, int dtype, const void* A, const void* x, void* y, int reps,
+                                   int* variant_out, double* ms_out, void* stream) {
+    if (n < 2 || (n & 1) || nrhs < 1 || !A || !x || !y || reps < 1 || (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("autotune_matvec: bad argument");
+        return MPA_ERR_ARG;
+    }
+    const int es = dtype == MPA_C128 ? 8 : 4;
+    const int c = gemv_class(nrhs);
+    hipStream_t st = (hipStream_t)stream;
+    MvDev J{A, x, y, n, nrhs, 0, -1};
+    hipEvent_t e0, e1;
+    MPA_HIP_CHECK(hipEventCreate(&e0));
+    MPA_HIP_CHECK(hipEventCreate(&e1));
+    int best = 0;
+    double best_ms = 1e30;
+    const int saved = g_variant[es == 8][c];
+    int rc = MPA_OK;
+    for (int v = 0; v < nvariants(c) && rc == MPA_OK; v++) {
+        g_variant[es == 8][c] = v;
+        rc = gemv_launch1(es, J, st);  // warm-up
+        if (rc) break;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && rc == MPA_OK; r++) rc = gemv_launch1(es, J, st);
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = MPA_ERR_HIP;
+        float ms = 0;
+        if (rc == MPA_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = MPA_ERR_HIP;
+        if (rc == MPA_OK && ms / reps < best_ms) {
+            best_ms = ms / reps;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_variant[es == 8][c] = rc == MPA_OK ? best : saved;
+    if (rc) return rc;
+    if (variant_out) *variant_out = best;
+    if (ms_out) *ms_out = best_ms;
+    return MPA_OK;
+}
+
+extern "C" int mpa_matvec_variant(int dtype, int nrhs, int set_to) {
+    if (dtype != MPA_C128 && dtype != MPA_C64) return -1;
+    const int c = gemv_class(nrhs < 1 ? 1 : nrhs);
+    int& v = g_variant[dtype == MPA_C128][c];
+    if (set_to >= 0 && set_to < nvariants(c)) v = set_to;
+    return v;
+}
+
+extern "C" const char* mpa_matvec_variant_name(int dtype, int nrhs, int variant) {
+    const int c = gemv_class(nrhs < 1 ? 1 : nrhs);
+    if (variant < 0 || variant >= nvariants(c) || (dtype != MPA_C128 && dtype != MPA_C64)) return "";
+    return kVariants[c][variant].name;
 }
 
 // Plain 16-byte copy of `bytes` (a multiple of 16): the reference point for

@@ -158,6 +158,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __r
     const int nmu = 2 * lmax + 1;
     const int JB = pairs_per_batch(rmax);
     const int zs = JB * rmax;  // z / reduction stride per (rhs, polarization)
+    const double2* zpre = J.tmat ? J.part + (size_t)nchunks * J.S * n : nullptr;  // z = Tm x
     PairGeo* geo = (PairGeo*)lds;
     double2* HL = (double2*)(geo + JB);
     double2* eph = HL + JB * Lp;
@@ -191,8 +192,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __r
                 const int b = e >= rmax ? 1 : 0, s = e - b * rmax;
                 const int v = (int)sqrtf((float)(s + 1) + 0.5f);
                 const int j = j0 + jj;
-                zr[(q * 2 + b) * zs + jj * rmax + s] =
-                    cmul(J.mie[((size_t)j * 2 + b) * lmax + v - 1], J.x[(size_t)(s0 + q) * n + (size_t)j * bs + e]);
+                const size_t xo = (size_t)(s0 + q) * n + (size_t)j * bs + e;
+                zr[(q * 2 + b) * zs + jj * rmax + s] = zpre ? zpre[xo] : cmul(J.mie[((size_t)j * 2 + b) * lmax + v - 1], J.x[xo]);
             }
             __syncthreads();
             for (int u = t; u < jc * (pmax + 1); u += kThreads) {
@@ -255,8 +256,15 @@ __global__ __launch_bounds__(kThreads) void k_matfree_sum(MfJob J, const SysDev*
 }  // namespace
 
 size_t matfree_workspace_bytes(int N, int lmax, int S) {
-    return sizeof(double2) * (size_t)2 * lmax_to_rmax(lmax) * N * S * chunks_for(N);
+    return sizeof(double2) * (size_t)2 * lmax_to_rmax(lmax) * N * S * (chunks_for(N) + 1);
 }
+
+}  // namespace mpa
+
+int mpa_internal_tmatrix_apply(int N, int lmax, int nrhs, const double* tmatrix, const void* x, void* y,
+                               hipStream_t st);
+
+namespace mpa {
 
 int matfree_launch(const MfJob& J, const SysDev* sys, hipStream_t st) {
     const TransTable* T = mpa_get_table(J.lmax);
@@ -266,6 +274,12 @@ int matfree_launch(const MfJob& J, const SysDev* sys, hipStream_t st) {
     const long long nblocks = (long long)J.N * nchunks;
     const size_t total = (size_t)J.S * 2 * lmax_to_rmax(J.lmax) * J.N;
     const unsigned sum_blocks = (unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads);
+    if (J.tmat) {
+        const size_t n = (size_t)2 * lmax_to_rmax(J.lmax) * J.N;
+        double2* z = J.part + (size_t)nchunks * J.S * n;
+        const int rc = mpa_internal_tmatrix_apply(J.N, J.lmax, J.S, (const double*)J.tmat, J.x, z, st);
+        if (rc) return rc;
+    }
     const TabDev tab{T->cmeta, T->ccoef, T->clidx};
     const int SG = sg_for(J.S);
     const size_t lds = lds_bytes(J.lmax, SG, T->nterms);
@@ -294,7 +308,8 @@ extern "C" size_t mpa_matfree_workspace_bytes(int N, int lmax, int nrhs) {
 
 extern "C" int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,
                                   void* stream) {
-    if (!op || op->N < 1 || nrhs < 1 || !op->positions || !op->mie || !op->workspace || !x || !y || !(op->k > 0) ||
+    if (!op || op->N < 1 || nrhs < 1 || !op->positions || !op->mie == !op->tmatrix || !op->workspace || !x || !y ||
+        !(op->k > 0) ||
         op->lmax < 1 || op->lmax > MPA_LMAX_MAX) {
         mpa_set_error("matfree_matvec: bad argument");
         return MPA_ERR_ARG;
@@ -304,6 +319,6 @@ extern "C" int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int 
         return MPA_ERR_ARG;
     }
     MfJob J{op->N, op->lmax, nrhs, add_identity ? 1 : 0, op->positions, (const double2*)op->mie, op->k,
-            (const double2*)x, (double2*)y, (double2*)op->workspace, -1, 0};
+            (const double2*)x, (double2*)y, (double2*)op->workspace, -1, 0, (const double2*)op->tmatrix};
     return matfree_launch(J, nullptr, (hipStream_t)stream);
 }

@@ -14,12 +14,13 @@ struct MfJob {
     double k;
     const double2* x;    // [S, n]
     double2* y;          // [S, n]
-    double2* part;       // [chunks, S, n]
+    double2* part;       // [chunks, S, n], then [S, n] for z = Tm x (general T-matrices)
     int skip;            // SysDev index whose all-done flag skips the job, or -1
     int pad;
+    const double2* tmat;  // [N, 2 rmax, 2 rmax] per-particle T-matrices, or null (spheres: mie)
 };
 
-// Workspace bytes of a job (the per-chunk partial rows).
+// Workspace bytes of a job (the per-chunk partial rows and the z buffer).
 size_t matfree_workspace_bytes(int N, int lmax, int S);
 
 // Launches the two kernels of one job; `sys` may be null when skip < 0.

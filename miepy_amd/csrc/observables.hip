@@ -11,6 +11,8 @@
 // All in FP64.  One thread per (particle, source); totals are reduced in a
 // fixed order by one workgroup per source.
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace {
@@ -201,7 +203,46 @@ __global__ __launch_bounds__(kThreads) void k_totals(int N, int lmax, const doub
         for (int c = 0; c < 3; c++) C_total[s * 3 + c] = red[c][0];
 }
 
+// y_j = Tm_j x_j for every particle and right-hand side (block-diagonal
+// product): p_scat = T p_inc of cpp's `cluster` path (src/miepy/cluster.py:657,
+// einsum "naibj,nbj->nai").  One thread per output entry; Tm_j rows are read
+// through the caches.
+__global__ __launch_bounds__(256) void k_tmatrix_apply(int N, int bs, int S, const double2* __restrict__ tm,
+                                                       const double2* __restrict__ x, double2* __restrict__ y) {
+    const size_t n = (size_t)bs * N;
+    const size_t total = (size_t)S * n;
+    for (size_t idx = (size_t)blockIdx.x * 256 + threadIdx.x; idx < total; idx += (size_t)gridDim.x * 256) {
+        const size_t s = idx / n, rest = idx - s * n;
+        const size_t j = rest / bs, e = rest - j * bs;
+        const double2* trow = tm + (j * bs + e) * bs;
+        const double2* xj = x + s * n + j * bs;
+        double2 acc = make_double2(0.0, 0.0);
+        for (int c = 0; c < bs; c++) cfma(acc, trow[c], xj[c]);
+        y[idx] = acc;
+    }
+}
+
 }  // namespace
+
+int mpa_internal_tmatrix_apply(int N, int lmax, int nrhs, const double* tmatrix, const void* x, void* y,
+                               hipStream_t st) {
+    const int bs = 2 * lmax_to_rmax(lmax);
+    const size_t total = (size_t)nrhs * bs * N;
+    const unsigned blocks = (unsigned)std::min<size_t>(4096, (total + 255) / 256);
+    hipLaunchKernelGGL(k_tmatrix_apply, dim3(blocks), dim3(256), 0, st, N, bs, nrhs, (const double2*)tmatrix,
+                       (const double2*)x, (double2*)y);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+extern "C" int mpa_tmatrix_apply(int N, int lmax, int nrhs, const double* tmatrix, const void* x, void* y,
+                                 void* stream) {
+    if (N < 1 || nrhs < 1 || lmax < 1 || lmax > MPA_LMAX_MAX || !tmatrix || !x || !y || x == y) {
+        mpa_set_error("tmatrix_apply: bad argument");
+        return MPA_ERR_ARG;
+    }
+    return mpa_internal_tmatrix_apply(N, lmax, nrhs, tmatrix, x, y, (hipStream_t)stream);
+}
 
 extern "C" int mpa_cluster_observables(int N, int lmax, int nsrc, const double* p_inc, const double* p_src,
                                        const double* mie_scat, const double* mie_int, double k, double eps_b,

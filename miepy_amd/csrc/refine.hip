@@ -174,7 +174,7 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             cw.push_back(make_int2(i, s));
             for (int g = 0; g < d.G; g++) vw.push_back(make_int3(i, s, g));
         }
-        const int rb = gemv_row_blocks(n, S);
+        const int rb = gemv_row_blocks(n, S, 8);  // the FP64 residual matvec
         for (int b = 0; b < rb; b++) mw[gemv_class(S)].push_back(make_int2(i, b));
         total += S;
     }
@@ -267,7 +267,7 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             }
         hipLaunchKernelGGL(k_ref_accum, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         MPA_LAUNCH_CHECK();
-        rc = gemv_launch<double>(d_mv, nullptr, d_mw, n_mw, st);
+        rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st);
         if (rc) return rc;
         hipLaunchKernelGGL(k_ref_residual, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         hipLaunchKernelGGL(k_ref_norms, dim3(nc), dim3(kThreads), 0, st, d_ref, d_cw);
@@ -293,7 +293,7 @@ size_t refine_scratch_bytes(int nsys, const mpa_refine_system* R) {
     for (int i = 0; i < nsys; i++) {
         vw += (size_t)R[i].nrhs * groups_for(R[i].n);
         cw += R[i].nrhs;
-        mw += gemv_row_blocks(R[i].n, R[i].nrhs);
+        mw += gemv_row_blocks(R[i].n, R[i].nrhs, 8);
         inner[i].n = R[i].n;
         inner[i].nrhs = R[i].nrhs;
     }

@@ -66,6 +66,46 @@ def assemble(positions, mie, k, dtype=C128, add_identity=False, out=None):
     return out
 
 
+def assemble_tmatrix(positions, tmatrix, k, dtype=C128, add_identity=False, out=None):
+    """Aggregate T-matrix [n, n] (or I + T) of particles with general
+    T-matrices: positions [N, 3] f64, tmatrix [N, 2, rmax, 2, rmax] c128."""
+    lib = L.load()
+    pos = to_device(positions, torch.float64).reshape(-1, 3)
+    N = pos.shape[0]
+    tm = to_device(tmatrix, C128).reshape(N, -1)
+    bs = int(round(tm.shape[1] ** 0.5))
+    if bs * bs != tm.shape[1] or bs % 2:
+        raise ValueError("tmatrix must be [N, 2, rmax, 2, rmax]")
+    rmax = bs // 2
+    lmax = int(round((rmax + 1) ** 0.5)) - 1
+    if lmax * (lmax + 2) != rmax:
+        raise ValueError(f"rmax={rmax} is not lmax(lmax+2)")
+    tm = tm.reshape(N, bs, bs).contiguous()
+    n = bs * N
+    if out is None:
+        out = torch.empty((n, n), dtype=dtype, device=pos.device)
+    L.check(lib.mpa_particle_aggregate_tmatrix(N, lmax, _ptr(pos), _ptr(tm), float(k), DTYPE_CODE[dtype],
+                                               1 if add_identity else 0, _ptr(out), _stream()),
+            "particle_aggregate_tmatrix")
+    return out
+
+
+def tmatrix_apply(tmatrix, x, out=None):
+    """y_j = Tm_j x_j per particle: tmatrix [N, 2, rmax, 2, rmax], x [S, N, 2, rmax] (or [N, 2, rmax])."""
+    lib = L.load()
+    tm = to_device(tmatrix, C128)
+    N = tm.shape[0]
+    bs = int(round(tm.reshape(N, -1).shape[1] ** 0.5))
+    rmax = bs // 2
+    lmax = int(round((rmax + 1) ** 0.5)) - 1
+    xd = to_device(x, C128)
+    x2 = xd.reshape(-1, N * bs)
+    if out is None:
+        out = torch.empty_like(x2)
+    L.check(lib.mpa_tmatrix_apply(N, lmax, x2.shape[0], _ptr(tm), _ptr(x2), _ptr(out), _stream()), "tmatrix_apply")
+    return out.reshape(xd.shape)
+
+
 def assemble_dual(positions, mie, k, add_identity=False, out64=None, out32=None):
     """One assembly pass writing the c128 matrix and its complex64 rounding."""
     lib = L.load()
@@ -96,6 +136,39 @@ def matvec(A, x, add_identity=False, out=None):
     L.check(lib.mpa_matvec(n, x2.shape[0], DTYPE_CODE[A.dtype], _ptr(A), 1 if add_identity else 0, _ptr(x2.contiguous()),
                            _ptr(y), _stream()), "matvec")
     return y.reshape(x.shape)
+
+
+def autotune_matvec(A, nrhs=1, reps=3):
+    """Time every matvec tile variant for A's dtype and the RHS-count class of
+    `nrhs` on A itself and keep the fastest for the process.  Returns
+    (variant name, ms per launch)."""
+    lib = L.load()
+    n = A.shape[0]
+    code = DTYPE_CODE[A.dtype]
+    g = torch.Generator(device=A.device).manual_seed(0)
+    x = torch.randn((nrhs, n), dtype=A.dtype, device=A.device, generator=g)
+    y = torch.empty_like(x)
+    v = ctypes.c_int(0)
+    ms = ctypes.c_double(0)
+    L.check(lib.mpa_autotune_matvec(n, nrhs, code, _ptr(A), _ptr(x), _ptr(y), int(reps), ctypes.byref(v),
+                                    ctypes.byref(ms), _stream()), "autotune_matvec")
+    return lib.mpa_matvec_variant_name(code, nrhs, v.value).decode(), ms.value
+
+
+def autotune_assembly(positions, mie, k, out, reps=2):
+    """Time every assembly tile variant on this cluster (`out` [n, n] is
+    overwritten) and keep the fastest for the process.  Returns
+    (tile column blocks in use, ms per launch)."""
+    lib = L.load()
+    pos = to_device(positions, torch.float64).reshape(-1, 3)
+    N = pos.shape[0]
+    mie_d = to_device(mie, C128).reshape(N, 2, -1)
+    lmax = mie_d.shape[-1]
+    v = ctypes.c_int(0)
+    ms = ctypes.c_double(0)
+    L.check(lib.mpa_autotune_assembly(N, lmax, _ptr(pos), _ptr(mie_d), float(k), DTYPE_CODE[out.dtype], _ptr(out),
+                                      int(reps), ctypes.byref(v), ctypes.byref(ms), _stream()), "autotune_assembly")
+    return lib.mpa_assembly_tile(lmax), ms.value
 
 
 class SolveReport:
@@ -227,12 +300,22 @@ class ClusterOperator:
     workspace for up to `max_rhs` right-hand sides; T itself is never
     stored, so memory is O(n)."""
 
-    def __init__(self, positions, mie, k, max_rhs=1):
+    def __init__(self, positions, mie, k, max_rhs=1, tmatrix=None):
         lib = L.load()
         self.pos = to_device(positions, torch.float64).reshape(-1, 3)
         self.N = self.pos.shape[0]
-        self.mie = to_device(mie, C128).reshape(self.N, 2, -1)
-        self.lmax = self.mie.shape[-1]
+        if (mie is None) == (tmatrix is None):
+            raise ValueError("give exactly one of mie (spheres) or tmatrix (general particles)")
+        if mie is not None:
+            self.mie = to_device(mie, C128).reshape(self.N, 2, -1)
+            self.lmax = self.mie.shape[-1]
+            self.tmatrix = None
+        else:
+            self.mie = None
+            self.tmatrix = to_device(tmatrix, C128).reshape(self.N, -1)
+            bs = int(round(self.tmatrix.shape[1] ** 0.5))
+            self.lmax = int(round((bs // 2 + 1) ** 0.5)) - 1
+            self.tmatrix = self.tmatrix.reshape(self.N, bs, bs).contiguous()
         self.k = float(k)
         self.n = 2 * lmax_to_rmax(self.lmax) * self.N
         self.shape = (self.n, self.n)
@@ -243,12 +326,14 @@ class ClusterOperator:
         if nb == 0:
             raise ValueError(f"matrix-free operator: unsupported N={self.N}, lmax={self.lmax}")
         self.workspace = torch.empty(nb, dtype=torch.uint8, device=self.device)
-        self.struct = L.MpaClusterOperator(self.N, self.lmax, self.pos.data_ptr(), self.mie.data_ptr(), self.k,
-                                           self.workspace.data_ptr())
+        self.struct = L.MpaClusterOperator(self.N, self.lmax, self.pos.data_ptr(),
+                                           None if self.mie is None else self.mie.data_ptr(), self.k,
+                                           self.workspace.data_ptr(),
+                                           None if self.tmatrix is None else self.tmatrix.data_ptr())
 
     def reserve(self, nrhs):
         if nrhs > self.max_rhs:
-            self.__init__(self.pos, self.mie, self.k, nrhs)
+            self.__init__(self.pos, self.mie, self.k, nrhs, self.tmatrix)
 
 
 def matfree_matvec(op, x, add_identity=False, out=None):

@@ -75,11 +75,13 @@ def test_single_particle_and_aligned_pairs(dev, oracle):
     assert np.max(np.abs(got - ref)) <= 1e-13 * np.abs(ref).max()
 
 
-@pytest.mark.parametrize("S", [1, 2, 3, 8, 11])
+@pytest.mark.parametrize("S", [1, 2, 3, 5, 8, 11, 16, 21])
 def test_matvec(dev, S):
-    import torch
+    """Every tile class: vector-ALU tiles (S <= 4) and the matrix-core GEMM
+    (S >= 5, 16 RHS per pass, K steps of 32 columns with a ragged tail,
+    partial row blocks)."""
     rng = np.random.default_rng(S)
-    for n in (64, 600, 1000):
+    for n in (30, 64, 130, 600, 1000):
         A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
         x = rng.normal(size=(S, n)) + 1j * rng.normal(size=(S, n))
         y = dev.matvec(dev.to_device(A, dev.C128), dev.to_device(x, dev.C128)).cpu().numpy()
@@ -284,3 +286,60 @@ def test_dual_assembly_equals_single(dev, oracle):
     d64, d32 = dev.assemble_dual(pos, mie, k, add_identity=True)
     assert np.array_equal(d64.cpu().numpy(), a64)
     assert np.array_equal(d32.cpu().numpy(), a64.astype(np.complex64))
+
+
+def test_every_matvec_variant_and_autotune(dev):
+    """Each tile variant the autotuner may pick gives the product (c128 1e-12,
+    c64 1e-5 relative), and the autotuner leaves a valid variant in place."""
+    import miepy_amd._lib as L
+    lib = L.load()
+    rng = np.random.default_rng(7)
+    n = 610
+    A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
+    for dtype, code, tol in ((dev.C128, L.MPA_C128, 1e-12), (dev.C64, L.MPA_C64, 1e-5)):
+        Ad = dev.to_device(A, dtype)
+        for S in (1, 2, 4, 9):
+            x = rng.normal(size=(S, n)) + 1j * rng.normal(size=(S, n))
+            ref = x @ A.T
+            xd = dev.to_device(x, dtype)
+            saved = lib.mpa_matvec_variant(code, S, -1)
+            v = 0
+            while lib.mpa_matvec_variant_name(code, S, v):
+                assert lib.mpa_matvec_variant(code, S, v) == v
+                y = dev.matvec(Ad, xd, add_identity=True).cpu().numpy()
+                err = np.max(np.abs(y - ref - x)) / np.abs(ref).max()
+                assert err <= tol, (dtype, S, lib.mpa_matvec_variant_name(code, S, v).decode(), err)
+                v += 1
+            assert v >= 2
+            lib.mpa_matvec_variant(code, S, saved)
+            name, ms = dev.autotune_matvec(Ad, S, reps=2)
+            assert name and ms > 0
+            y = dev.matvec(Ad, xd).cpu().numpy()
+            assert np.max(np.abs(y - ref)) <= tol * np.abs(ref).max()
+
+
+def test_assembly_autotune_variants_match_oracle(dev, oracle):
+    """Every assembly tile variant (tile width, table in LDS or not) equals the
+    oracle's matrix to 1e-13; the autotuner keeps one of them."""
+    import ctypes
+    import torch
+    import miepy_amd._lib as L
+    lib = L.load()
+    for N, lmax in ((23, 2), (19, 3), (9, 4)):
+        pos, mie = _cluster(N, lmax, seed=40 + lmax)
+        k = 2 * np.pi / (600 * nm)
+        ref = oracle.sphere_aggregate_tmatrix(pos, mie, k)
+        n = ref.shape[0]
+        cnt = ctypes.c_int(0)
+        saved = lib.mpa_assembly_variant(lmax, -1, ctypes.byref(cnt))
+        assert cnt.value >= 2
+        for v in range(cnt.value):
+            assert lib.mpa_assembly_variant(lmax, v, None) == v
+            got = dev.assemble(pos, mie, k).cpu().numpy()
+            assert np.max(np.abs(got - ref)) <= 1e-13 * np.abs(ref).max(), (lmax, v)
+        lib.mpa_assembly_variant(lmax, saved, None)
+        out = torch.empty((n, n), dtype=dev.C128, device="cuda")
+        J, ms = dev.autotune_assembly(pos, mie, k, out, reps=1)
+        assert J >= 1 and ms > 0
+        got = dev.assemble(pos, mie, k).cpu().numpy()
+        assert np.max(np.abs(got - ref)) <= 1e-13 * np.abs(ref).max()

@@ -168,3 +168,23 @@ def test_particle_tmatrix_reduces_to_sphere(oracle):
     T1 = oracle.sphere_aggregate_tmatrix(pos, mie, k)
     T2 = oracle.particle_aggregate_tmatrix(pos, tm, k)
     assert np.max(np.abs(T1 - T2)) <= 1e-13 * np.max(np.abs(T1))
+
+
+def test_particle_tmatrix_is_bare_blocks_times_tmatrix(oracle):
+    """For dense T-matrices each block is the pair's bare translation block
+    (the sphere assembly with unit Mie coefficients) times the column
+    particle's T-matrix (cpp/src/vsh_translation.cpp:129-157)."""
+    lmax, k = 2, 2 * np.pi / 0.7
+    rmax = lmax * (lmax + 2)
+    bs = 2 * rmax
+    pos = np.array([[0.0, 0.0, 0.0], [0.35, 0.1, -0.2], [-0.1, 0.4, 0.3], [0.2, -0.3, 0.25]])
+    N = pos.shape[0]
+    rng = np.random.default_rng(12)
+    tm = rng.normal(size=(N, bs, bs)) + 1j * rng.normal(size=(N, bs, bs))
+    X = oracle.sphere_aggregate_tmatrix(pos, np.ones((N, 2, lmax), dtype=complex), k)
+    T = oracle.particle_aggregate_tmatrix(pos, tm.reshape(N, 2, rmax, 2, rmax), k)
+    for i in range(N):
+        for j in range(N):
+            blk = T[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
+            ref = X[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs] @ tm[j]
+            assert np.max(np.abs(blk - ref)) <= 1e-12 * max(np.max(np.abs(ref)), 1e-300)
