@@ -42,6 +42,7 @@ def parse():
     ap.add_argument("--precision", default="all", choices=["all", "both", "fp64", "mixed", "matfree"],
                     help="all = fp64, mixed and matfree; both = fp64 and mixed")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tiles (no autotuning in warmup)")
     ap.add_argument("--cpu-gemvs", type=int, default=3, help="timed oracle GEMVs in the CPU sample")
     ap.add_argument("--batch-count", type=int, default=256, help="clusters in the batch256 workload")
     ap.add_argument("--serve-rate", type=float, default=20.0, help="request arrivals per second (Poisson)")
@@ -110,9 +111,15 @@ def bench_batch(args):
     precs = {"all": ["fp64", "mixed", "matfree"], "both": ["fp64", "mixed"]}.get(args.precision, [args.precision])
     for prec in precs:
         B = ClusterBatch(ws, precision=prec)
-        lib_prof = {}
-        for _ in range(args.warmup):
+        for i in range(max(args.warmup, 1)):
             B.solve()
+            if i == 0 and not args.no_autotune and prec != "matfree":
+                import miepy_amd.device as D
+                big = max(B.clusters, key=lambda c: c.n)
+                tuned = {"matvec_c128": D.autotune_matvec(big.A64, big.S)}
+                if prec == "mixed":
+                    tuned["matvec_c64"] = D.autotune_matvec(big.A32, big.S)
+                log(f"{prec}: autotuned on n={big.n}: {tuned}")
         torch.cuda.synchronize()
         lat = []
         t0 = time.perf_counter()
@@ -189,6 +196,24 @@ class Solver:
         self.x = torch.empty((S, n), dtype=D.C128, device=dev)
         self.report = None
         self.obs = None
+        self.tuned = None
+
+    def tune(self):
+        """Autotune the assembly and matvec tiles on this workload's own
+        matrices (outside the timed region); returns the choices."""
+        D, p, w = self.D, self.prob, self.prob.w
+        out = {}
+        if self.A64 is not None:
+            J, ms = D.autotune_assembly(p.pos_d, p.ms_d, w.k, self.A64, reps=2)
+            out["assembly_tile"] = {"J": J, "ms": ms}
+            self.step()  # re-assemble the tuned-over matrices
+            name, ms = D.autotune_matvec(self.A64, p.S)
+            out["matvec_c128"] = {"variant": name, "ms": ms}
+        if self.A32 is not None:
+            name, ms = D.autotune_matvec(self.A32, p.S)
+            out["matvec_c64"] = {"variant": name, "ms": ms}
+        log(f"{self.precision}: autotuned {out}")
+        return out
 
     def step(self):
         D, p, w = self.D, self.prob, self.prob.w
@@ -206,12 +231,14 @@ class Solver:
         self.obs = D.observables(self.x.reshape(p.S, w.N, 2, w.rmax), p.src_d, p.ms_d, p.mi_d, w.k, w.medium_eps)
 
 
-def run_timed(solver, steps, warmup, profile=False):
+def run_timed(solver, steps, warmup, profile=False, tune=True):
     import torch
     L = solver.L
     lib = L.load()
-    for _ in range(warmup):
+    for i in range(max(warmup, 1 if tune else 0)):
         solver.step()
+        if i == 0 and tune:
+            solver.tuned = solver.tune()
     torch.cuda.synchronize()
     if profile:
         lib.mpa_profile_enable(1)
@@ -346,8 +373,8 @@ def main():
     precs = {"all": ["fp64", "mixed", "matfree"], "both": ["fp64", "mixed"]}.get(args.precision, [args.precision])
     for prec in precs:
         s = Solver(prob, prec)
-        total, lat, prof = run_timed(s, args.steps, args.warmup, profile=True)
-        res = dict(total=total, lat=lat, prof=prof, report=s.report,
+        total, lat, prof = run_timed(s, args.steps, args.warmup, profile=True, tune=not args.no_autotune)
+        res = dict(total=total, lat=lat, prof=prof, report=s.report, tuned=s.tuned,
                    obs={k: (v.cpu().numpy() if v is not None else None) for k, v in s.obs.items()})
         results[prec] = res
         log(f"{prec}: {args.steps} steps in {total:.3f} s ({total / args.steps * 1e3:.1f} ms/step), "
@@ -397,6 +424,7 @@ def main():
             "outer": results["mixed"]["report"].outer,
             "inner_iterations": results["mixed"]["report"].iterations,
         }
+    line["autotuned"] = {k: v["tuned"] for k, v in results.items() if v.get("tuned")}
     # roofline of the dominant kernel (largest summed time in the timed region)
     prof = r["prof"]
     dom = max(prof, key=lambda k: prof[k]["total_ms"]) if prof else None

@@ -344,7 +344,7 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
             const mpa_cluster_operator& o = *q.op;
             if (!mpa_get_table(o.lmax)) return MPA_ERR_ARG;  // built here, never inside a graph capture
             const MfJob base{o.N, o.lmax, S, q.add_identity ? 1 : 0, o.positions, (const double2*)o.mie, o.k,
-                             nullptr, nullptr, (double2*)o.workspace, i, 0, (const double2*)o.tmatrix};
+                             nullptr, nullptr, (double2*)o.workspace, i, 0, (const double2*)o.tmatrix, nullptr};
             MfJob j = base;
             j.x = (const double2*)d.p;
             j.y = (double2*)d.v;

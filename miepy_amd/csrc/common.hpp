@@ -94,6 +94,11 @@ struct TransTable {
     int2* cmeta = nullptr;            // device, [rmax*rmax]
     double2* ccoef = nullptr;         // device, [nterms]
     unsigned short* clidx = nullptr;  // device, [nterms]
+    // contraction form (factorized matrix-free operator, matfree.hip):
+    int nentries = 0;
+    int2* wmeta = nullptr;     // device, [nentries] {first term, count | kind << 16}
+    int* wsrc = nullptr;       // device, column mode s of each term
+    double2* wcoef = nullptr;  // device, coefficient of each term
 };
 
 // Returns the (cached, per device) table for lmax; nullptr on failure.

@@ -164,69 +164,87 @@ __device__ inline R xprime(R re, R im, int odd, int part) {
     return odd ? (part ? re : -im) : (part ? im : re);
 }
 
+// A tile of a K step in LDS: rows x (32 complex + 1 pad chunk); the odd
+// 16-byte pad spreads the rows of one MFMA operand read over the banks.
+constexpr int kPadC64 = 2;  // floats of padding per row (c64: 64 + 4 floats -> stride 68 dwords)
+
 __device__ inline void gemm_rows_c64(const MvDev& J, int rb) {
-    __shared__ float2 xs[2][kGemmS][kGemmKB + 1];
+    constexpr int RW = 128, RS = 2 * kGemmKB + 2 * kPadC64;  // rows per workgroup, row stride (floats)
+    __shared__ float As[RW * RS];
+    __shared__ float2 xs[kGemmS][kGemmKB + 1];
     const int n = J.n;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int h = lane >> 5, col = lane & 31, s_of = col >> 1, part = col & 1;
-    const int row = rb * 128 + w * 32 + (lane & 31);
-    const fvec4* Ar = (const fvec4*)((const float2*)J.A + (size_t)min(row, n - 1) * n);
+    const float2* A = (const float2*)J.A;
     const float2* X = (const float2*)J.x;
     const int nkb = (n + kGemmKB - 1) / kGemmKB;
+    // global -> register staging of one K step: thread loads 8 chunks of 16 B
+    // (chunk id = it * 256 + tid: 16 chunks per row, a wave covers 4 full rows)
     for (int s0 = 0; s0 < J.S; s0 += kGemmS) {
-        auto stage = [&](int kb, int buf) {
-            const int j0 = kb * kGemmKB;
-            for (int u = threadIdx.x; u < kGemmS * kGemmKB; u += kThreads) {
-                const int s = u / kGemmKB, jj = u % kGemmKB;
-                float2 v = make_float2(0.f, 0.f);
-                if (s0 + s < J.S && j0 + jj < n) v = X[(size_t)(s0 + s) * n + j0 + jj];
-                xs[buf][s][jj] = v;
+        auto load_a = [&](int kb, fvec4 (&ra)[8]) {
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int id = it * kThreads + tid;
+                const int r = id >> 4, c16 = id & 15;
+                const int row = min(rb * RW + r, n - 1);
+                const int j = kb * kGemmKB + 2 * c16;  // first complex column of the chunk (even; n even)
+                fvec4 z = {0.f, 0.f, 0.f, 0.f};
+                ra[it] = j < n ? __builtin_nontemporal_load((const fvec4*)(A + (size_t)row * n + j)) : z;
             }
         };
-        auto load_a = [&](int kb, fvec4 (&a)[8]) {
-            const int j0 = kb * kGemmKB + 16 * h;  // this lane's first complex column (even)
-            if (kb * kGemmKB + kGemmKB <= n) {
+        auto load_x = [&](int kb, float2 (&rx)[2]) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) a[q] = __builtin_nontemporal_load(Ar + j0 / 2 + q);
-            } else {
+            for (int it = 0; it < 2; it++) {
+                const int u = it * kThreads + tid;
+                const int s = u / kGemmKB, jj = u % kGemmKB, j = kb * kGemmKB + jj;
+                rx[it] = (s0 + s < J.S && j < n) ? X[(size_t)(s0 + s) * n + j] : make_float2(0.f, 0.f);
+            }
+        };
+        auto store_lds = [&](const fvec4 (&ra)[8], const float2 (&rx)[2]) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    fvec4 z = {0.f, 0.f, 0.f, 0.f};
-                    a[q] = (j0 + 2 * q < n) ? __builtin_nontemporal_load(Ar + j0 / 2 + q) : z;
-                }
+            for (int it = 0; it < 8; it++) {
+                const int id = it * kThreads + tid;
+                *(fvec4*)(As + (id >> 4) * RS + (id & 15) * 4) = ra[it];
+            }
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                const int u = it * kThreads + tid;
+                xs[u / kGemmKB][u % kGemmKB] = rx[it];
             }
         };
         f16v acc;
 #pragma unroll
         for (int i = 0; i < 16; i++) acc[i] = 0.f;
-        fvec4 a[8];
-        load_a(0, a);
-        stage(0, 0);
-        __syncthreads();
+        fvec4 ra[8];
+        float2 rx[2];
+        load_a(0, ra);
+        load_x(0, rx);
         for (int kb = 0; kb < nkb; kb++) {
-            const int buf = kb & 1;
-            fvec4 an[8];
-            if (kb + 1 < nkb) {
-                load_a(kb + 1, an);
-                stage(kb + 1, buf ^ 1);
-            }
-#pragma unroll
-            for (int t = 0; t < 32; t++) {
-                const float2 xv = xs[buf][s_of][16 * h + (t >> 1)];
-                const float bf = xprime<float>(xv.x, xv.y, t & 1, part);
-                acc = __builtin_amdgcn_mfma_f32_32x32x2f32(a[t >> 2][t & 3], bf, acc, 0, 0, 0);
-            }
-            if (kb + 1 < nkb) {
-#pragma unroll
-                for (int q = 0; q < 8; q++) a[q] = an[q];
-            }
+            __syncthreads();  // the previous step's reads of As / xs are done
+            store_lds(ra, rx);
             __syncthreads();
+            if (kb + 1 < nkb) {
+                load_a(kb + 1, ra);
+                load_x(kb + 1, rx);
+            }
+            const float* arow = As + (w * 32 + (lane & 31)) * RS + h * 32;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const fvec4 av = *(const fvec4*)(arow + 4 * q);
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const int t = 4 * q + e;
+                    const float2 xv = xs[s_of][16 * h + (t >> 1)];
+                    acc = __builtin_amdgcn_mfma_f32_32x32x2f32(av[e], xprime<float>(xv.x, xv.y, t & 1, part), acc, 0, 0,
+                                                               0);
+                }
+            }
         }
         const int s = s0 + s_of;
 #pragma unroll
         for (int i = 0; i < 16; i++) {
-            const int r = rb * 128 + w * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
-            if (r < n && s < J.S) {
+            const int r = rb * RW + w * 32 + 8 * (i >> 2) + 4 * h + (i & 3);
+            if (r < n && s < J.S && s_of < kGemmS) {
                 const size_t o = 2 * ((size_t)s * n + r) + part;
                 float v = acc[i];
                 if (J.add_identity) v += ((const float*)J.x)[o];
@@ -236,69 +254,86 @@ __device__ inline void gemm_rows_c64(const MvDev& J, int rb) {
     }
 }
 
+constexpr int kPadC128 = 2;  // doubles of padding per row (c128: 64 + 2 doubles -> stride 132 dwords)
+
 __device__ inline void gemm_rows_c128(const MvDev& J, int rb) {
-    __shared__ double2 xs[2][kGemmS][kGemmKB + 1];
+    constexpr int RW = 64, RS = 2 * kGemmKB + kPadC128;  // rows per workgroup, row stride (doubles)
+    __shared__ double As[RW * RS];
+    __shared__ double2 xs[kGemmS][kGemmKB + 1];
     const int n = J.n;
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, col = lane & 15, s_of = col >> 1, part = col & 1;
-    const int row = rb * 64 + w * 16 + (lane & 15);
-    const dvec2* Ar = (const dvec2*)((const double2*)J.A + (size_t)min(row, n - 1) * n);
+    const double2* A = (const double2*)J.A;
     const double2* X = (const double2*)J.x;
     const int nkb = (n + kGemmKB - 1) / kGemmKB;
+    // 64 rows x 32 complex = 2048 chunks of 16 B (one complex each): thread loads 8
+    // (chunk id = it * 256 + tid: 32 chunks per row, a wave covers 2 full rows)
     for (int s0 = 0; s0 < J.S; s0 += kGemmS) {
-        auto stage = [&](int kb, int buf) {
-            const int j0 = kb * kGemmKB;
-            for (int u = threadIdx.x; u < kGemmS * kGemmKB; u += kThreads) {
-                const int s = u / kGemmKB, jj = u % kGemmKB;
-                double2 v = make_double2(0.0, 0.0);
-                if (s0 + s < J.S && j0 + jj < n) v = X[(size_t)(s0 + s) * n + j0 + jj];
-                xs[buf][s][jj] = v;
+        auto load_a = [&](int kb, dvec2 (&ra)[8]) {
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int id = it * kThreads + tid;
+                const int r = id >> 5, c = id & 31;
+                const int row = min(rb * RW + r, n - 1);
+                const int j = kb * kGemmKB + c;
+                dvec2 z = {0.0, 0.0};
+                ra[it] = j < n ? __builtin_nontemporal_load((const dvec2*)(A + (size_t)row * n + j)) : z;
             }
         };
-        auto load_a = [&](int kb, dvec2 (&a)[8]) {
-            const int j0 = kb * kGemmKB + 8 * g;  // this lane's first complex column
-            if (kb * kGemmKB + kGemmKB <= n) {
+        auto load_x = [&](int kb, double2 (&rx)[2]) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) a[q] = __builtin_nontemporal_load(Ar + j0 + q);
-            } else {
+            for (int it = 0; it < 2; it++) {
+                const int u = it * kThreads + tid;
+                const int s = u / kGemmKB, jj = u % kGemmKB, j = kb * kGemmKB + jj;
+                rx[it] = (s0 + s < J.S && j < n) ? X[(size_t)(s0 + s) * n + j] : make_double2(0.0, 0.0);
+            }
+        };
+        auto store_lds = [&](const dvec2 (&ra)[8], const double2 (&rx)[2]) {
 #pragma unroll
-                for (int q = 0; q < 8; q++) {
-                    dvec2 z = {0.0, 0.0};
-                    a[q] = (j0 + q < n) ? __builtin_nontemporal_load(Ar + j0 + q) : z;
-                }
+            for (int it = 0; it < 8; it++) {
+                const int id = it * kThreads + tid;
+                *(dvec2*)(As + (id >> 5) * RS + (id & 31) * 2) = ra[it];
+            }
+#pragma unroll
+            for (int it = 0; it < 2; it++) {
+                const int u = it * kThreads + tid;
+                xs[u / kGemmKB][u % kGemmKB] = rx[it];
             }
         };
         d4v acc0, acc1;
 #pragma unroll
         for (int i = 0; i < 4; i++) acc0[i] = acc1[i] = 0.0;
-        dvec2 a[8];
-        load_a(0, a);
-        stage(0, 0);
-        __syncthreads();
+        dvec2 ra[8];
+        double2 rx[2];
+        load_a(0, ra);
+        load_x(0, rx);
         for (int kb = 0; kb < nkb; kb++) {
-            const int buf = kb & 1;
-            dvec2 an[8];
-            if (kb + 1 < nkb) {
-                load_a(kb + 1, an);
-                stage(kb + 1, buf ^ 1);
-            }
-#pragma unroll
-            for (int t = 0; t < 16; t++) {
-                const double af = a[t >> 1][t & 1];
-                const double2 x0 = xs[buf][s_of][8 * g + (t >> 1)];
-                const double2 x1 = xs[buf][8 + s_of][8 * g + (t >> 1)];
-                acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(af, xprime<double>(x0.x, x0.y, t & 1, part), acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(af, xprime<double>(x1.x, x1.y, t & 1, part), acc1, 0, 0, 0);
-            }
-            if (kb + 1 < nkb) {
-#pragma unroll
-                for (int q = 0; q < 8; q++) a[q] = an[q];
-            }
             __syncthreads();
+            store_lds(ra, rx);
+            __syncthreads();
+            if (kb + 1 < nkb) {
+                load_a(kb + 1, ra);
+                load_x(kb + 1, rx);
+            }
+            const double* arow = As + (w * 16 + (lane & 15)) * RS + g * 16;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const dvec2 av = *(const dvec2*)(arow + 2 * q);
+#pragma unroll
+                for (int e = 0; e < 2; e++) {
+                    const int t = 2 * q + e;
+                    const double2 x0 = xs[s_of][8 * g + (t >> 1)];
+                    const double2 x1 = xs[8 + s_of][8 * g + (t >> 1)];
+                    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xprime<double>(x0.x, x0.y, t & 1, part), acc0, 0,
+                                                                0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xprime<double>(x1.x, x1.y, t & 1, part), acc1, 0,
+                                                                0, 0);
+                }
+            }
         }
 #pragma unroll
         for (int i = 0; i < 4; i++) {
-            const int r = rb * 64 + w * 16 + 4 * i + g;
+            const int r = rb * RW + w * 16 + 4 * i + g;
 #pragma unroll
             for (int c = 0; c < 2; c++) {
                 const int s = s0 + 8 * c + s_of;

@@ -27,6 +27,9 @@
 // kernel adds the chunks in a fixed order plus the identity term, so the
 // result is deterministic.
 
+#include <algorithm>
+#include <utility>
+
 #include "matfree.hpp"
 #include "translation.hpp"
 
@@ -158,7 +161,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __r
     const int nmu = 2 * lmax + 1;
     const int JB = pairs_per_batch(rmax);
     const int zs = JB * rmax;  // z / reduction stride per (rhs, polarization)
-    const double2* zpre = J.tmat ? J.part + (size_t)nchunks * J.S * n : nullptr;  // z = Tm x
+    const double2* zpre = J.zpre;  // z = Tm x (general T-matrices) or null
     PairGeo* geo = (PairGeo*)lds;
     double2* HL = (double2*)(geo + JB);
     double2* eph = HL + JB * Lp;
@@ -242,6 +245,212 @@ __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __r
     }
 }
 
+// ---- factorized operator (lmax <= 3, one lane per row particle) -------------
+//
+// With HLE_ij[p(p+1) + mu] = h_p(k|d|) P_p^|mu|(cos theta) e^{i mu phi} for
+// signed mu, every block row is a contraction over entries e = (r, kind, mu, p):
+//   y_i[a, r] = sum_j sum_{e of row r} HLE_ij[p, mu] W_j[e][a],
+//   W_j[e][a] = sum over the table terms (s, c) of e of c z_j[a', s]
+// (a' = a for A entries, 1 - a for B entries), because each q-term of
+// X_A[r,s] / X_B[r,s] is a coefficient times one HLE value.  W_j depends on
+// x but not on i, so it is formed once per matvec (O(N) work, k_matfree_w).
+// In the pair kernel a lane owns one row particle i and walks the column
+// particles j of its chunk: it forms its pair's HLE in registers (no LDS),
+// and the fully unrolled entry loop multiplies them by W_j, whose address is
+// the same for the whole wave and is read through the scalar cache; the
+// accumulators of the 2 rmax rows of y_i stay in registers.  The entry list
+// is enumerated at compile time in the order tables.cpp stores W's terms.
+
+constexpr int kFacMaxLmax = 3;
+constexpr int kFacMinN = 128;  // smaller clusters keep the LDS kernel (too few lanes)
+
+constexpr int fac_n_of(int r) {
+    int n = 1;
+    while ((n + 1) * (n + 1) - 1 <= r) n++;
+    return n;
+}
+constexpr int fac_m_of(int r) { return r - (fac_n_of(r) * fac_n_of(r) + fac_n_of(r) - 1); }
+constexpr int fac_min(int a, int b) { return a < b ? a : b; }
+constexpr int fac_abs(int a) { return a < 0 ? -a : a; }
+
+template <int L>
+constexpr bool fac_has_entry(int kind, int r, int mu, int p) {
+    const int n = fac_n_of(r), m = fac_m_of(r);
+    for (int v = 1; v <= L; v++)
+        for (int u = -v; u <= v; u++) {
+            if (u - m != mu) continue;
+            const int qa = fac_min(fac_min(n, v), (n + v - fac_abs(mu)) / 2);
+            const int qb = fac_min(fac_min(n, v), (n + v + 1 - fac_abs(mu)) / 2);
+            if (kind == 0) {
+                for (int q = 0; q <= qa; q++)
+                    if (n + v - 2 * q == p) return true;
+            } else {
+                for (int q = 1; q <= qb; q++)
+                    if (n + v - 2 * q + 1 == p) return true;
+            }
+        }
+    return false;
+}
+
+template <int L>
+struct FacEntries {
+    static constexpr int R = L * (L + 2);
+    static constexpr int MAXE = 2 * R * (4 * L + 1) * (2 * L + 1);
+    int count;
+    int r[MAXE];
+    int h[MAXE];
+    constexpr FacEntries() : count(0), r(), h() {
+        for (int rr = 0; rr < R; rr++)
+            for (int kind = 0; kind < 2; kind++)
+                for (int mu = -2 * L; mu <= 2 * L; mu++)
+                    for (int p = fac_abs(mu); p <= 2 * L; p++)
+                        if (fac_has_entry<L>(kind, rr, mu, p)) {
+                            r[count] = rr;
+                            h[count] = p * (p + 1) + mu;
+                            count++;
+                        }
+    }
+};
+template <int L>
+constexpr FacEntries<L> kFacEntries{};
+
+template <int L, int I>
+__device__ __forceinline__ void fac_step(const double2 (&hle)[(2 * L + 1) * (2 * L + 1)],
+                                        const double2* __restrict__ W, double2 (&acc)[2][L * (L + 2)]) {
+    constexpr int r = kFacEntries<L>.r[I];
+    constexpr int hi = kFacEntries<L>.h[I];
+    const double2 w0 = W[2 * I], w1 = W[2 * I + 1];
+    cfma(acc[0][r], hle[hi], w0);
+    cfma(acc[1][r], hle[hi], w1);
+}
+
+template <int L, int... Is>
+__device__ __forceinline__ void fac_entries(const double2 (&hle)[(2 * L + 1) * (2 * L + 1)],
+                                           const double2* __restrict__ W, double2 (&acc)[2][L * (L + 2)],
+                                           std::integer_sequence<int, Is...>) {
+    (fac_step<L, Is>(hle, W, acc), ...);
+}
+
+// W[s][j][e] = (w0, w1) for every RHS s, particle j and entry e.
+__global__ __launch_bounds__(kThreads) void k_matfree_w(MfJob J, const SysDev* __restrict__ sys, int ne,
+                                                        const int2* __restrict__ wmeta, const int* __restrict__ wsrc,
+                                                        const double2* __restrict__ wcoef, double2* __restrict__ W) {
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int lmax = J.lmax, rmax = lmax_to_rmax(lmax), bs = 2 * rmax;
+    const size_t n = (size_t)bs * J.N;
+    const size_t total = (size_t)J.S * J.N * ne;
+    for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kThreads) {
+        const size_t sj = idx / ne;
+        const int e = (int)(idx - sj * ne);
+        const size_t s = sj / J.N, j = sj - s * J.N;
+        const int2 m = wmeta[e];
+        const int cnt = m.y & 0xffff, kind = m.y >> 16;
+        const double2* xj = J.x + s * n + j * bs;
+        const double2* zj = J.zpre ? J.zpre + s * n + j * bs : nullptr;
+        double2 w0 = make_double2(0, 0), w1 = make_double2(0, 0);
+        for (int k = m.x; k < m.x + cnt; k++) {
+            const int sidx = wsrc[k];
+            const int v = (int)sqrtf((float)(sidx + 1) + 0.5f);
+            // z_j[b, s] = mie_j[b, v] x_j[b, s] (spheres) or (Tm_j x_j)[b, s]
+            const double2 z0 = zj ? zj[sidx] : cmul(J.mie[(j * 2 + 0) * lmax + v - 1], xj[sidx]);
+            const double2 z1 = zj ? zj[rmax + sidx] : cmul(J.mie[(j * 2 + 1) * lmax + v - 1], xj[rmax + sidx]);
+            cfma(w0, wcoef[k], kind ? z1 : z0);
+            cfma(w1, wcoef[k], kind ? z0 : z1);
+        }
+        W[2 * idx] = w0;
+        W[2 * idx + 1] = w1;
+    }
+}
+
+// Partial rows part[chunk][s][row] of y = T x; blockIdx.y is the RHS.
+template <int L>
+__global__ __launch_bounds__(kThreads) void k_matfree_fac(MfJob J, const SysDev* __restrict__ sys, int nchunks,
+                                                          int chunk, const double2* __restrict__ Wall) {
+    constexpr int R = L * (L + 2);
+    constexpr int PM = 2 * L;
+    constexpr int NE = kFacEntries<L>.count;
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int N = J.N;
+    const int s = blockIdx.y;
+    const int c = blockIdx.x % nchunks;
+    const int i = (blockIdx.x / nchunks) * kThreads + threadIdx.x;
+    const bool live = i < N;
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const double2* W = Wall + (size_t)s * N * NE * 2;
+    double2 acc[2][R];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int q = 0; q < R; q++) acc[a][q] = make_double2(0, 0);
+    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
+    for (int j = jbeg; j < jend; j++) {
+        if (j == ii) continue;
+        const double dx = xi - J.pos[3 * j], dy = yi - J.pos[3 * j + 1], dz = zi - J.pos[3 * j + 2];
+        const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+        const double ct = dz / rad, st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
+        const double zz = J.k * rad, rz = 1.0 / zz;
+        double sz, cz;
+        sincos(zz, &sz, &cz);
+        const double rho = sqrt(dx * dx + dy * dy);
+        const double2 e1 = rho > 0 ? make_double2(dx / rho, dy / rho) : make_double2(1.0, 0.0);
+        double2 h[PM + 1];
+        h[0] = make_double2(sz * rz, -cz * rz);
+        h[1] = make_double2(h[0].x * rz + h[0].y, h[0].y * rz - h[0].x);
+#pragma unroll
+        for (int l = 2; l <= PM; l++)
+            h[l] = make_double2((2 * l - 1) * rz * h[l - 1].x - h[l - 2].x, (2 * l - 1) * rz * h[l - 1].y - h[l - 2].y);
+        double2 e[PM + 1];
+        e[0] = make_double2(1.0, 0.0);
+#pragma unroll
+        for (int m = 1; m <= PM; m++) e[m] = cmul(e[m - 1], e1);
+        double2 hle[(2 * L + 1) * (2 * L + 1)];
+#pragma unroll
+        for (int m = 0; m <= PM; m++) {
+            double pmm = 1.0;
+#pragma unroll
+            for (int q = 1; q <= PM; q++)
+                if (q <= m) pmm *= (2 * q - 1) * st;
+            double p2 = 0.0, p1 = 0.0;
+#pragma unroll
+            for (int l = 0; l <= PM; l++) {
+                if (l < m) continue;
+                const double P = (l == m)       ? pmm
+                                 : (l == m + 1) ? ct * (2 * m + 1) * pmm
+                                                : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) / (l - m);
+                p2 = p1;
+                p1 = P;
+                const double2 hp = cscale(h[l], P);
+                hle[l * (l + 1) + m] = cmul(hp, e[m]);
+                if (m > 0) hle[l * (l + 1) - m] = cmul(hp, cconj(e[m]));
+            }
+        }
+        fac_entries<L>(hle, W + (size_t)j * NE * 2, acc, std::make_integer_sequence<int, NE>{});
+    }
+    if (live) {
+        const size_t n = (size_t)2 * R * N;
+        double2* out = J.part + ((size_t)c * J.S + s) * n + (size_t)i * 2 * R;
+#pragma unroll
+        for (int a = 0; a < 2; a++)
+#pragma unroll
+            for (int q = 0; q < R; q++) out[a * R + q] = acc[a][q];
+    }
+}
+
+inline bool fac_path(int N, int lmax) { return lmax <= kFacMaxLmax && N >= kFacMinN; }
+
+// Column chunks of the factorized kernel: about 512 workgroups in total.
+inline int fac_chunks(int N) {
+    const int groups = (N + kThreads - 1) / kThreads;
+    int c = (512 + groups - 1) / groups;
+    if (c > N) c = N;
+    return c < 1 ? 1 : c;
+}
+
+inline int fac_entries_of(int lmax) {
+    return lmax == 1 ? kFacEntries<1>.count : lmax == 2 ? kFacEntries<2>.count : kFacEntries<3>.count;
+}
+
 // y = (add_identity ? x : 0) + sum of the chunk partials in chunk order
 __global__ __launch_bounds__(kThreads) void k_matfree_sum(MfJob J, const SysDev* __restrict__ sys, int nchunks) {
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
@@ -255,8 +464,15 @@ __global__ __launch_bounds__(kThreads) void k_matfree_sum(MfJob J, const SysDev*
 
 }  // namespace
 
+// Workspace: partial rows [slots][S][n], then z = Tm x [S][n], then (the
+// factorized path) W [S][N][entries][2].
+inline int partial_slots(int N) { return std::max(chunks_for(N), fac_chunks(N)); }
+
 size_t matfree_workspace_bytes(int N, int lmax, int S) {
-    return sizeof(double2) * (size_t)2 * lmax_to_rmax(lmax) * N * S * (chunks_for(N) + 1);
+    const size_t n = (size_t)2 * lmax_to_rmax(lmax) * N;
+    size_t b = sizeof(double2) * n * S * (partial_slots(N) + 1);
+    if (fac_path(N, lmax)) b += sizeof(double2) * 2 * (size_t)S * N * fac_entries_of(lmax);
+    return b;
 }
 
 }  // namespace mpa
@@ -266,20 +482,45 @@ int mpa_internal_tmatrix_apply(int N, int lmax, int nrhs, const double* tmatrix,
 
 namespace mpa {
 
-int matfree_launch(const MfJob& J, const SysDev* sys, hipStream_t st) {
-    const TransTable* T = mpa_get_table(J.lmax);
+int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
+    const TransTable* T = mpa_get_table(J0.lmax);
     if (!T) return MPA_ERR_ARG;
+    MfJob J = J0;
+    const size_t n = (size_t)2 * lmax_to_rmax(J.lmax) * J.N;
+    const size_t total = (size_t)J.S * n;
+    const unsigned sum_blocks = (unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads);
+    double2* zbuf = J.part + (size_t)partial_slots(J.N) * J.S * n;
+    J.zpre = nullptr;
+    if (J.tmat) {
+        const int rc = mpa_internal_tmatrix_apply(J.N, J.lmax, J.S, (const double*)J.tmat, J.x, zbuf, st);
+        if (rc) return rc;
+        J.zpre = zbuf;
+    }
+    if (fac_path(J.N, J.lmax)) {
+        const int ne = fac_entries_of(J.lmax);
+        if (ne != T->nentries) {
+            mpa_set_error("matfree: contraction table has %d entries, the kernel %d", T->nentries, ne);
+            return MPA_ERR_ARG;
+        }
+        double2* W = zbuf + total;
+        const int nch = fac_chunks(J.N);
+        const int chunk = (J.N + nch - 1) / nch;
+        const int groups = (J.N + kThreads - 1) / kThreads;
+        const size_t wtot = (size_t)J.S * J.N * ne;
+        ProfScope prof(4, st);
+        hipLaunchKernelGGL(k_matfree_w, dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)),
+                           dim3(kThreads), 0, st, J, sys, ne, T->wmeta, T->wsrc, T->wcoef, W);
+        const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
+        if (J.lmax == 1) hipLaunchKernelGGL(k_matfree_fac<1>, grid, block, 0, st, J, sys, nch, chunk, W);
+        else if (J.lmax == 2) hipLaunchKernelGGL(k_matfree_fac<2>, grid, block, 0, st, J, sys, nch, chunk, W);
+        else hipLaunchKernelGGL(k_matfree_fac<3>, grid, block, 0, st, J, sys, nch, chunk, W);
+        hipLaunchKernelGGL(k_matfree_sum, dim3(sum_blocks), dim3(kThreads), 0, st, J, sys, nch);
+        MPA_LAUNCH_CHECK();
+        return MPA_OK;
+    }
     const int nchunks = chunks_for(J.N);
     const int chunk = (J.N + nchunks - 1) / nchunks;
     const long long nblocks = (long long)J.N * nchunks;
-    const size_t total = (size_t)J.S * 2 * lmax_to_rmax(J.lmax) * J.N;
-    const unsigned sum_blocks = (unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads);
-    if (J.tmat) {
-        const size_t n = (size_t)2 * lmax_to_rmax(J.lmax) * J.N;
-        double2* z = J.part + (size_t)nchunks * J.S * n;
-        const int rc = mpa_internal_tmatrix_apply(J.N, J.lmax, J.S, (const double*)J.tmat, J.x, z, st);
-        if (rc) return rc;
-    }
     const TabDev tab{T->cmeta, T->ccoef, T->clidx};
     const int SG = sg_for(J.S);
     const size_t lds = lds_bytes(J.lmax, SG, T->nterms);
@@ -319,6 +560,6 @@ extern "C" int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int 
         return MPA_ERR_ARG;
     }
     MfJob J{op->N, op->lmax, nrhs, add_identity ? 1 : 0, op->positions, (const double2*)op->mie, op->k,
-            (const double2*)x, (double2*)y, (double2*)op->workspace, -1, 0, (const double2*)op->tmatrix};
+            (const double2*)x, (double2*)y, (double2*)op->workspace, -1, 0, (const double2*)op->tmatrix, nullptr};
     return matfree_launch(J, nullptr, (hipStream_t)stream);
 }

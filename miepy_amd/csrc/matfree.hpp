@@ -18,6 +18,7 @@ struct MfJob {
     int skip;            // SysDev index whose all-done flag skips the job, or -1
     int pad;
     const double2* tmat;  // [N, 2 rmax, 2 rmax] per-particle T-matrices, or null (spheres: mie)
+    const double2* zpre;  // z = Tm x when tmat is set (filled by matfree_launch), else null
 };
 
 // Workspace bytes of a job (the per-chunk partial rows and the z buffer).

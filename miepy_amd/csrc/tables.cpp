@@ -155,6 +155,40 @@ TransTable* build_table(int lmax) {
             clidx.push_back((unsigned short)leg_index(mt.w - 2 * q + 1, amu));
         }
     }
+    // contraction form for the factorized matrix-free operator: entries
+    // e = (r, kind, mu, p) in the canonical order (r, kind A then B, mu from
+    // -2 lmax, p ascending; matfree.hip enumerates the same order at compile
+    // time), each with the (s, coefficient) terms that map to it:
+    // X_kind[r,s] = sum over those terms of coef * h_p P_p^|mu| e^{i mu phi}.
+    std::vector<int2> wmeta;
+    std::vector<int> wsrc;
+    std::vector<double2> wcoef;
+    for (int r = 0; r < rmax; r++)
+        for (int kind = 0; kind < 2; kind++)
+            for (int mu = -2 * lmax; mu <= 2 * lmax; mu++)
+                for (int p = std::abs(mu); p <= 2 * lmax; p++) {
+                    const int start = (int)wsrc.size();
+                    for (int sidx = 0; sidx < rmax; sidx++) {
+                        const int t = r * rmax + sidx;
+                        const int4 mt = meta[t];
+                        if (mt.z != mu) continue;
+                        if (kind == 0) {
+                            for (int q = 0; q < mt.x; q++)
+                                if (mt.w - 2 * q == p) {
+                                    wsrc.push_back(sidx);
+                                    wcoef.push_back(coef[(size_t)t * coef_slots + q]);
+                                }
+                        } else {
+                            for (int q = 1; q <= mt.y; q++)
+                                if (mt.w - 2 * q + 1 == p) {
+                                    wsrc.push_back(sidx);
+                                    wcoef.push_back(coef[(size_t)t * coef_slots + qa_slots + q - 1]);
+                                }
+                        }
+                    }
+                    const int cnt = (int)wsrc.size() - start;
+                    if (cnt) wmeta.push_back(make_int2(start, cnt | (kind << 16)));
+                }
     TransTable* T = new TransTable();
     T->lmax = lmax;
     T->rmax = rmax;
@@ -171,6 +205,17 @@ TransTable* build_table(int lmax) {
         hipMalloc(&T->ccoef, ccoef.size() * sizeof(double2)) != hipSuccess ||
         hipMalloc(&T->clidx, clidx.size() * sizeof(unsigned short)) != hipSuccess) {
         mpa_set_error("hipMalloc for the lmax=%d translation table failed", lmax);
+        delete T;
+        return nullptr;
+    }
+    T->nentries = (int)wmeta.size();
+    if (hipMalloc(&T->wmeta, wmeta.size() * sizeof(int2)) != hipSuccess ||
+        hipMalloc(&T->wsrc, wsrc.size() * sizeof(int)) != hipSuccess ||
+        hipMalloc(&T->wcoef, wcoef.size() * sizeof(double2)) != hipSuccess ||
+        hipMemcpy(T->wmeta, wmeta.data(), wmeta.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(T->wsrc, wsrc.data(), wsrc.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(T->wcoef, wcoef.data(), wcoef.size() * sizeof(double2), hipMemcpyHostToDevice) != hipSuccess) {
+        mpa_set_error("contraction table for lmax=%d failed", lmax);
         delete T;
         return nullptr;
     }

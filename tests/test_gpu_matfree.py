@@ -160,3 +160,37 @@ def test_matfree_bad_arguments(dev):
     assert lib.mpa_matfree_workspace_bytes(10, 7, 1) == 0
     op = L.MpaClusterOperator(4, 9, None, None, 1.0, None)
     assert lib.mpa_matfree_matvec(ctypes.byref(op), 1, 0, None, None, None) == L.MPA_ERR_ARG
+
+
+@pytest.mark.parametrize("N,lmax,S", [(300, 1, 1), (200, 2, 1), (140, 3, 1), (150, 3, 2)])
+def test_factorized_operator_matches_dense(dev, N, lmax, S):
+    """Clusters of 128+ particles with lmax <= 3 take the factorized kernel
+    (one lane per row particle, per-particle contraction W_j): equal to the
+    stored matrix's product to 1e-12, spheres and general T-matrices."""
+    pos, mie = _cluster(N, lmax, seed=200 + N + lmax, spacing=300 * nm)
+    k = 2 * np.pi / (600 * nm)
+    n = 2 * lmax * (lmax + 2) * N
+    rng = np.random.default_rng(N)
+    X = dev.to_device(rng.normal(size=(S, n)) + 1j * rng.normal(size=(S, n)), dev.C128)
+    A = dev.assemble(pos, mie, k, add_identity=True)
+    ref = dev.matvec(A, X)
+    got = dev.matfree_matvec(dev.ClusterOperator(pos, mie, k, max_rhs=S), X, add_identity=True)
+    err = (got - ref).abs().max().item() / ref.abs().max().item()
+    assert err <= 1e-12, err
+    rmax = lmax * (lmax + 2)
+    tm = 0.05 * (rng.normal(size=(N, 2 * rmax, 2 * rmax)) + 1j * rng.normal(size=(N, 2 * rmax, 2 * rmax)))
+    At = dev.assemble_tmatrix(pos, tm.reshape(N, 2, rmax, 2, rmax), k, add_identity=True)
+    ref = dev.matvec(At, X)
+    got = dev.matfree_matvec(dev.ClusterOperator(pos, None, k, max_rhs=S, tmatrix=tm), X, add_identity=True)
+    err = (got - ref).abs().max().item() / ref.abs().max().item()
+    assert err <= 1e-12, err
+
+
+def test_factorized_operator_matches_oracle(dev, oracle):
+    N, lmax = 130, 2
+    pos, mie = _cluster(N, lmax, seed=9, spacing=300 * nm)
+    k = 2 * np.pi / (600 * nm)
+    T = oracle.sphere_aggregate_tmatrix(pos, mie, k)
+    x = np.random.default_rng(4).normal(size=T.shape[0]) + 0j
+    got = dev.matfree_matvec(dev.ClusterOperator(pos, mie, k), dev.to_device(x.reshape(1, -1), dev.C128))
+    assert _rel_err(got.cpu().numpy()[0], oracle.gemv(T, x)) <= 1e-12
