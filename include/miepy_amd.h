@@ -148,6 +148,13 @@ size_t mpa_matfree_workspace_bytes(int N, int lmax, int nrhs);
 int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,
                        void* stream);
 
+/* The same product in complex64 on the FP32 matrix cores (translation
+ * special functions in FP64, products and sums in FP32), x and y [nrhs, n]
+ * c64: the operator of mixed-precision inner solves without a stored
+ * matrix.  Spheres with lmax <= 3. */
+int mpa_matfree_matvec_c64(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,
+                           void* stream);
+
 /* BiCGSTAB (as mpa_bicgstab, FP64) on (add_identity ? I + T : T) x = b with
  * the matrix-free operator; workspace: mpa_bicgstab_workspace_bytes(n,
  * nrhs, MPA_C128) bytes, n = 2 lmax (lmax + 2) N; op->workspace sized for
@@ -209,6 +216,9 @@ typedef struct {
     void* state;
     double tol, inner_tol;
     int maxiter, max_refine;
+    const mpa_cluster_operator* op; /* host pointer or NULL: with op set, A32 and A64 are NULL, the
+                                       corrections use the complex64 matrix-core operator and the
+                                       residual the FP64 one (no matrix is stored; spheres, lmax <= 3) */
 } mpa_refine_system;
 
 size_t mpa_refine_state_bytes(int n, int nrhs);

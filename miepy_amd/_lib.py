@@ -94,7 +94,7 @@ class MpaRefineSystem(ctypes.Structure):
     _fields_ = [("n", ctypes.c_int), ("nrhs", ctypes.c_int), ("add_identity", ctypes.c_int), ("pad", ctypes.c_int),
                 ("A32", ctypes.c_void_p), ("A64", ctypes.c_void_p), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
                 ("state", ctypes.c_void_p), ("tol", ctypes.c_double), ("inner_tol", ctypes.c_double),
-                ("maxiter", ctypes.c_int), ("max_refine", ctypes.c_int)]
+                ("maxiter", ctypes.c_int), ("max_refine", ctypes.c_int), ("op", ctypes.c_void_p)]
 
 
 _ps = ctypes.POINTER(MpaSystem)
@@ -105,6 +105,7 @@ SIGNATURES.update({
     "mpa_tmatrix_apply": (_i, [_i, _i, _i, _v, _v, _v, _v]),
     "mpa_matfree_workspace_bytes": (_sz, [_i, _i, _i]),
     "mpa_matfree_matvec": (_i, [_pop, _i, _i, _v, _v, _v]),
+    "mpa_matfree_matvec_c64": (_i, [_pop, _i, _i, _v, _v, _v]),
     "mpa_bicgstab_matfree": (_i, [_pop, _i, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_system_state_bytes": (_sz, [_i, _i, _i]),
     "mpa_batch_scratch_bytes": (_sz, [_i, _ps, _i]),

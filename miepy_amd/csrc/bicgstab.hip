@@ -344,7 +344,8 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
             const mpa_cluster_operator& o = *q.op;
             if (!mpa_get_table(o.lmax)) return MPA_ERR_ARG;  // built here, never inside a graph capture
             const MfJob base{o.N, o.lmax, S, q.add_identity ? 1 : 0, o.positions, (const double2*)o.mie, o.k,
-                             nullptr, nullptr, (double2*)o.workspace, i, 0, (const double2*)o.tmatrix, nullptr};
+                             nullptr, nullptr, (double2*)o.workspace, i, 0, (const double2*)o.tmatrix, nullptr,
+                             dtype == MPA_C64 ? 1 : 0, 0};
             MfJob j = base;
             j.x = (const double2*)d.p;
             j.y = (double2*)d.v;
@@ -564,10 +565,11 @@ bool valid_systems(int nsys, const mpa_system* s, int dtype) {
             return false;
         if (s[i].op) {
             const mpa_cluster_operator& o = *s[i].op;
-            if (s[i].A || dtype != MPA_C128 || o.N < 1 || o.lmax < 1 || o.lmax > MPA_LMAX_MAX || !o.positions ||
+            if (s[i].A || o.N < 1 || o.lmax < 1 || o.lmax > MPA_LMAX_MAX || !o.positions ||
                 !o.mie == !o.tmatrix || !o.workspace || !(o.k > 0) ||
                 (long long)2 * lmax_to_rmax(o.lmax) * o.N != s[i].n)
                 return false;
+            if (dtype == MPA_C64 && !matfree_c64_supported(o.N, o.lmax, o.tmatrix != nullptr)) return false;
         } else if (!s[i].A) {
             return false;
         }

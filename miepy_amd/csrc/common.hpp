@@ -99,6 +99,8 @@ struct TransTable {
     int2* wmeta = nullptr;     // device, [nentries] {first term, count | kind << 16}
     int* wsrc = nullptr;       // device, column mode s of each term
     double2* wcoef = nullptr;  // device, coefficient of each term
+    int nh = 0;                // HLE values per pair, (2 lmax + 1)^2
+    int2* hmap = nullptr;      // device, [rmax][nh]: the A and B entry of (r, h), or -1
 };
 
 // Returns the (cached, per device) table for lmax; nullptr on failure.

@@ -261,6 +261,15 @@ __global__ __launch_bounds__(kThreads) void k_matfree(MfJob J, const SysDev* __r
 // accumulators of the 2 rmax rows of y_i stay in registers.  The entry list
 // is enumerated at compile time in the order tables.cpp stores W's terms.
 
+typedef float f16v __attribute__((ext_vector_type(16)));
+
+// element of the real expansion X' of a complex B operand (see gemv.hip) for
+// K parity `odd` (the Im part of the A value) and output part `part`
+template <typename Rt>
+__device__ inline Rt xprime(Rt re, Rt im, int odd, int part) {
+    return odd ? (part ? re : -im) : (part ? im : re);
+}
+
 constexpr int kFacMaxLmax = 3;
 constexpr int kFacMinN = 128;  // smaller clusters keep the LDS kernel (too few lanes)
 
@@ -439,6 +448,198 @@ __global__ __launch_bounds__(kThreads) void k_matfree_fac(MfJob J, const SysDev*
 
 inline bool fac_path(int N, int lmax) { return lmax <= kFacMaxLmax && N >= kFacMinN; }
 
+// ---- complex64 operator on the FP32 matrix cores (mixed-precision inner solves)
+//
+// The same contraction as the factorized FP64 operator, grouped by HLE index:
+//   y_i[a,r] = sum_j sum_h HLE_ij[h] Wm_j[h][a,r],  Wm_j[h][a,r] = sum of W_j over
+// the (at most two: A and B) entries of (r, h),
+// i.e. per column particle j a complex product [rows i] x [nh] times [nh] x [2 rmax].
+// With the real expansion used by the matvec GEMM (a complex HLE becomes the
+// two K positions (Re, Im); Wm the matching B columns X'), each j is nh
+// v_mfma_f32_32x32x2f32 steps per 16 complex output columns for 32 row
+// particles.  Lanes l and l + 32 of a wave own the same row particle and
+// supply its Re and Im HLE values (computed in FP64 by both, rounded to FP32);
+// Wm_j (FP32, formed once per matvec) is staged in LDS per workgroup and
+// read as the B operand; the accumulators stay in registers across the
+// workgroup's chunk of column particles.
+
+// Wm[s][j][h][cc], cc = a rmax + r: thread per entry.
+__global__ __launch_bounds__(kThreads) void k_matfree_w32(MfJob J, const SysDev* __restrict__ sys, int nh,
+                                                          const int2* __restrict__ hmap,
+                                                          const int2* __restrict__ wmeta, const int* __restrict__ wsrc,
+                                                          const double2* __restrict__ wcoef, float2* __restrict__ Wm) {
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int lmax = J.lmax, rmax = lmax_to_rmax(lmax), bs = 2 * rmax;
+    const size_t n = (size_t)bs * J.N;
+    const size_t total = (size_t)J.S * J.N * nh * bs;
+    const float2* X = (const float2*)J.x;
+    for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kThreads) {
+        const int cc = (int)(idx % bs);
+        const size_t rest = idx / bs;
+        const int h = (int)(rest % nh);
+        const size_t sj = rest / nh;
+        const size_t s = sj / J.N, j = sj - s * J.N;
+        const int a = cc >= rmax ? 1 : 0, r = cc - a * rmax;
+        const int2 es = hmap[(size_t)r * nh + h];
+        const float2* xj = X + s * n + j * bs;
+        double2 w = make_double2(0, 0);
+        for (int kind = 0; kind < 2; kind++) {
+            const int e = kind ? es.y : es.x;
+            if (e < 0) continue;
+            const int2 m = wmeta[e];
+            const int cnt = m.y & 0xffff;
+            const int src_pol = kind ? 1 - a : a;  // B entries couple the other polarization
+            for (int k = m.x; k < m.x + cnt; k++) {
+                const int sidx = wsrc[k];
+                const int v = (int)sqrtf((float)(sidx + 1) + 0.5f);
+                const float2 xv = xj[src_pol * rmax + sidx];
+                const double2 z = cmul(J.mie[(j * 2 + src_pol) * lmax + v - 1], make_double2(xv.x, xv.y));
+                cfma(w, wcoef[k], z);
+            }
+        }
+        Wm[idx] = make_float2((float)w.x, (float)w.y);
+    }
+}
+
+// Partial rows part32[chunk][s][row] (complex64); blockIdx.y is the RHS.
+template <int L>
+__global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev* __restrict__ sys, int nchunks,
+                                                           int chunk, const float2* __restrict__ Wm,
+                                                           float2* __restrict__ part32) {
+    constexpr int R = L * (L + 2);
+    constexpr int NC = 2 * R;              // complex output columns per particle
+    constexpr int NT = (NC + 15) / 16;     // 32-column MFMA tiles
+    constexpr int PM = 2 * L;
+    constexpr int NH = (2 * L + 1) * (2 * L + 1);
+    __shared__ float2 Ws[2][NH][NT * 16 + 1];
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int N = J.N;
+    const int s = blockIdx.y;
+    const int c = blockIdx.x % nchunks;
+    const int group = blockIdx.x / nchunks;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int kh = lane >> 5, col = lane & 31, part = col & 1;
+    const int i = group * 128 + w * 32 + (lane & 31);
+    const bool live = i < N;
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const float2* Wall = Wm + (size_t)s * N * NH * NC;
+    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
+    f16v acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) acc[t][q] = 0.f;
+    auto stage = [&](int j, int buf) {
+        const float2* src = Wall + (size_t)j * NH * NC;
+        for (int u = tid; u < NH * NT * 16; u += kThreads) {
+            const int h = u / (NT * 16), cc = u - h * (NT * 16);
+            Ws[buf][h][cc] = cc < NC ? src[h * NC + cc] : make_float2(0.f, 0.f);
+        }
+    };
+    if (jbeg < jend) stage(jbeg, 0);
+    __syncthreads();
+    for (int j = jbeg; j < jend; j++) {
+        const int buf = (j - jbeg) & 1;
+        if (j + 1 < jend) stage(j + 1, buf ^ 1);
+        // this lane's half (Re for kh = 0, Im for kh = 1) of the pair's HLE values
+        float af[NH];
+        {
+            const double dx = xi - J.pos[3 * j], dy = yi - J.pos[3 * j + 1], dz = zi - J.pos[3 * j + 2];
+            const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ct = dz / rad, st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
+            const double zz = J.k * rad, rz = 1.0 / zz;
+            double sz, cz;
+            sincos(zz, &sz, &cz);
+            const double rho = sqrt(dx * dx + dy * dy);
+            const double2 e1 = rho > 0 ? make_double2(dx / rho, dy / rho) : make_double2(1.0, 0.0);
+            double2 hh[PM + 1];
+            hh[0] = make_double2(sz * rz, -cz * rz);
+            hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
+#pragma unroll
+            for (int l = 2; l <= PM; l++)
+                hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
+                                     (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
+            double2 e = make_double2(1.0, 0.0);
+            const bool on = live && j != ii;
+#pragma unroll
+            for (int m = 0; m <= PM; m++) {
+                if (m > 0) e = cmul(e, e1);
+                double pmm = 1.0;
+#pragma unroll
+                for (int q = 1; q <= PM; q++)
+                    if (q <= m) pmm *= (2 * q - 1) * st;
+                double p2 = 0.0, p1 = 0.0;
+#pragma unroll
+                for (int l = 0; l <= PM; l++) {
+                    if (l < m) continue;
+                    const double P = (l == m)       ? pmm
+                                     : (l == m + 1) ? ct * (2 * m + 1) * pmm
+                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) / (l - m);
+                    p2 = p1;
+                    p1 = P;
+                    const double2 hp = cscale(hh[l], P);
+                    const double2 vp = cmul(hp, e);
+                    af[l * (l + 1) + m] = on ? (float)(kh ? vp.y : vp.x) : 0.f;
+                    if (m > 0) {
+                        const double2 vm = cmul(hp, cconj(e));
+                        af[l * (l + 1) - m] = on ? (float)(kh ? vm.y : vm.x) : 0.f;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NH; t++)
+#pragma unroll
+            for (int tile = 0; tile < NT; tile++) {
+                const float2 wv = Ws[buf][t][tile * 16 + (col >> 1)];
+                acc[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[t], xprime<float>(wv.x, wv.y, kh, part),
+                                                                 acc[tile], 0, 0, 0);
+            }
+        __syncthreads();
+    }
+    const size_t n = (size_t)NC * N;
+#pragma unroll
+    for (int tile = 0; tile < NT; tile++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int ip = group * 128 + w * 32 + 8 * (q >> 2) + 4 * kh + (q & 3);
+            const int cc = tile * 16 + (col >> 1);
+            if (ip < N && cc < NC)
+                ((float*)part32)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = acc[tile][q];
+        }
+}
+
+// y32 = (add_identity ? x32 : 0) + sum of the complex64 chunk partials in chunk order (FP64 sum)
+__global__ __launch_bounds__(kThreads) void k_matfree_sum32(MfJob J, const SysDev* __restrict__ sys, int nchunks,
+                                                            const float2* __restrict__ part32) {
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const size_t total = (size_t)J.S * 2 * lmax_to_rmax(J.lmax) * J.N;
+    const float2* X = (const float2*)J.x;
+    float2* Y = (float2*)J.y;
+    for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kThreads) {
+        double ax = 0, ay = 0;
+        if (J.add_identity) {
+            ax = X[idx].x;
+            ay = X[idx].y;
+        }
+        for (int cc = 0; cc < nchunks; cc++) {
+            const float2 v = part32[(size_t)cc * total + idx];
+            ax += v.x;
+            ay += v.y;
+        }
+        Y[idx] = make_float2((float)ax, (float)ay);
+    }
+}
+
+// Column chunks of the complex64 kernel: about 1024 workgroups in total.
+inline int mf32_chunks(int N) {
+    const int groups = (N + 127) / 128;
+    int c = (1024 + groups - 1) / groups;
+    if (c > N) c = N;
+    return c < 1 ? 1 : c;
+}
+
 // Column chunks of the factorized kernel: about 512 workgroups in total.
 inline int fac_chunks(int N) {
     const int groups = (N + kThreads - 1) / kThreads;
@@ -472,7 +673,41 @@ size_t matfree_workspace_bytes(int N, int lmax, int S) {
     const size_t n = (size_t)2 * lmax_to_rmax(lmax) * N;
     size_t b = sizeof(double2) * n * S * (partial_slots(N) + 1);
     if (fac_path(N, lmax)) b += sizeof(double2) * 2 * (size_t)S * N * fac_entries_of(lmax);
+    if (matfree_c64_supported(N, lmax, false)) {  // the complex64 layout: partials, then Wm
+        const int nh = (2 * lmax + 1) * (2 * lmax + 1);
+        const size_t b32 = sizeof(float2) * (n * S * mf32_chunks(N) + (size_t)S * N * nh * n / N);
+        b = std::max(b, b32);
+    }
     return b;
+}
+
+bool matfree_c64_supported(int N, int lmax, bool tmatrix) { return lmax <= kFacMaxLmax && N >= 1 && !tmatrix; }
+
+int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const TransTable* T) {
+    if (!matfree_c64_supported(J.N, J.lmax, J.tmat != nullptr)) {
+        mpa_set_error("matfree: the complex64 operator covers spheres with lmax <= %d", kFacMaxLmax);
+        return MPA_ERR_ARG;
+    }
+    const int nh = T->nh;
+    const size_t n = (size_t)2 * lmax_to_rmax(J.lmax) * J.N;
+    const size_t total = (size_t)J.S * n;
+    const int nch = mf32_chunks(J.N);
+    const int chunk = (J.N + nch - 1) / nch;
+    float2* part32 = (float2*)J.part;
+    float2* Wm = part32 + (size_t)nch * total;
+    const size_t wtot = (size_t)J.S * J.N * nh * (n / J.N);
+    const int groups = (J.N + 127) / 128;
+    ProfScope prof(5, st);
+    hipLaunchKernelGGL(k_matfree_w32, dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc, T->wcoef, Wm);
+    const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
+    if (J.lmax == 1) hipLaunchKernelGGL(k_matfree_mf32<1>, grid, block, 0, st, J, sys, nch, chunk, Wm, part32);
+    else if (J.lmax == 2) hipLaunchKernelGGL(k_matfree_mf32<2>, grid, block, 0, st, J, sys, nch, chunk, Wm, part32);
+    else hipLaunchKernelGGL(k_matfree_mf32<3>, grid, block, 0, st, J, sys, nch, chunk, Wm, part32);
+    hipLaunchKernelGGL(k_matfree_sum32, dim3((unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads)),
+                       dim3(kThreads), 0, st, J, sys, nch, part32);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
 }
 
 }  // namespace mpa
@@ -485,6 +720,7 @@ namespace mpa {
 int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
     const TransTable* T = mpa_get_table(J0.lmax);
     if (!T) return MPA_ERR_ARG;
+    if (J0.c64) return matfree_launch_c64(J0, sys, st, T);
     MfJob J = J0;
     const size_t n = (size_t)2 * lmax_to_rmax(J.lmax) * J.N;
     const size_t total = (size_t)J.S * n;
@@ -545,6 +781,18 @@ using namespace mpa;
 extern "C" size_t mpa_matfree_workspace_bytes(int N, int lmax, int nrhs) {
     if (N < 1 || nrhs < 1 || lmax < 1 || lmax > MPA_LMAX_MAX) return 0;
     return matfree_workspace_bytes(N, lmax, nrhs);
+}
+
+extern "C" int mpa_matfree_matvec_c64(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x,
+                                      void* y, void* stream) {
+    if (!op || op->N < 1 || nrhs < 1 || !op->positions || !op->mie || op->tmatrix || !op->workspace || !x || !y ||
+        !(op->k > 0) || op->lmax < 1 || op->lmax > MPA_LMAX_MAX) {
+        mpa_set_error("matfree_matvec_c64: bad argument (spheres only)");
+        return MPA_ERR_ARG;
+    }
+    MfJob J{op->N, op->lmax, nrhs, add_identity ? 1 : 0, op->positions, (const double2*)op->mie, op->k,
+            (const double2*)x, (double2*)y, (double2*)op->workspace, -1, 0, nullptr, nullptr, 1, 0};
+    return matfree_launch(J, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,

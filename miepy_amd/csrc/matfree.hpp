@@ -19,7 +19,12 @@ struct MfJob {
     int pad;
     const double2* tmat;  // [N, 2 rmax, 2 rmax] per-particle T-matrices, or null (spheres: mie)
     const double2* zpre;  // z = Tm x when tmat is set (filled by matfree_launch), else null
+    int c64;              // 1: x, y are complex64 and the product runs on the FP32 matrix cores
+    int pad2;
 };
+
+// Whether the complex64 matrix-core operator covers (N, lmax, T-matrices).
+bool matfree_c64_supported(int N, int lmax, bool tmatrix);
 
 // Workspace bytes of a job (the per-chunk partial rows and the z buffer).
 size_t matfree_workspace_bytes(int N, int lmax, int S);

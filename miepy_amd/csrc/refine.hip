@@ -15,6 +15,7 @@
 
 #include "gemv.hpp"
 #include "krylov.hpp"
+#include "matfree.hpp"
 
 using namespace mpa;
 
@@ -123,10 +124,20 @@ size_t own_bytes(int n, int S) {
 
 bool valid(int nsys, const mpa_refine_system* s) {
     if (nsys < 1 || !s) return false;
-    for (int i = 0; i < nsys; i++)
-        if (s[i].n < 2 || (s[i].n & 1) || s[i].nrhs < 1 || !s[i].A32 || !s[i].A64 || !s[i].b || !s[i].x || !s[i].state ||
+    for (int i = 0; i < nsys; i++) {
+        if (s[i].n < 2 || (s[i].n & 1) || s[i].nrhs < 1 || !s[i].b || !s[i].x || !s[i].state ||
             !(s[i].tol > 0) || !(s[i].inner_tol > 0) || s[i].maxiter < 1 || s[i].max_refine < 1)
             return false;
+        if (s[i].op) {
+            const mpa_cluster_operator& o = *s[i].op;
+            if (s[i].A32 || s[i].A64 || o.N < 1 || o.lmax < 1 || o.lmax > MPA_LMAX_MAX || !o.positions || !o.mie ||
+                o.tmatrix || !o.workspace || !(o.k > 0) || (long long)2 * lmax_to_rmax(o.lmax) * o.N != s[i].n ||
+                !matfree_c64_supported(o.N, o.lmax, false))
+                return false;
+        } else if (!s[i].A32 || !s[i].A64) {
+            return false;
+        }
+    }
     return true;
 }
 
@@ -139,6 +150,7 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
     std::vector<int3> vw;
     std::vector<int2> cw;
     std::vector<int2> mw[kTileClasses];
+    std::vector<MfJob> mf;
     int total = 0;
     for (int i = 0; i < nsys; i++) {
         const mpa_refine_system& q = R[i];
@@ -163,7 +175,8 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
         in.n = n;
         in.nrhs = S;
         in.add_identity = q.add_identity;
-        in.A = q.A32;
+        in.A = q.op ? nullptr : q.A32;
+        in.op = q.op;
         in.b = d.r32;
         in.x = d.d32;
         in.state = p;
@@ -174,8 +187,14 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             cw.push_back(make_int2(i, s));
             for (int g = 0; g < d.G; g++) vw.push_back(make_int3(i, s, g));
         }
-        const int rb = gemv_row_blocks(n, S, 8);  // the FP64 residual matvec
-        for (int b = 0; b < rb; b++) mw[gemv_class(S)].push_back(make_int2(i, b));
+        if (q.op) {  // FP64 residual with the matrix-free operator
+            const mpa_cluster_operator& o = *q.op;
+            mf.push_back(MfJob{o.N, o.lmax, S, q.add_identity ? 1 : 0, o.positions, (const double2*)o.mie, o.k,
+                               (const double2*)q.x, d.y, (double2*)o.workspace, -1, 0, nullptr, nullptr, 0, 0});
+        } else {
+            const int rb = gemv_row_blocks(n, S, 8);  // the FP64 residual matvec
+            for (int b = 0; b < rb; b++) mw[gemv_class(S)].push_back(make_int2(i, b));
+        }
         total += S;
     }
     char* p = (char*)scratch;
@@ -267,8 +286,16 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             }
         hipLaunchKernelGGL(k_ref_accum, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         MPA_LAUNCH_CHECK();
-        rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st);
-        if (rc) return rc;
+        bool any_mv = false;
+        for (int c = 0; c < kTileClasses; c++) any_mv |= n_mw[c] > 0;
+        if (any_mv) {
+            rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st);
+            if (rc) return rc;
+        }
+        for (const MfJob& j : mf) {
+            rc = matfree_launch(j, nullptr, st);
+            if (rc) return rc;
+        }
         hipLaunchKernelGGL(k_ref_residual, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         hipLaunchKernelGGL(k_ref_norms, dim3(nc), dim3(kThreads), 0, st, d_ref, d_cw);
         MPA_LAUNCH_CHECK();

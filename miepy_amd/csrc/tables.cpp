@@ -163,6 +163,9 @@ TransTable* build_table(int lmax) {
     std::vector<int2> wmeta;
     std::vector<int> wsrc;
     std::vector<double2> wcoef;
+    // and, per (row mode r, HLE index h = p(p+1) + mu), its A and B entries (or -1)
+    const int nh = (2 * lmax + 1) * (2 * lmax + 1);
+    std::vector<int2> hmap((size_t)rmax * nh, make_int2(-1, -1));
     for (int r = 0; r < rmax; r++)
         for (int kind = 0; kind < 2; kind++)
             for (int mu = -2 * lmax; mu <= 2 * lmax; mu++)
@@ -187,7 +190,12 @@ TransTable* build_table(int lmax) {
                         }
                     }
                     const int cnt = (int)wsrc.size() - start;
-                    if (cnt) wmeta.push_back(make_int2(start, cnt | (kind << 16)));
+                    if (cnt) {
+                        const int hh = p * (p + 1) + mu;
+                        int2& slot = hmap[(size_t)r * nh + hh];
+                        (kind == 0 ? slot.x : slot.y) = (int)wmeta.size();
+                        wmeta.push_back(make_int2(start, cnt | (kind << 16)));
+                    }
                 }
     TransTable* T = new TransTable();
     T->lmax = lmax;
@@ -209,6 +217,13 @@ TransTable* build_table(int lmax) {
         return nullptr;
     }
     T->nentries = (int)wmeta.size();
+    T->nh = nh;
+    if (hipMalloc(&T->hmap, hmap.size() * sizeof(int2)) != hipSuccess ||
+        hipMemcpy(T->hmap, hmap.data(), hmap.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess) {
+        mpa_set_error("contraction map for lmax=%d failed", lmax);
+        delete T;
+        return nullptr;
+    }
     if (hipMalloc(&T->wmeta, wmeta.size() * sizeof(int2)) != hipSuccess ||
         hipMalloc(&T->wsrc, wsrc.size() * sizeof(int)) != hipSuccess ||
         hipMalloc(&T->wcoef, wcoef.size() * sizeof(double2)) != hipSuccess ||

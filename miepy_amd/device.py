@@ -337,17 +337,27 @@ class ClusterOperator:
 
 
 def matfree_matvec(op, x, add_identity=False, out=None):
-    """y = (I +) T x with T recomputed inside the product; x [nrhs, n] c128."""
+    """y = (I +) T x with T recomputed inside the product; x [nrhs, n]
+    complex128 (FP64 operator) or complex64 (FP32 matrix-core operator)."""
     lib = L.load()
     x2 = x.reshape(-1, op.n)
-    if x2.dtype != C128 or not x2.is_contiguous():
-        raise ValueError("x must be contiguous complex128 [nrhs, n]")
+    if x2.dtype not in (C128, C64) or not x2.is_contiguous():
+        raise ValueError("x must be contiguous complex128 or complex64 [nrhs, n]")
     op.reserve(x2.shape[0])
     if out is None:
         out = torch.empty_like(x2)
-    L.check(lib.mpa_matfree_matvec(ctypes.byref(op.struct), x2.shape[0], 1 if add_identity else 0, _ptr(x2),
-                                   _ptr(out), _stream()), "matfree_matvec")
+    fn = lib.mpa_matfree_matvec if x2.dtype == C128 else lib.mpa_matfree_matvec_c64
+    L.check(fn(ctypes.byref(op.struct), x2.shape[0], 1 if add_identity else 0, _ptr(x2), _ptr(out), _stream()),
+            "matfree_matvec")
     return out.reshape(x.shape)
+
+
+def solve_refined_matfree(op, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refine=8):
+    """Mixed precision without a stored matrix: complex64 corrections with the
+    FP32 matrix-core operator, FP64 residuals with the FP64 operator."""
+    xs, reps = solve_refined_batched([op], [None], [b.reshape(-1, op.n)], tol=tol, inner_tol=inner_tol,
+                                     maxiter=maxiter, max_refine=max_refine, add_identity=True)
+    return xs[0].reshape(b.shape), reps[0]
 
 
 def bicgstab_matfree(op, b, tol=1e-5, maxiter=1000, add_identity=True, x=None, workspace=None, sync_every=1):
@@ -445,7 +455,10 @@ def solve_refined_batched(A32s, A64s, rhs, tol=1e-10, inner_tol=1e-5, maxiter=10
                           add_identity=False):
     """Mixed-precision solve of several systems at once (c64 corrections, FP64 refinement).
 
-    A32s: list of [n_i, n_i] c64, A64s: list of [n_i, n_i] c128, rhs: list of [S_i, n_i] c128."""
+    A32s: list of [n_i, n_i] c64, A64s: list of [n_i, n_i] c128, rhs: list of [S_i, n_i] c128.
+    An entry of A32s may be a ClusterOperator (its A64s entry None): that
+    system's corrections use the complex64 matrix-core operator and its
+    residual the FP64 one, with no stored matrix."""
     lib = L.load()
     k = len(A32s)
     tols = tol if isinstance(tol, (list, tuple)) else [tol] * k
@@ -454,18 +467,28 @@ def solve_refined_batched(A32s, A64s, rhs, tol=1e-10, inner_tol=1e-5, maxiter=10
     for i, (A32, A64, b) in enumerate(zip(A32s, A64s, rhs)):
         n = A32.shape[0]
         b2 = b.reshape(-1, n)
-        if A32.dtype != C64 or A64.dtype != C128 or b2.dtype != C128 or not b2.is_contiguous():
-            raise ValueError("solve_refined_batched needs c64 / c128 matrices and contiguous c128 right-hand sides")
+        op = A32 if isinstance(A32, ClusterOperator) else None
+        if op is None and (A32.dtype != C64 or A64.dtype != C128):
+            raise ValueError("solve_refined_batched needs c64 / c128 matrices")
+        if b2.dtype != C128 or not b2.is_contiguous():
+            raise ValueError("solve_refined_batched needs contiguous c128 right-hand sides")
+        if op is not None:
+            op.reserve(b2.shape[0])
         x = torch.empty_like(b2)
-        st = torch.empty(lib.mpa_refine_state_bytes(n, b2.shape[0]), dtype=torch.uint8, device=A32.device)
+        st = torch.empty(lib.mpa_refine_state_bytes(n, b2.shape[0]), dtype=torch.uint8, device=b2.device)
         xs.append(x)
         states.append(st)
         s = L.MpaRefineSystem()
         s.n, s.nrhs, s.add_identity = n, b2.shape[0], 1 if add_identity else 0
-        s.A32, s.A64, s.b, s.x, s.state = A32.data_ptr(), A64.data_ptr(), b2.data_ptr(), x.data_ptr(), st.data_ptr()
+        if op is not None:
+            s.A32 = s.A64 = None
+            s.op = ctypes.cast(ctypes.pointer(op.struct), ctypes.c_void_p)
+        else:
+            s.A32, s.A64 = A32.data_ptr(), A64.data_ptr()
+        s.b, s.x, s.state = b2.data_ptr(), x.data_ptr(), st.data_ptr()
         s.tol, s.inner_tol, s.maxiter, s.max_refine = float(tols[i]), float(inner_tol), int(maxiter), int(max_refine)
         structs[i] = s
-    scratch = torch.empty(lib.mpa_refine_scratch_bytes(k, structs), dtype=torch.uint8, device=A32s[0].device)
+    scratch = torch.empty(lib.mpa_refine_scratch_bytes(k, structs), dtype=torch.uint8, device=rhs[0].device)
     total = sum(x.shape[0] for x in xs)
     outer = (ctypes.c_int * total)()
     inner = (ctypes.c_int * total)()

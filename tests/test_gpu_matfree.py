@@ -194,3 +194,48 @@ def test_factorized_operator_matches_oracle(dev, oracle):
     x = np.random.default_rng(4).normal(size=T.shape[0]) + 0j
     got = dev.matfree_matvec(dev.ClusterOperator(pos, mie, k), dev.to_device(x.reshape(1, -1), dev.C128))
     assert _rel_err(got.cpu().numpy()[0], oracle.gemv(T, x)) <= 1e-12
+
+
+@pytest.mark.parametrize("N,lmax,S", [(7, 1, 1), (20, 2, 2), (45, 3, 1), (300, 3, 1), (160, 2, 3)])
+def test_c64_matrix_core_operator(dev, N, lmax, S):
+    """The complex64 operator on the FP32 matrix cores (special functions in
+    FP64) equals the FP64 operator to FP32 accuracy: 2e-6 relative to the
+    largest entry of the product."""
+    import torch
+    pos, mie = _cluster(N, lmax, seed=300 + N, spacing=300 * nm)
+    k = 2 * np.pi / (600 * nm)
+    n = 2 * lmax * (lmax + 2) * N
+    rng = np.random.default_rng(N + 1)
+    X = rng.normal(size=(S, n)) + 1j * rng.normal(size=(S, n))
+    op = dev.ClusterOperator(pos, mie, k, max_rhs=S)
+    ref = dev.matfree_matvec(op, dev.to_device(X, dev.C128), add_identity=True).cpu().numpy()
+    got = dev.matfree_matvec(op, dev.to_device(X, dev.C64), add_identity=True).cpu().numpy()
+    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+    assert err <= 2e-6, err
+
+
+def test_matrix_free_mixed_precision_solve(dev, H):
+    """FP32 matrix-core corrections + FP64 matrix-free refinement reach the
+    FP64 solution (tol 1e-10 -> 1e-8 relative), alone and batched with a
+    stored-matrix system."""
+    from miepy_amd.workloads import rand1000
+    w = rand1000()
+    ms, _ = H.mie_coefficients(w.radius, w.eps, np.ones(w.N), w.medium_eps, 1.0, w.k, w.lmax)
+    b = dev.to_device(np.stack([s.structure(w.position, w.k, w.lmax) for s in w.sources]).reshape(1, -1), dev.C128)
+    op = dev.ClusterOperator(w.position, ms, w.k)
+    x64, rep64 = dev.bicgstab_matfree(op, b, tol=1e-10)
+    x, rep = dev.solve_refined_matfree(op, b, tol=1e-10)
+    assert rep.converged, rep
+    x64, x = x64.cpu().numpy(), x.cpu().numpy()
+    assert np.linalg.norm(x - x64) / np.linalg.norm(x64) <= 1e-8
+    # batched with a stored system
+    N2, lmax2 = 12, 2
+    pos2, _ = _cluster(N2, lmax2, seed=3)
+    ms2, _ = H.mie_coefficients(np.full(N2, 75 * nm), np.full(N2, -15 + 1j), np.ones(N2), 1.0, 1.0, w.k, lmax2)
+    A64, A32 = dev.assemble_dual(pos2, ms2, w.k, add_identity=True)
+    b2 = dev.to_device(np.random.default_rng(1).normal(size=(1, A64.shape[0])) + 0j, dev.C128)
+    xs, reps = dev.solve_refined_batched([op, A32], [None, A64], [b, b2], tol=1e-10)
+    assert all(r.converged for r in reps)
+    assert np.array_equal(xs[0].cpu().numpy(), x)
+    ref2 = np.linalg.solve(A64.cpu().numpy(), b2.cpu().numpy()[0])
+    assert np.linalg.norm(xs[1].cpu().numpy()[0] - ref2) / np.linalg.norm(ref2) <= 1e-8
