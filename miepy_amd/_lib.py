@@ -21,6 +21,7 @@ MPA_ERR_OOM = 6
 MPA_C128 = 0
 MPA_C64 = 1
 MPA_OPT_GRAPHS = 0
+MPA_OPT_MATFREE_FP64_KERNEL = 1
 
 STATUS_NAMES = {
     MPA_OK: "ok",

@@ -446,7 +446,15 @@ __global__ __launch_bounds__(kThreads) void k_matfree_fac(MfJob J, const SysDev*
     }
 }
 
-inline bool fac_path(int N, int lmax) { return lmax <= kFacMaxLmax && N >= kFacMinN; }
+// The factorized FP64 kernel is opt-in (MPA_OPT_MATFREE_FP64_KERNEL = 1): measured
+// on MI355X at n = 30000 it runs 1.57 ms per product against 1.13 ms for the
+// LDS kernel (one wave per SIMD at 256 VGPRs leaves the scalar loads of W_j
+// uncovered).
+inline bool fac_path(int N, int lmax) {
+    return lmax <= kFacMaxLmax && N >= kFacMinN && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 1;
+}
+inline bool mf64_path(int lmax) { return lmax <= kFacMaxLmax && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 2; }
+inline bool fac_possible(int N, int lmax) { return lmax <= kFacMaxLmax && N >= kFacMinN; }
 
 // ---- complex64 operator on the FP32 matrix cores (mixed-precision inner solves)
 //
@@ -463,16 +471,19 @@ inline bool fac_path(int N, int lmax) { return lmax <= kFacMaxLmax && N >= kFacM
 // read as the B operand; the accumulators stay in registers across the
 // workgroup's chunk of column particles.
 
-// Wm[s][j][h][cc], cc = a rmax + r: thread per entry.
-__global__ __launch_bounds__(kThreads) void k_matfree_w32(MfJob J, const SysDev* __restrict__ sys, int nh,
-                                                          const int2* __restrict__ hmap,
-                                                          const int2* __restrict__ wmeta, const int* __restrict__ wsrc,
-                                                          const double2* __restrict__ wcoef, float2* __restrict__ Wm) {
+// Wm[s][j][h][cc], cc = a rmax + r: thread per entry.  T: element type of x
+// and Wm (float: complex64 operator, double: FP64 matrix-core operator).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_matfree_wm(MfJob J, const SysDev* __restrict__ sys, int nh,
+                                                         const int2* __restrict__ hmap,
+                                                         const int2* __restrict__ wmeta, const int* __restrict__ wsrc,
+                                                         const double2* __restrict__ wcoef, cplx_t<T>* __restrict__ Wm) {
+    using C = cplx_t<T>;
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const int lmax = J.lmax, rmax = lmax_to_rmax(lmax), bs = 2 * rmax;
     const size_t n = (size_t)bs * J.N;
     const size_t total = (size_t)J.S * J.N * nh * bs;
-    const float2* X = (const float2*)J.x;
+    const C* X = (const C*)J.x;
     for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kThreads) {
         const int cc = (int)(idx % bs);
         const size_t rest = idx / bs;
@@ -481,7 +492,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_w32(MfJob J, const SysDev*
         const size_t s = sj / J.N, j = sj - s * J.N;
         const int a = cc >= rmax ? 1 : 0, r = cc - a * rmax;
         const int2 es = hmap[(size_t)r * nh + h];
-        const float2* xj = X + s * n + j * bs;
+        const C* xj = X + s * n + j * bs;
         double2 w = make_double2(0, 0);
         for (int kind = 0; kind < 2; kind++) {
             const int e = kind ? es.y : es.x;
@@ -492,12 +503,13 @@ __global__ __launch_bounds__(kThreads) void k_matfree_w32(MfJob J, const SysDev*
             for (int k = m.x; k < m.x + cnt; k++) {
                 const int sidx = wsrc[k];
                 const int v = (int)sqrtf((float)(sidx + 1) + 0.5f);
-                const float2 xv = xj[src_pol * rmax + sidx];
-                const double2 z = cmul(J.mie[(j * 2 + src_pol) * lmax + v - 1], make_double2(xv.x, xv.y));
+                const C xv = xj[src_pol * rmax + sidx];
+                const double2 z = J.zpre ? J.zpre[s * n + j * bs + src_pol * rmax + sidx]
+                                         : cmul(J.mie[(j * 2 + src_pol) * lmax + v - 1], make_double2(xv.x, xv.y));
                 cfma(w, wcoef[k], z);
             }
         }
-        Wm[idx] = make_float2((float)w.x, (float)w.y);
+        Wm[idx] = cmake<T>((T)w.x, (T)w.y);
     }
 }
 
@@ -610,6 +622,136 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
         }
 }
 
+// FP64 variant on the FP64 matrix cores (v_mfma_f64_16x16x4f64): 16 row
+// particles per wave; the 98 real K positions of a pair's HLE values are
+// spread over the four 16-lane groups g (position 4t + g: HLE index
+// h = 2t + g/2, Re for even g, Im for odd g), so each lane keeps 25 of them;
+// 4 output tiles of 8 complex columns; FP64 accumulation.
+typedef double d4v __attribute__((ext_vector_type(4)));
+
+template <int L>
+__global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev* __restrict__ sys, int nchunks,
+                                                           int chunk, const double2* __restrict__ Wm) {
+    constexpr int R = L * (L + 2);
+    constexpr int NC = 2 * R;               // complex output columns per particle
+    constexpr int NT = (NC + 7) / 8;        // 16-column MFMA tiles
+    constexpr int PM = 2 * L;
+    constexpr int NH = (2 * L + 1) * (2 * L + 1);
+    constexpr int KT = (2 * NH + 3) / 4;    // K steps of 4 real positions
+    constexpr int HP = 2 * KT;              // HLE rows staged (zero-padded past NH)
+    __shared__ double2 Ws[2][HP][NT * 8 + 1];
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int N = J.N;
+    const int s = blockIdx.y;
+    const int c = blockIdx.x % nchunks;
+    const int group = blockIdx.x / nchunks;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, col = lane & 15, part = col & 1;
+    const int i = group * 64 + w * 16 + (lane & 15);
+    const bool live = i < N;
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const double2* Wall = Wm + (size_t)s * N * NH * NC;
+    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
+    d4v acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[t][q] = 0.0;
+    auto stage = [&](int j, int buf) {
+        const double2* src = Wall + (size_t)j * NH * NC;
+        for (int u = tid; u < HP * NT * 8; u += kThreads) {
+            const int h = u / (NT * 8), cc = u - h * (NT * 8);
+            Ws[buf][h][cc] = (h < NH && cc < NC) ? src[h * NC + cc] : make_double2(0.0, 0.0);
+        }
+    };
+    if (jbeg < jend) stage(jbeg, 0);
+    __syncthreads();
+    const int hpar = g >> 1, im = g & 1;  // this lane keeps HLE values with h % 2 == hpar, part im
+    for (int j = jbeg; j < jend; j++) {
+        const int buf = (j - jbeg) & 1;
+        if (j + 1 < jend) stage(j + 1, buf ^ 1);
+        double af[KT];
+#pragma unroll
+        for (int t = 0; t < KT; t++) af[t] = 0.0;
+        {
+            const double dx = xi - J.pos[3 * j], dy = yi - J.pos[3 * j + 1], dz = zi - J.pos[3 * j + 2];
+            const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ct = dz / rad, st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
+            const double zz = J.k * rad, rz = 1.0 / zz;
+            double sz, cz;
+            sincos(zz, &sz, &cz);
+            const double rho = sqrt(dx * dx + dy * dy);
+            const double2 e1 = rho > 0 ? make_double2(dx / rho, dy / rho) : make_double2(1.0, 0.0);
+            double2 hh[PM + 1];
+            hh[0] = make_double2(sz * rz, -cz * rz);
+            hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
+#pragma unroll
+            for (int l = 2; l <= PM; l++)
+                hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
+                                     (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
+            double2 e = make_double2(1.0, 0.0);
+            const bool on = live && j != ii;
+#pragma unroll
+            for (int m = 0; m <= PM; m++) {
+                if (m > 0) e = cmul(e, e1);
+                double pmm = 1.0;
+#pragma unroll
+                for (int q = 1; q <= PM; q++)
+                    if (q <= m) pmm *= (2 * q - 1) * st;
+                double p2 = 0.0, p1 = 0.0;
+#pragma unroll
+                for (int l = 0; l <= PM; l++) {
+                    if (l < m) continue;
+                    const double P = (l == m)       ? pmm
+                                     : (l == m + 1) ? ct * (2 * m + 1) * pmm
+                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) / (l - m);
+                    p2 = p1;
+                    p1 = P;
+                    const double2 hp = cscale(hh[l], P);
+                    const double2 vp = cmul(hp, e);
+                    const int hpos = l * (l + 1) + m;
+                    if ((hpos & 1) == hpar) af[hpos >> 1] = on ? (im ? vp.y : vp.x) : 0.0;
+                    if (m > 0) {
+                        const double2 vm = cmul(hp, cconj(e));
+                        const int hneg = l * (l + 1) - m;
+                        if ((hneg & 1) == hpar) af[hneg >> 1] = on ? (im ? vm.y : vm.x) : 0.0;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < KT; t++) {
+            const int h = 2 * t + hpar;
+#pragma unroll
+            for (int tile = 0; tile < NT; tile++) {
+                const double2 wv = Ws[buf][h][tile * 8 + (col >> 1)];
+                acc[tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], xprime<double>(wv.x, wv.y, im, part),
+                                                                 acc[tile], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    const size_t n = (size_t)NC * N;
+#pragma unroll
+    for (int tile = 0; tile < NT; tile++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int ip = group * 64 + w * 16 + 4 * q + g;
+            const int cc = tile * 8 + (col >> 1);
+            if (ip < N && cc < NC)
+                ((double*)J.part)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = acc[tile][q];
+        }
+}
+
+// Column chunks of the FP64 matrix-core kernel: about 1024 workgroups in total.
+inline int mf64_chunks(int N) {
+    const int groups = (N + 63) / 64;
+    int c = (1024 + groups - 1) / groups;
+    if (c > N) c = N;
+    return c < 1 ? 1 : c;
+}
+
 // y32 = (add_identity ? x32 : 0) + sum of the complex64 chunk partials in chunk order (FP64 sum)
 __global__ __launch_bounds__(kThreads) void k_matfree_sum32(MfJob J, const SysDev* __restrict__ sys, int nchunks,
                                                             const float2* __restrict__ part32) {
@@ -667,12 +809,17 @@ __global__ __launch_bounds__(kThreads) void k_matfree_sum(MfJob J, const SysDev*
 
 // Workspace: partial rows [slots][S][n], then z = Tm x [S][n], then (the
 // factorized path) W [S][N][entries][2].
-inline int partial_slots(int N) { return std::max(chunks_for(N), fac_chunks(N)); }
+inline int partial_slots(int N) { return std::max(std::max(chunks_for(N), fac_chunks(N)), mf64_chunks(N)); }
 
 size_t matfree_workspace_bytes(int N, int lmax, int S) {
     const size_t n = (size_t)2 * lmax_to_rmax(lmax) * N;
     size_t b = sizeof(double2) * n * S * (partial_slots(N) + 1);
-    if (fac_path(N, lmax)) b += sizeof(double2) * 2 * (size_t)S * N * fac_entries_of(lmax);
+    if (lmax <= kFacMaxLmax) {  // W of the factorized kernel or Wm of the matrix-core kernel
+        const int nh = (2 * lmax + 1) * (2 * lmax + 1);
+        size_t w = sizeof(double2) * (size_t)S * nh * n;
+        if (fac_possible(N, lmax)) w = std::max(w, sizeof(double2) * 2 * (size_t)S * N * fac_entries_of(lmax));
+        b += w;
+    }
     if (matfree_c64_supported(N, lmax, false)) {  // the complex64 layout: partials, then Wm
         const int nh = (2 * lmax + 1) * (2 * lmax + 1);
         const size_t b32 = sizeof(float2) * (n * S * mf32_chunks(N) + (size_t)S * N * nh * n / N);
@@ -698,7 +845,7 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     const size_t wtot = (size_t)J.S * J.N * nh * (n / J.N);
     const int groups = (J.N + 127) / 128;
     ProfScope prof(5, st);
-    hipLaunchKernelGGL(k_matfree_w32, dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)),
+    hipLaunchKernelGGL(k_matfree_wm<float>, dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)),
                        dim3(kThreads), 0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc, T->wcoef, Wm);
     const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
     if (J.lmax == 1) hipLaunchKernelGGL(k_matfree_mf32<1>, grid, block, 0, st, J, sys, nch, chunk, Wm, part32);
@@ -731,6 +878,25 @@ int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
         const int rc = mpa_internal_tmatrix_apply(J.N, J.lmax, J.S, (const double*)J.tmat, J.x, zbuf, st);
         if (rc) return rc;
         J.zpre = zbuf;
+    }
+    if (mf64_path(J.lmax)) {
+        const int nh = T->nh;
+        double2* Wm = zbuf + total;
+        const int nch = mf64_chunks(J.N);
+        const int chunk = (J.N + nch - 1) / nch;
+        const int groups = (J.N + 63) / 64;
+        const size_t wtot = (size_t)J.S * J.N * nh * (n / J.N);
+        ProfScope prof(4, st);
+        hipLaunchKernelGGL(k_matfree_wm<double>,
+                           dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)), dim3(kThreads),
+                           0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc, T->wcoef, Wm);
+        const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
+        if (J.lmax == 1) hipLaunchKernelGGL(k_matfree_mf64<1>, grid, block, 0, st, J, sys, nch, chunk, Wm);
+        else if (J.lmax == 2) hipLaunchKernelGGL(k_matfree_mf64<2>, grid, block, 0, st, J, sys, nch, chunk, Wm);
+        else hipLaunchKernelGGL(k_matfree_mf64<3>, grid, block, 0, st, J, sys, nch, chunk, Wm);
+        hipLaunchKernelGGL(k_matfree_sum, dim3(sum_blocks), dim3(kThreads), 0, st, J, sys, nch);
+        MPA_LAUNCH_CHECK();
+        return MPA_OK;
     }
     if (fac_path(J.N, J.lmax)) {
         const int ne = fac_entries_of(J.lmax);

@@ -162,11 +162,22 @@ def test_matfree_bad_arguments(dev):
     assert lib.mpa_matfree_matvec(ctypes.byref(op), 1, 0, None, None, None) == L.MPA_ERR_ARG
 
 
-@pytest.mark.parametrize("N,lmax,S", [(300, 1, 1), (200, 2, 1), (140, 3, 1), (150, 3, 2)])
-def test_factorized_operator_matches_dense(dev, N, lmax, S):
-    """Clusters of 128+ particles with lmax <= 3 take the factorized kernel
-    (one lane per row particle, per-particle contraction W_j): equal to the
-    stored matrix's product to 1e-12, spheres and general T-matrices."""
+@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("N,lmax,S", [(300, 1, 1), (200, 2, 1), (140, 3, 1), (150, 3, 2), (33, 3, 3)])
+def test_factorized_operator_matches_dense(dev, N, lmax, S, kernel):
+    """The contraction kernels for lmax <= 3 (1: factorized, one lane per row
+    particle, 128+ particles; 2: FP64 matrix cores): equal to the stored
+    matrix's product to 1e-12, spheres and general T-matrices."""
+    import miepy_amd._lib as L
+    lib = L.load()
+    lib.mpa_set_option(L.MPA_OPT_MATFREE_FP64_KERNEL, kernel)
+    try:
+        _factorized_vs_dense(dev, N, lmax, S)
+    finally:
+        lib.mpa_set_option(L.MPA_OPT_MATFREE_FP64_KERNEL, 0)
+
+
+def _factorized_vs_dense(dev, N, lmax, S):
     pos, mie = _cluster(N, lmax, seed=200 + N + lmax, spacing=300 * nm)
     k = 2 * np.pi / (600 * nm)
     n = 2 * lmax * (lmax + 2) * N
@@ -186,7 +197,18 @@ def test_factorized_operator_matches_dense(dev, N, lmax, S):
     assert err <= 1e-12, err
 
 
-def test_factorized_operator_matches_oracle(dev, oracle):
+@pytest.mark.parametrize("kernel", [1, 2])
+def test_factorized_operator_matches_oracle(dev, oracle, kernel):
+    import miepy_amd._lib as L
+    lib = L.load()
+    lib.mpa_set_option(L.MPA_OPT_MATFREE_FP64_KERNEL, kernel)
+    try:
+        _factorized_vs_oracle(dev, oracle)
+    finally:
+        lib.mpa_set_option(L.MPA_OPT_MATFREE_FP64_KERNEL, 0)
+
+
+def _factorized_vs_oracle(dev, oracle):
     N, lmax = 130, 2
     pos, mie = _cluster(N, lmax, seed=9, spacing=300 * nm)
     k = 2 * np.pi / (600 * nm)
@@ -231,11 +253,11 @@ def test_matrix_free_mixed_precision_solve(dev, H):
     # batched with a stored system
     N2, lmax2 = 12, 2
     pos2, _ = _cluster(N2, lmax2, seed=3)
-    ms2, _ = H.mie_coefficients(np.full(N2, 75 * nm), np.full(N2, -15 + 1j), np.ones(N2), 1.0, 1.0, w.k, lmax2)
+    ms2, _ = H.mie_coefficients(np.full(N2, 75 * nm), np.full(N2, 2.5 + 0.1j), np.ones(N2), 1.0, 1.0, w.k, lmax2)
     A64, A32 = dev.assemble_dual(pos2, ms2, w.k, add_identity=True)
     b2 = dev.to_device(np.random.default_rng(1).normal(size=(1, A64.shape[0])) + 0j, dev.C128)
     xs, reps = dev.solve_refined_batched([op, A32], [None, A64], [b, b2], tol=1e-10)
-    assert all(r.converged for r in reps)
+    assert all(r.converged for r in reps), reps
     assert np.array_equal(xs[0].cpu().numpy(), x)
     ref2 = np.linalg.solve(A64.cpu().numpy(), b2.cpu().numpy()[0])
     assert np.linalg.norm(xs[1].cpu().numpy()[0] - ref2) / np.linalg.norm(ref2) <= 1e-8
