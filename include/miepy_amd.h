@@ -47,8 +47,9 @@ extern "C" {
                             launches.  Results are identical either way.  Graphs are not used while
                             launch timing (mpa_profile_enable) is on. */
 #define MPA_OPT_MATFREE_FP64_KERNEL 1 /* kernel of the FP64 matrix-free product for lmax <= 3: 0 (default)
-                                         the LDS kernel, 1 the factorized register-resident kernel (N >= 128),
-                                         2 the FP64 matrix-core kernel.  Results agree to rounding. */
+                                         the FP64 matrix-core kernel, 1 the LDS kernel (always used for
+                                         lmax > 3), 2 the factorized register-resident kernel (N >= 128).
+                                         Results agree to rounding. */
 #define MPA_OPT_COUNT 2
 
 const char* mpa_version(void);

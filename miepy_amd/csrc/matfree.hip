@@ -426,7 +426,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_fac(MfJob J, const SysDev*
                 if (l < m) continue;
                 const double P = (l == m)       ? pmm
                                  : (l == m + 1) ? ct * (2 * m + 1) * pmm
-                                                : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) / (l - m);
+                                                : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.0 / (l - m));  // l - m constant: a multiply
                 p2 = p1;
                 p1 = P;
                 const double2 hp = cscale(h[l], P);
@@ -446,14 +446,14 @@ __global__ __launch_bounds__(kThreads) void k_matfree_fac(MfJob J, const SysDev*
     }
 }
 
-// The factorized FP64 kernel is opt-in (MPA_OPT_MATFREE_FP64_KERNEL = 1): measured
-// on MI355X at n = 30000 it runs 1.57 ms per product against 1.13 ms for the
-// LDS kernel (one wave per SIMD at 256 VGPRs leaves the scalar loads of W_j
-// uncovered).
+// FP64 kernel choice for lmax <= 3 (MPA_OPT_MATFREE_FP64_KERNEL): measured on
+// MI355X at n = 30000, one RHS: matrix cores 0.45 ms (default), LDS kernel
+// 1.17 ms, factorized register-resident kernel 1.57 ms (one wave per SIMD at
+// 256 VGPRs leaves the scalar loads of W_j uncovered).
 inline bool fac_path(int N, int lmax) {
-    return lmax <= kFacMaxLmax && N >= kFacMinN && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 1;
+    return lmax <= kFacMaxLmax && N >= kFacMinN && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 2;
 }
-inline bool mf64_path(int lmax) { return lmax <= kFacMaxLmax && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 2; }
+inline bool mf64_path(int lmax) { return lmax <= kFacMaxLmax && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 0; }
 inline bool fac_possible(int N, int lmax) { return lmax <= kFacMaxLmax && N >= kFacMinN; }
 
 // ---- complex64 operator on the FP32 matrix cores (mixed-precision inner solves)
@@ -537,11 +537,11 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
     const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
     const float2* Wall = Wm + (size_t)s * N * NH * NC;
     const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
-    f16v acc[NT];
+    f16v acc[NT][2];  // two partial sums per tile (even / odd K steps): independent MFMA chains
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-        for (int q = 0; q < 16; q++) acc[t][q] = 0.f;
+        for (int q = 0; q < 16; q++) acc[t][0][q] = acc[t][1][q] = 0.f;
     auto stage = [&](int j, int buf) {
         const float2* src = Wall + (size_t)j * NH * NC;
         for (int u = tid; u < NH * NT * 16; u += kThreads) {
@@ -587,7 +587,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
                     if (l < m) continue;
                     const double P = (l == m)       ? pmm
                                      : (l == m + 1) ? ct * (2 * m + 1) * pmm
-                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) / (l - m);
+                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.0 / (l - m));  // l - m constant: a multiply
                     p2 = p1;
                     p1 = P;
                     const double2 hp = cscale(hh[l], P);
@@ -605,8 +605,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
 #pragma unroll
             for (int tile = 0; tile < NT; tile++) {
                 const float2 wv = Ws[buf][t][tile * 16 + (col >> 1)];
-                acc[tile] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[t], xprime<float>(wv.x, wv.y, kh, part),
-                                                                 acc[tile], 0, 0, 0);
+                acc[tile][t & 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[t], xprime<float>(wv.x, wv.y, kh, part),
+                                                                        acc[tile][t & 1], 0, 0, 0);
             }
         __syncthreads();
     }
@@ -618,7 +618,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
             const int ip = group * 128 + w * 32 + 8 * (q >> 2) + 4 * kh + (q & 3);
             const int cc = tile * 16 + (col >> 1);
             if (ip < N && cc < NC)
-                ((float*)part32)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = acc[tile][q];
+                ((float*)part32)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] =
+                    acc[tile][0][q] + acc[tile][1][q];
         }
 }
 
@@ -653,11 +654,11 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
     const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
     const double2* Wall = Wm + (size_t)s * N * NH * NC;
     const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
-    d4v acc[NT];
+    d4v acc[NT][2];  // even / odd K steps: independent MFMA chains
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[t][q] = 0.0;
+        for (int q = 0; q < 4; q++) acc[t][0][q] = acc[t][1][q] = 0.0;
     auto stage = [&](int j, int buf) {
         const double2* src = Wall + (size_t)j * NH * NC;
         for (int u = tid; u < HP * NT * 8; u += kThreads) {
@@ -705,7 +706,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
                     if (l < m) continue;
                     const double P = (l == m)       ? pmm
                                      : (l == m + 1) ? ct * (2 * m + 1) * pmm
-                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) / (l - m);
+                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.0 / (l - m));  // l - m constant: a multiply
                     p2 = p1;
                     p1 = P;
                     const double2 hp = cscale(hh[l], P);
@@ -726,8 +727,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
 #pragma unroll
             for (int tile = 0; tile < NT; tile++) {
                 const double2 wv = Ws[buf][h][tile * 8 + (col >> 1)];
-                acc[tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], xprime<double>(wv.x, wv.y, im, part),
-                                                                 acc[tile], 0, 0, 0);
+                acc[tile][t & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], xprime<double>(wv.x, wv.y, im, part),
+                                                                        acc[tile][t & 1], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -740,7 +741,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
             const int ip = group * 64 + w * 16 + 4 * q + g;
             const int cc = tile * 8 + (col >> 1);
             if (ip < N && cc < NC)
-                ((double*)J.part)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = acc[tile][q];
+                ((double*)J.part)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] =
+                    acc[tile][0][q] + acc[tile][1][q];
         }
 }
 

@@ -395,6 +395,7 @@ def _sys_struct(A, b, x, state, tol, maxiter, add_identity):
     s.add_identity = 1 if add_identity else 0
     if isinstance(A, ClusterOperator):
         A.reserve(s.nrhs)
+        s.add_identity = 1  # a ClusterOperator stands for the cluster's T: its system is always I + T
         s.A = None
         s.op = ctypes.cast(ctypes.pointer(A.struct), ctypes.c_void_p)
         s.b = b.data_ptr()
@@ -457,8 +458,9 @@ def solve_refined_batched(A32s, A64s, rhs, tol=1e-10, inner_tol=1e-5, maxiter=10
 
     A32s: list of [n_i, n_i] c64, A64s: list of [n_i, n_i] c128, rhs: list of [S_i, n_i] c128.
     An entry of A32s may be a ClusterOperator (its A64s entry None): that
-    system's corrections use the complex64 matrix-core operator and its
-    residual the FP64 one, with no stored matrix."""
+    system (always I + T, whatever add_identity says for the stored ones)
+    uses the complex64 matrix-core operator for its corrections and the FP64
+    one for its residual, with no stored matrix."""
     lib = L.load()
     k = len(A32s)
     tols = tol if isinstance(tol, (list, tuple)) else [tol] * k
@@ -479,7 +481,8 @@ def solve_refined_batched(A32s, A64s, rhs, tol=1e-10, inner_tol=1e-5, maxiter=10
         xs.append(x)
         states.append(st)
         s = L.MpaRefineSystem()
-        s.n, s.nrhs, s.add_identity = n, b2.shape[0], 1 if add_identity else 0
+        # a ClusterOperator stands for the cluster's T: its system is always I + T
+        s.n, s.nrhs, s.add_identity = n, b2.shape[0], 1 if (add_identity or op is not None) else 0
         if op is not None:
             s.A32 = s.A64 = None
             s.op = ctypes.cast(ctypes.pointer(op.struct), ctypes.c_void_p)

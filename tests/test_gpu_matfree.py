@@ -162,12 +162,12 @@ def test_matfree_bad_arguments(dev):
     assert lib.mpa_matfree_matvec(ctypes.byref(op), 1, 0, None, None, None) == L.MPA_ERR_ARG
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [0, 1, 2])
 @pytest.mark.parametrize("N,lmax,S", [(300, 1, 1), (200, 2, 1), (140, 3, 1), (150, 3, 2), (33, 3, 3)])
 def test_factorized_operator_matches_dense(dev, N, lmax, S, kernel):
-    """The contraction kernels for lmax <= 3 (1: factorized, one lane per row
-    particle, 128+ particles; 2: FP64 matrix cores): equal to the stored
-    matrix's product to 1e-12, spheres and general T-matrices."""
+    """Every FP64 kernel for lmax <= 3 (0: matrix cores, 1: LDS kernel, 2:
+    factorized, one lane per row particle, 128+ particles): equal to the
+    stored matrix's product to 1e-12, spheres and general T-matrices."""
     import miepy_amd._lib as L
     lib = L.load()
     lib.mpa_set_option(L.MPA_OPT_MATFREE_FP64_KERNEL, kernel)
@@ -197,7 +197,7 @@ def _factorized_vs_dense(dev, N, lmax, S):
     assert err <= 1e-12, err
 
 
-@pytest.mark.parametrize("kernel", [1, 2])
+@pytest.mark.parametrize("kernel", [0, 1, 2])
 def test_factorized_operator_matches_oracle(dev, oracle, kernel):
     import miepy_amd._lib as L
     lib = L.load()
