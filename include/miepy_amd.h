@@ -169,6 +169,13 @@ int mpa_bicgstab_matfree(const mpa_cluster_operator* op, int nrhs, int add_ident
                          double tol, int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out,
                          int sync_every, void* stream);
 
+/* Autotuner for the matrix-core kernels of the matrix-free operator (no
+ * counterpart in the reference): times their variants (one or two
+ * accumulator chains per tile) on the caller's operator and vectors x, y
+ * ([nrhs, n] of dtype; y is overwritten) and keeps the fastest for dtype. */
+int mpa_autotune_matfree(const mpa_cluster_operator* op, int nrhs, int dtype, const void* x, void* y, int reps,
+                         int* variant_out, double* ms_out, void* stream);
+
 /* ---- batched solves: many independent systems, one set of launches -------- */
 
 /* One system (I + T) x = b with nrhs right-hand sides (host struct; all

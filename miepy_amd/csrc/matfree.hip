@@ -271,6 +271,11 @@ __device__ inline Rt xprime(Rt re, Rt im, int odd, int part) {
 }
 
 constexpr int kFacMaxLmax = 3;
+
+// Matrix-core contraction variants (mpa_autotune_matfree): [c64, c128].
+// c64: 0 one accumulator chain per tile, 1 two (K steps alternate);
+// c128: 0 two chains, 1 one.
+int g_mf_variant[2] = {0, 0};
 constexpr int kFacMinN = 128;  // smaller clusters keep the LDS kernel (too few lanes)
 
 constexpr int fac_n_of(int r) {
@@ -513,101 +518,240 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wm(MfJob J, const SysDev* 
     }
 }
 
+// B operands of the complex64 kernel, expanded per lane: Wx[s][j][tile][q][lane]
+// (4 floats: h = 4q + e, e = 0..3) = X'[k = lane / 32][col = lane % 32] of
+// the 32-column tile, i.e. each lane's B value of the MFMA step of HLE
+// index h, zero where h >= nh or the complex column tile * 16 + col / 2 is
+// past 2 rmax.  One thread per complex (h, column) writes its four lanes.
+template <int NT, int NQ>
+__global__ __launch_bounds__(kThreads) void k_matfree_wx(MfJob J, const SysDev* __restrict__ sys, int nh,
+                                                         const int2* __restrict__ hmap,
+                                                         const int2* __restrict__ wmeta, const int* __restrict__ wsrc,
+                                                         const double2* __restrict__ wcoef, float* __restrict__ Wx) {
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int lmax = J.lmax, rmax = lmax_to_rmax(lmax), bs = 2 * rmax;
+    const size_t n = (size_t)bs * J.N;
+    constexpr int NCP = NT * 16, NHP = NQ * 4;
+    const size_t total = (size_t)J.S * J.N * NHP * NCP;
+    const float2* X = (const float2*)J.x;
+    for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kThreads) {
+        const int cc = (int)(idx % NCP);
+        const size_t rest = idx / NCP;
+        const int h = (int)(rest % NHP);
+        const size_t sj = rest / NHP;
+        const size_t s = sj / J.N, j = sj - s * J.N;
+        double2 w = make_double2(0, 0);
+        if (h < nh && cc < bs) {
+            const int a = cc >= rmax ? 1 : 0, r = cc - a * rmax;
+            const int2 es = hmap[(size_t)r * nh + h];
+            const float2* xj = X + s * n + j * bs;
+            for (int kind = 0; kind < 2; kind++) {
+                const int e = kind ? es.y : es.x;
+                if (e < 0) continue;
+                const int2 m = wmeta[e];
+                const int cnt = m.y & 0xffff;
+                const int src_pol = kind ? 1 - a : a;  // B entries couple the other polarization
+                for (int k = m.x; k < m.x + cnt; k++) {
+                    const int sidx = wsrc[k];
+                    const int v = (int)sqrtf((float)(sidx + 1) + 0.5f);
+                    const float2 xv = xj[src_pol * rmax + sidx];
+                    cfma(w, wcoef[k], cmul(J.mie[(j * 2 + src_pol) * lmax + v - 1], make_double2(xv.x, xv.y)));
+                }
+            }
+        }
+        const float re = (float)w.x, im = (float)w.y;
+        const int tile = cc / 16, c2 = 2 * (cc % 16);
+        float* dst = Wx + (sj * NT + tile) * (size_t)NQ * 256 + (size_t)(h / 4) * 256 + (h % 4);
+        dst[4 * c2] = re;             // k = 0, part 0
+        dst[4 * (c2 + 1)] = im;       // k = 0, part 1
+        dst[4 * (32 + c2)] = -im;     // k = 1, part 0
+        dst[4 * (32 + c2 + 1)] = re;  // k = 1, part 1
+    }
+}
+
+typedef float f4v __attribute__((ext_vector_type(4)));
+
+// sin and cos of 0 <= x < 1e6 without branches: Cody-Waite reduction by pi/2
+// (two-term constant, fused multiply-adds), Taylor polynomials through r^13
+// and r^14 on |r| <= pi/4 (truncation below 3e-14), quadrant by selects.
+__device__ __forceinline__ void sincos_reduced(double x, double* sp, double* cp) {
+    const double q = rint(x * 0.63661977236758134308);
+    double r = fma(-q, 1.5707963267948966, x);
+    r = fma(-q, 6.123233995736766e-17, r);
+    const double r2 = r * r;
+    double ps = -1.0 / 6227020800.0;
+    ps = fma(ps, r2, 1.0 / 39916800.0);
+    ps = fma(ps, r2, -1.0 / 362880.0);
+    ps = fma(ps, r2, 1.0 / 5040.0);
+    ps = fma(ps, r2, -1.0 / 120.0);
+    ps = fma(ps, r2, 1.0 / 6.0);
+    const double S = fma(-ps * r2, r, r);
+    double pc = 1.0 / 87178291200.0;
+    pc = fma(pc, r2, -1.0 / 479001600.0);
+    pc = fma(pc, r2, 1.0 / 3628800.0);
+    pc = fma(pc, r2, -1.0 / 40320.0);
+    pc = fma(pc, r2, 1.0 / 720.0);
+    pc = fma(pc, r2, -1.0 / 24.0);
+    pc = fma(pc, r2, 0.5);
+    const double C = fma(-pc, r2, 1.0);
+    const int quad = (int)q & 3;
+    const double s0 = (quad & 1) ? C : S, c0 = (quad & 1) ? S : C;
+    *sp = (quad & 2) ? -s0 : s0;
+    *cp = ((quad + 1) & 2) ? -c0 : c0;
+}
+
+// The real part this lane supplies (Re for kh = 0, Im for kh = 1) of the
+// HLE values of pair (i, j), FP64 rounded to FP32; af[h] = 0 for h >= nh or
+// when !on.  With hp = h_p(kd) = (a, b) and e^{i m phi} = (c, s):
+//   Re HLE[p, +-m] = P (a c -+ b s),  Im HLE[p, +-m] = P (b c +- a s),
+// so each lane forms (alpha c + beta s'), (alpha c - beta s') with
+// (alpha, beta, s') = (a, b, -s) or (b, a, s).
+template <int L, int NHP>
+__device__ __forceinline__ void hle_lane(double xi, double yi, double zi, const double* __restrict__ pos, int j,
+                                         double k, int kh, bool on, float (&af)[NHP]) {
+    constexpr int PM = 2 * L;
+    constexpr int NH = (2 * L + 1) * (2 * L + 1);
+#pragma unroll
+    for (int t = NH; t < NHP; t++) af[t] = 0.f;
+    const double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+    const double rho2 = dx * dx + dy * dy;
+    const double rad = sqrt(rho2 + dz * dz);
+    const double rinv = 1.0 / rad;
+    const double ct = dz * rinv, st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
+    const double zz = k * rad, rz = rinv / k;
+    double sz, cz;
+    sincos_reduced(zz, &sz, &cz);
+    const double rho = sqrt(rho2);
+    const double rhoinv = 1.0 / fmax(rho, 1e-300);
+    const double2 e1 = rho > 0 ? make_double2(dx * rhoinv, dy * rhoinv) : make_double2(1.0, 0.0);
+    double2 hh[PM + 1];
+    hh[0] = make_double2(sz * rz, -cz * rz);
+    hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
+#pragma unroll
+    for (int l = 2; l <= PM; l++)
+        hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
+                             (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
+    double al[PM + 1], be[PM + 1];
+#pragma unroll
+    for (int l = 0; l <= PM; l++) {
+        al[l] = kh ? hh[l].y : hh[l].x;
+        be[l] = kh ? hh[l].x : hh[l].y;
+    }
+    double2 e = make_double2(1.0, 0.0);
+    double pmm = 1.0;
+#pragma unroll
+    for (int m = 0; m <= PM; m++) {
+        if (m > 0) {
+            e = cmul(e, e1);
+            pmm *= (2 * m - 1) * st;
+        }
+        const double sp = kh ? e.y : -e.y;
+        double p2 = 0.0, p1 = 0.0;
+#pragma unroll
+        for (int l = 0; l <= PM; l++) {
+            if (l < m) continue;
+            const double P = (l == m)       ? pmm
+                             : (l == m + 1) ? ct * (2 * m + 1) * pmm
+                                            : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.0 / (l - m));
+            p2 = p1;
+            p1 = P;
+            const double ac = al[l] * e.x, bs = be[l] * sp;
+            af[l * (l + 1) + m] = on ? (float)(P * (ac + bs)) : 0.f;
+            if (m > 0) af[l * (l + 1) - m] = on ? (float)(P * (ac - bs)) : 0.f;
+        }
+    }
+}
+
+template <int PER, int PFN>
+__device__ __forceinline__ void wx_fetch(f4v (&pf)[PFN], const f4v* __restrict__ src, int tid) {
+#pragma unroll
+    for (int q = 0; q < PFN; q++) {
+        const int u = tid + q * kThreads;
+        pf[q] = src[u < PER ? u : 0];
+    }
+}
+template <int PER, int PFN>
+__device__ __forceinline__ void wx_put(const f4v (&pf)[PFN], f4v* dst, int tid) {
+#pragma unroll
+    for (int q = 0; q < PFN; q++) {
+        const int u = tid + q * kThreads;
+        if (q < PFN - 1 || u < PER) dst[u] = pf[q];
+    }
+}
+
 // Partial rows part32[chunk][s][row] (complex64); blockIdx.y is the RHS.
-template <int L>
+// Software-pipelined over the workgroup's columns j: the HLE values of
+// column j + 1 (VALU) and the prefetch of its Wx block (global -> registers)
+// are issued between the MFMAs of column j; the Wx block goes to the other
+// LDS buffer before the one barrier per column.  KS accumulator chains per
+// tile (K steps t % KS).
+template <int L, int KS>
 __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev* __restrict__ sys, int nchunks,
-                                                           int chunk, const float2* __restrict__ Wm,
+                                                           int chunk, const f4v* __restrict__ Wx,
                                                            float2* __restrict__ part32) {
     constexpr int R = L * (L + 2);
     constexpr int NC = 2 * R;              // complex output columns per particle
     constexpr int NT = (NC + 15) / 16;     // 32-column MFMA tiles
-    constexpr int PM = 2 * L;
     constexpr int NH = (2 * L + 1) * (2 * L + 1);
-    __shared__ float2 Ws[2][NH][NT * 16 + 1];
+    constexpr int NQ = (NH + 3) / 4;
+    constexpr int PER = NT * NQ * 64;      // f4v per column particle
+    constexpr int PFN = (PER + kThreads - 1) / kThreads;
+    __shared__ f4v Ws[2][PER];
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const int N = J.N;
     const int s = blockIdx.y;
     const int c = blockIdx.x % nchunks;
     const int group = blockIdx.x / nchunks;
+    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
+    if (jbeg >= jend) return;  // uniform over the workgroup
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int kh = lane >> 5, col = lane & 31, part = col & 1;
-    const int i = group * 128 + w * 32 + (lane & 31);
+    const int i = group * 128 + w * 32 + col;
     const bool live = i < N;
     const int ii = live ? i : N - 1;
     const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
-    const float2* Wall = Wm + (size_t)s * N * NH * NC;
-    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
-    f16v acc[NT][2];  // two partial sums per tile (even / odd K steps): independent MFMA chains
+    const f4v* Wall = Wx + (size_t)s * N * PER;
+    f16v acc[NT][KS];
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-        for (int q = 0; q < 16; q++) acc[t][0][q] = acc[t][1][q] = 0.f;
-    auto stage = [&](int j, int buf) {
-        const float2* src = Wall + (size_t)j * NH * NC;
-        for (int u = tid; u < NH * NT * 16; u += kThreads) {
-            const int h = u / (NT * 16), cc = u - h * (NT * 16);
-            Ws[buf][h][cc] = cc < NC ? src[h * NC + cc] : make_float2(0.f, 0.f);
-        }
-    };
-    if (jbeg < jend) stage(jbeg, 0);
+        for (int k = 0; k < KS; k++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) acc[t][k][q] = 0.f;
+    f4v pf[PFN];
+    wx_fetch<PER, PFN>(pf, Wall + (size_t)jbeg * PER, tid);
+    wx_put<PER, PFN>(pf, Ws[0], tid);
+    float af[NQ * 4];
+    hle_lane<L, NQ * 4>(xi, yi, zi, J.pos, jbeg, J.k, kh, live && jbeg != ii, af);
     __syncthreads();
     for (int j = jbeg; j < jend; j++) {
         const int buf = (j - jbeg) & 1;
-        if (j + 1 < jend) stage(j + 1, buf ^ 1);
-        // this lane's half (Re for kh = 0, Im for kh = 1) of the pair's HLE values
-        float af[NH];
-        {
-            const double dx = xi - J.pos[3 * j], dy = yi - J.pos[3 * j + 1], dz = zi - J.pos[3 * j + 2];
-            const double rad = sqrt(dx * dx + dy * dy + dz * dz);
-            const double ct = dz / rad, st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
-            const double zz = J.k * rad, rz = 1.0 / zz;
-            double sz, cz;
-            sincos(zz, &sz, &cz);
-            const double rho = sqrt(dx * dx + dy * dy);
-            const double2 e1 = rho > 0 ? make_double2(dx / rho, dy / rho) : make_double2(1.0, 0.0);
-            double2 hh[PM + 1];
-            hh[0] = make_double2(sz * rz, -cz * rz);
-            hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
+        const int jn = min(j + 1, jend - 1);
+        wx_fetch<PER, PFN>(pf, Wall + (size_t)jn * PER, tid);
+        float afn[NQ * 4];
+        hle_lane<L, NQ * 4>(xi, yi, zi, J.pos, jn, J.k, kh, live && jn != ii, afn);
 #pragma unroll
-            for (int l = 2; l <= PM; l++)
-                hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
-                                     (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
-            double2 e = make_double2(1.0, 0.0);
-            const bool on = live && j != ii;
-#pragma unroll
-            for (int m = 0; m <= PM; m++) {
-                if (m > 0) e = cmul(e, e1);
-                double pmm = 1.0;
-#pragma unroll
-                for (int q = 1; q <= PM; q++)
-                    if (q <= m) pmm *= (2 * q - 1) * st;
-                double p2 = 0.0, p1 = 0.0;
-#pragma unroll
-                for (int l = 0; l <= PM; l++) {
-                    if (l < m) continue;
-                    const double P = (l == m)       ? pmm
-                                     : (l == m + 1) ? ct * (2 * m + 1) * pmm
-                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.0 / (l - m));  // l - m constant: a multiply
-                    p2 = p1;
-                    p1 = P;
-                    const double2 hp = cscale(hh[l], P);
-                    const double2 vp = cmul(hp, e);
-                    af[l * (l + 1) + m] = on ? (float)(kh ? vp.y : vp.x) : 0.f;
-                    if (m > 0) {
-                        const double2 vm = cmul(hp, cconj(e));
-                        af[l * (l + 1) - m] = on ? (float)(kh ? vm.y : vm.x) : 0.f;
-                    }
-                }
-            }
-        }
-#pragma unroll
-        for (int t = 0; t < NH; t++)
+        for (int q = 0; q < NQ; q++)
 #pragma unroll
             for (int tile = 0; tile < NT; tile++) {
-                const float2 wv = Ws[buf][t][tile * 16 + (col >> 1)];
-                acc[tile][t & 1] = __builtin_amdgcn_mfma_f32_32x32x2f32(af[t], xprime<float>(wv.x, wv.y, kh, part),
-                                                                        acc[tile][t & 1], 0, 0, 0);
+                const f4v b = Ws[buf][(tile * NQ + q) * 64 + lane];
+#pragma unroll
+                for (int e4 = 0; e4 < 4; e4++) {
+                    const int t = 4 * q + e4;
+                    if (t < NH)
+                        acc[tile][t % KS] =
+                            __builtin_amdgcn_mfma_f32_32x32x2f32(af[t], b[e4], acc[tile][t % KS], 0, 0, 0);
+                }
             }
+        // scheduling hint: one MFMA, then up to five VALU instructions of the next column's HLE values
+#pragma unroll
+        for (int t = 0; t < NH * NT; t++) {
+            __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+            __builtin_amdgcn_sched_group_barrier(0x002, 5, 0);
+        }
+        wx_put<PER, PFN>(pf, Ws[buf ^ 1], tid);
+#pragma unroll
+        for (int t = 0; t < NQ * 4; t++) af[t] = afn[t];
         __syncthreads();
     }
     const size_t n = (size_t)NC * N;
@@ -617,9 +761,12 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
         for (int q = 0; q < 16; q++) {
             const int ip = group * 128 + w * 32 + 8 * (q >> 2) + 4 * kh + (q & 3);
             const int cc = tile * 16 + (col >> 1);
-            if (ip < N && cc < NC)
-                ((float*)part32)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] =
-                    acc[tile][0][q] + acc[tile][1][q];
+            if (ip < N && cc < NC) {
+                float v = acc[tile][0][q];
+#pragma unroll
+                for (int k = 1; k < KS; k++) v += acc[tile][k][q];
+                ((float*)part32)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = v;
+            }
         }
 }
 
@@ -630,7 +777,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
 // 4 output tiles of 8 complex columns; FP64 accumulation.
 typedef double d4v __attribute__((ext_vector_type(4)));
 
-template <int L>
+template <int L, int KS>
 __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev* __restrict__ sys, int nchunks,
                                                            int chunk, const double2* __restrict__ Wm) {
     constexpr int R = L * (L + 2);
@@ -654,11 +801,13 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
     const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
     const double2* Wall = Wm + (size_t)s * N * NH * NC;
     const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
-    d4v acc[NT][2];  // even / odd K steps: independent MFMA chains
+    d4v acc[NT][KS];  // KS partial sums per tile (K steps t % KS): independent MFMA chains
 #pragma unroll
     for (int t = 0; t < NT; t++)
 #pragma unroll
-        for (int q = 0; q < 4; q++) acc[t][0][q] = acc[t][1][q] = 0.0;
+        for (int k = 0; k < KS; k++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[t][k][q] = 0.0;
     auto stage = [&](int j, int buf) {
         const double2* src = Wall + (size_t)j * NH * NC;
         for (int u = tid; u < HP * NT * 8; u += kThreads) {
@@ -727,8 +876,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
 #pragma unroll
             for (int tile = 0; tile < NT; tile++) {
                 const double2 wv = Ws[buf][h][tile * 8 + (col >> 1)];
-                acc[tile][t & 1] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], xprime<double>(wv.x, wv.y, im, part),
-                                                                        acc[tile][t & 1], 0, 0, 0);
+                acc[tile][t % KS] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], xprime<double>(wv.x, wv.y, im, part),
+                                                                         acc[tile][t % KS], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -740,9 +889,12 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
         for (int q = 0; q < 4; q++) {
             const int ip = group * 64 + w * 16 + 4 * q + g;
             const int cc = tile * 8 + (col >> 1);
-            if (ip < N && cc < NC)
-                ((double*)J.part)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] =
-                    acc[tile][0][q] + acc[tile][1][q];
+            if (ip < N && cc < NC) {
+                double v = acc[tile][0][q];
+#pragma unroll
+                for (int k = 1; k < KS; k++) v += acc[tile][k][q];
+                ((double*)J.part)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = v;
+            }
         }
 }
 
@@ -776,10 +928,11 @@ __global__ __launch_bounds__(kThreads) void k_matfree_sum32(MfJob J, const SysDe
     }
 }
 
-// Column chunks of the complex64 kernel: about 1024 workgroups in total.
+// Column chunks of the complex64 kernel: about 512 workgroups in total (two
+// per CU at the kernel's register count, one wave of workgroups).
 inline int mf32_chunks(int N) {
     const int groups = (N + 127) / 128;
-    int c = (1024 + groups - 1) / groups;
+    int c = (512 + groups - 1) / groups;
     if (c > N) c = N;
     return c < 1 ? 1 : c;
 }
@@ -790,6 +943,12 @@ inline int fac_chunks(int N) {
     int c = (512 + groups - 1) / groups;
     if (c > N) c = N;
     return c < 1 ? 1 : c;
+}
+
+// f4v entries of Wx per column particle: tiles x K quads x 64 lanes
+inline int wx_per(int lmax) {
+    const int nc = 2 * lmax_to_rmax(lmax), nh = (2 * lmax + 1) * (2 * lmax + 1);
+    return ((nc + 15) / 16) * ((nh + 3) / 4) * 64;
 }
 
 inline int fac_entries_of(int lmax) {
@@ -822,9 +981,8 @@ size_t matfree_workspace_bytes(int N, int lmax, int S) {
         if (fac_possible(N, lmax)) w = std::max(w, sizeof(double2) * 2 * (size_t)S * N * fac_entries_of(lmax));
         b += w;
     }
-    if (matfree_c64_supported(N, lmax, false)) {  // the complex64 layout: partials, then Wm
-        const int nh = (2 * lmax + 1) * (2 * lmax + 1);
-        const size_t b32 = sizeof(float2) * (n * S * mf32_chunks(N) + (size_t)S * N * nh * n / N);
+    if (matfree_c64_supported(N, lmax, false)) {  // the complex64 layout: partials, then Wx
+        const size_t b32 = sizeof(float2) * n * S * mf32_chunks(N) + sizeof(float) * 4 * (size_t)S * N * wx_per(lmax);
         b = std::max(b, b32);
     }
     return b;
@@ -840,19 +998,28 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     const int nh = T->nh;
     const size_t n = (size_t)2 * lmax_to_rmax(J.lmax) * J.N;
     const size_t total = (size_t)J.S * n;
-    const int nch = mf32_chunks(J.N);
-    const int chunk = (J.N + nch - 1) / nch;
+    const int chunk = (J.N + mf32_chunks(J.N) - 1) / mf32_chunks(J.N);
+    const int nch = (J.N + chunk - 1) / chunk;  // every chunk non-empty: each writes its partial rows
     float2* part32 = (float2*)J.part;
-    float2* Wm = part32 + (size_t)nch * total;
-    const size_t wtot = (size_t)J.S * J.N * nh * (n / J.N);
+    f4v* Wx = (f4v*)(part32 + (size_t)nch * total);
     const int groups = (J.N + 127) / 128;
     ProfScope prof(5, st);
-    hipLaunchKernelGGL(k_matfree_wm<float>, dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)),
-                       dim3(kThreads), 0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc, T->wcoef, Wm);
     const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
-    if (J.lmax == 1) hipLaunchKernelGGL(k_matfree_mf32<1>, grid, block, 0, st, J, sys, nch, chunk, Wm, part32);
-    else if (J.lmax == 2) hipLaunchKernelGGL(k_matfree_mf32<2>, grid, block, 0, st, J, sys, nch, chunk, Wm, part32);
-    else hipLaunchKernelGGL(k_matfree_mf32<3>, grid, block, 0, st, J, sys, nch, chunk, Wm, part32);
+    const bool ks2 = g_mf_variant[0] == 1;
+#define MPA_MF32(L_)                                                                                               \
+    do {                                                                                                           \
+        constexpr int NT_ = (2 * L_ * (L_ + 2) + 15) / 16, NQ_ = ((2 * L_ + 1) * (2 * L_ + 1) + 3) / 4;            \
+        const size_t wtot = (size_t)J.S * J.N * NQ_ * 4 * NT_ * 16;                                                \
+        hipLaunchKernelGGL((k_matfree_wx<NT_, NQ_>),                                                               \
+                           dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)), dim3(kThreads), \
+                           0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc, T->wcoef, (float*)Wx);                   \
+        if (ks2) hipLaunchKernelGGL((k_matfree_mf32<L_, 2>), grid, block, 0, st, J, sys, nch, chunk, Wx, part32);  \
+        else hipLaunchKernelGGL((k_matfree_mf32<L_, 1>), grid, block, 0, st, J, sys, nch, chunk, Wx, part32);      \
+    } while (0)
+    if (J.lmax == 1) MPA_MF32(1);
+    else if (J.lmax == 2) MPA_MF32(2);
+    else MPA_MF32(3);
+#undef MPA_MF32
     hipLaunchKernelGGL(k_matfree_sum32, dim3((unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads)),
                        dim3(kThreads), 0, st, J, sys, nch, part32);
     MPA_LAUNCH_CHECK();
@@ -893,9 +1060,16 @@ int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
                            dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)), dim3(kThreads),
                            0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc, T->wcoef, Wm);
         const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
-        if (J.lmax == 1) hipLaunchKernelGGL(k_matfree_mf64<1>, grid, block, 0, st, J, sys, nch, chunk, Wm);
-        else if (J.lmax == 2) hipLaunchKernelGGL(k_matfree_mf64<2>, grid, block, 0, st, J, sys, nch, chunk, Wm);
-        else hipLaunchKernelGGL(k_matfree_mf64<3>, grid, block, 0, st, J, sys, nch, chunk, Wm);
+        const bool ks2 = g_mf_variant[1] == 0;
+#define MPA_MF64(L_)                                                                                          \
+    do {                                                                                                      \
+        if (ks2) hipLaunchKernelGGL((k_matfree_mf64<L_, 2>), grid, block, 0, st, J, sys, nch, chunk, Wm);     \
+        else hipLaunchKernelGGL((k_matfree_mf64<L_, 1>), grid, block, 0, st, J, sys, nch, chunk, Wm);         \
+    } while (0)
+        if (J.lmax == 1) MPA_MF64(1);
+        else if (J.lmax == 2) MPA_MF64(2);
+        else MPA_MF64(3);
+#undef MPA_MF64
         hipLaunchKernelGGL(k_matfree_sum, dim3(sum_blocks), dim3(kThreads), 0, st, J, sys, nch);
         MPA_LAUNCH_CHECK();
         return MPA_OK;
@@ -978,4 +1152,49 @@ extern "C" int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int 
     MfJob J{op->N, op->lmax, nrhs, add_identity ? 1 : 0, op->positions, (const double2*)op->mie, op->k,
             (const double2*)x, (double2*)y, (double2*)op->workspace, -1, 0, (const double2*)op->tmatrix, nullptr};
     return matfree_launch(J, nullptr, (hipStream_t)stream);
+}
+
+// Autotuner for the matrix-core contraction kernels: times each variant on
+// the caller's operator and vectors (y is overwritten) and keeps the fastest
+// for the element type of x (c64 or c128).
+extern "C" int mpa_autotune_matfree(const mpa_cluster_operator* op, int nrhs, int dtype, const void* x, void* y,
+                                    int reps, int* variant_out, double* ms_out, void* stream) {
+    if (!op || nrhs < 1 || !x || !y || reps < 1 || (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("autotune_matfree: bad argument");
+        return MPA_ERR_ARG;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    hipEvent_t e0, e1;
+    MPA_HIP_CHECK(hipEventCreate(&e0));
+    MPA_HIP_CHECK(hipEventCreate(&e1));
+    const int idx = dtype == MPA_C64 ? 0 : 1;
+    const int saved = g_mf_variant[idx];
+    int best = 0, rc = MPA_OK;
+    double best_ms = 1e30;
+    for (int v = 0; v < 2 && rc == MPA_OK; v++) {
+        g_mf_variant[idx] = v;
+        auto run = [&]() {
+            return dtype == MPA_C64 ? mpa_matfree_matvec_c64(op, nrhs, 1, x, y, stream)
+                                    : mpa_matfree_matvec(op, nrhs, 1, x, y, stream);
+        };
+        rc = run();
+        if (rc) break;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && rc == MPA_OK; r++) rc = run();
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = MPA_ERR_HIP;
+        float ms = 0;
+        if (rc == MPA_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = MPA_ERR_HIP;
+        if (rc == MPA_OK && ms / reps < best_ms) {
+            best_ms = ms / reps;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    g_mf_variant[idx] = rc == MPA_OK ? best : saved;
+    if (rc) return rc;
+    if (variant_out) *variant_out = best;
+    if (ms_out) *ms_out = best_ms;
+    return MPA_OK;
 }

@@ -352,6 +352,21 @@ def matfree_matvec(op, x, add_identity=False, out=None):
     return out.reshape(x.shape)
 
 
+def autotune_matfree(op, nrhs=1, dtype=C64, reps=3):
+    """Time the matrix-core contraction variants of the matrix-free operator
+    on `op` and keep the fastest for dtype.  Returns (variant, ms)."""
+    lib = L.load()
+    op.reserve(nrhs)
+    g = torch.Generator(device=op.device).manual_seed(0)
+    x = torch.randn((nrhs, op.n), dtype=dtype, device=op.device, generator=g)
+    y = torch.empty_like(x)
+    v = ctypes.c_int(0)
+    ms = ctypes.c_double(0)
+    L.check(lib.mpa_autotune_matfree(ctypes.byref(op.struct), nrhs, DTYPE_CODE[dtype], _ptr(x), _ptr(y), int(reps),
+                                     ctypes.byref(v), ctypes.byref(ms), _stream()), "autotune_matfree")
+    return v.value, ms.value
+
+
 def solve_refined_matfree(op, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refine=8):
     """Mixed precision without a stored matrix: complex64 corrections with the
     FP32 matrix-core operator, FP64 residuals with the FP64 operator."""
