@@ -50,7 +50,11 @@ extern "C" {
                                          the FP64 matrix-core kernel, 1 the LDS kernel (always used for
                                          lmax > 3), 2 the factorized register-resident kernel (N >= 128).
                                          Results agree to rounding. */
-#define MPA_OPT_COUNT 2
+#define MPA_OPT_MATFREE_C64_KERNEL 2 /* kernel of the complex64 matrix-free product: 0 (default) FP16
+                                        matrix cores on hi/lo split operands (three products, FP32
+                                        accumulation, per-pair and per-column scaling; relative error
+                                        per term about 2^-21), 1 FP32 matrix cores. */
+#define MPA_OPT_COUNT 3
 
 const char* mpa_version(void);
 const char* mpa_last_error(void);

@@ -22,6 +22,7 @@ MPA_C128 = 0
 MPA_C64 = 1
 MPA_OPT_GRAPHS = 0
 MPA_OPT_MATFREE_FP64_KERNEL = 1
+MPA_OPT_MATFREE_C64_KERNEL = 2
 
 STATUS_NAMES = {
     MPA_OK: "ok",

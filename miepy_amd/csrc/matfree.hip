@@ -770,6 +770,324 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
         }
 }
 
+// ---- complex64 operator on the FP16 matrix cores with hi/lo split operands
+//
+// The contraction transposed, D[real output column][row particle i] =
+// sum_k A[col][k] B[k][i], on v_mfma_f32_32x32x16_f16: B = the pair's HLE
+// values (each lane's column is its own particle i, so per-pair factors are
+// undone lane-locally), A = W_j from LDS.  K slots: the (p, m >= 0) combos
+// u in m-major order; lane half 0 supplies HLE[p, +m], half 1 HLE[p, -m]
+// (zero for m = 0), each as (Re, Im) -> 2 K positions; 4 slots per 16-K
+// block.  Scaling keeps every FP16 value within [-1, 1]:
+//   B' = HLE c_pm g_ij,  c_pm = sqrt((p-m)!/(p+m)!),  g_ij = 1 / max_p |h_p(k d_ij)|
+//   A' = W / (c_pm tau_j),  tau_j = max |W_j / c_pm|
+// and each pair's block sum is scaled by tau_j / g_ij into the FP32 total.
+// Each operand is split v = hi + lo (hi = v with the mantissa cut to 10
+// bits, lo = v - hi, both exact in FP16 in the normal range); the products
+// hi*hi + hi*lo + lo*hi leave a relative error of about 2^-21 per term.
+
+typedef _Float16 h8v __attribute__((ext_vector_type(8)));
+typedef _Float16 h2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+
+template <int L>
+struct F16Shape {
+    static constexpr int PM = 2 * L;
+    static constexpr int NCOMBO = (PM + 1) * (PM + 2) / 2;  // (p, m >= 0) combos
+    static constexpr int NKB = (NCOMBO + 3) / 4;             // 16-K blocks
+    static constexpr int NC = 2 * L * (L + 2);               // complex output columns
+    static constexpr int NT = (2 * NC + 31) / 32;            // 32-row tiles of real output columns
+    static constexpr int PER = 2 * NT * NKB * 64;            // 16-byte vectors per column particle (hi, lo)
+};
+
+__device__ __forceinline__ void split_hilo(f2v v, h2v& hi, h2v& lo) {
+    f2v h;
+    h.x = __uint_as_float(__float_as_uint(v.x) & 0xFFFFE000u);
+    h.y = __uint_as_float(__float_as_uint(v.y) & 0xFFFFE000u);
+    hi = __builtin_convertvector(h, h2v);
+    lo = __builtin_convertvector(v - h, h2v);
+}
+
+// combo u (m-major: m = 0, p = 0..PM; m = 1, p = 1..PM; ...) -> (p, m)
+__device__ __forceinline__ void combo_pm(int u, int PM, int& p, int& m) {
+    m = 0;
+    int base = 0;
+    while (u >= base + PM + 1 - m) {
+        base += PM + 1 - m;
+        m++;
+    }
+    p = m + (u - base);
+}
+
+// sqrt((p-m)!/(p+m)!)
+__device__ __forceinline__ double legendre_norm(int p, int m) {
+    double c = 1.0;
+    for (int q = p - m + 1; q <= p + m; q++) c /= q;
+    return sqrt(c);
+}
+
+// Per column particle j (blockIdx.x) and RHS (blockIdx.y): W_j / c_pm into
+// LDS, tau_j = its largest modulus, then the A operands (hi and lo) in lane
+// order Wh[s][j][hl][tile][kb][lane] (8 halves each) and tau[s][j].
+template <int L>
+__global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* __restrict__ sys, int nh,
+                                                         const int2* __restrict__ hmap,
+                                                         const int2* __restrict__ wmeta, const int* __restrict__ wsrc,
+                                                         const double2* __restrict__ wcoef, h8v* __restrict__ Wh,
+                                                         float* __restrict__ tau) {
+    using Sh = F16Shape<L>;
+    constexpr int NHm = (2 * L + 1) * (2 * L + 1);
+    constexpr int NC = Sh::NC;
+    __shared__ float2 Wn[NHm][NC];
+    __shared__ float red[kThreads / 64];
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int j = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    const int lmax = J.lmax, rmax = lmax_to_rmax(lmax);
+    const size_t n = (size_t)NC * J.N;
+    const float2* xj = (const float2*)J.x + s * n + (size_t)j * NC;
+    float mx = 0.f;
+    for (int idx = tid; idx < NHm * NC; idx += kThreads) {
+        const int h = idx / NC, cc = idx - h * NC;
+        const int a = cc >= rmax ? 1 : 0, r = cc - a * rmax;
+        const int2 es = hmap[(size_t)r * nh + h];
+        double2 w = make_double2(0, 0);
+        for (int kind = 0; kind < 2; kind++) {
+            const int e = kind ? es.y : es.x;
+            if (e < 0) continue;
+            const int2 m = wmeta[e];
+            const int cnt = m.y & 0xffff;
+            const int src_pol = kind ? 1 - a : a;  // B entries couple the other polarization
+            for (int k = m.x; k < m.x + cnt; k++) {
+                const int sidx = wsrc[k];
+                const int v = (int)sqrtf((float)(sidx + 1) + 0.5f);
+                const float2 xv = xj[src_pol * rmax + sidx];
+                cfma(w, wcoef[k], cmul(J.mie[((size_t)j * 2 + src_pol) * lmax + v - 1], make_double2(xv.x, xv.y)));
+            }
+        }
+        const int p = (int)sqrtf((float)h + 0.5f), mu = h - p * (p + 1);
+        const double cinv = 1.0 / legendre_norm(p, mu < 0 ? -mu : mu);
+        const float2 wn = make_float2((float)(w.x * cinv), (float)(w.y * cinv));
+        Wn[h][cc] = wn;
+        mx = fmaxf(mx, fmaxf(fabsf(wn.x), fabsf(wn.y)));
+    }
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
+    if ((tid & 63) == 0) red[tid >> 6] = mx;
+    __syncthreads();
+    float t = red[0];
+    for (int q = 1; q < kThreads / 64; q++) t = fmaxf(t, red[q]);
+    t = t > 0.f ? t : 1.f;
+    if (tid == 0) tau[(size_t)s * J.N + j] = t;
+    const float tinv = 1.f / t;
+    h8v* dst = Wh + ((size_t)s * J.N + j) * Sh::PER;
+    for (int item = tid; item < Sh::NT * Sh::NKB * 64; item += kThreads) {
+        const int lane = item & 63, tk = item >> 6;
+        const int kb = tk % Sh::NKB, tile = tk / Sh::NKB;
+        const int kh = lane >> 5, Rr = 32 * tile + (lane & 31), cc = Rr >> 1, part = Rr & 1;
+        h8v hv, lv;
+#pragma unroll
+        for (int e2 = 0; e2 < 4; e2++) {
+            const int u = 4 * kb + e2;
+            float2 wv = make_float2(0.f, 0.f);
+            if (cc < NC && u < Sh::NCOMBO) {
+                int p, m;
+                combo_pm(u, Sh::PM, p, m);
+                if (!(kh && m == 0)) wv = Wn[p * (p + 1) + (kh ? -m : m)][cc];
+            }
+            // A[col = 2cc + part][K = (Re, Im) of the slot]
+            f2v a;
+            a.x = (part ? wv.y : wv.x) * tinv;
+            a.y = (part ? wv.x : -wv.y) * tinv;
+            h2v hi, lo;
+            split_hilo(a, hi, lo);
+            hv[2 * e2] = hi.x;
+            hv[2 * e2 + 1] = hi.y;
+            lv[2 * e2] = lo.x;
+            lv[2 * e2 + 1] = lo.y;
+        }
+        dst[tk * 64 + lane] = hv;
+        dst[(Sh::NT * Sh::NKB + tk) * 64 + lane] = lv;
+    }
+}
+
+// This lane's B operands (hi, lo) for pair (i, j): slots of lane half kh,
+// and ginv = 1 / g_ij (0 when !on; the geometry is replaced by a unit
+// distance there so nothing non-finite arises).
+template <int L>
+__device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const double* __restrict__ pos, int j, double k,
+                                        int kh, bool on, h8v (&bh)[F16Shape<L>::NKB], h8v (&bl)[F16Shape<L>::NKB],
+                                        float& ginv) {
+    using Sh = F16Shape<L>;
+    constexpr int PM = Sh::PM, NKB = Sh::NKB;
+    double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+    dx = on ? dx : 1.0;
+    dy = on ? dy : 0.0;
+    dz = on ? dz : 0.0;
+    const double rho2 = dx * dx + dy * dy;
+    const double rad = sqrt(rho2 + dz * dz);
+    const double rinv = 1.0 / rad;
+    const double zz = k * rad, rz = rinv / k;
+    double sz, cz;
+    sincos_reduced(zz, &sz, &cz);
+    double2 hh[PM + 1];
+    hh[0] = make_double2(sz * rz, -cz * rz);
+    hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
+#pragma unroll
+    for (int l = 2; l <= PM; l++)
+        hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
+                             (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
+    double hm2 = 0.0;
+#pragma unroll
+    for (int l = 0; l <= PM; l++) hm2 = fmax(hm2, hh[l].x * hh[l].x + hh[l].y * hh[l].y);
+    const double hmax = sqrt(hm2);
+    const double g = on ? 1.0 / hmax : 0.0;
+    ginv = on ? (float)hmax : 0.f;
+    float hr[PM + 1], hq[PM + 1];
+#pragma unroll
+    for (int l = 0; l <= PM; l++) {
+        hr[l] = (float)(hh[l].x * g);
+        hq[l] = (float)(hh[l].y * g);
+    }
+    const float ct = (float)(dz * rinv);
+    const float st = sqrtf(fmaxf(0.f, (1.f - ct) * (1.f + ct)));
+    const double rho = sqrt(rho2);
+    const double rhoinv = 1.0 / fmax(rho, 1e-300);
+    const float e1x = rho > 0 ? (float)(dx * rhoinv) : 1.f;
+    const float e1y = rho > 0 ? (float)((kh ? -dy : dy) * rhoinv) : 0.f;  // e^{-i phi} on lane half 1
+    float vals[NKB * 8];
+#pragma unroll
+    for (int t = 0; t < NKB * 8; t++) vals[t] = 0.f;
+    float ex = kh ? 0.f : 1.f, ey = 0.f;  // m = 0 slots belong to lane half 0
+    float pmm = 1.f;
+    int u = 0;
+#pragma unroll
+    for (int m = 0; m <= PM; m++) {
+        if (m > 0) {
+            const float bx = m == 1 ? 1.f : ex, by = m == 1 ? 0.f : ey;
+            ex = bx * e1x - by * e1y;
+            ey = bx * e1y + by * e1x;
+            pmm *= (2 * m - 1) * st;
+        }
+        float p2 = 0.f, p1 = 0.f;
+#pragma unroll
+        for (int l = 0; l <= PM; l++) {
+            if (l < m) continue;
+            const float P = (l == m)       ? pmm
+                            : (l == m + 1) ? ct * (2 * m + 1) * pmm
+                                           : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.f / (l - m));
+            p2 = p1;
+            p1 = P;
+            double c = 1.0;
+#pragma unroll
+            for (int q = l - m + 1; q <= l + m; q++) c /= q;
+            const float Pn = P * (float)__builtin_sqrt(c);
+            const float ar = hr[l] * Pn, ai = hq[l] * Pn;
+            vals[2 * u] = ar * ex - ai * ey;
+            vals[2 * u + 1] = ar * ey + ai * ex;
+            u++;
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+        for (int e2 = 0; e2 < 4; e2++) {
+            const f2v v = {vals[kb * 8 + 2 * e2], vals[kb * 8 + 2 * e2 + 1]};
+            h2v hi, lo;
+            split_hilo(v, hi, lo);
+            bh[kb][2 * e2] = hi.x;
+            bh[kb][2 * e2 + 1] = hi.y;
+            bl[kb][2 * e2] = lo.x;
+            bl[kb][2 * e2 + 1] = lo.y;
+        }
+}
+
+// Partial rows part32[chunk][s][row] (complex64); blockIdx.y is the RHS.  Per
+// column j: prefetch of the next Wh block (global -> registers), the pair's
+// B operands, 3 NKB NT FP16 MFMAs into KS chains per tile, the block sum
+// scaled by tau_j / g_ij into the total, the prefetch to the other LDS
+// buffer, one barrier.
+template <int L, int KS>
+__global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev* __restrict__ sys, int nchunks,
+                                                           int chunk, const f4v* __restrict__ Wh,
+                                                           const float* __restrict__ tau, float2* __restrict__ part32) {
+    using Sh = F16Shape<L>;
+    constexpr int NC = Sh::NC, NT = Sh::NT, NKB = Sh::NKB, PER = Sh::PER;
+    constexpr int PFN = (PER + kThreads - 1) / kThreads;
+    __shared__ f4v Ws[2][PER];
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int N = J.N;
+    const int s = blockIdx.y;
+    const int c = blockIdx.x % nchunks;
+    const int group = blockIdx.x / nchunks;
+    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
+    if (jbeg >= jend) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int kh = lane >> 5, col = lane & 31;
+    const int i = group * 128 + w * 32 + col;
+    const bool live = i < N;
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const f4v* Wall = Wh + (size_t)s * N * PER;
+    const float* tau_s = tau + (size_t)s * N;
+    f16v tot[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) tot[t][q] = 0.f;
+    f4v pf[PFN];
+    wx_fetch<PER, PFN>(pf, Wall + (size_t)jbeg * PER, tid);
+    wx_put<PER, PFN>(pf, Ws[0], tid);
+    __syncthreads();
+    for (int j = jbeg; j < jend; j++) {
+        const int buf = (j - jbeg) & 1;
+        const int jn = min(j + 1, jend - 1);
+        wx_fetch<PER, PFN>(pf, Wall + (size_t)jn * PER, tid);
+        h8v bh[NKB], bl[NKB];
+        float ginv;
+        hle_f16<L>(xi, yi, zi, J.pos, j, J.k, kh, live && j != ii, bh, bl, ginv);
+        f16v acc[NT][KS];
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int k = 0; k < KS; k++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) acc[t][k][q] = 0.f;
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const h8v ah = __builtin_bit_cast(h8v, Ws[buf][(t * NKB + kb) * 64 + lane]);
+                const h8v al = __builtin_bit_cast(h8v, Ws[buf][((NT + t) * NKB + kb) * 64 + lane]);
+                f16v& a = acc[t][kb % KS];
+                a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[kb], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[kb], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[kb], a, 0, 0, 0);
+            }
+        const float sc = tau_s[j] * ginv;
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                float a = acc[t][0][q];
+#pragma unroll
+                for (int k = 1; k < KS; k++) a += acc[t][k][q];
+                tot[t][q] = fmaf(sc, a, tot[t][q]);
+            }
+        wx_put<PER, PFN>(pf, Ws[buf ^ 1], tid);
+        __syncthreads();
+    }
+    if (live) {
+        const size_t n = (size_t)NC * N;
+        float* out = (float*)part32 + 2 * (((size_t)c * J.S + s) * n + (size_t)i * NC);
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int Rr = 32 * t + 8 * (q >> 2) + 4 * kh + (q & 3);  // real output column 2 cc + part
+                if (Rr < 2 * NC) out[Rr] = tot[t][q];
+            }
+    }
+}
+
 // FP64 variant on the FP64 matrix cores (v_mfma_f64_16x16x4f64): 16 row
 // particles per wave; the 98 real K positions of a pair's HLE values are
 // spread over the four 16-lane groups g (position 4t + g: HLE index
@@ -951,6 +1269,11 @@ inline int wx_per(int lmax) {
     return ((nc + 15) / 16) * ((nh + 3) / 4) * 64;
 }
 
+// 16-byte vectors of Wh per column particle (FP16 kernel)
+inline int wh_per(int lmax) {
+    return lmax == 1 ? F16Shape<1>::PER : lmax == 2 ? F16Shape<2>::PER : F16Shape<3>::PER;
+}
+
 inline int fac_entries_of(int lmax) {
     return lmax == 1 ? kFacEntries<1>.count : lmax == 2 ? kFacEntries<2>.count : kFacEntries<3>.count;
 }
@@ -982,7 +1305,9 @@ size_t matfree_workspace_bytes(int N, int lmax, int S) {
         b += w;
     }
     if (matfree_c64_supported(N, lmax, false)) {  // the complex64 layout: partials, then Wx
-        const size_t b32 = sizeof(float2) * n * S * mf32_chunks(N) + sizeof(float) * 4 * (size_t)S * N * wx_per(lmax);
+        const size_t ops = std::max(sizeof(float) * 4 * (size_t)S * N * wx_per(lmax),
+                                    (size_t)16 * S * N * wh_per(lmax) + sizeof(float) * S * N);
+        const size_t b32 = sizeof(float2) * n * S * mf32_chunks(N) + ops;
         b = std::max(b, b32);
     }
     return b;
@@ -1006,6 +1331,28 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     ProfScope prof(5, st);
     const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
     const bool ks2 = g_mf_variant[0] == 1;
+    if (mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0) {
+        h8v* Whp = (h8v*)Wx;
+        float* taup = (float*)(Whp + (size_t)J.S * J.N * wh_per(J.lmax));
+        const dim3 wgrid((unsigned)J.N, (unsigned)J.S);
+#define MPA_MF16(L_)                                                                                                \
+    do {                                                                                                            \
+        hipLaunchKernelGGL((k_matfree_wh<L_>), wgrid, block, 0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc,         \
+                           T->wcoef, Whp, taup);                                                                    \
+        if (ks2) hipLaunchKernelGGL((k_matfree_mf16<L_, 2>), grid, block, 0, st, J, sys, nch, chunk, (const f4v*)Whp, \
+                                    taup, part32);                                                                  \
+        else hipLaunchKernelGGL((k_matfree_mf16<L_, 1>), grid, block, 0, st, J, sys, nch, chunk, (const f4v*)Whp,    \
+                                taup, part32);                                                                      \
+    } while (0)
+        if (J.lmax == 1) MPA_MF16(1);
+        else if (J.lmax == 2) MPA_MF16(2);
+        else MPA_MF16(3);
+#undef MPA_MF16
+        hipLaunchKernelGGL(k_matfree_sum32, dim3((unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads)),
+                           dim3(kThreads), 0, st, J, sys, nch, part32);
+        MPA_LAUNCH_CHECK();
+        return MPA_OK;
+    }
 #define MPA_MF32(L_)                                                                                               \
     do {                                                                                                           \
         constexpr int NT_ = (2 * L_ * (L_ + 2) + 15) / 16, NQ_ = ((2 * L_ + 1) * (2 * L_ + 1) + 3) / 4;            \

@@ -12,7 +12,7 @@
 namespace {
 
 std::mutex g_mu;
-int g_options[MPA_OPT_COUNT] = {1, 0};  // graphs on, FP64 matrix-free kernel: matrix cores
+int g_options[MPA_OPT_COUNT] = {1, 0, 0};  // graphs on; matrix-free kernels: FP64 and FP16-split matrix cores
 std::map<int, hipStream_t> g_capture;  // per device
 void* g_pinned = nullptr;
 size_t g_pinned_bytes = 0;

@@ -218,22 +218,74 @@ def _factorized_vs_oracle(dev, oracle):
     assert _rel_err(got.cpu().numpy()[0], oracle.gemv(T, x)) <= 1e-12
 
 
-@pytest.mark.parametrize("N,lmax,S", [(7, 1, 1), (20, 2, 2), (45, 3, 1), (300, 3, 1), (160, 2, 3)])
-def test_c64_matrix_core_operator(dev, N, lmax, S):
-    """The complex64 operator on the FP32 matrix cores (special functions in
-    FP64) equals the FP64 operator to FP32 accuracy: 2e-6 relative to the
-    largest entry of the product."""
-    import torch
-    pos, mie = _cluster(N, lmax, seed=300 + N, spacing=300 * nm)
-    k = 2 * np.pi / (600 * nm)
-    n = 2 * lmax * (lmax + 2) * N
-    rng = np.random.default_rng(N + 1)
+@pytest.fixture(params=[0, 1], ids=["fp16split", "fp32"])
+def c64_kernel(request, dev):
+    """Selects the complex64 matrix-free kernel (FP16 matrix cores on hi/lo
+    split operands, or FP32 matrix cores) and restores the default after."""
+    import miepy_amd._lib as L
+    lib = L.load()
+    old = lib.mpa_get_option(L.MPA_OPT_MATFREE_C64_KERNEL)
+    lib.mpa_set_option(L.MPA_OPT_MATFREE_C64_KERNEL, request.param)
+    yield request.param
+    lib.mpa_set_option(L.MPA_OPT_MATFREE_C64_KERNEL, old)
+
+
+def _c64_vs_c128(dev, pos, mie, k, S, seed, identity):
+    n = 2 * mie.shape[2] * (mie.shape[2] + 2) * pos.shape[0]
+    rng = np.random.default_rng(seed)
     X = rng.normal(size=(S, n)) + 1j * rng.normal(size=(S, n))
     op = dev.ClusterOperator(pos, mie, k, max_rhs=S)
-    ref = dev.matfree_matvec(op, dev.to_device(X, dev.C128), add_identity=True).cpu().numpy()
-    got = dev.matfree_matvec(op, dev.to_device(X, dev.C64), add_identity=True).cpu().numpy()
-    err = np.max(np.abs(got - ref)) / np.max(np.abs(ref))
-    assert err <= 2e-6, err
+    ref = dev.matfree_matvec(op, dev.to_device(X, dev.C128), add_identity=identity).cpu().numpy()
+    got = dev.matfree_matvec(op, dev.to_device(X, dev.C64), add_identity=identity).cpu().numpy()
+    assert np.all(np.isfinite(got))
+    return np.max(np.abs(got - ref)) / np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("N,lmax,S", [(7, 1, 1), (20, 2, 2), (45, 3, 1), (300, 3, 1), (160, 2, 3), (130, 1, 2)])
+def test_c64_matrix_core_operator(dev, c64_kernel, N, lmax, S):
+    """The complex64 operator (special functions in FP64/FP32, FP16-split or
+    FP32 matrix cores) equals the FP64 operator to FP32 accuracy: 2e-6
+    relative to the largest entry of (I + T) x, 5e-6 for T x alone."""
+    pos, mie = _cluster(N, lmax, seed=300 + N, spacing=300 * nm)
+    k = 2 * np.pi / (600 * nm)
+    assert _c64_vs_c128(dev, pos, mie, k, S, N + 1, True) <= 2e-6
+    assert _c64_vs_c128(dev, pos, mie, k, S, N + 2, False) <= 5e-6
+
+
+@pytest.mark.parametrize("kd_near,kd_far", [(0.05, 5.0), (2.0, 2000.0)])
+def test_c64_operator_extreme_separations(dev, c64_kernel, kd_near, kd_far):
+    """Pairs far inside the near field (|h_6(kd)| ~ 1e15 at kd = 0.05, far
+    beyond the FP16 range before scaling) next to well separated pairs: the
+    per-pair and per-column scaling keeps the FP16-split operator at FP32
+    accuracy."""
+    lmax = 3
+    k = 2 * np.pi / (600 * nm)
+    rng = np.random.default_rng(11)
+    centers = rng.normal(size=(6, 3))
+    centers *= kd_far / k / np.max(np.linalg.norm(centers[:, None] - centers[None], axis=-1))
+    offs = np.array([[0, 0, 0], [kd_near / k, 0, 0], [0, kd_near / k, kd_near / k]])
+    pos = (centers[:, None, :] + offs[None]).reshape(-1, 3)
+    mie = (rng.normal(size=(len(pos), 2, lmax)) + 1j * rng.normal(size=(len(pos), 2, lmax))) * 1e-3
+    assert _c64_vs_c128(dev, pos, mie, k, 1, 5, False) <= 5e-6
+
+
+def test_matfree_autotune_keeps_results(dev):
+    """Every accumulator-chain variant of the matrix-core kernels gives the
+    same product to rounding; the autotuner keeps one of them."""
+    N, lmax = 200, 3
+    pos, mie = _cluster(N, lmax, seed=21, spacing=300 * nm)
+    k = 2 * np.pi / (600 * nm)
+    op = dev.ClusterOperator(pos, mie, k, max_rhs=2)
+    n = op.n
+    rng = np.random.default_rng(3)
+    X = dev.to_device(rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n)), dev.C128)
+    ref64 = dev.matfree_matvec(op, X).cpu().numpy()
+    ref32 = dev.matfree_matvec(op, X.to(dev.C64)).cpu().numpy()
+    for dt, ref, tol in ((dev.C64, ref32, 1e-5), (dev.C128, ref64, 1e-13)):
+        v, ms = dev.autotune_matfree(op, 2, dt, reps=2)
+        assert v in (0, 1) and ms > 0
+        got = dev.matfree_matvec(op, X.to(dt)).cpu().numpy()
+        assert np.max(np.abs(got - ref)) <= tol * np.max(np.abs(ref))
 
 
 def test_matrix_free_mixed_precision_solve(dev, H):
