@@ -782,9 +782,9 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
 //   B' = HLE c_pm g_ij,  c_pm = sqrt((p-m)!/(p+m)!),  g_ij = 1 / max_p |h_p(k d_ij)|
 //   A' = W / (c_pm tau_j),  tau_j = max |W_j / c_pm|
 // and each pair's block sum is scaled by tau_j / g_ij into the FP32 total.
-// Each operand is split v = hi + lo (hi = v with the mantissa cut to 10
-// bits, lo = v - hi, both exact in FP16 in the normal range); the products
-// hi*hi + hi*lo + lo*hi leave a relative error of about 2^-21 per term.
+// Each operand is split v = hi + lo (hi = v rounded to FP16, lo = v - hi
+// rounded to FP16); the products hi*hi + hi*lo + lo*hi leave a relative
+// error of about 2^-21 per term.
 
 typedef _Float16 h8v __attribute__((ext_vector_type(8)));
 typedef _Float16 h2v __attribute__((ext_vector_type(2)));
@@ -801,11 +801,14 @@ struct F16Shape {
 };
 
 __device__ __forceinline__ void split_hilo(f2v v, h2v& hi, h2v& lo) {
-    f2v h;
-    h.x = __uint_as_float(__float_as_uint(v.x) & 0xFFFFE000u);
-    h.y = __uint_as_float(__float_as_uint(v.y) & 0xFFFFE000u);
-    hi = __builtin_convertvector(h, h2v);
-    lo = __builtin_convertvector(v - h, h2v);
+    hi = __builtin_convertvector(v, h2v);  // round to nearest
+    // lo = v - hi, the FP16 hi read back by the mixed-precision FMA (one instruction per value)
+    const unsigned hb = __builtin_bit_cast(unsigned, hi);
+    float l0, l1;
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel_hi:[1,0,0]" : "=v"(l0) : "v"(hb), "v"(v.x));
+    asm("v_fma_mix_f32 %0, -%1, 1.0, %2 op_sel:[1,0,0] op_sel_hi:[1,0,0]" : "=v"(l1) : "v"(hb), "v"(v.y));
+    const f2v lv = {l0, l1};
+    lo = __builtin_convertvector(lv, h2v);
 }
 
 // combo u (m-major: m = 0, p = 0..PM; m = 1, p = 1..PM; ...) -> (p, m)
@@ -819,16 +822,12 @@ __device__ __forceinline__ void combo_pm(int u, int PM, int& p, int& m) {
     p = m + (u - base);
 }
 
-// sqrt((p-m)!/(p+m)!)
-__device__ __forceinline__ double legendre_norm(int p, int m) {
-    double c = 1.0;
-    for (int q = p - m + 1; q <= p + m; q++) c /= q;
-    return sqrt(c);
-}
+// Per group of kWhJ column particles j (blockIdx.x) and RHS (blockIdx.y):
+// W_j / c_pm into LDS, tau_j = its largest modulus, then the A operands (hi
+// and lo) in lane order Wh[s][j][hl][tile][kb][lane] (8 halves each) and
+// tau[s][j].  Each table term is read once for the group's columns.
+constexpr int kWhJ = 1;
 
-// Per column particle j (blockIdx.x) and RHS (blockIdx.y): W_j / c_pm into
-// LDS, tau_j = its largest modulus, then the A operands (hi and lo) in lane
-// order Wh[s][j][hl][tile][kb][lane] (8 halves each) and tau[s][j].
 template <int L>
 __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* __restrict__ sys, int nh,
                                                          const int2* __restrict__ hmap,
@@ -837,50 +836,80 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
                                                          float* __restrict__ tau) {
     using Sh = F16Shape<L>;
     constexpr int NHm = (2 * L + 1) * (2 * L + 1);
-    constexpr int NC = Sh::NC;
-    __shared__ float2 Wn[NHm][NC];
-    __shared__ float red[kThreads / 64];
+    constexpr int NC = Sh::NC, JB = kWhJ;
+    __shared__ float2 Wn[JB][NHm][NC];
+    __shared__ double2 zs[JB][NC];  // Mie coefficient times x of column j, per (polarization, r)
+    __shared__ double cinv_s[NHm];  // 1 / c_pm = sqrt((p+m)!/(p-m)!) per HLE index
+    __shared__ float red[JB][kThreads / 64];
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
-    const int j = blockIdx.x, s = blockIdx.y, tid = threadIdx.x;
+    const int j0 = blockIdx.x * JB, s = blockIdx.y, tid = threadIdx.x;
     const int lmax = J.lmax, rmax = lmax_to_rmax(lmax);
     const size_t n = (size_t)NC * J.N;
-    const float2* xj = (const float2*)J.x + s * n + (size_t)j * NC;
-    float mx = 0.f;
+    if (tid < JB * NC) {
+        const int b = tid / NC, q = tid - b * NC, j = j0 + b;
+        double2 z = make_double2(0, 0);
+        if (j < J.N) {
+            const float2 xv = ((const float2*)J.x)[s * n + (size_t)j * NC + q];
+            const int a = q >= rmax ? 1 : 0, r = q - a * rmax;
+            const int v = (int)sqrtf((float)(r + 1) + 0.5f);
+            z = cmul(J.mie[((size_t)j * 2 + a) * lmax + v - 1], make_double2(xv.x, xv.y));
+        }
+        zs[b][q] = z;
+    }
+    if (tid < NHm) {
+        const int p = (int)sqrtf((float)tid + 0.5f), mu = tid - p * (p + 1), m = mu < 0 ? -mu : mu;
+        double prod = 1.0;
+        for (int q = p - m + 1; q <= p + m; q++) prod *= q;
+        cinv_s[tid] = sqrt(prod);
+    }
+    __syncthreads();
+    float mx[JB];
+#pragma unroll
+    for (int b = 0; b < JB; b++) mx[b] = 0.f;
     for (int idx = tid; idx < NHm * NC; idx += kThreads) {
         const int h = idx / NC, cc = idx - h * NC;
         const int a = cc >= rmax ? 1 : 0, r = cc - a * rmax;
         const int2 es = hmap[(size_t)r * nh + h];
-        double2 w = make_double2(0, 0);
+        double2 w[JB];
+#pragma unroll
+        for (int b = 0; b < JB; b++) w[b] = make_double2(0, 0);
         for (int kind = 0; kind < 2; kind++) {
             const int e = kind ? es.y : es.x;
             if (e < 0) continue;
             const int2 m = wmeta[e];
             const int cnt = m.y & 0xffff;
-            const int src_pol = kind ? 1 - a : a;  // B entries couple the other polarization
+            const int zo = (kind ? 1 - a : a) * rmax;  // B entries couple the other polarization
             for (int k = m.x; k < m.x + cnt; k++) {
-                const int sidx = wsrc[k];
-                const int v = (int)sqrtf((float)(sidx + 1) + 0.5f);
-                const float2 xv = xj[src_pol * rmax + sidx];
-                cfma(w, wcoef[k], cmul(J.mie[((size_t)j * 2 + src_pol) * lmax + v - 1], make_double2(xv.x, xv.y)));
+                const double2 cf = wcoef[k];
+                const int sidx = zo + wsrc[k];
+#pragma unroll
+                for (int b = 0; b < JB; b++) cfma(w[b], cf, zs[b][sidx]);
             }
         }
-        const int p = (int)sqrtf((float)h + 0.5f), mu = h - p * (p + 1);
-        const double cinv = 1.0 / legendre_norm(p, mu < 0 ? -mu : mu);
-        const float2 wn = make_float2((float)(w.x * cinv), (float)(w.y * cinv));
-        Wn[h][cc] = wn;
-        mx = fmaxf(mx, fmaxf(fabsf(wn.x), fabsf(wn.y)));
+        const double cinv = cinv_s[h];
+#pragma unroll
+        for (int b = 0; b < JB; b++) {
+            const float2 wn = make_float2((float)(w[b].x * cinv), (float)(w[b].y * cinv));
+            Wn[b][h][cc] = wn;
+            mx[b] = fmaxf(mx[b], fmaxf(fabsf(wn.x), fabsf(wn.y)));
+        }
     }
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off));
-    if ((tid & 63) == 0) red[tid >> 6] = mx;
+#pragma unroll
+    for (int b = 0; b < JB; b++) {
+        for (int off = 32; off > 0; off >>= 1) mx[b] = fmaxf(mx[b], __shfl_xor(mx[b], off));
+        if ((tid & 63) == 0) red[b][tid >> 6] = mx[b];
+    }
     __syncthreads();
-    float t = red[0];
-    for (int q = 1; q < kThreads / 64; q++) t = fmaxf(t, red[q]);
-    t = t > 0.f ? t : 1.f;
-    if (tid == 0) tau[(size_t)s * J.N + j] = t;
-    const float tinv = 1.f / t;
-    h8v* dst = Wh + ((size_t)s * J.N + j) * Sh::PER;
-    for (int item = tid; item < Sh::NT * Sh::NKB * 64; item += kThreads) {
-        const int lane = item & 63, tk = item >> 6;
+    for (int item = tid; item < JB * Sh::NT * Sh::NKB * 64; item += kThreads) {
+        const int lane = item & 63, tkb = item >> 6;
+        const int b = tkb / (Sh::NT * Sh::NKB), tk = tkb - b * (Sh::NT * Sh::NKB);
+        const int j = j0 + b;
+        if (j >= J.N) break;  // items run in b order: the rest are past N too
+        float t = red[b][0];
+        for (int q = 1; q < kThreads / 64; q++) t = fmaxf(t, red[b][q]);
+        t = t > 0.f ? t : 1.f;
+        if (tk == 0 && lane == 0) tau[(size_t)s * J.N + j] = t;
+        const float tinv = 1.f / t;
         const int kb = tk % Sh::NKB, tile = tk / Sh::NKB;
         const int kh = lane >> 5, Rr = 32 * tile + (lane & 31), cc = Rr >> 1, part = Rr & 1;
         h8v hv, lv;
@@ -891,19 +920,20 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
             if (cc < NC && u < Sh::NCOMBO) {
                 int p, m;
                 combo_pm(u, Sh::PM, p, m);
-                if (!(kh && m == 0)) wv = Wn[p * (p + 1) + (kh ? -m : m)][cc];
+                if (!(kh && m == 0)) wv = Wn[b][p * (p + 1) + (kh ? -m : m)][cc];
             }
             // A[col = 2cc + part][K = (Re, Im) of the slot]
-            f2v a;
-            a.x = (part ? wv.y : wv.x) * tinv;
-            a.y = (part ? wv.x : -wv.y) * tinv;
+            f2v av;
+            av.x = (part ? wv.y : wv.x) * tinv;
+            av.y = (part ? wv.x : -wv.y) * tinv;
             h2v hi, lo;
-            split_hilo(a, hi, lo);
+            split_hilo(av, hi, lo);
             hv[2 * e2] = hi.x;
             hv[2 * e2 + 1] = hi.y;
             lv[2 * e2] = lo.x;
             lv[2 * e2 + 1] = lo.y;
         }
+        h8v* dst = Wh + ((size_t)s * J.N + j) * Sh::PER;
         dst[tk * 64 + lane] = hv;
         dst[(Sh::NT * Sh::NKB + tk) * 64 + lane] = lv;
     }
@@ -911,48 +941,64 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
 
 // This lane's B operands (hi, lo) for pair (i, j): slots of lane half kh,
 // and ginv = 1 / g_ij (0 when !on; the geometry is replaced by a unit
-// distance there so nothing non-finite arises).
+// distance there so nothing non-finite arises).  FP64: the separation, k d
+// and its reduction modulo pi/2 (Cody-Waite, two-term constant); FP32: sin
+// and cos of the reduced argument (Taylor through r^9 / r^10, truncation
+// below 2e-9), the Hankel recurrence, and the normalized Legendre functions
+//   Pn_m^m = Pn_{m-1}^{m-1} st sqrt((2m-1)/(2m)),  Pn_{m+1}^m = ct sqrt(2m+1) Pn_m^m,
+//   Pn_l^m = (2l-1)/sqrt((l-m)(l+m)) ct Pn_{l-1}^m
+//            - sqrt((l+m-1)(l-m-1)/((l-m)(l+m))) Pn_{l-2}^m,
+// Pn = c_lm P (the standard recurrence of P scaled by c_lm).  g_ij = 1 / max
+// of |Re|, |Im| of h_PM, the largest order.
 template <int L>
 __device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const double* __restrict__ pos, int j, double k,
-                                        int kh, bool on, h8v (&bh)[F16Shape<L>::NKB], h8v (&bl)[F16Shape<L>::NKB],
-                                        float& ginv) {
+                                        float kinv, int kh, bool on, h8v (&bh)[F16Shape<L>::NKB],
+                                        h8v (&bl)[F16Shape<L>::NKB], float& ginv) {
     using Sh = F16Shape<L>;
     constexpr int PM = Sh::PM, NKB = Sh::NKB;
     double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
     dx = on ? dx : 1.0;
     dy = on ? dy : 0.0;
     dz = on ? dz : 0.0;
-    const double rho2 = dx * dx + dy * dy;
-    const double rad = sqrt(rho2 + dz * dz);
-    const double rinv = 1.0 / rad;
-    const double zz = k * rad, rz = rinv / k;
-    double sz, cz;
-    sincos_reduced(zz, &sz, &cz);
-    double2 hh[PM + 1];
-    hh[0] = make_double2(sz * rz, -cz * rz);
-    hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
+    const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+    const double zz = k * rad;
+    const double qd = rint(zz * 0.63661977236758134308);
+    double rr = fma(-qd, 1.5707963267948966, zz);
+    rr = fma(-qd, 6.123233995736766e-17, rr);
+    const float r = (float)rr, r2 = r * r;
+    const float S = r + r * r2 * (-1.f / 6.f + r2 * (1.f / 120.f + r2 * (-1.f / 5040.f + r2 * (1.f / 362880.f))));
+    const float C = 1.f + r2 * (-0.5f + r2 * (1.f / 24.f + r2 * (-1.f / 720.f + r2 * (1.f / 40320.f -
+                                                                                       r2 * (1.f / 3628800.f)))));
+    const int quad = (int)qd & 3;
+    const float s0 = (quad & 1) ? C : S, c0 = (quad & 1) ? S : C;
+    const float sz = (quad & 2) ? -s0 : s0, cz = ((quad + 1) & 2) ? -c0 : c0;
+    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+    const float rinv = __builtin_amdgcn_rcpf((float)rad);
+    const float rz = rinv * kinv;
+    float hx[PM + 1], hy[PM + 1];
+    hx[0] = sz * rz;
+    hy[0] = -cz * rz;
+    hx[1] = hx[0] * rz + hy[0];
+    hy[1] = hy[0] * rz - hx[0];
 #pragma unroll
-    for (int l = 2; l <= PM; l++)
-        hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
-                             (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
-    double hm2 = 0.0;
-#pragma unroll
-    for (int l = 0; l <= PM; l++) hm2 = fmax(hm2, hh[l].x * hh[l].x + hh[l].y * hh[l].y);
-    const double hmax = sqrt(hm2);
-    const double g = on ? 1.0 / hmax : 0.0;
-    ginv = on ? (float)hmax : 0.f;
-    float hr[PM + 1], hq[PM + 1];
+    for (int l = 2; l <= PM; l++) {
+        hx[l] = (2 * l - 1) * rz * hx[l - 1] - hx[l - 2];
+        hy[l] = (2 * l - 1) * rz * hy[l - 1] - hy[l - 2];
+    }
+    const float hmax = fmaxf(fabsf(hx[PM]), fabsf(hy[PM]));
+    const float g = on ? __builtin_amdgcn_rcpf(hmax) : 0.f;
+    ginv = on ? hmax : 0.f;
 #pragma unroll
     for (int l = 0; l <= PM; l++) {
-        hr[l] = (float)(hh[l].x * g);
-        hq[l] = (float)(hh[l].y * g);
+        hx[l] *= g;
+        hy[l] *= g;
     }
-    const float ct = (float)(dz * rinv);
-    const float st = sqrtf(fmaxf(0.f, (1.f - ct) * (1.f + ct)));
-    const double rho = sqrt(rho2);
-    const double rhoinv = 1.0 / fmax(rho, 1e-300);
-    const float e1x = rho > 0 ? (float)(dx * rhoinv) : 1.f;
-    const float e1y = rho > 0 ? (float)((kh ? -dy : dy) * rhoinv) : 0.f;  // e^{-i phi} on lane half 1
+    const float rho2 = fx * fx + fy * fy;
+    const float rhoinv = rho2 > 0.f ? __builtin_amdgcn_rsqf(rho2) : 0.f;
+    const float ct = fz * rinv;
+    const float st = rho2 * rhoinv * rinv;
+    const float e1x = rho2 > 0.f ? fx * rhoinv : 1.f;
+    const float e1y = (kh ? -fy : fy) * rhoinv;  // e^{-i phi} on lane half 1
     float vals[NKB * 8];
 #pragma unroll
     for (int t = 0; t < NKB * 8; t++) vals[t] = 0.f;
@@ -965,22 +1011,25 @@ __device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const d
             const float bx = m == 1 ? 1.f : ex, by = m == 1 ? 0.f : ey;
             ex = bx * e1x - by * e1y;
             ey = bx * e1y + by * e1x;
-            pmm *= (2 * m - 1) * st;
+            pmm *= st * (float)__builtin_sqrt((2.0 * m - 1) / (2.0 * m));
         }
         float p2 = 0.f, p1 = 0.f;
 #pragma unroll
         for (int l = 0; l <= PM; l++) {
             if (l < m) continue;
-            const float P = (l == m)       ? pmm
-                            : (l == m + 1) ? ct * (2 * m + 1) * pmm
-                                           : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.f / (l - m));
+            float P;
+            if (l == m) {
+                P = pmm;
+            } else if (l == m + 1) {
+                P = ct * (float)__builtin_sqrt(2.0 * m + 1) * pmm;
+            } else {
+                const float A = (float)((2.0 * l - 1) / __builtin_sqrt((double)(l - m) * (l + m)));
+                const float B = (float)__builtin_sqrt((double)(l + m - 1) * (l - m - 1) / ((double)(l - m) * (l + m)));
+                P = A * (ct * p1) - B * p2;
+            }
             p2 = p1;
             p1 = P;
-            double c = 1.0;
-#pragma unroll
-            for (int q = l - m + 1; q <= l + m; q++) c /= q;
-            const float Pn = P * (float)__builtin_sqrt(c);
-            const float ar = hr[l] * Pn, ai = hq[l] * Pn;
+            const float ar = hx[l] * P, ai = hy[l] * P;
             vals[2 * u] = ar * ex - ai * ey;
             vals[2 * u + 1] = ar * ey + ai * ex;
             u++;
@@ -1028,6 +1077,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev
     const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
     const f4v* Wall = Wh + (size_t)s * N * PER;
     const float* tau_s = tau + (size_t)s * N;
+    const float kinv = (float)(1.0 / J.k);
     f16v tot[NT];
 #pragma unroll
     for (int t = 0; t < NT; t++)
@@ -1043,7 +1093,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev
         wx_fetch<PER, PFN>(pf, Wall + (size_t)jn * PER, tid);
         h8v bh[NKB], bl[NKB];
         float ginv;
-        hle_f16<L>(xi, yi, zi, J.pos, j, J.k, kh, live && j != ii, bh, bl, ginv);
+        hle_f16<L>(xi, yi, zi, J.pos, j, J.k, kinv, kh, live && j != ii, bh, bl, ginv);
         f16v acc[NT][KS];
 #pragma unroll
         for (int t = 0; t < NT; t++)
@@ -1237,7 +1287,19 @@ __global__ __launch_bounds__(kThreads) void k_matfree_sum32(MfJob J, const SysDe
             ax = X[idx].x;
             ay = X[idx].y;
         }
-        for (int cc = 0; cc < nchunks; cc++) {
+        // loads issued 8 at a time (latency overlap), added in chunk order
+        int cc = 0;
+        for (; cc + 8 <= nchunks; cc += 8) {
+            float2 v[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = part32[(size_t)(cc + q) * total + idx];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                ax += v[q].x;
+                ay += v[q].y;
+            }
+        }
+        for (; cc < nchunks; cc++) {
             const float2 v = part32[(size_t)cc * total + idx];
             ax += v.x;
             ay += v.y;
@@ -1334,7 +1396,7 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     if (mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0) {
         h8v* Whp = (h8v*)Wx;
         float* taup = (float*)(Whp + (size_t)J.S * J.N * wh_per(J.lmax));
-        const dim3 wgrid((unsigned)J.N, (unsigned)J.S);
+        const dim3 wgrid((unsigned)((J.N + kWhJ - 1) / kWhJ), (unsigned)J.S);
 #define MPA_MF16(L_)                                                                                                \
     do {                                                                                                            \
         hipLaunchKernelGGL((k_matfree_wh<L_>), wgrid, block, 0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc,         \
