@@ -48,9 +48,11 @@ struct MvDev {
 
 inline size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-// Workgroups per RHS for the vector kernels of an n-long system.
+// Workgroups per RHS for the vector kernels of an n-long system: about 1024
+// elements (4 per thread) each, so a single system of moderate n still
+// spreads over tens of CUs and each thread's loads overlap.
 inline int groups_for(int n) {
-    int g = (n + 4095) / 4096;
+    int g = (n + 1023) / 1024;
     return g < 1 ? 1 : (g > kMaxG ? kMaxG : g);
 }
 
