@@ -36,6 +36,18 @@
 namespace mpa {
 namespace {
 
+// Workgroup id -> work id so that consecutive work ids run on one XCD:
+// workgroups are dealt round-robin to the 8 XCDs (id % 8), so XCD x gets the
+// contiguous work range [x G8 / 8, (x + 1) G8 / 8) of the first G8 = G & ~7
+// workgroups (the tail keeps its id).  With work id = chunk * groups + row
+// group, the row groups that read the same column chunk's operands share an
+// XCD's L2.
+__device__ __forceinline__ int xcd_swizzle(int b, int G) {
+    const int full = G & ~7;
+    return b < full ? (b & 7) * (full >> 3) + (b >> 3) : b;
+}
+
+
 constexpr int kJB = 16;   // most pairs per batch
 constexpr int kSMax = 4;  // right-hand sides per pass (largest SG)
 
@@ -700,8 +712,10 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const int N = J.N;
     const int s = blockIdx.y;
-    const int c = blockIdx.x % nchunks;
-    const int group = blockIdx.x / nchunks;
+    const int groups = (N + 127) / 128;
+    const int v = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int c = v / groups;
+    const int group = v - c * groups;
     const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
     if (jbeg >= jend) return;  // uniform over the workgroup
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1065,8 +1079,10 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const int N = J.N;
     const int s = blockIdx.y;
-    const int c = blockIdx.x % nchunks;
-    const int group = blockIdx.x / nchunks;
+    const int groups = (N + 127) / 128;
+    const int v = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int c = v / groups;
+    const int group = v - c * groups;
     const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
     if (jbeg >= jend) return;  // uniform over the workgroup
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
@@ -1159,8 +1175,10 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const int N = J.N;
     const int s = blockIdx.y;
-    const int c = blockIdx.x % nchunks;
-    const int group = blockIdx.x / nchunks;
+    const int groups = (N + 63) / 64;
+    const int v = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int c = v / groups;
+    const int group = v - c * groups;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int g = lane >> 4, col = lane & 15, part = col & 1;
     const int i = group * 64 + w * 16 + (lane & 15);
