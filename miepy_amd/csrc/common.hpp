@@ -101,6 +101,12 @@ struct TransTable {
     double2* wcoef = nullptr;  // device, coefficient of each term
     int nh = 0;                // HLE values per pair, (2 lmax + 1)^2
     int2* hmap = nullptr;      // device, [rmax][nh]: the A and B entry of (r, h), or -1
+    // the same terms flattened per (r, h): hlist[r * nh + h] = (first, count)
+    // into hterm_src (sidx | kind << 8) and hterm_coef, A terms before B terms
+    int nhterms = 0;
+    int2* hlist = nullptr;
+    int* hterm_src = nullptr;
+    double2* hterm_coef = nullptr;
 };
 
 // Returns the (cached, per device) table for lmax; nullptr on failure.

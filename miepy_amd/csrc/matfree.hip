@@ -839,32 +839,43 @@ __device__ __forceinline__ void combo_pm(int u, int PM, int& p, int& m) {
 // Per group of kWhJ column particles j (blockIdx.x) and RHS (blockIdx.y):
 // W_j / c_pm into LDS, tau_j = its largest modulus, then the A operands (hi
 // and lo) in lane order Wh[s][j][hl][tile][kb][lane] (8 halves each) and
-// tau[s][j].  Each table term is read once for the group's columns.
-constexpr int kWhJ = 1;
+// tau[s][j].  The flattened term lists of every (r, h) are staged in LDS
+// first (a few coalesced loads per thread), so the per-entry lookups are LDS
+// reads instead of chains of dependent global loads.
+constexpr int kWhJ = 2;
+constexpr int kWhMaxTerms = 1024;  // LDS term capacity (lmax 3 has 789)
 
 template <int L>
 __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* __restrict__ sys, int nh,
-                                                         const int2* __restrict__ hmap,
-                                                         const int2* __restrict__ wmeta, const int* __restrict__ wsrc,
-                                                         const double2* __restrict__ wcoef, h8v* __restrict__ Wh,
+                                                         int nterms, const int2* __restrict__ hlist,
+                                                         const int* __restrict__ hsrc,
+                                                         const double2* __restrict__ hcoef, h8v* __restrict__ Wh,
                                                          float* __restrict__ tau) {
     using Sh = F16Shape<L>;
     constexpr int NHm = (2 * L + 1) * (2 * L + 1);
-    constexpr int NC = Sh::NC, JB = kWhJ;
+    constexpr int NC = Sh::NC, JB = kWhJ, RM = L * (L + 2);
     __shared__ float2 Wn[JB][NHm][NC];
     __shared__ double2 zs[JB][NC];  // Mie coefficient times x of column j, per (polarization, r)
     __shared__ double cinv_s[NHm];  // 1 / c_pm = sqrt((p+m)!/(p-m)!) per HLE index
+    __shared__ int2 hl_s[RM * NHm];
+    __shared__ int hs_s[kWhMaxTerms];
+    __shared__ double2 hc_s[kWhMaxTerms];
     __shared__ float red[JB][kThreads / 64];
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const int j0 = blockIdx.x * JB, s = blockIdx.y, tid = threadIdx.x;
-    const int lmax = J.lmax, rmax = lmax_to_rmax(lmax);
+    const int lmax = J.lmax;
     const size_t n = (size_t)NC * J.N;
+    for (int q = tid; q < RM * NHm; q += kThreads) hl_s[q] = hlist[q];
+    for (int q = tid; q < nterms; q += kThreads) {
+        hs_s[q] = hsrc[q];
+        hc_s[q] = hcoef[q];
+    }
     if (tid < JB * NC) {
         const int b = tid / NC, q = tid - b * NC, j = j0 + b;
         double2 z = make_double2(0, 0);
         if (j < J.N) {
             const float2 xv = ((const float2*)J.x)[s * n + (size_t)j * NC + q];
-            const int a = q >= rmax ? 1 : 0, r = q - a * rmax;
+            const int a = q >= RM ? 1 : 0, r = q - a * RM;
             const int v = (int)sqrtf((float)(r + 1) + 0.5f);
             z = cmul(J.mie[((size_t)j * 2 + a) * lmax + v - 1], make_double2(xv.x, xv.y));
         }
@@ -882,23 +893,17 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
     for (int b = 0; b < JB; b++) mx[b] = 0.f;
     for (int idx = tid; idx < NHm * NC; idx += kThreads) {
         const int h = idx / NC, cc = idx - h * NC;
-        const int a = cc >= rmax ? 1 : 0, r = cc - a * rmax;
-        const int2 es = hmap[(size_t)r * nh + h];
+        const int a = cc >= RM ? 1 : 0, r = cc - a * RM;
+        const int2 se = hl_s[r * nh + h];
         double2 w[JB];
 #pragma unroll
         for (int b = 0; b < JB; b++) w[b] = make_double2(0, 0);
-        for (int kind = 0; kind < 2; kind++) {
-            const int e = kind ? es.y : es.x;
-            if (e < 0) continue;
-            const int2 m = wmeta[e];
-            const int cnt = m.y & 0xffff;
-            const int zo = (kind ? 1 - a : a) * rmax;  // B entries couple the other polarization
-            for (int k = m.x; k < m.x + cnt; k++) {
-                const double2 cf = wcoef[k];
-                const int sidx = zo + wsrc[k];
+        for (int k = se.x; k < se.x + se.y; k++) {
+            const int src = hs_s[k];
+            const int zi = ((src >> 8) ? 1 - a : a) * RM + (src & 0xff);  // B terms couple the other polarization
+            const double2 cf = hc_s[k];
 #pragma unroll
-                for (int b = 0; b < JB; b++) cfma(w[b], cf, zs[b][sidx]);
-            }
+            for (int b = 0; b < JB; b++) cfma(w[b], cf, zs[b][zi]);
         }
         const double cinv = cinv_s[h];
 #pragma unroll
@@ -1412,13 +1417,17 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
     const bool ks2 = g_mf_variant[0] == 1;
     if (mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0) {
+        if (T->nhterms > kWhMaxTerms) {
+            mpa_set_error("matfree: %d contraction terms exceed the operand kernel's %d", T->nhterms, kWhMaxTerms);
+            return MPA_ERR_ARG;
+        }
         h8v* Whp = (h8v*)Wx;
         float* taup = (float*)(Whp + (size_t)J.S * J.N * wh_per(J.lmax));
         const dim3 wgrid((unsigned)((J.N + kWhJ - 1) / kWhJ), (unsigned)J.S);
 #define MPA_MF16(L_)                                                                                                \
     do {                                                                                                            \
-        hipLaunchKernelGGL((k_matfree_wh<L_>), wgrid, block, 0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc,         \
-                           T->wcoef, Whp, taup);                                                                    \
+        hipLaunchKernelGGL((k_matfree_wh<L_>), wgrid, block, 0, st, J, sys, nh, T->nhterms, T->hlist,               \
+                           T->hterm_src, T->hterm_coef, Whp, taup);                                                 \
         if (ks2) hipLaunchKernelGGL((k_matfree_mf16<L_, 2>), grid, block, 0, st, J, sys, nch, chunk, (const f4v*)Whp, \
                                     taup, part32);                                                                  \
         else hipLaunchKernelGGL((k_matfree_mf16<L_, 1>), grid, block, 0, st, J, sys, nch, chunk, (const f4v*)Whp,    \

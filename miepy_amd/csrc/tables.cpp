@@ -224,6 +224,38 @@ TransTable* build_table(int lmax) {
         delete T;
         return nullptr;
     }
+    {
+        std::vector<int2> hlist((size_t)rmax * nh, make_int2(0, 0));
+        std::vector<int> hsrc;
+        std::vector<double2> hcoef;
+        for (int r = 0; r < rmax; r++)
+            for (int h = 0; h < nh; h++) {
+                const int2 es = hmap[(size_t)r * nh + h];
+                hlist[(size_t)r * nh + h].x = (int)hsrc.size();
+                for (int kind = 0; kind < 2; kind++) {
+                    const int e = kind ? es.y : es.x;
+                    if (e < 0) continue;
+                    const int2 m = wmeta[e];
+                    for (int k = m.x; k < m.x + (m.y & 0xffff); k++) {
+                        hsrc.push_back(wsrc[k] | (kind << 8));
+                        hcoef.push_back(wcoef[k]);
+                    }
+                }
+                hlist[(size_t)r * nh + h].y = (int)hsrc.size() - hlist[(size_t)r * nh + h].x;
+            }
+        T->nhterms = (int)hsrc.size();
+        if (hipMalloc(&T->hlist, hlist.size() * sizeof(int2)) != hipSuccess ||
+            hipMalloc(&T->hterm_src, std::max<size_t>(1, hsrc.size()) * sizeof(int)) != hipSuccess ||
+            hipMalloc(&T->hterm_coef, std::max<size_t>(1, hcoef.size()) * sizeof(double2)) != hipSuccess ||
+            hipMemcpy(T->hlist, hlist.data(), hlist.size() * sizeof(int2), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(T->hterm_src, hsrc.data(), hsrc.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(T->hterm_coef, hcoef.data(), hcoef.size() * sizeof(double2), hipMemcpyHostToDevice) !=
+                hipSuccess) {
+            mpa_set_error("flattened contraction table for lmax=%d failed", lmax);
+            delete T;
+            return nullptr;
+        }
+    }
     if (hipMalloc(&T->wmeta, wmeta.size() * sizeof(int2)) != hipSuccess ||
         hipMalloc(&T->wsrc, wsrc.size() * sizeof(int)) != hipSuccess ||
         hipMalloc(&T->wcoef, wcoef.size() * sizeof(double2)) != hipSuccess ||
