@@ -109,12 +109,13 @@ def bench_batch(args):
         f"largest n = {max(w.n for w in ws)}")
     out = {}
     res = {}
-    precs = {"all": ["fp64", "mixed", "matfree"], "both": ["fp64", "mixed"]}.get(args.precision, [args.precision])
+    precs = {"all": ["fp64", "mixed", "matfree", "mixed_matfree"], "both": ["fp64", "mixed"]}.get(
+        args.precision, [args.precision])
     for prec in precs:
         B = ClusterBatch(ws, precision=prec)
         for i in range(max(args.warmup, 1)):
             B.solve()
-            if i == 0 and not args.no_autotune and prec != "matfree":
+            if i == 0 and not args.no_autotune and prec in ("fp64", "mixed"):
                 import miepy_amd.device as D
                 big = max(B.clusters, key=lambda c: c.n)
                 tuned = {"matvec_c128": D.autotune_matvec(big.A64, big.S)}
@@ -139,23 +140,40 @@ def bench_batch(args):
         log(f"{prec}: {out[prec]}")
         del B
         torch.cuda.empty_cache()
-    head = "mixed" if "mixed" in out else "fp64"
+    # errors against an FP64 reference solved to 1e-4 x each cluster's tolerance (untimed), so that the
+    # comparison measures each path's own error rather than the FP64 run's stopping error
+    ref = None
+    if any(p in res for p in ("fp64", "mixed", "matfree", "mixed_matfree")):
+        R = ClusterBatch(ws, precision="fp64", tol_scale=1e-4)
+        R.solve()
+        ref = R.host_results()
+        del R
+        torch.cuda.empty_cache()
+    errs = {}
+    for p in res:
+        e = [float(np.max(np.abs(a["C_total"] - b["C_total"]) / np.abs(b["C_total"]).max()))
+             for a, b in zip(res[p], ref)]
+        errs[p] = max(e)
+        out[p]["cross_section_max_rel_err_vs_fp64_reference"] = errs[p]
+    cands = [p for p in ("mixed", "mixed_matfree") if p in out and errs.get(p, 1.0) <= 1e-8]
+    head = min(cands, key=lambda p: out[p]["ms_per_step"]) if cands else ("fp64" if "fp64" in out else precs[0])
     line = {
         "metric": "cluster solves/s (FP32+refine vs FP64, <=1e-8 rel err); p50/p99 solve latency",
         "value": out[head]["clusters_per_s"], "unit": "cluster solves/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": out[head]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c64+fp64-refine" if head == "mixed" else "c128",
+        "vs_baseline": None,
+        "dtype": {"mixed": "c64+fp64-refine", "mixed_matfree": "c64(fp16x3-split MFMA)+fp64-refine"}.get(head, "c128"),
         "data": "synthetic (seeded random clusters)",
         "config": {"workload": "batch256", "clusters": len(ws), "rhs_per_cluster": 8, "tol": 1e-8,
                    "precision": head},
+        "latency_ms": {"p50": out[head]["p50_ms"], "p99": out[head]["p99_ms"]},
         "rhs_per_s": out[head]["clusters_per_s"] * 8, "runs": out,
+        "error_reference": "FP64 batched BiCGSTAB to 1e-12 (1e-4 x tol), untimed",
     }
-    for other in ("mixed", "matfree"):
-        if "fp64" in res and other in res:
-            errs = [float(np.max(np.abs(a["C_total"] - b["C_total"]) / np.abs(b["C_total"]).max()))
-                    for a, b in zip(res[other], res["fp64"])]
+    for other in ("mixed", "matfree", "mixed_matfree"):
+        if "fp64" in out and other in out:
             line[f"{other}_vs_fp64"] = {"speedup": out["fp64"]["ms_per_step"] / out[other]["ms_per_step"],
-                                        "cross_section_max_rel_err": max(errs)}
+                                        "cross_section_max_rel_err": errs[other]}
     print(json.dumps(line), flush=True)
 
 

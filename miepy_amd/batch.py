@@ -4,9 +4,11 @@ Each cluster (positions, radii, permittivities, wavelength, lmax, sources)
 becomes one system (I + T_c) p_inc = p_src with one right-hand side per
 source.  Host work (Mie coefficients, p_src) is done once in `prepare`; the
 device work (assembly of every matrix, one batched BiCGSTAB or one batched
-complex64 + FP64-refinement solve over all clusters, or one batched
-matrix-free FP64 BiCGSTAB with no stored matrices; observables) is
-`ClusterBatch.solve`.  The batched engine covers all clusters with one launch
+complex64 + FP64-refinement solve over all clusters, one batched matrix-free
+FP64 BiCGSTAB with no stored matrices, or one batched refinement whose
+complex64 corrections and FP64 residuals both use the matrix-free operator
+(clusters with lmax <= 3; larger lmax keep stored matrices in the same
+batch); observables) is `ClusterBatch.solve`.  The batched engine covers all clusters with one launch
 per kernel and iteration, and a cluster that converged costs nothing further.
 """
 
@@ -39,26 +41,27 @@ class PreparedCluster:
         self.op = None
 
     def matrix_bytes(self, precision):
-        if precision == "matfree":
+        if precision == "matfree" or (precision == "mixed_matfree" and self.w.lmax <= 3):
             return 0
-        return 16 * self.n * self.n + (8 * self.n * self.n if precision == "mixed" else 0)
+        return 16 * self.n * self.n + (8 * self.n * self.n if precision.startswith("mixed") else 0)
 
 
 class ClusterBatch:
-    def __init__(self, workloads, precision="mixed", inner_tol=1e-5, maxiter=2000, max_refine=8):
-        if precision not in ("fp64", "mixed", "matfree"):
-            raise ValueError("precision must be 'fp64', 'mixed' or 'matfree'")
+    def __init__(self, workloads, precision="mixed", inner_tol=1e-5, maxiter=2000, max_refine=8, tol_scale=1.0):
+        if precision not in ("fp64", "mixed", "matfree", "mixed_matfree"):
+            raise ValueError("precision must be 'fp64', 'mixed', 'matfree' or 'mixed_matfree'")
         self.precision = precision
         self.inner_tol = inner_tol
         self.maxiter = maxiter
         self.max_refine = max_refine
+        self.tol_scale = tol_scale  # solve to tol_scale x each workload's tolerance (reference solutions)
         self.clusters = [PreparedCluster(w) for w in workloads]
         for c in self.clusters:
-            if precision == "matfree":
+            if precision == "matfree" or (precision == "mixed_matfree" and c.w.lmax <= 3):
                 c.op = D.ClusterOperator(c.pos, c.mie_scat, c.k, max_rhs=c.S)
                 continue
             c.A64 = torch.empty((c.n, c.n), dtype=D.C128, device=c.pos.device)
-            if precision == "mixed":
+            if precision.startswith("mixed"):
                 c.A32 = torch.empty((c.n, c.n), dtype=D.C64, device=c.pos.device)
         self.reports = None
         self.results = None
@@ -66,22 +69,25 @@ class ClusterBatch:
     def solve(self):
         """Assemble, solve and reduce every cluster; results stay on the device."""
         cs = self.clusters
+        tols = [c.w.tol * self.tol_scale for c in cs]
+        rhs = [c.p_src.reshape(c.S, c.n) for c in cs]
         if self.precision == "matfree":
-            xs, self.reports = D.bicgstab_batched([c.op for c in cs], [c.p_src.reshape(c.S, c.n) for c in cs],
-                                                  tol=[c.w.tol for c in cs], maxiter=self.maxiter, add_identity=True)
+            xs, self.reports = D.bicgstab_batched([c.op for c in cs], rhs, tol=tols, maxiter=self.maxiter,
+                                                  add_identity=True)
             return self._reduce(xs)
         for c in cs:
-            if self.precision == "mixed":
+            if c.op is not None:
+                continue
+            if c.A32 is not None:
                 D.assemble_dual(c.pos, c.mie_scat, c.k, add_identity=True, out64=c.A64, out32=c.A32)
             else:
                 D.assemble(c.pos, c.mie_scat, c.k, dtype=D.C128, add_identity=True, out=c.A64)
-        rhs = [c.p_src.reshape(c.S, c.n) for c in cs]
-        tols = [c.w.tol for c in cs]
         if self.precision == "fp64":
             xs, self.reports = D.bicgstab_batched([c.A64 for c in cs], rhs, tol=tols, maxiter=self.maxiter)
         else:
-            xs, self.reports = D.solve_refined_batched([c.A32 for c in cs], [c.A64 for c in cs], rhs, tol=tols,
-                                                       inner_tol=self.inner_tol, maxiter=self.maxiter,
+            xs, self.reports = D.solve_refined_batched([c.op if c.op is not None else c.A32 for c in cs],
+                                                       [None if c.op is not None else c.A64 for c in cs], rhs,
+                                                       tol=tols, inner_tol=self.inner_tol, maxiter=self.maxiter,
                                                        max_refine=self.max_refine)
         return self._reduce(xs)
 

@@ -92,7 +92,8 @@ def test_batched_mixed_status(dev):
 def test_cluster_batch_matches_single_clusters(dev):
     """ClusterBatch (all clusters in one batched solve) vs one sphere_cluster
     per cluster: cross-sections and forces per source within 1e-9 (FP64 and
-    matrix-free FP64) and 1e-8 (complex64 + FP64 refinement)."""
+    matrix-free FP64) and 1e-8 (complex64 + FP64 refinement, stored or with
+    the matrix-free operators for lmax <= 3 beside stored lmax-4 systems)."""
     import miepy_amd as M
     from miepy_amd.batch import ClusterBatch
     from miepy_amd.workloads import batch256
@@ -102,7 +103,8 @@ def test_cluster_batch_matches_single_clusters(dev):
         w.N = w.position.shape[0]
         w.radius, w.eps = w.radius[: w.N], w.eps[: w.N]
         w.tol = 1e-11
-    for precision, rtol in (("fp64", 1e-9), ("mixed", 1e-8), ("matfree", 1e-9)):
+    assert {w.lmax for w in ws} >= {4} and min(w.lmax for w in ws) <= 3
+    for precision, rtol in (("fp64", 1e-9), ("mixed", 1e-8), ("matfree", 1e-9), ("mixed_matfree", 1e-8)):
         B = ClusterBatch(ws, precision=precision)
         B.solve()
         assert B.converged()
