@@ -179,6 +179,10 @@ int mpa_bicgstab_matfree(const mpa_cluster_operator* op, int nrhs, int add_ident
  * ([nrhs, n] of dtype; y is overwritten) and keeps the fastest for dtype. */
 int mpa_autotune_matfree(const mpa_cluster_operator* op, int nrhs, int dtype, const void* x, void* y, int reps,
                          int* variant_out, double* ms_out, void* stream);
+/* Matrix-core kernel variant of the matrix-free product in use for dtype;
+ * set_to >= 0 selects one; *count_out (may be NULL) receives the number of
+ * variants; -1 on a bad argument. */
+int mpa_matfree_variant(int dtype, int set_to, int* count_out);
 
 /* ---- batched solves: many independent systems, one set of launches -------- */
 

@@ -109,6 +109,7 @@ SIGNATURES.update({
     "mpa_matfree_matvec": (_i, [_pop, _i, _i, _v, _v, _v]),
     "mpa_matfree_matvec_c64": (_i, [_pop, _i, _i, _v, _v, _v]),
     "mpa_autotune_matfree": (_i, [_pop, _i, _i, _v, _v, _i, _pi, _pd, _v]),
+    "mpa_matfree_variant": (_i, [_i, _i, _pi]),
     "mpa_bicgstab_matfree": (_i, [_pop, _i, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_system_state_bytes": (_sz, [_i, _i, _i]),
     "mpa_batch_scratch_bytes": (_sz, [_i, _ps, _i]),

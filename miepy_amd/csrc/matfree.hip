@@ -285,9 +285,12 @@ __device__ inline Rt xprime(Rt re, Rt im, int odd, int part) {
 constexpr int kFacMaxLmax = 3;
 
 // Matrix-core contraction variants (mpa_autotune_matfree): [c64, c128].
-// c64: 0 one accumulator chain per tile, 1 two (K steps alternate);
-// c128: 0 two chains, 1 one.
+// c64: 0 one accumulator chain per tile, 1 two (K steps alternate), 2 and 3
+// one chain with the next column's special functions interleaved with the
+// MFMAs (8 or 12 VALU instructions per MFMA; FP16-split kernel; the FP32
+// kernel treats 2, 3 as 0); c128: 0 two chains, 1 one.
 int g_mf_variant[2] = {0, 0};
+constexpr int kMfVariants[2] = {4, 2};
 constexpr int kFacMinN = 128;  // smaller clusters keep the LDS kernel (too few lanes)
 
 constexpr int fac_n_of(int r) {
@@ -1073,7 +1076,7 @@ __device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const d
 // B operands, 3 NKB NT FP16 MFMAs into KS chains per tile, the block sum
 // scaled by tau_j / g_ij into the total, the prefetch to the other LDS
 // buffer, one barrier.
-template <int L, int KS>
+template <int L, int KS, int SGB>
 __global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev* __restrict__ sys, int nchunks,
                                                            int chunk, const f4v* __restrict__ Wh,
                                                            const float* __restrict__ tau, float2* __restrict__ part32) {
@@ -1107,14 +1110,21 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev
     f4v pf[PFN];
     wx_fetch<PER, PFN>(pf, Wall + (size_t)jbeg * PER, tid);
     wx_put<PER, PFN>(pf, Ws[0], tid);
+    h8v bh[NKB], bl[NKB];
+    float ginv;
+    if (SGB) hle_f16<L>(xi, yi, zi, J.pos, jbeg, J.k, kinv, kh, live && jbeg != ii, bh, bl, ginv);
     __syncthreads();
     for (int j = jbeg; j < jend; j++) {
         const int buf = (j - jbeg) & 1;
         const int jn = min(j + 1, jend - 1);
         wx_fetch<PER, PFN>(pf, Wall + (size_t)jn * PER, tid);
-        h8v bh[NKB], bl[NKB];
-        float ginv;
-        hle_f16<L>(xi, yi, zi, J.pos, j, J.k, kinv, kh, live && j != ii, bh, bl, ginv);
+        // SGB > 0: the next column's B operands are computed here and their
+        // VALU work is interleaved with this column's MFMAs (SGB VALU
+        // instructions per MFMA); SGB = 0: this column's, before its MFMAs
+        h8v bhn[NKB], bln[NKB];
+        float ginvn;
+        if (SGB) hle_f16<L>(xi, yi, zi, J.pos, jn, J.k, kinv, kh, live && jn != ii, bhn, bln, ginvn);
+        else hle_f16<L>(xi, yi, zi, J.pos, j, J.k, kinv, kh, live && j != ii, bh, bl, ginv);
         f16v acc[NT][KS];
 #pragma unroll
         for (int t = 0; t < NT; t++)
@@ -1133,6 +1143,13 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev
                 a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[kb], a, 0, 0, 0);
                 a = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[kb], a, 0, 0, 0);
             }
+        if (SGB) {
+#pragma unroll
+            for (int t = 0; t < 3 * NKB * NT; t++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x002, SGB, 0);
+            }
+        }
         const float sc = tau_s[j] * ginv;
 #pragma unroll
         for (int t = 0; t < NT; t++)
@@ -1144,6 +1161,14 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev
                 tot[t][q] = fmaf(sc, a, tot[t][q]);
             }
         wx_put<PER, PFN>(pf, Ws[buf ^ 1], tid);
+        if (SGB) {
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                bh[kb] = bhn[kb];
+                bl[kb] = bln[kb];
+            }
+            ginv = ginvn;
+        }
         __syncthreads();
     }
     if (live) {
@@ -1428,10 +1453,16 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     do {                                                                                                            \
         hipLaunchKernelGGL((k_matfree_wh<L_>), wgrid, block, 0, st, J, sys, nh, T->nhterms, T->hlist,               \
                            T->hterm_src, T->hterm_coef, Whp, taup);                                                 \
-        if (ks2) hipLaunchKernelGGL((k_matfree_mf16<L_, 2>), grid, block, 0, st, J, sys, nch, chunk, (const f4v*)Whp, \
-                                    taup, part32);                                                                  \
-        else hipLaunchKernelGGL((k_matfree_mf16<L_, 1>), grid, block, 0, st, J, sys, nch, chunk, (const f4v*)Whp,    \
-                                taup, part32);                                                                      \
+        switch (g_mf_variant[0]) {                                                                                  \
+            case 1: hipLaunchKernelGGL((k_matfree_mf16<L_, 2, 0>), grid, block, 0, st, J, sys, nch, chunk,           \
+                                       (const f4v*)Whp, taup, part32); break;                                       \
+            case 2: hipLaunchKernelGGL((k_matfree_mf16<L_, 1, 8>), grid, block, 0, st, J, sys, nch, chunk,           \
+                                       (const f4v*)Whp, taup, part32); break;                                       \
+            case 3: hipLaunchKernelGGL((k_matfree_mf16<L_, 1, 12>), grid, block, 0, st, J, sys, nch, chunk,          \
+                                       (const f4v*)Whp, taup, part32); break;                                       \
+            default: hipLaunchKernelGGL((k_matfree_mf16<L_, 1, 0>), grid, block, 0, st, J, sys, nch, chunk,          \
+                                        (const f4v*)Whp, taup, part32);                                             \
+        }                                                                                                           \
     } while (0)
         if (J.lmax == 1) MPA_MF16(1);
         else if (J.lmax == 2) MPA_MF16(2);
@@ -1590,6 +1621,17 @@ extern "C" int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int 
     return matfree_launch(J, nullptr, (hipStream_t)stream);
 }
 
+// Variant of the matrix-core contraction kernels in use for dtype; set_to >=
+// 0 selects one; *count_out (may be NULL) receives the number of variants.
+extern "C" int mpa_matfree_variant(int dtype, int set_to, int* count_out) {
+    if (dtype != MPA_C128 && dtype != MPA_C64) return -1;
+    const int idx = dtype == MPA_C64 ? 0 : 1;
+    if (count_out) *count_out = kMfVariants[idx];
+    if (set_to >= kMfVariants[idx]) return -1;
+    if (set_to >= 0) g_mf_variant[idx] = set_to;
+    return g_mf_variant[idx];
+}
+
 // Autotuner for the matrix-core contraction kernels: times each variant on
 // the caller's operator and vectors (y is overwritten) and keeps the fastest
 // for the element type of x (c64 or c128).
@@ -1607,7 +1649,7 @@ extern "C" int mpa_autotune_matfree(const mpa_cluster_operator* op, int nrhs, in
     const int saved = g_mf_variant[idx];
     int best = 0, rc = MPA_OK;
     double best_ms = 1e30;
-    for (int v = 0; v < 2 && rc == MPA_OK; v++) {
+    for (int v = 0; v < kMfVariants[idx] && rc == MPA_OK; v++) {
         g_mf_variant[idx] = v;
         auto run = [&]() {
             return dtype == MPA_C64 ? mpa_matfree_matvec_c64(op, nrhs, 1, x, y, stream)
