@@ -9,12 +9,15 @@ the solve of :664-717 running as HIP kernels on the device:
 
   host:   material data, Mie coefficients (vectorized scipy), p_src
   device: assemble (I + T) -> BiCGSTAB (FP64) or complex64 BiCGSTAB + FP64
-          refinement ('mixed') -> p_scat / p_int, cross-sections, force,
-          torque in one fused kernel.
+          refinement ('mixed'); or, with no stored matrix, BiCGSTAB on the
+          FP64 matrix-free operator ('matfree') or complex64 corrections on
+          the FP16-split matrix-free operator + FP64 matrix-free refinement
+          ('mixed_matfree', lmax <= 3) -> p_scat / p_int, cross-sections,
+          force, torque in one fused kernel.
 
 Additional keyword arguments (the reference hard-codes these,
-cpp/src/interactions.cpp:362-363): precision ('fp64' | 'mixed'), tol,
-maxiter.  `source` may also be a list of sources, solved together as
+cpp/src/interactions.cpp:362-363): precision ('fp64' | 'mixed' | 'matfree'
+| 'mixed_matfree'), tol, maxiter.  `source` may also be a list of sources, solved together as
 several right-hand sides sharing one matrix; results then carry a leading
 source axis.
 """
@@ -69,8 +72,10 @@ class sphere_cluster:
         self.tol = tol
         self.maxiter = maxiter
         self.refine_tol = refine_tol
-        if precision not in ("fp64", "mixed"):
-            raise ValueError("precision must be 'fp64' or 'mixed'")
+        if precision not in ("fp64", "mixed", "matfree", "mixed_matfree"):
+            raise ValueError("precision must be 'fp64', 'mixed', 'matfree' or 'mixed_matfree'")
+        if precision == "mixed_matfree" and lmax > 3:
+            raise ValueError("precision 'mixed_matfree' covers lmax <= 3")
 
         self.auto_origin = False
         if origin is None:
@@ -124,6 +129,14 @@ class sphere_cluster:
                 A = D.assemble(self.position, mie_scat_d, k, dtype=D.C128, add_identity=True)
                 x, self.report = D.bicgstab(A, b, tol=tol, maxiter=maxiter)
                 del A
+            elif self.precision in ("matfree", "mixed_matfree"):
+                op = D.ClusterOperator(self.position, mie_scat_d, k, max_rhs=S)
+                if self.precision == "matfree":
+                    x, self.report = D.bicgstab(op, b, tol=tol, maxiter=maxiter, add_identity=True)
+                else:
+                    rt = tol if self.refine_tol is None else self.refine_tol
+                    x, self.report = D.solve_refined_matfree(op, b, tol=rt, maxiter=maxiter)
+                del op
             else:
                 A64, A32 = D.assemble_dual(self.position, mie_scat_d, k, add_identity=True)
                 rt = tol if self.refine_tol is None else self.refine_tol

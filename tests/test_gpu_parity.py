@@ -226,15 +226,20 @@ def test_end_to_end_hex37(dev, H):
     assert abs(c.cross_sections().absorption) < 1e-9 * c.cross_sections().extinction
 
 
-def test_end_to_end_mixed_matches_fp64(dev, H):
+@pytest.mark.parametrize("precision", ["mixed", "matfree", "mixed_matfree"])
+def test_end_to_end_mixed_matches_fp64(dev, H, precision):
+    """sphere_cluster with complex64 + FP64 refinement (stored or matrix-free)
+    and the FP64 matrix-free solve against the stored FP64 solve (1e-8) and
+    the oracle's LU solve (1e-9 for FP64)."""
     from miepy_amd.workloads import scaled_random
     w = scaled_random(60, 3, seed=1, tol=1e-10)
     f64 = _gpu_cluster(w, sources=w.sources[0])
-    mix = _gpu_cluster(w, precision="mixed", sources=w.sources[0])
+    mix = _gpu_cluster(w, precision=precision, sources=w.sources[0])
     assert mix.report.converged, mix.report
     err = np.linalg.norm(mix.p_scat - f64.p_scat) / np.linalg.norm(f64.p_scat)
     assert err < 1e-8, err
     np.testing.assert_allclose(mix.cross_sections(), f64.cross_sections(), rtol=1e-8)
+    assert np.max(np.abs(mix.force() - f64.force())) <= 1e-8 * np.abs(f64.force()).max()
     ref = _oracle_cluster(H, w, method="exact")
     np.testing.assert_allclose(f64.cross_sections(), ref["cross_sections"], rtol=1e-9)
 
