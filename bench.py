@@ -51,6 +51,12 @@ def parse():
     ap.add_argument("--serve-nmax", type=int, default=1000)
     ap.add_argument("--serve-budget-gb", type=float, default=200.0)
     ap.add_argument("--serve-wall", type=float, default=120.0, help="wall-clock cap of the serving run")
+    ap.add_argument("--serve-matrix-free", default="auto", choices=["auto", "on", "off"],
+                    help="matrix-free systems: auto (lmax <= 3 or matrices over a quarter of the budget), all, none")
+    ap.add_argument("--serve-precision", default="mixed", choices=["mixed", "fp64"],
+                    help="serving solves: FP32 + FP64 refinement (default) or FP64 BiCGSTAB")
+    ap.add_argument("--serve-baseline-seconds", type=float, default=20.0,
+                    help="time budget of the one-at-a-time FP64 (stored) comparison run")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 4 GiB device copy (reference for HBM byte counters) and exit")
     return ap.parse_args()
@@ -69,12 +75,14 @@ def bench_serving(args):
     """Poisson stream of cluster-solve requests served with continuous batching."""
     import torch
     from miepy_amd import workloads as W
-    from miepy_amd.serve import ServingEngine, latency_summary
+    from miepy_amd.serve import Request, ServingEngine, latency_summary
     arrivals = [(t, ws) for t, ws in W.serving_requests(rate=args.serve_rate, duration=args.serve_seconds,
                                                         Nmax=args.serve_nmax)]
     log(f"serving: {len(arrivals)} requests over {args.serve_seconds} s at {args.serve_rate}/s, "
         f"{sum(len(ws) for _, ws in arrivals)} (cluster, wavelength) systems")
-    eng = ServingEngine(hbm_budget_bytes=args.serve_budget_gb * 1e9, iterations_per_step=2)
+    mf = {"auto": "auto", "on": True, "off": False}[args.serve_matrix_free]
+    eng = ServingEngine(hbm_budget_bytes=args.serve_budget_gb * 1e9, iterations_per_step=2, matrix_free=mf,
+                        precision=args.serve_precision)
     t0 = time.perf_counter()
     reqs = eng.run_stream(arrivals, max_wall=args.serve_wall,
                           progress=lambda t, e: log(f"serving t={t:.0f}s: {e.solves_done} systems done, "
@@ -83,19 +91,53 @@ def bench_serving(args):
     torch.cuda.synchronize()
     lat = latency_summary(reqs)
     done_sys = eng.solves_done
+    n_mf = sum(1 for r in reqs for res in r.results if isinstance(res, dict) and res.get("matrix_free"))
+    steps = eng.steps
+    # one-at-a-time FP64 on stored matrices (the reference's mode: one cluster per solve), on the
+    # stream's systems in arrival order until the time budget is spent; systems whose matrix does not
+    # fit the budget are counted as not servable that way
+    del eng
+    torch.cuda.empty_cache()
+    base_done, base_t, base_skip = 0, 0.0, 0
+    for _, ws in arrivals:
+        for w in ws:
+            if base_t >= args.serve_baseline_seconds:
+                break
+            one = ServingEngine(hbm_budget_bytes=args.serve_budget_gb * 1e9, iterations_per_step=8,
+                                matrix_free=False)
+            r = Request(0, [w], 0.0)
+            s0 = time.perf_counter()
+            one.submit(r)
+            one.drain()
+            torch.cuda.synchronize()
+            base_t += time.perf_counter() - s0
+            if r.status == "done":
+                base_done += 1
+            else:
+                base_skip += 1
+            del one
+        if base_t >= args.serve_baseline_seconds:
+            break
     line = {
         "metric": "cluster solves/s (FP32+refine vs FP64, <=1e-8 rel err); p50/p99 solve latency",
-        "value": done_sys / wall, "unit": "cluster solves/s", "n_gpus": 1, "steps": eng.steps, "warmup": 0,
-        "ms_per_step": wall / max(eng.steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "c128", "data": "synthetic (seeded Poisson request stream)",
+        "value": done_sys / wall, "unit": "cluster solves/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": wall / max(steps, 1) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "c64+fp64-refine" if args.serve_precision == "mixed" else "c128",
+        "data": "synthetic (seeded Poisson request stream)",
         "config": {"workload": "serving", "rate_per_s": args.serve_rate, "arrival_seconds": args.serve_seconds,
+                   "precision": args.serve_precision,
                    "Nmax": args.serve_nmax, "hbm_budget_gb": args.serve_budget_gb, "wall_cap_s": args.serve_wall},
         "latency_ms": {k: v for k, v in lat.items() if k.endswith("_ms")},
         "requests": {"arrived": len(reqs), "completed": lat.get("completed", 0),
                      "rejected_too_large": sum(r.status == "too_large" for r in reqs),
                      "unfinished": sum(r.status in ("queued", "running") for r in reqs)},
+        "systems": {"done": done_sys, "matrix_free": n_mf, "policy": args.serve_matrix_free},
         "wall_s": wall,
+        "one_at_a_time_fp64": {"systems": base_done, "not_servable": base_skip, "seconds": base_t,
+                               "solves_per_s": base_done / base_t if base_t > 0 else None},
     }
+    if base_t > 0 and base_done:
+        line["speedup_vs_one_at_a_time_fp64"] = (done_sys / wall) / (base_done / base_t)
     print(json.dumps(line), flush=True)
 
 

@@ -247,6 +247,23 @@ size_t mpa_refine_scratch_bytes(int nsys, const mpa_refine_system* systems);
 int mpa_solve_refined_batched(int nsys, const mpa_refine_system* systems, void* scratch, int* outer_out,
                               int* inner_out, double* resid_out, int* status_out, void* stream);
 
+/* Iteration-level mixed-precision refinement (no counterpart in the
+ * reference; the serving loop's form of mpa_solve_refined_batched): systems
+ * join and leave between calls.  mpa_refine_systems_start sets x = 0 and
+ * starts each system's first complex64 correction; each
+ * mpa_refine_systems_advance call runs `iterations` BiCGSTAB iterations of
+ * the running corrections of the given systems (any subset, in any order of
+ * calls), and a system whose correction finished gets its FP64 update,
+ * residual, per-RHS decision and next correction in the same call.
+ * done_out[k] = 1 once every RHS of the system is decided; outer, inner,
+ * resid and status as mpa_solve_refined_batched.  Same arguments and
+ * scratch (mpa_refine_scratch_bytes) as mpa_solve_refined_batched; the
+ * per-RHS control records live in each system's state. */
+int mpa_refine_systems_start(int nsys, const mpa_refine_system* systems, void* scratch, void* stream);
+int mpa_refine_systems_advance(int nsys, const mpa_refine_system* systems, void* scratch, int iterations,
+                               int* done_out, int* outer_out, int* inner_out, double* resid_out, int* status_out,
+                               void* stream);
+
 /* Bytes of device workspace mpa_solve_refined needs. */
 size_t mpa_refine_workspace_bytes(int n, int nrhs);
 

@@ -110,6 +110,8 @@ SIGNATURES.update({
     "mpa_matfree_matvec_c64": (_i, [_pop, _i, _i, _v, _v, _v]),
     "mpa_autotune_matfree": (_i, [_pop, _i, _i, _v, _v, _i, _pi, _pd, _v]),
     "mpa_matfree_variant": (_i, [_i, _i, _pi]),
+    "mpa_refine_systems_start": (_i, [_i, _prs, _v, _v]),
+    "mpa_refine_systems_advance": (_i, [_i, _prs, _v, _i, _pi, _pi, _pi, _pd, _pi, _v]),
     "mpa_bicgstab_matfree": (_i, [_pop, _i, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_system_state_bytes": (_sz, [_i, _i, _i]),
     "mpa_batch_scratch_bytes": (_sz, [_i, _ps, _i]),

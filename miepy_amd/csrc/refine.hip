@@ -117,9 +117,17 @@ __global__ __launch_bounds__(kThreads) void k_ref_residual(const RefDev* __restr
 size_t v64(int n, int S) { return align256(sizeof(double2) * (size_t)n * S); }
 size_t v32(int n, int S) { return align256(sizeof(float2) * (size_t)n * S); }
 
+// Per-RHS control record of the iteration-level refinement
+// (mpa_refine_systems_start / _advance), kept in the system's state between
+// calls.  The system's phase is held by its RHS 0.
+struct RefCtl {
+    double bn2, rn2, prev, inner_tol;
+    int outer, inner_it, status, phase;  // phase: 0 decide, 1 correction running, 2 finished
+};
+
 size_t own_bytes(int n, int S) {
     return 2 * v64(n, S) + 2 * v32(n, S) + align256(sizeof(double2) * (size_t)S * groups_for(n)) +
-           3 * align256(sizeof(double) * S);
+           3 * align256(sizeof(double) * S) + align256(sizeof(RefCtl) * S);
 }
 
 bool valid(int nsys, const mpa_refine_system* s) {
@@ -167,6 +175,7 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
         d.norms = (double*)p; p += align256(sizeof(double) * S);
         d.scale = (double*)p; p += align256(sizeof(double) * S);
         d.unscale = (double*)p; p += align256(sizeof(double) * S);
+        p += align256(sizeof(RefCtl) * S);  // control records of the iteration-level driver
         d.n = n;
         d.S = S;
         d.G = groups_for(n);
@@ -314,6 +323,296 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
     return worst;
 }
 
+// ---- iteration-level refinement (serving: systems join and leave between calls) ----
+//
+// The same algorithm as refine_batch, split into calls: each call advances
+// the running corrections of all given systems by `iterations` complex64
+// BiCGSTAB iterations; a system whose correction finished gets its FP64
+// update, residual and per-RHS decision in the same call, and its next
+// correction is started (or the system is finished).  Per-RHS control
+// records live in each system's state, so any subset of systems can be
+// advanced together in any call.
+
+struct RefView {
+    RefDev d;
+    RefCtl* ctl;      // device, [S]
+    void* inner_state;
+};
+
+RefView ref_view(const mpa_refine_system& q) {
+    RefView v{};
+    const int n = q.n, S = q.nrhs;
+    char* p = (char*)q.state;
+    v.d.b = (const double2*)q.b;
+    v.d.x = (double2*)q.x;
+    v.d.r = (double2*)p; p += v64(n, S);
+    v.d.y = (double2*)p; p += v64(n, S);
+    v.d.r32 = (float2*)p; p += v32(n, S);
+    v.d.d32 = (float2*)p; p += v32(n, S);
+    v.d.part = (double2*)p; p += align256(sizeof(double2) * (size_t)S * groups_for(n));
+    v.d.norms = (double*)p; p += align256(sizeof(double) * S);
+    v.d.scale = (double*)p; p += align256(sizeof(double) * S);
+    v.d.unscale = (double*)p; p += align256(sizeof(double) * S);
+    v.ctl = (RefCtl*)p; p += align256(sizeof(RefCtl) * S);
+    v.inner_state = p;
+    v.d.n = n;
+    v.d.S = S;
+    v.d.G = groups_for(n);
+    return v;
+}
+
+mpa_system inner_system(const mpa_refine_system& q, const RefView& v, double tol) {
+    mpa_system in{};
+    in.n = q.n;
+    in.nrhs = q.nrhs;
+    in.add_identity = q.add_identity;
+    in.A = q.op ? nullptr : q.A32;
+    in.op = q.op;
+    in.b = v.d.r32;
+    in.x = v.d.d32;
+    in.state = v.inner_state;
+    in.tol = tol;
+    in.maxiter = q.maxiter;
+    return in;
+}
+
+// Work lists over a subset of systems (indices into the full descriptor array).
+struct SubLists {
+    std::vector<int3> vw;
+    std::vector<int2> cw;
+    std::vector<int2> mw[kTileClasses];
+    std::vector<MfJob> mf;
+};
+
+SubLists sub_lists(const std::vector<int>& idx, const mpa_refine_system* R, const std::vector<RefView>& V) {
+    SubLists L;
+    for (int i : idx) {
+        const mpa_refine_system& q = R[i];
+        for (int s = 0; s < q.nrhs; s++) {
+            L.cw.push_back(make_int2(i, s));
+            for (int g = 0; g < V[i].d.G; g++) L.vw.push_back(make_int3(i, s, g));
+        }
+        if (q.op) {
+            const mpa_cluster_operator& o = *q.op;
+            L.mf.push_back(MfJob{o.N, o.lmax, q.nrhs, q.add_identity ? 1 : 0, o.positions, (const double2*)o.mie, o.k,
+                                 (const double2*)q.x, V[i].d.y, (double2*)o.workspace, -1, 0, nullptr, nullptr, 0,
+                                 0});
+        } else {
+            const int rb = gemv_row_blocks(q.n, q.nrhs, 8);
+            for (int b = 0; b < rb; b++) L.mw[gemv_class(q.nrhs)].push_back(make_int2(i, b));
+        }
+    }
+    return L;
+}
+
+int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool start, int iterations, int* done_out,
+                   int* outer_out, int* inner_out, double* resid_out, int* status_out, hipStream_t st) {
+    std::vector<RefView> V(nsys);
+    std::vector<RefDev> ref(nsys);
+    std::vector<MvDev> mv(nsys);
+    std::vector<int> first(nsys + 1, 0);
+    for (int i = 0; i < nsys; i++) {
+        V[i] = ref_view(R[i]);
+        ref[i] = V[i].d;
+        mv[i] = MvDev{R[i].A64, R[i].x, V[i].d.y, R[i].n, R[i].nrhs, R[i].add_identity, -1};
+        first[i + 1] = first[i] + R[i].nrhs;
+    }
+    const int total = first[nsys];
+    // scratch: descriptors, one region each for the (subset) work lists, then the inner engine's
+    char* p = (char*)scratch;
+    RefDev* d_ref = (RefDev*)p; p += align256(sizeof(RefDev) * nsys);
+    MvDev* d_mv = (MvDev*)p; p += align256(sizeof(MvDev) * nsys);
+    size_t nvw = 0, nmw = 0;
+    for (int i = 0; i < nsys; i++) {
+        nvw += (size_t)R[i].nrhs * V[i].d.G;
+        nmw += gemv_row_blocks(R[i].n, R[i].nrhs, 8);
+    }
+    int3* d_vw = (int3*)p; p += align256(sizeof(int3) * nvw);
+    int2* d_cw = (int2*)p; p += align256(sizeof(int2) * total);
+    int2* d_mwbase = (int2*)p; p += kTileClasses * 256 + align256(sizeof(int2) * nmw);
+    void* inner_scratch = p;
+    MPA_HIP_CHECK(hipMemcpyAsync(d_ref, ref.data(), sizeof(RefDev) * nsys, hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(d_mv, mv.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
+    auto upload = [&](const SubLists& L, int2** d_mw, int* n_mw) -> int {
+        if (!L.vw.empty())
+            MPA_HIP_CHECK(hipMemcpyAsync(d_vw, L.vw.data(), sizeof(int3) * L.vw.size(), hipMemcpyHostToDevice, st));
+        if (!L.cw.empty())
+            MPA_HIP_CHECK(hipMemcpyAsync(d_cw, L.cw.data(), sizeof(int2) * L.cw.size(), hipMemcpyHostToDevice, st));
+        char* q = (char*)d_mwbase;
+        for (int c = 0; c < kTileClasses; c++) {
+            d_mw[c] = (int2*)q;
+            n_mw[c] = (int)L.mw[c].size();
+            if (n_mw[c])
+                MPA_HIP_CHECK(hipMemcpyAsync(d_mw[c], L.mw[c].data(), sizeof(int2) * n_mw[c], hipMemcpyHostToDevice, st));
+            q += align256(sizeof(int2) * L.mw[c].size());
+        }
+        return MPA_OK;
+    };
+    int2* d_mw[kTileClasses];
+    int n_mw[kTileClasses];
+    std::vector<RefCtl> ctl(total);
+    std::vector<double> nrm(total);
+    auto read_ctl = [&]() -> int {
+        for (int i = 0; i < nsys; i++)
+            MPA_HIP_CHECK(hipMemcpyAsync(&ctl[first[i]], V[i].ctl, sizeof(RefCtl) * R[i].nrhs, hipMemcpyDeviceToHost, st));
+        MPA_HIP_CHECK(hipStreamSynchronize(st));
+        return MPA_OK;
+    };
+    auto read_norms = [&](const std::vector<int>& idx) -> int {
+        for (int i : idx)
+            MPA_HIP_CHECK(hipMemcpyAsync(&nrm[first[i]], V[i].d.norms, sizeof(double) * R[i].nrhs,
+                                         hipMemcpyDeviceToHost, st));
+        MPA_HIP_CHECK(hipStreamSynchronize(st));
+        return MPA_OK;
+    };
+    int rc;
+    std::vector<int> all(nsys);
+    for (int i = 0; i < nsys; i++) all[i] = i;
+    if (start) {
+        // x = 0, r = b, ||b||^2; every system starts in the decision phase
+        SubLists L = sub_lists(all, R, V);
+        if ((rc = upload(L, d_mw, n_mw))) return rc;
+        hipLaunchKernelGGL(k_ref_start, dim3((unsigned)L.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
+        hipLaunchKernelGGL(k_ref_norms, dim3((unsigned)L.cw.size()), dim3(kThreads), 0, st, d_ref, d_cw);
+        MPA_LAUNCH_CHECK();
+        if ((rc = read_norms(all))) return rc;
+        for (int k = 0; k < total; k++)
+            ctl[k] = RefCtl{nrm[k], nrm[k], INFINITY, 0.0, 0, 0, MPA_NOT_CONVERGED, 0};
+    } else {
+        if ((rc = read_ctl())) return rc;
+    }
+    // A. advance the running corrections
+    std::vector<int> running, finished_inner;
+    for (int i = 0; i < nsys; i++)
+        if (ctl[first[i]].phase == 1) running.push_back(i);
+    if (!running.empty() && iterations > 0) {
+        std::vector<mpa_system> in;
+        int cnt = 0;
+        for (int i : running) {
+            in.push_back(inner_system(R[i], V[i], ctl[first[i]].inner_tol));
+            cnt += R[i].nrhs;
+        }
+        std::vector<int> dn(cnt), its(cnt), stt(cnt);
+        std::vector<double> res(cnt);
+        rc = mpa_systems_advance((int)in.size(), in.data(), MPA_C64, inner_scratch, iterations, dn.data(), its.data(),
+                                 res.data(), stt.data(), st);
+        if (rc) return rc;
+        int o = 0;
+        for (int i : running) {
+            bool all_done = true;
+            for (int s = 0; s < R[i].nrhs; s++) all_done &= dn[o + s] != 0;
+            if (all_done) {
+                for (int s = 0; s < R[i].nrhs; s++) {
+                    RefCtl& c = ctl[first[i] + s];
+                    if (c.status == -1) {  // this RHS had a correction running
+                        c.inner_it += its[o + s];
+                        c.outer += 1;
+                    }
+                }
+                finished_inner.push_back(i);
+            }
+            o += R[i].nrhs;
+        }
+    }
+    // B. finished corrections: x += ||r|| d, FP64 residual and norms
+    if (!finished_inner.empty()) {
+        SubLists L = sub_lists(finished_inner, R, V);
+        if ((rc = upload(L, d_mw, n_mw))) return rc;
+        hipLaunchKernelGGL(k_ref_accum, dim3((unsigned)L.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
+        MPA_LAUNCH_CHECK();
+        bool any_mv = false;
+        for (int c = 0; c < kTileClasses; c++) any_mv |= n_mw[c] > 0;
+        if (any_mv && (rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st))) return rc;
+        for (const MfJob& j : L.mf)
+            if ((rc = matfree_launch(j, nullptr, st))) return rc;
+        hipLaunchKernelGGL(k_ref_residual, dim3((unsigned)L.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
+        hipLaunchKernelGGL(k_ref_norms, dim3((unsigned)L.cw.size()), dim3(kThreads), 0, st, d_ref, d_cw);
+        MPA_LAUNCH_CHECK();
+        if ((rc = read_norms(finished_inner))) return rc;
+        for (int i : finished_inner) {
+            for (int s = 0; s < R[i].nrhs; s++) {
+                RefCtl& c = ctl[first[i] + s];
+                c.prev = c.rn2;
+                c.rn2 = nrm[first[i] + s];
+            }
+            ctl[first[i]].phase = 0;
+        }
+    }
+    // C. decisions: per RHS converged / stalled / out of corrections, or the next correction
+    std::vector<int> deciding;
+    for (int i = 0; i < nsys; i++)
+        if (ctl[first[i]].phase == 0) deciding.push_back(i);
+    if (!deciding.empty()) {
+        std::vector<int> starting;
+        for (int i : deciding) {
+            bool any = false;
+            double need = INFINITY;
+            for (int s = 0; s < R[i].nrhs; s++) {
+                // every RHS is decided afresh, as in refine_batch: one that took no part in the last
+                // correction keeps its residual, so a converged RHS stays converged and a stalled one
+                // stays stalled
+                RefCtl& c = ctl[first[i] + s];
+                const double tol = R[i].tol;
+                int d;
+                if (!std::isfinite(c.rn2)) d = MPA_NONFINITE;
+                else if (c.rn2 <= tol * tol * c.bn2) d = MPA_OK;
+                else if (c.outer > 0 && c.rn2 > 0.0625 * c.prev) d = MPA_NOT_CONVERGED;  // < 4x per step
+                else if (c.outer >= R[i].max_refine) d = MPA_NOT_CONVERGED;
+                else d = -1;
+                c.status = d;
+                if (d == -1) {
+                    any = true;
+                    need = std::min(need, 0.25 * tol * std::sqrt(c.bn2 / c.rn2));
+                }
+            }
+            if (any) {
+                const double itol = std::isfinite(need) ? std::max(R[i].inner_tol, need) : R[i].inner_tol;
+                std::vector<double> sc(R[i].nrhs), usc(R[i].nrhs);
+                for (int s = 0; s < R[i].nrhs; s++) {
+                    RefCtl& c = ctl[first[i] + s];
+                    const bool active = c.status == -1;
+                    sc[s] = active ? 1.0 / std::sqrt(c.rn2) : 0.0;
+                    usc[s] = active ? std::sqrt(c.rn2) : 0.0;
+                    c.inner_tol = itol;
+                }
+                MPA_HIP_CHECK(hipMemcpyAsync(V[i].d.scale, sc.data(), sizeof(double) * R[i].nrhs,
+                                             hipMemcpyHostToDevice, st));
+                MPA_HIP_CHECK(hipMemcpyAsync(V[i].d.unscale, usc.data(), sizeof(double) * R[i].nrhs,
+                                             hipMemcpyHostToDevice, st));
+                MPA_HIP_CHECK(hipStreamSynchronize(st));  // sc / usc are host locals
+                ctl[first[i]].phase = 1;
+                starting.push_back(i);
+            } else {
+                ctl[first[i]].phase = 2;
+            }
+        }
+        if (!starting.empty()) {
+            SubLists L = sub_lists(starting, R, V);
+            if ((rc = upload(L, d_mw, n_mw))) return rc;
+            hipLaunchKernelGGL(k_ref_to32, dim3((unsigned)L.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
+            MPA_LAUNCH_CHECK();
+            std::vector<mpa_system> in;
+            for (int i : starting) in.push_back(inner_system(R[i], V[i], ctl[first[i]].inner_tol));
+            if ((rc = mpa_systems_start((int)in.size(), in.data(), MPA_C64, inner_scratch, st))) return rc;
+        }
+    }
+    for (int i = 0; i < nsys; i++)
+        MPA_HIP_CHECK(hipMemcpyAsync(V[i].ctl, &ctl[first[i]], sizeof(RefCtl) * R[i].nrhs, hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipStreamSynchronize(st));
+    for (int i = 0; i < nsys; i++)
+        for (int s = 0; s < R[i].nrhs; s++) {
+            const int k = first[i] + s;
+            const RefCtl& c = ctl[k];
+            const bool fin = ctl[first[i]].phase == 2;
+            if (done_out) done_out[k] = fin ? 1 : 0;
+            if (outer_out) outer_out[k] = c.outer;
+            if (inner_out) inner_out[k] = c.inner_it;
+            if (resid_out) resid_out[k] = c.bn2 > 0 ? std::sqrt(c.rn2 / c.bn2) : 0.0;
+            if (status_out) status_out[k] = c.status == -1 ? MPA_NOT_CONVERGED : c.status;
+        }
+    return MPA_OK;
+}
+
 size_t refine_scratch_bytes(int nsys, const mpa_refine_system* R) {
     size_t vw = 0, cw = 0, mw = 0;
     std::vector<mpa_system> inner(nsys);
@@ -346,6 +645,26 @@ int mpa_solve_refined_batched(int nsys, const mpa_refine_system* systems, void* 
         return MPA_ERR_ARG;
     }
     return refine_batch(nsys, systems, scratch, outer_out, inner_out, resid_out, status_out, (hipStream_t)stream);
+}
+
+int mpa_refine_systems_start(int nsys, const mpa_refine_system* systems, void* scratch, void* stream) {
+    if (!valid(nsys, systems) || !scratch) {
+        mpa_set_error("refine_systems_start: bad argument");
+        return MPA_ERR_ARG;
+    }
+    return refine_iterate(nsys, systems, scratch, true, 0, nullptr, nullptr, nullptr, nullptr, nullptr,
+                          (hipStream_t)stream);
+}
+
+int mpa_refine_systems_advance(int nsys, const mpa_refine_system* systems, void* scratch, int iterations,
+                               int* done_out, int* outer_out, int* inner_out, double* resid_out, int* status_out,
+                               void* stream) {
+    if (!valid(nsys, systems) || !scratch || iterations < 0) {
+        mpa_set_error("refine_systems_advance: bad argument");
+        return MPA_ERR_ARG;
+    }
+    return refine_iterate(nsys, systems, scratch, false, iterations, done_out, outer_out, inner_out, resid_out,
+                          status_out, (hipStream_t)stream);
 }
 
 size_t mpa_refine_workspace_bytes(int n, int nrhs) {

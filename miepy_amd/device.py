@@ -300,7 +300,10 @@ class ClusterOperator:
     workspace for up to `max_rhs` right-hand sides; T itself is never
     stored, so memory is O(n)."""
 
-    def __init__(self, positions, mie, k, max_rhs=1, tmatrix=None):
+    def __init__(self, positions, mie, k, max_rhs=1, tmatrix=None, workspace=None):
+        """`workspace` (optional): a contiguous uint8 device tensor of at least
+        mpa_matfree_workspace_bytes(N, lmax, max_rhs) bytes to use instead of
+        allocating one (e.g. a block of the serving engine's arena)."""
         lib = L.load()
         self.pos = to_device(positions, torch.float64).reshape(-1, 3)
         self.N = self.pos.shape[0]
@@ -325,7 +328,12 @@ class ClusterOperator:
         nb = lib.mpa_matfree_workspace_bytes(self.N, self.lmax, self.max_rhs)
         if nb == 0:
             raise ValueError(f"matrix-free operator: unsupported N={self.N}, lmax={self.lmax}")
-        self.workspace = torch.empty(nb, dtype=torch.uint8, device=self.device)
+        if workspace is not None:
+            if workspace.dtype != torch.uint8 or not workspace.is_contiguous() or workspace.numel() < nb:
+                raise ValueError(f"matrix-free workspace needs {nb} contiguous bytes")
+            self.workspace = workspace
+        else:
+            self.workspace = torch.empty(nb, dtype=torch.uint8, device=self.device)
         self.struct = L.MpaClusterOperator(self.N, self.lmax, self.pos.data_ptr(),
                                            None if self.mie is None else self.mie.data_ptr(), self.k,
                                            self.workspace.data_ptr(),
@@ -334,6 +342,10 @@ class ClusterOperator:
     def reserve(self, nrhs):
         if nrhs > self.max_rhs:
             self.__init__(self.pos, self.mie, self.k, nrhs, self.tmatrix)
+
+    @staticmethod
+    def workspace_bytes(N, lmax, nrhs):
+        return int(L.load().mpa_matfree_workspace_bytes(int(N), int(lmax), int(nrhs)))
 
 
 def matfree_matvec(op, x, add_identity=False, out=None):

@@ -6,9 +6,14 @@ with one right-hand side per source, and the response carries, per
 wavelength and source, the cross-sections and the per-particle forces.
 
 The engine admits systems first-come first-served while their HBM footprint
-(matrix + Krylov state + solution) fits in a free block of the engine's
-arena (one device allocation of the HBM budget), assembles them and joins
-them to the running batch; every engine step advances all active systems by
+(matrix or matrix-free operator workspace + Krylov state + solution) fits in
+a free block of the engine's arena (one device allocation of the HBM
+budget), assembles them (stored systems) and joins them to the running
+batch.  A system runs matrix-free (FP64 operator recomputed in every
+product, O(n) memory) when `matrix_free` is True, or, with "auto" (the
+default), when lmax <= 3 (the FP64 matrix-core operator is faster than
+reading a stored matrix) or its matrix would take more than a quarter of
+the budget; stored and matrix-free systems share every batched launch; every engine step advances all active systems by
 a few BiCGSTAB iterations in one batched launch sequence
 (`mpa_systems_advance`), retires the systems that converged (observables,
 memory freed) and admits waiting ones into the freed memory.  Requests that
@@ -17,6 +22,13 @@ up, and their iterations overlap with everyone else's.  Matrices, Krylov
 vectors and per-RHS scalars stay on the device for the life of a system;
 each system's arithmetic does not depend on what else is in the batch, so a
 request's result is the same alone and in the mix.
+
+With precision="mixed" every system is solved as FP32 + FP64 refinement
+at the iteration level (`mpa_refine_systems_advance`): complex64 BiCGSTAB
+corrections on a stored complex64 matrix or, for lmax <= 3, the FP16-split
+matrix-free operator, FP64 residuals on the stored complex128 matrix or the
+FP64 matrix-free operator; a system leaves the batch once every source
+reached the tolerance in FP64.
 
 A system whose footprint alone exceeds the budget is rejected with status
 "too_large" instead of blocking the queue.
@@ -36,7 +48,7 @@ from .mie import mie_coefficients
 
 class _System:
     __slots__ = ("req", "index", "w", "n", "S", "k", "eps_b", "pos", "mie_scat", "mie_int", "p_src", "A", "x",
-                 "state", "struct", "done", "iters", "footprint", "block")
+                 "state", "struct", "done", "iters", "footprint", "block", "matrix_free")
 
 
 class Arena:
@@ -84,6 +96,11 @@ class Arena:
     def view(self, off, nbytes, dtype, shape):
         return self.buf[off:off + nbytes].view(dtype).reshape(shape)
 
+    def check_inside(self, block, end):
+        """Raise if [block, end) runs past the block allocated at `block`."""
+        if end > block + self.live[block]:
+            raise RuntimeError(f"arena block at {block} overrun: {end - block} > {self.live[block]} bytes")
+
     @property
     def used(self):
         return sum(self.live.values())
@@ -109,16 +126,33 @@ def _aligned(nbytes):
     return (int(nbytes) + Arena.ALIGN - 1) // Arena.ALIGN * Arena.ALIGN
 
 
-def system_footprint(w):
-    """Arena bytes of one system: c128 matrix, Krylov state and solution."""
+def system_footprint(w, matrix_free=False, precision="fp64"):
+    """Arena bytes of one system: matrices (c128, plus c64 for "mixed") or the
+    matrix-free operator's workspace, solver state and solution."""
     n, S = w.n, len(w.sources)
-    state = L.load().mpa_system_state_bytes(n, S, L.MPA_C128)
-    return _aligned(16 * n * n) + _aligned(state) + _aligned(16 * S * n)
+    lib = L.load()
+    if precision == "mixed":
+        state = lib.mpa_refine_state_bytes(n, S)
+        mats = _aligned(16 * n * n) + _aligned(8 * n * n)  # each matrix at an aligned offset
+    else:
+        state = lib.mpa_system_state_bytes(n, S, L.MPA_C128)
+        mats = _aligned(16 * n * n)
+    op = _aligned(D.ClusterOperator.workspace_bytes(w.N, w.lmax, S)) if matrix_free else mats
+    return op + _aligned(state) + _aligned(16 * S * n)
 
 
 class ServingEngine:
-    def __init__(self, hbm_budget_bytes=200e9, iterations_per_step=2, maxiter=2000, clock=time.perf_counter):
+    def __init__(self, hbm_budget_bytes=200e9, iterations_per_step=2, maxiter=2000, clock=time.perf_counter,
+                 matrix_free="auto", precision="fp64", inner_tol=1e-5, max_refine=8):
         D.require_gpu()
+        if matrix_free not in (True, False, "auto"):
+            raise ValueError("matrix_free must be True, False or 'auto'")
+        if precision not in ("fp64", "mixed"):
+            raise ValueError("precision must be 'fp64' or 'mixed'")
+        self.matrix_free = matrix_free
+        self.precision = precision
+        self.inner_tol = float(inner_tol)
+        self.max_refine = int(max_refine)
         self.arena = Arena(hbm_budget_bytes)
         self.budget = float(self.arena.size)
         self.k = int(iterations_per_step)
@@ -135,11 +169,23 @@ class ServingEngine:
 
     # ---- intake --------------------------------------------------------------------
 
+    def _use_matrix_free(self, w):
+        if self.precision == "mixed":
+            # the complex64 matrix-free operator covers lmax <= 3; larger lmax keep stored matrices
+            if w.lmax > 3 or self.matrix_free is False:
+                return False
+            return self.matrix_free is True or w.n >= 6000 or 24.0 * w.n * w.n > 0.25 * self.budget
+        if self.matrix_free != "auto":
+            return bool(self.matrix_free)
+        # per-system matrix-free launches cost more than one batched product over many small matrices
+        return (w.lmax <= 3 and w.n >= 6000) or 16.0 * w.n * w.n > 0.25 * self.budget
+
     def submit(self, request):
         for i, w in enumerate(request.workloads):
             s = _System()
             s.req, s.index, s.w = request, i, w
-            s.footprint = system_footprint(w)
+            s.matrix_free = self._use_matrix_free(w)
+            s.footprint = system_footprint(w, s.matrix_free, self.precision)
             if s.footprint > self.budget:
                 request.status = "too_large"
                 request.results[i] = {"status": "too_large"}
@@ -163,12 +209,53 @@ class ServingEngine:
         s.mie_int = D.to_device(mi, D.C128)
         s.p_src = D.to_device(p_src, D.C128)
         off = s.block
-        nA, nst = _aligned(16 * s.n * s.n), _aligned(self.lib.mpa_system_state_bytes(s.n, s.S, L.MPA_C128))
-        A = self.arena.view(off, 16 * s.n * s.n, D.C128, (s.n, s.n))
-        s.A = D.assemble(s.pos, s.mie_scat, s.k, dtype=D.C128, add_identity=True, out=A)
+        if self.precision == "mixed":
+            self._prepare_mixed(s, off)
+            return
+        nst = _aligned(self.lib.mpa_system_state_bytes(s.n, s.S, L.MPA_C128))
+        if s.matrix_free:
+            nw = D.ClusterOperator.workspace_bytes(w.N, w.lmax, s.S)
+            ws = self.arena.view(off, nw, torch.uint8, (nw,))
+            s.A = D.ClusterOperator(s.pos, s.mie_scat, s.k, max_rhs=s.S, workspace=ws)
+            nA = _aligned(nw)
+        else:
+            nA = _aligned(16 * s.n * s.n)
+            A = self.arena.view(off, 16 * s.n * s.n, D.C128, (s.n, s.n))
+            s.A = D.assemble(s.pos, s.mie_scat, s.k, dtype=D.C128, add_identity=True, out=A)
         s.state = self.arena.view(off + nA, nst, torch.uint8, (nst,))
         s.x = self.arena.view(off + nA + nst, 16 * s.S * s.n, D.C128, (s.S, s.n))
+        self.arena.check_inside(off, off + nA + nst + 16 * s.S * s.n)
         s.struct = D._sys_struct(s.A, s.p_src.reshape(s.S, s.n), s.x, s.state, w.tol, self.maxiter, False)
+        s.done = False
+        s.iters = 0
+
+    def _prepare_mixed(self, s, off):
+        w = s.w
+        nst = _aligned(self.lib.mpa_refine_state_bytes(s.n, s.S))
+        rs = L.MpaRefineSystem()
+        # the stored matrices hold I + T; the matrix-free operator stands for T
+        rs.n, rs.nrhs, rs.add_identity = s.n, s.S, 1 if s.matrix_free else 0
+        if s.matrix_free:
+            nw = D.ClusterOperator.workspace_bytes(w.N, w.lmax, s.S)
+            ws = self.arena.view(off, nw, torch.uint8, (nw,))
+            s.A = D.ClusterOperator(s.pos, s.mie_scat, s.k, max_rhs=s.S, workspace=ws)
+            nA = _aligned(nw)
+            rs.A32 = rs.A64 = None
+            rs.op = ctypes.cast(ctypes.pointer(s.A.struct), ctypes.c_void_p)
+        else:
+            n64 = _aligned(16 * s.n * s.n)
+            A64 = self.arena.view(off, 16 * s.n * s.n, D.C128, (s.n, s.n))
+            A32 = self.arena.view(off + n64, 8 * s.n * s.n, D.C64, (s.n, s.n))
+            D.assemble_dual(s.pos, s.mie_scat, s.k, add_identity=True, out64=A64, out32=A32)
+            s.A = (A64, A32)
+            nA = n64 + _aligned(8 * s.n * s.n)
+            rs.A32, rs.A64 = A32.data_ptr(), A64.data_ptr()
+        s.state = self.arena.view(off + nA, nst, torch.uint8, (nst,))
+        s.x = self.arena.view(off + nA + nst, 16 * s.S * s.n, D.C128, (s.S, s.n))
+        self.arena.check_inside(off, off + nA + nst + 16 * s.S * s.n)
+        rs.b, rs.x, rs.state = s.p_src.data_ptr(), s.x.data_ptr(), s.state.data_ptr()
+        rs.tol, rs.inner_tol, rs.maxiter, rs.max_refine = float(w.tol), self.inner_tol, self.maxiter, self.max_refine
+        s.struct = rs
         s.done = False
         s.iters = 0
 
@@ -190,15 +277,24 @@ class ServingEngine:
                 s.req.status = "running"
             new.append(s)
         if new:
-            arr = (L.MpaSystem * len(new))(*[s.struct for s in new])
-            scratch = self._scratch_for(arr, len(new))
-            L.check(self.lib.mpa_systems_start(len(new), arr, L.MPA_C128, D._ptr(scratch), D._stream()),
-                    "systems_start")
+            if self.precision == "mixed":
+                arr = (L.MpaRefineSystem * len(new))(*[s.struct for s in new])
+                scratch = self._scratch_for(arr, len(new))
+                L.check(self.lib.mpa_refine_systems_start(len(new), arr, D._ptr(scratch), D._stream()),
+                        "refine_systems_start")
+            else:
+                arr = (L.MpaSystem * len(new))(*[s.struct for s in new])
+                scratch = self._scratch_for(arr, len(new))
+                L.check(self.lib.mpa_systems_start(len(new), arr, L.MPA_C128, D._ptr(scratch), D._stream()),
+                        "systems_start")
             self.active.extend(new)
         return len(new)
 
     def _scratch_for(self, arr, count):
-        need = self.lib.mpa_batch_scratch_bytes(count, arr, L.MPA_C128)
+        if self.precision == "mixed":
+            need = self.lib.mpa_refine_scratch_bytes(count, arr)
+        else:
+            need = self.lib.mpa_batch_scratch_bytes(count, arr, L.MPA_C128)
         if self._scratch.numel() < need:
             self._scratch = torch.empty(int(need * 1.5) + 4096, dtype=torch.uint8, device="cuda")
         return self._scratch
@@ -211,15 +307,22 @@ class ServingEngine:
         if not self.active:
             return False
         count = len(self.active)
-        arr = (L.MpaSystem * count)(*[s.struct for s in self.active])
         total = sum(s.S for s in self.active)
         done = (ctypes.c_int * total)()
         its = (ctypes.c_int * total)()
         res = (ctypes.c_double * total)()
         stt = (ctypes.c_int * total)()
-        scratch = self._scratch_for(arr, count)
-        L.check(self.lib.mpa_systems_advance(count, arr, L.MPA_C128, D._ptr(scratch), self.k, done, its, res, stt,
-                                             D._stream()), "systems_advance")
+        if self.precision == "mixed":
+            arr = (L.MpaRefineSystem * count)(*[s.struct for s in self.active])
+            outer = (ctypes.c_int * total)()
+            scratch = self._scratch_for(arr, count)
+            L.check(self.lib.mpa_refine_systems_advance(count, arr, D._ptr(scratch), self.k, done, outer, its, res,
+                                                        stt, D._stream()), "refine_systems_advance")
+        else:
+            arr = (L.MpaSystem * count)(*[s.struct for s in self.active])
+            scratch = self._scratch_for(arr, count)
+            L.check(self.lib.mpa_systems_advance(count, arr, L.MPA_C128, D._ptr(scratch), self.k, done, its, res,
+                                                 stt, D._stream()), "systems_advance")
         self.steps += 1
         keep, o = [], 0
         for s in self.active:
@@ -240,7 +343,7 @@ class ServingEngine:
             "wavelength": s.w.wavelength,
             "cross_sections": obs["C_total"].cpu().numpy(),  # [S, 3]
             "force": obs["F"].cpu().numpy(),                  # [S, N, 3]
-            "iterations": iters, "residual": resid, "status": status,
+            "iterations": iters, "residual": resid, "status": status, "matrix_free": s.matrix_free,
         }
         self.solves_done += 1
         s.A = s.x = s.state = s.pos = s.mie_scat = s.mie_int = s.p_src = None

@@ -35,17 +35,19 @@ def _results(req):
     return [(r["cross_sections"], r["force"]) for r in req.results]
 
 
-def test_alone_equals_mixed(serve):
+@pytest.mark.parametrize("matrix_free", [False, True, "auto"])
+def test_alone_equals_mixed(serve, matrix_free):
+    """Stored, matrix-free and mixed (auto: matrix-free for lmax <= 3) systems."""
     reqs = _requests(11, 6)
     alone = []
     for t, ws in reqs:
-        eng = serve.ServingEngine(hbm_budget_bytes=50e9, iterations_per_step=3)
+        eng = serve.ServingEngine(hbm_budget_bytes=50e9, iterations_per_step=3, matrix_free=matrix_free)
         r = serve.Request(0, ws, 0.0)
         eng.submit(r)
         eng.drain()
         assert r.status == "done"
         alone.append(_results(r))
-    eng = serve.ServingEngine(hbm_budget_bytes=50e9, iterations_per_step=1)
+    eng = serve.ServingEngine(hbm_budget_bytes=50e9, iterations_per_step=1, matrix_free=matrix_free)
     mixed = []
     for i, (t, ws) in enumerate(reqs):
         r = serve.Request(i, ws, 0.0)
@@ -64,7 +66,7 @@ def test_admission_by_footprint_and_rejection(serve):
     from miepy_amd import workloads as W
     reqs = _requests(12, 5)
     fp = max(serve.system_footprint(w) for _, ws in reqs for w in ws)
-    eng = serve.ServingEngine(hbm_budget_bytes=1.5 * fp, iterations_per_step=2)
+    eng = serve.ServingEngine(hbm_budget_bytes=1.5 * fp, iterations_per_step=2, matrix_free=False)
     rs = []
     for i, (t, ws) in enumerate(reqs):
         r = serve.Request(i, ws, 0.0)
@@ -80,3 +82,77 @@ def test_admission_by_footprint_and_rejection(serve):
     rb = serve.Request(99, [big], 0.0)
     eng.submit(rb)
     assert rb.status == "too_large" and not eng.queue
+
+
+def test_matrix_free_admits_what_a_matrix_cannot(serve):
+    """Under a budget far below its matrix (12.5 GB), an lmax-5 system of
+    n = 28000 runs matrix-free (O(n) footprint) and matches the FP64
+    matrix-free solve of the same cluster; stored small systems share the
+    batch with it."""
+    from miepy_amd import workloads as W
+    import miepy_amd.device as D
+    from miepy_amd.mie import mie_coefficients
+    big = W.scaled_random(400, 5, seed=1)
+    big.tol = 1e-8
+    small = [ws[0] for _, ws in _requests(13, 2)]
+    eng = serve.ServingEngine(hbm_budget_bytes=2e9, iterations_per_step=4)
+    assert serve.system_footprint(big, True) < 0.1 * eng.budget < 16 * big.n ** 2
+    rb = serve.Request(0, [big], 0.0)
+    eng.submit(rb)
+    rs = [serve.Request(i + 1, [w], 0.0) for i, w in enumerate(small)]
+    for r in rs:
+        eng.submit(r)
+    eng.drain()
+    assert rb.status == "done" and all(r.status == "done" for r in rs)
+    # the same system solved directly with the FP64 matrix-free operator
+    cond = np.zeros(big.N, dtype=bool)
+    ms, mi = mie_coefficients(big.radius, big.eps, np.ones(big.N), big.medium_eps, 1.0, big.k, big.lmax, cond)
+    op = D.ClusterOperator(big.position, ms, big.k, max_rhs=len(big.sources))
+    b = D.to_device(np.stack([s.structure(big.position, big.k, big.lmax) for s in big.sources]), D.C128)
+    x, rep = D.bicgstab(op, b.reshape(len(big.sources), -1), tol=big.tol, maxiter=2000, add_identity=True)
+    obs = D.observables(x.reshape(len(big.sources), big.N, 2, big.rmax), b.reshape(len(big.sources), big.N, 2,
+                        big.rmax), D.to_device(ms, D.C128), D.to_device(mi, D.C128), big.k,
+                        float(np.real(big.medium_eps)), want_int=False, want_torque=False)
+    np.testing.assert_allclose(rb.results[0]["cross_sections"], obs["C_total"].cpu().numpy(), rtol=1e-12)
+
+
+def test_mixed_precision_serving(serve):
+    """precision='mixed': iteration-level complex64 corrections + FP64
+    refinement (stored matrices, and the FP16-split matrix-free operator for
+    lmax <= 3).  A request's results are the same alone and in the mix (bit
+    for bit) and match the FP64 engine's to 1e-9 (both solved to 1e-11)."""
+    reqs = _requests(21, 5)
+    for _, ws in reqs:
+        for w in ws:
+            w.tol = 1e-11
+    ref = []
+    for _, ws in reqs:
+        e64 = serve.ServingEngine(hbm_budget_bytes=50e9, iterations_per_step=4, matrix_free=False)
+        r = serve.Request(0, ws, 0.0)
+        e64.submit(r)
+        e64.drain()
+        ref.append(_results(r))
+    for mf in (False, "auto", True):
+        alone = []
+        for _, ws in reqs:
+            eng = serve.ServingEngine(hbm_budget_bytes=50e9, iterations_per_step=3, precision="mixed",
+                                      matrix_free=mf)
+            r = serve.Request(0, ws, 0.0)
+            eng.submit(r)
+            eng.drain()
+            assert r.status == "done"
+            alone.append(_results(r))
+        eng = serve.ServingEngine(hbm_budget_bytes=50e9, iterations_per_step=2, precision="mixed", matrix_free=mf)
+        mixed = []
+        for i, (_, ws) in enumerate(reqs):
+            r = serve.Request(i, ws, 0.0)
+            eng.submit(r)
+            mixed.append(r)
+            eng.step()
+        eng.drain()
+        for a, r, f in zip(alone, mixed, ref):
+            assert r.status == "done"
+            for (ca, fa), (cm, fm), (c64, f64) in zip(a, _results(r), f):
+                assert np.array_equal(ca, cm) and np.array_equal(fa, fm)
+                np.testing.assert_allclose(cm, c64, rtol=1e-9, atol=1e-9 * np.abs(c64).max())
+                assert np.max(np.abs(fm - f64)) <= 1e-8 * np.abs(f64).max()
