@@ -7,8 +7,10 @@ device work (assembly of every matrix, one batched BiCGSTAB or one batched
 complex64 + FP64-refinement solve over all clusters, one batched matrix-free
 FP64 BiCGSTAB with no stored matrices, or one batched refinement whose
 complex64 corrections and FP64 residuals both use the matrix-free operator
-(clusters with lmax <= 3; larger lmax keep stored matrices in the same
-batch); observables) is `ClusterBatch.solve`.  The batched engine covers all clusters with one launch
+for clusters with lmax <= 4 and n >= matfree_min_n (each matrix-free
+system launches its own operator kernels, so small clusters keep stored
+matrices, whose products run as one batched launch); observables) is
+`ClusterBatch.solve`.  The batched engine covers all clusters with one launch
 per kernel and iteration, and a cluster that converged costs nothing further.
 """
 
@@ -40,14 +42,16 @@ class PreparedCluster:
         self.A32 = None
         self.op = None
 
-    def matrix_bytes(self, precision):
-        if precision == "matfree" or (precision == "mixed_matfree" and self.w.lmax <= 3):
+    def matrix_bytes(self, precision, matfree_min_n=6000):
+        if precision == "matfree" or (precision == "mixed_matfree" and self.w.lmax <= 4 and
+                                      self.n >= matfree_min_n):
             return 0
         return 16 * self.n * self.n + (8 * self.n * self.n if precision.startswith("mixed") else 0)
 
 
 class ClusterBatch:
-    def __init__(self, workloads, precision="mixed", inner_tol=1e-5, maxiter=2000, max_refine=8, tol_scale=1.0):
+    def __init__(self, workloads, precision="mixed", inner_tol=1e-5, maxiter=2000, max_refine=8, tol_scale=1.0,
+                 matfree_min_n=6000):
         if precision not in ("fp64", "mixed", "matfree", "mixed_matfree"):
             raise ValueError("precision must be 'fp64', 'mixed', 'matfree' or 'mixed_matfree'")
         self.precision = precision
@@ -57,7 +61,8 @@ class ClusterBatch:
         self.tol_scale = tol_scale  # solve to tol_scale x each workload's tolerance (reference solutions)
         self.clusters = [PreparedCluster(w) for w in workloads]
         for c in self.clusters:
-            if precision == "matfree" or (precision == "mixed_matfree" and c.w.lmax <= 3):
+            if precision == "matfree" or (precision == "mixed_matfree" and c.w.lmax <= 4 and
+                                          c.n >= matfree_min_n):
                 c.op = D.ClusterOperator(c.pos, c.mie_scat, c.k, max_rhs=c.S)
                 continue
             c.A64 = torch.empty((c.n, c.n), dtype=D.C128, device=c.pos.device)

@@ -839,14 +839,20 @@ __device__ __forceinline__ void combo_pm(int u, int PM, int& p, int& m) {
     p = m + (u - base);
 }
 
-// Per group of kWhJ column particles j (blockIdx.x) and RHS (blockIdx.y):
+// Per group of WhShape<L>::JB column particles j (blockIdx.x) and RHS (blockIdx.y):
 // W_j / c_pm into LDS, tau_j = its largest modulus, then the A operands (hi
 // and lo) in lane order Wh[s][j][hl][tile][kb][lane] (8 halves each) and
 // tau[s][j].  The flattened term lists of every (r, h) are staged in LDS
 // first (a few coalesced loads per thread), so the per-entry lookups are LDS
 // reads instead of chains of dependent global loads.
-constexpr int kWhJ = 2;
-constexpr int kWhMaxTerms = 1024;  // LDS term capacity (lmax 3 has 789)
+constexpr int kWhMaxTerms = 1024;  // LDS term capacity (lmax 3 has 789); lmax 4 reads its terms through the caches
+
+// column particles per workgroup and whether the term lists are staged in LDS
+template <int L>
+struct WhShape {
+    static constexpr int JB = L <= 3 ? 2 : 1;
+    static constexpr bool TLDS = L <= 3;
+};
 
 template <int L>
 __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* __restrict__ sys, int nh,
@@ -856,23 +862,28 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
                                                          float* __restrict__ tau) {
     using Sh = F16Shape<L>;
     constexpr int NHm = (2 * L + 1) * (2 * L + 1);
-    constexpr int NC = Sh::NC, JB = kWhJ, RM = L * (L + 2);
+    constexpr int NC = Sh::NC, JB = WhShape<L>::JB, RM = L * (L + 2);
+    constexpr bool TLDS = WhShape<L>::TLDS;
     __shared__ float2 Wn[JB][NHm][NC];
     __shared__ double2 zs[JB][NC];  // Mie coefficient times x of column j, per (polarization, r)
     __shared__ double cinv_s[NHm];  // 1 / c_pm = sqrt((p+m)!/(p-m)!) per HLE index
     __shared__ int2 hl_s[RM * NHm];
-    __shared__ int hs_s[kWhMaxTerms];
-    __shared__ double2 hc_s[kWhMaxTerms];
+    __shared__ int slot_h[2][Sh::NKB * 4];  // HLE index of K slot u for lane half kh, -1 for the zero slots
+    __shared__ int hs_s[TLDS ? kWhMaxTerms : 1];
+    __shared__ double2 hc_s[TLDS ? kWhMaxTerms : 1];
     __shared__ float red[JB][kThreads / 64];
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const int j0 = blockIdx.x * JB, s = blockIdx.y, tid = threadIdx.x;
     const int lmax = J.lmax;
     const size_t n = (size_t)NC * J.N;
     for (int q = tid; q < RM * NHm; q += kThreads) hl_s[q] = hlist[q];
-    for (int q = tid; q < nterms; q += kThreads) {
-        hs_s[q] = hsrc[q];
-        hc_s[q] = hcoef[q];
-    }
+    if (TLDS)
+        for (int q = tid; q < nterms; q += kThreads) {
+            hs_s[q] = hsrc[q];
+            hc_s[q] = hcoef[q];
+        }
+    const int* tsrc = TLDS ? hs_s : hsrc;
+    const double2* tcoef = TLDS ? hc_s : hcoef;
     if (tid < JB * NC) {
         const int b = tid / NC, q = tid - b * NC, j = j0 + b;
         double2 z = make_double2(0, 0);
@@ -890,6 +901,16 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
         for (int q = p - m + 1; q <= p + m; q++) prod *= q;
         cinv_s[tid] = sqrt(prod);
     }
+    if (tid < 2 * Sh::NKB * 4) {
+        const int kh = tid / (Sh::NKB * 4), u = tid - kh * (Sh::NKB * 4);
+        int h = -1;
+        if (u < Sh::NCOMBO) {
+            int p, m;
+            combo_pm(u, Sh::PM, p, m);
+            if (!(kh && m == 0)) h = p * (p + 1) + (kh ? -m : m);
+        }
+        slot_h[kh][u] = h;
+    }
     __syncthreads();
     float mx[JB];
 #pragma unroll
@@ -902,9 +923,9 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
 #pragma unroll
         for (int b = 0; b < JB; b++) w[b] = make_double2(0, 0);
         for (int k = se.x; k < se.x + se.y; k++) {
-            const int src = hs_s[k];
+            const int src = tsrc[k];
             const int zi = ((src >> 8) ? 1 - a : a) * RM + (src & 0xff);  // B terms couple the other polarization
-            const double2 cf = hc_s[k];
+            const double2 cf = tcoef[k];
 #pragma unroll
             for (int b = 0; b < JB; b++) cfma(w[b], cf, zs[b][zi]);
         }
@@ -937,13 +958,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
         h8v hv, lv;
 #pragma unroll
         for (int e2 = 0; e2 < 4; e2++) {
-            const int u = 4 * kb + e2;
-            float2 wv = make_float2(0.f, 0.f);
-            if (cc < NC && u < Sh::NCOMBO) {
-                int p, m;
-                combo_pm(u, Sh::PM, p, m);
-                if (!(kh && m == 0)) wv = Wn[b][p * (p + 1) + (kh ? -m : m)][cc];
-            }
+            const int h = slot_h[kh][4 * kb + e2];
+            const float2 wv = (cc < NC && h >= 0) ? Wn[b][h][cc] : make_float2(0.f, 0.f);
             // A[col = 2cc + part][K = (Re, Im) of the slot]
             f2v av;
             av.x = (part ? wv.y : wv.x) * tinv;
@@ -1381,7 +1397,10 @@ inline int wx_per(int lmax) {
 
 // 16-byte vectors of Wh per column particle (FP16 kernel)
 inline int wh_per(int lmax) {
-    return lmax == 1 ? F16Shape<1>::PER : lmax == 2 ? F16Shape<2>::PER : F16Shape<3>::PER;
+    return lmax == 1 ? F16Shape<1>::PER
+           : lmax == 2 ? F16Shape<2>::PER
+           : lmax == 3 ? F16Shape<3>::PER
+                       : F16Shape<4>::PER;
 }
 
 inline int fac_entries_of(int lmax) {
@@ -1423,11 +1442,13 @@ size_t matfree_workspace_bytes(int N, int lmax, int S) {
     return b;
 }
 
-bool matfree_c64_supported(int N, int lmax, bool tmatrix) { return lmax <= kFacMaxLmax && N >= 1 && !tmatrix; }
+// lmax <= 3: FP16-split or FP32 matrix-core kernel; lmax 4: FP16-split kernel (whatever the option says)
+constexpr int kC64MaxLmax = 4;
+bool matfree_c64_supported(int N, int lmax, bool tmatrix) { return lmax <= kC64MaxLmax && N >= 1 && !tmatrix; }
 
 int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const TransTable* T) {
     if (!matfree_c64_supported(J.N, J.lmax, J.tmat != nullptr)) {
-        mpa_set_error("matfree: the complex64 operator covers spheres with lmax <= %d", kFacMaxLmax);
+        mpa_set_error("matfree: the complex64 operator covers spheres with lmax <= %d", kC64MaxLmax);
         return MPA_ERR_ARG;
     }
     const int nh = T->nh;
@@ -1441,14 +1462,15 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     ProfScope prof(5, st);
     const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
     const bool ks2 = g_mf_variant[0] == 1;
-    if (mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0) {
-        if (T->nhterms > kWhMaxTerms) {
+    if (mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0 || J.lmax > kFacMaxLmax) {
+        if (J.lmax <= 3 && T->nhterms > kWhMaxTerms) {
             mpa_set_error("matfree: %d contraction terms exceed the operand kernel's %d", T->nhterms, kWhMaxTerms);
             return MPA_ERR_ARG;
         }
         h8v* Whp = (h8v*)Wx;
         float* taup = (float*)(Whp + (size_t)J.S * J.N * wh_per(J.lmax));
-        const dim3 wgrid((unsigned)((J.N + kWhJ - 1) / kWhJ), (unsigned)J.S);
+        const int jb = J.lmax <= 3 ? WhShape<3>::JB : WhShape<4>::JB;
+        const dim3 wgrid((unsigned)((J.N + jb - 1) / jb), (unsigned)J.S);
 #define MPA_MF16(L_)                                                                                                \
     do {                                                                                                            \
         hipLaunchKernelGGL((k_matfree_wh<L_>), wgrid, block, 0, st, J, sys, nh, T->nhterms, T->hlist,               \
@@ -1466,7 +1488,8 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     } while (0)
         if (J.lmax == 1) MPA_MF16(1);
         else if (J.lmax == 2) MPA_MF16(2);
-        else MPA_MF16(3);
+        else if (J.lmax == 3) MPA_MF16(3);
+        else MPA_MF16(4);
 #undef MPA_MF16
         hipLaunchKernelGGL(k_matfree_sum32, dim3((unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads)),
                            dim3(kThreads), 0, st, J, sys, nch, part32);

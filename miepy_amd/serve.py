@@ -9,11 +9,12 @@ The engine admits systems first-come first-served while their HBM footprint
 (matrix or matrix-free operator workspace + Krylov state + solution) fits in
 a free block of the engine's arena (one device allocation of the HBM
 budget), assembles them (stored systems) and joins them to the running
-batch.  A system runs matrix-free (FP64 operator recomputed in every
+batch.  A system runs matrix-free (the operator recomputed in every
 product, O(n) memory) when `matrix_free` is True, or, with "auto" (the
-default), when lmax <= 3 (the FP64 matrix-core operator is faster than
-reading a stored matrix) or its matrix would take more than a quarter of
-the budget; stored and matrix-free systems share every batched launch; every engine step advances all active systems by
+default), when it is large enough for the matrix-free kernels to beat
+reading a stored matrix (FP64: lmax <= 3 and n >= 6000; mixed: lmax <= 4
+and n >= 6000) or its matrices would take more than a quarter of the
+budget; stored and matrix-free systems share every batched launch; every engine step advances all active systems by
 a few BiCGSTAB iterations in one batched launch sequence
 (`mpa_systems_advance`), retires the systems that converged (observables,
 memory freed) and admits waiting ones into the freed memory.  Requests that
@@ -25,7 +26,7 @@ request's result is the same alone and in the mix.
 
 With precision="mixed" every system is solved as FP32 + FP64 refinement
 at the iteration level (`mpa_refine_systems_advance`): complex64 BiCGSTAB
-corrections on a stored complex64 matrix or, for lmax <= 3, the FP16-split
+corrections on a stored complex64 matrix or, for lmax <= 4, the FP16-split
 matrix-free operator, FP64 residuals on the stored complex128 matrix or the
 FP64 matrix-free operator; a system leaves the batch once every source
 reached the tolerance in FP64.
@@ -171,8 +172,8 @@ class ServingEngine:
 
     def _use_matrix_free(self, w):
         if self.precision == "mixed":
-            # the complex64 matrix-free operator covers lmax <= 3; larger lmax keep stored matrices
-            if w.lmax > 3 or self.matrix_free is False:
+            # the complex64 matrix-free operator covers lmax <= 4; larger lmax keep stored matrices
+            if w.lmax > 4 or self.matrix_free is False:
                 return False
             return self.matrix_free is True or w.n >= 6000 or 24.0 * w.n * w.n > 0.25 * self.budget
         if self.matrix_free != "auto":

@@ -105,7 +105,7 @@ def test_cluster_batch_matches_single_clusters(dev):
         w.tol = 1e-11
     assert {w.lmax for w in ws} >= {4} and min(w.lmax for w in ws) <= 3
     for precision, rtol in (("fp64", 1e-9), ("mixed", 1e-8), ("matfree", 1e-9), ("mixed_matfree", 1e-8)):
-        B = ClusterBatch(ws, precision=precision)
+        B = ClusterBatch(ws, precision=precision, matfree_min_n=1000)  # the larger clusters run matrix-free
         B.solve()
         assert B.converged()
         res = B.host_results()

@@ -241,7 +241,8 @@ def _c64_vs_c128(dev, pos, mie, k, S, seed, identity):
     return np.max(np.abs(got - ref)) / np.max(np.abs(ref))
 
 
-@pytest.mark.parametrize("N,lmax,S", [(7, 1, 1), (20, 2, 2), (45, 3, 1), (300, 3, 1), (160, 2, 3), (130, 1, 2)])
+@pytest.mark.parametrize("N,lmax,S", [(7, 1, 1), (20, 2, 2), (45, 3, 1), (300, 3, 1), (160, 2, 3), (130, 1, 2),
+                                     (9, 4, 1), (150, 4, 2)])
 def test_c64_matrix_core_operator(dev, c64_kernel, N, lmax, S):
     """The complex64 operator (special functions in FP64/FP32, FP16-split or
     FP32 matrix cores) equals the FP64 operator to FP32 accuracy: 2e-6
@@ -252,13 +253,14 @@ def test_c64_matrix_core_operator(dev, c64_kernel, N, lmax, S):
     assert _c64_vs_c128(dev, pos, mie, k, S, N + 2, False) <= 5e-6
 
 
+@pytest.mark.parametrize("lmax", [3, 4])
 @pytest.mark.parametrize("kd_near,kd_far", [(0.05, 5.0), (2.0, 2000.0)])
-def test_c64_operator_extreme_separations(dev, c64_kernel, kd_near, kd_far):
+def test_c64_operator_extreme_separations(dev, c64_kernel, kd_near, kd_far, lmax):
     """Pairs far inside the near field (|h_6(kd)| ~ 1e15 at kd = 0.05, far
     beyond the FP16 range before scaling) next to well separated pairs: the
     per-pair and per-column scaling keeps the FP16-split operator at FP32
-    accuracy."""
-    lmax = 3
+    accuracy (lmax 3 and 4; lmax 4 runs the FP16-split kernel whatever the
+    kernel option)."""
     k = 2 * np.pi / (600 * nm)
     rng = np.random.default_rng(11)
     centers = rng.normal(size=(6, 3))
