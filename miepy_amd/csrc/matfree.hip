@@ -473,7 +473,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_fac(MfJob J, const SysDev*
 inline bool fac_path(int N, int lmax) {
     return lmax <= kFacMaxLmax && N >= kFacMinN && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 2;
 }
-inline bool mf64_path(int lmax) { return lmax <= kFacMaxLmax && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 0; }
+constexpr int kMf64MaxLmax = 4;  // FP64 matrix-core kernel: lmax <= 4 (128 KB of LDS at lmax 4)
+inline bool mf64_path(int lmax) { return lmax <= kMf64MaxLmax && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 0; }
 inline bool fac_possible(int N, int lmax) { return lmax <= kFacMaxLmax && N >= kFacMinN; }
 
 // ---- complex64 operator on the FP32 matrix cores (mixed-precision inner solves)
@@ -1427,7 +1428,7 @@ inline int partial_slots(int N) { return std::max(std::max(chunks_for(N), fac_ch
 size_t matfree_workspace_bytes(int N, int lmax, int S) {
     const size_t n = (size_t)2 * lmax_to_rmax(lmax) * N;
     size_t b = sizeof(double2) * n * S * (partial_slots(N) + 1);
-    if (lmax <= kFacMaxLmax) {  // W of the factorized kernel or Wm of the matrix-core kernel
+    if (lmax <= kMf64MaxLmax) {  // W of the factorized kernel or Wm of the matrix-core kernel
         const int nh = (2 * lmax + 1) * (2 * lmax + 1);
         size_t w = sizeof(double2) * (size_t)S * nh * n;
         if (fac_possible(N, lmax)) w = std::max(w, sizeof(double2) * 2 * (size_t)S * N * fac_entries_of(lmax));
@@ -1558,7 +1559,8 @@ int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
     } while (0)
         if (J.lmax == 1) MPA_MF64(1);
         else if (J.lmax == 2) MPA_MF64(2);
-        else MPA_MF64(3);
+        else if (J.lmax == 3) MPA_MF64(3);
+        else MPA_MF64(4);
 #undef MPA_MF64
         hipLaunchKernelGGL(k_matfree_sum, dim3(sum_blocks), dim3(kThreads), 0, st, J, sys, nch);
         MPA_LAUNCH_CHECK();
