@@ -404,6 +404,63 @@ def cpu_baseline(w, prob, iters, gemvs):
                         f"-> {t_solve:.2f} s per solve"))
 
 
+def cpu_stages(w, prob, threads, min_repeats=5, warmup=2, min_seconds=0.1):
+    """Per-stage timing of the oracle's full CPU path (assembly, BiCGSTAB at the
+    reference default 1e-5 and at the workload tolerance, cross-sections,
+    forces) on `threads` OpenMP threads: `warmup` untimed runs, then at least
+    `min_repeats` timed runs and `min_seconds` in total; mean / p50 / p99 per
+    stage in ms.  For workloads whose full solve takes well under a second."""
+    import oracle as O
+    O.build()
+    O.set_threads(threads)
+    b = np.ascontiguousarray(prob.p_src[0].reshape(-1))
+    N, rmax = w.N, w.rmax
+    ms = prob.mie_scat
+    n_idx = np.concatenate([[n - 1] * (2 * n + 1) for n in range(1, w.lmax + 1)])
+    eps_b = float(np.real(w.medium_eps))
+
+    def one():
+        t = {}
+        s0 = time.perf_counter()
+        T = O.sphere_aggregate_tmatrix(w.position, ms, w.k)
+        t["assembly"] = time.perf_counter() - s0
+        s0 = time.perf_counter()
+        O.solve_linear_system(T, b, tol=1e-5, maxiter=1000)
+        t["solve_tol1e-5"] = time.perf_counter() - s0
+        s0 = time.perf_counter()
+        x, _, _ = O.solve_linear_system(T, b, tol=w.tol, maxiter=10 * w.n)
+        t["solve_config_tol"] = time.perf_counter() - s0
+        p_inc = x.reshape(N, 2, rmax)
+        p_scat = p_inc * ms[:, :, n_idx]
+        s0 = time.perf_counter()
+        for i in range(N):
+            O.particle_cross_sections(p_scat[i], p_inc[i], prob.p_src[0].reshape(N, 2, rmax)[i], w.k)
+        t["cross_sections"] = time.perf_counter() - s0
+        s0 = time.perf_counter()
+        O.force_all(p_scat, p_inc, w.k, eps_b, 1.0)
+        t["forces"] = time.perf_counter() - s0
+        return t
+
+    for _ in range(warmup):
+        one()
+    runs = []
+    t0 = time.perf_counter()
+    while len(runs) < min_repeats or time.perf_counter() - t0 < min_seconds:
+        runs.append(one())
+    O.set_threads(O.max_threads())
+    out = {}
+    for k in runs[0]:
+        v = np.array([r[k] for r in runs]) * 1e3
+        out[k] = {"mean_ms": float(v.mean()), "p50_ms": float(np.percentile(v, 50)),
+                  "p99_ms": float(np.percentile(v, 99))}
+    tot = np.array([r["assembly"] + r["solve_config_tol"] + r["cross_sections"] + r["forces"] for r in runs]) * 1e3
+    out["total_config_tol"] = {"mean_ms": float(tot.mean()), "p50_ms": float(np.percentile(tot, 50)),
+                               "p99_ms": float(np.percentile(tot, 99))}
+    out["threads"] = threads
+    out["repeats"] = len(runs)
+    return out
+
+
 def main():
     args = parse()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -547,7 +604,21 @@ def main():
     if not args.no_cpu_baseline and "fp64" in results:
         it = int(max(results["fp64"]["report"].iterations))
         log("cpu baseline (oracle) ...")
-        line["cpu_baseline"] = cpu_baseline(w, prob, it, args.cpu_gemvs)
+        if w.n <= 6000:
+            # the whole CPU path fits the time budget: per stage, all host threads and one thread
+            import oracle as O
+            O.build()
+            allt = cpu_stages(w, prob, O.max_threads())
+            one = cpu_stages(w, prob, 1)
+            per = allt["total_config_tol"]["mean_ms"] / 1e3
+            line["cpu_baseline"] = dict(
+                value=1.0 / per, unit="cluster solves/s", cores=allt["threads"], kind="port",
+                sample=(f"oracle FP64 full path on {allt['threads']} host threads, {allt['repeats']} timed runs after "
+                        f"2 warmups: assembly + BiCGSTAB to tol {w.tol} + cross-sections + forces "
+                        f"= {per * 1e3:.2f} ms mean per solve"),
+                stages_all_threads=allt, stages_one_thread=one)
+        else:
+            line["cpu_baseline"] = cpu_baseline(w, prob, it, args.cpu_gemvs)
     print(json.dumps(line), flush=True)
 
 

@@ -58,6 +58,7 @@ _SIGS = {
     "orc_force_all": (None, [_pd, _pd, _i, _i, _d, _d, _d, _pd]),
     "orc_torque_all": (None, [_pd, _pd, _i, _i, _d, _d, _d, _pd]),
     "orc_max_threads": (_i, []),
+    "orc_set_threads": (None, [_i]),
 }
 
 
@@ -317,3 +318,8 @@ def torque_all(p_scat, p_inc, k, eps_b=1.0, mu_b=1.0):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (timing only)."""
+    lib().orc_set_threads(int(n))

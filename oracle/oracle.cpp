@@ -974,5 +974,6 @@ void orc_torque_all(const double* p_scat_all, const double* p_inc_all, int N, in
 }
 
 int orc_max_threads() { return omp_get_max_threads(); }
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
 
 }  // extern "C"
