@@ -29,6 +29,8 @@ def mie_coefficients(radius, eps, mu, eps_b, mu_b, k, lmax, conducting=None):
 
     radius, eps, mu: [N]; eps_b, mu_b: medium; k: medium wavenumber;
     conducting: [N] bool (perfect-conductor limit, mie_sphere.py:34-36, 67-69).
+    k, eps_b and mu_b may also be [N] arrays (one value per particle), so that
+    particles of many clusters and wavelengths go through one call.
     """
     radius = np.asarray(radius, dtype=float)
     eps = np.asarray(eps, dtype=complex)
