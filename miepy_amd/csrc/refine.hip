@@ -311,6 +311,28 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
         prev = rn2;
         rc = read_norms(rn2);
         if (rc) return rc;
+        // a correction that left the residual larger (a failed inner solve) is taken back:
+        // x -= ||r|| d, and the RHS keeps its previous residual (it stops as not converged)
+        bool any_revert = false;
+        for (int j = 0; j < total; j++) {
+            const bool revert = usc[j] != 0.0 && !(rn2[j] <= prev[j]);
+            usc[j] = revert ? -usc[j] : 0.0;
+            if (revert) {
+                rn2[j] = prev[j];
+                any_revert = true;
+            }
+        }
+        if (any_revert) {
+            k = 0;
+            for (int i = 0; i < nsys; i++) {
+                MPA_HIP_CHECK(hipMemcpyAsync(ref[i].unscale, &usc[k], sizeof(double) * ref[i].S,
+                                             hipMemcpyHostToDevice, st));
+                k += ref[i].S;
+            }
+            hipLaunchKernelGGL(k_ref_accum, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
+            MPA_LAUNCH_CHECK();
+            MPA_HIP_CHECK(hipStreamSynchronize(st));  // usc is read by the copies above
+        }
     }
     int worst = MPA_OK;
     for (int j = 0; j < total; j++) {
@@ -529,13 +551,37 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
         hipLaunchKernelGGL(k_ref_norms, dim3((unsigned)L.cw.size()), dim3(kThreads), 0, st, d_ref, d_cw);
         MPA_LAUNCH_CHECK();
         if ((rc = read_norms(finished_inner))) return rc;
+        std::vector<int> revert_sys;
         for (int i : finished_inner) {
+            bool any = false;
+            std::vector<double> neg(R[i].nrhs, 0.0);
             for (int s = 0; s < R[i].nrhs; s++) {
                 RefCtl& c = ctl[first[i] + s];
-                c.prev = c.rn2;
-                c.rn2 = nrm[first[i] + s];
+                const double now = nrm[first[i] + s];
+                if (c.status == -1 && !(now <= c.rn2)) {
+                    // the correction left the residual larger (a failed inner solve): take it back
+                    neg[s] = -std::sqrt(c.rn2);
+                    c.prev = c.rn2;  // residual unchanged: the decision below stops this RHS
+                    any = true;
+                } else {
+                    c.prev = c.rn2;
+                    c.rn2 = now;
+                }
+            }
+            if (any) {
+                MPA_HIP_CHECK(hipMemcpyAsync(V[i].d.unscale, neg.data(), sizeof(double) * R[i].nrhs,
+                                             hipMemcpyHostToDevice, st));
+                MPA_HIP_CHECK(hipStreamSynchronize(st));
+                revert_sys.push_back(i);
             }
             ctl[first[i]].phase = 0;
+        }
+        if (!revert_sys.empty()) {
+            SubLists Lr = sub_lists(revert_sys, R, V);
+            if ((rc = upload(Lr, d_mw, n_mw))) return rc;
+            hipLaunchKernelGGL(k_ref_accum, dim3((unsigned)Lr.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
+            MPA_LAUNCH_CHECK();
+            MPA_HIP_CHECK(hipStreamSynchronize(st));
         }
     }
     // C. decisions: per RHS converged / stalled / out of corrections, or the next correction

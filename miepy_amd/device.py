@@ -250,11 +250,12 @@ def translation_table(lmax):
 
 
 class RefineReport:
-    def __init__(self, outer, inner, resid, status):
+    def __init__(self, outer, inner, resid, status, fp64_fallback=False):
         self.outer = outer
         self.iterations = inner
         self.residual = resid
         self.status = status
+        self.fp64_fallback = fp64_fallback  # True: refinement failed and the system was re-solved in FP64
 
     @property
     def converged(self):
@@ -262,14 +263,16 @@ class RefineReport:
 
     def __repr__(self):
         return (f"RefineReport(outer={self.outer}, inner_iterations={self.iterations}, residual={self.residual}, "
-                f"status={self.status})")
+                f"status={self.status}" + (", fp64_fallback=True)" if self.fp64_fallback else ")"))
 
 
 def solve_refined(A32, A64, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refine=8, add_identity=False,
-                  workspace=None, x=None):
+                  workspace=None, x=None, fallback=True):
     """Mixed-precision solve: c64 BiCGSTAB corrections, FP64 refinement.
 
-    A32 [n, n] c64, A64 [n, n] c128 (device), b [S, n] c128.  Returns (x c128, RefineReport)."""
+    A32 [n, n] c64, A64 [n, n] c128 (device), b [S, n] c128.  Returns (x c128, RefineReport).
+    fallback: a system whose refinement does not converge is re-solved with
+    FP64 BiCGSTAB on A64 (RefineReport.fp64_fallback)."""
     lib = L.load()
     n = A32.shape[0]
     b2 = b.reshape(-1, n).contiguous()
@@ -288,9 +291,23 @@ def solve_refined(A32, A64, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refi
     ai = 1 if add_identity else 0
     rc = lib.mpa_solve_refined(n, S, _ptr(A32), ai, _ptr(A64), ai, _ptr(b2), _ptr(x), float(tol), float(inner_tol),
                                int(maxiter), int(max_refine), _ptr(workspace), outer, inner, res, st, _stream())
-    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED):
+    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED, L.MPA_NONFINITE):
         L.check(rc, "solve_refined")
-    return x.reshape(b.shape), RefineReport(list(outer), list(inner), list(res), list(st))
+    rep = RefineReport(list(outer), list(inner), list(res), list(st))
+    if fallback and any(v != L.MPA_OK for v in rep.status):
+        x, rep = _fp64_fallback(A64, b2, x, tol, maxiter, add_identity, rep)
+    return x.reshape(b.shape), rep
+
+
+def _fp64_fallback(A64, b2, x, tol, maxiter, add_identity, rep):
+    """A refinement that did not reach tol (its complex64 corrections failed,
+    e.g. on an ill-conditioned system) is redone with FP64 BiCGSTAB on the
+    same FP64 operator, on the GPU."""
+    x64, r64 = bicgstab(A64, b2, tol=tol, maxiter=max(maxiter, 10 * b2.shape[1]), add_identity=add_identity,
+                        check=False)
+    x.copy_(x64.reshape(x.shape))
+    return x, RefineReport(rep.outer, [a + b for a, b in zip(rep.iterations, r64.iterations)], r64.residual,
+                           r64.status, fp64_fallback=True)
 
 
 class ClusterOperator:
@@ -379,11 +396,12 @@ def autotune_matfree(op, nrhs=1, dtype=C64, reps=3):
     return v.value, ms.value
 
 
-def solve_refined_matfree(op, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refine=8):
+def solve_refined_matfree(op, b, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refine=8, fallback=True):
     """Mixed precision without a stored matrix: complex64 corrections with the
-    FP32 matrix-core operator, FP64 residuals with the FP64 operator."""
+    FP16-split matrix-core operator, FP64 residuals with the FP64 operator
+    (FP64 matrix-free BiCGSTAB if the refinement does not converge)."""
     xs, reps = solve_refined_batched([op], [None], [b.reshape(-1, op.n)], tol=tol, inner_tol=inner_tol,
-                                     maxiter=maxiter, max_refine=max_refine, add_identity=True)
+                                     maxiter=maxiter, max_refine=max_refine, add_identity=True, fallback=fallback)
     return xs[0].reshape(b.shape), reps[0]
 
 
@@ -480,7 +498,7 @@ def bicgstab_batched(matrices, rhs, tol=1e-5, maxiter=1000, add_identity=False, 
 
 
 def solve_refined_batched(A32s, A64s, rhs, tol=1e-10, inner_tol=1e-5, maxiter=1000, max_refine=8,
-                          add_identity=False):
+                          add_identity=False, fallback=True):
     """Mixed-precision solve of several systems at once (c64 corrections, FP64 refinement).
 
     A32s: list of [n_i, n_i] c64, A64s: list of [n_i, n_i] c128, rhs: list of [S_i, n_i] c128.
@@ -525,12 +543,18 @@ def solve_refined_batched(A32s, A64s, rhs, tol=1e-10, inner_tol=1e-5, maxiter=10
     res = (ctypes.c_double * total)()
     stt = (ctypes.c_int * total)()
     rc = lib.mpa_solve_refined_batched(k, structs, _ptr(scratch), outer, inner, res, stt, _stream())
-    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED):
+    if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED, L.MPA_NONFINITE):
         L.check(rc, "solve_refined_batched")
     reports, o = [], 0
-    for x in xs:
+    for i, x in enumerate(xs):
         S = x.shape[0]
-        reports.append(RefineReport(list(outer[o:o + S]), list(inner[o:o + S]), list(res[o:o + S]),
-                                    list(stt[o:o + S])))
+        rep = RefineReport(list(outer[o:o + S]), list(inner[o:o + S]), list(res[o:o + S]), list(stt[o:o + S]))
+        if fallback and any(v != L.MPA_OK for v in rep.status):
+            # the operator that stands for the FP64 system: the cluster operator (always I + T) or A64
+            A = A32s[i] if isinstance(A32s[i], ClusterOperator) else A64s[i]
+            ai = True if isinstance(A32s[i], ClusterOperator) else add_identity
+            x, rep = _fp64_fallback(A, rhs[i].reshape(-1, x.shape[1]), x, tols[i], maxiter, ai, rep)
+            xs[i] = x
+        reports.append(rep)
         o += S
     return xs, reports

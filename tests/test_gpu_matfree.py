@@ -343,3 +343,71 @@ def test_matrix_free_mixed_precision_solve(dev, H):
     assert np.array_equal(xs[0].cpu().numpy(), x)
     ref2 = np.linalg.solve(A64.cpu().numpy(), b2.cpu().numpy()[0])
     assert np.linalg.norm(xs[1].cpu().numpy()[0] - ref2) / np.linalg.norm(ref2) <= 1e-8
+
+
+@pytest.mark.parametrize("N,lmax", [(1, 1), (1, 3), (2, 2), (2, 4), (3, 3)])
+def test_c64_operator_tiny_clusters(dev, c64_kernel, N, lmax):
+    """One particle (T = 0: the product is exactly 0, (I + T) x = x in
+    complex64), two and three particles (every chunk holds a self pair)."""
+    pos, mie = _cluster(N, lmax, seed=40 + N, spacing=300 * nm)
+    k = 2 * np.pi / (600 * nm)
+    n = 2 * lmax * (lmax + 2) * N
+    rng = np.random.default_rng(N)
+    X = rng.normal(size=(2, n)) + 1j * rng.normal(size=(2, n))
+    op = dev.ClusterOperator(pos, mie, k, max_rhs=2)
+    got = dev.matfree_matvec(op, dev.to_device(X, dev.C64), add_identity=False).cpu().numpy()
+    ref = dev.matfree_matvec(op, dev.to_device(X, dev.C128), add_identity=False).cpu().numpy()
+    assert np.all(np.isfinite(got))
+    if N == 1:
+        assert np.all(got == 0) and np.all(ref == 0)
+        gi = dev.matfree_matvec(op, dev.to_device(X, dev.C64), add_identity=True).cpu().numpy()
+        assert np.array_equal(gi, X.astype(np.complex64))
+    else:
+        assert np.max(np.abs(got - ref)) <= 5e-6 * np.max(np.abs(ref))
+
+
+def _physical_cluster(N, lmax, seed):
+    """Spheres of radius 75 nm, eps 2.5 + 0.1j, in vacuum at 600 nm, with their
+    Mie coefficients (random coefficients can make I + T nearly singular)."""
+    from miepy_amd.mie import mie_coefficients
+    from miepy_amd.workloads import random_cluster
+    rng = np.random.default_rng(seed)
+    pos = random_cluster(N, 300 * nm * N ** (1 / 3) * 1.5, 75 * nm, rng)
+    k = 2 * np.pi / (600 * nm)
+    mie, _ = mie_coefficients(np.full(N, 75 * nm), np.full(N, 2.5 + 0.1j), np.ones(N), 1.0, 1.0, k, lmax)
+    return pos, mie, k
+
+
+@pytest.mark.parametrize("N,lmax", [(1, 2), (2, 3), (2, 4), (5, 4)])
+def test_mixed_matfree_solve_tiny_clusters(dev, N, lmax):
+    """The matrix-free mixed solve (complex64 corrections, FP64 residuals) on
+    one to five particles equals the FP64 matrix-free BiCGSTAB (1e-9)."""
+    pos, mie, k = _physical_cluster(N, lmax, 50 + N)
+    n = 2 * lmax * (lmax + 2) * N
+    b = dev.to_device(np.random.default_rng(5).normal(size=(1, n)) + 0j, dev.C128)
+    op = dev.ClusterOperator(pos, mie, k)
+    x64, r64 = dev.bicgstab_matfree(op, b, tol=1e-12, maxiter=5000)
+    assert r64.converged, r64
+    x, rep = dev.solve_refined_matfree(op, b, tol=1e-11)
+    assert rep.converged and not rep.fp64_fallback, rep
+    assert np.max(np.abs(x.cpu().numpy() - x64.cpu().numpy())) <= 1e-9 * np.max(np.abs(x64.cpu().numpy()))
+
+
+def test_refinement_failure_keeps_best_iterate_and_falls_back(dev):
+    """A refinement that cannot reach its tolerance (inner solves cut at 2
+    iterations, 2 corrections) reports "not converged" and keeps an iterate no
+    worse than x = 0; with the FP64 safety net (the default) the system is
+    re-solved with FP64 BiCGSTAB and matches it."""
+    pos, mie, k = _physical_cluster(40, 3, 9)
+    n = 2 * 15 * 40
+    b = dev.to_device(np.random.default_rng(6).normal(size=(1, n)) + 0j, dev.C128)
+    op = dev.ClusterOperator(pos, mie, k)
+    raw, rep0 = dev.solve_refined_matfree(op, b, tol=1e-11, maxiter=2, max_refine=2, fallback=False)
+    assert not rep0.converged and not rep0.fp64_fallback
+    y = dev.matfree_matvec(op, raw.reshape(1, -1), add_identity=True).cpu().numpy()
+    bn = np.linalg.norm(b.cpu().numpy())
+    assert np.linalg.norm(b.cpu().numpy() - y) <= bn
+    x, rep = dev.solve_refined_matfree(op, b, tol=1e-11, maxiter=2, max_refine=2)
+    assert rep.converged and rep.fp64_fallback, rep
+    x64, _ = dev.bicgstab_matfree(op, b, tol=1e-11, maxiter=5000)
+    assert np.max(np.abs(x.cpu().numpy() - x64.cpu().numpy())) <= 1e-12 * np.max(np.abs(x64.cpu().numpy()))
