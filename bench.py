@@ -588,6 +588,9 @@ def main():
         line["speedup_vs_fp64_stored"] = paths[head].get("speedup_vs_fp64", 1.0)
         if "vs_fp64" in paths[head]:
             line["error_vs_fp64"] = paths[head]["vs_fp64"]
+    if "matfree" in paths and head != "matfree":
+        # the FP64 matrix-free path (FP64 matrix cores, no stored matrix) is this build's fastest FP64 path
+        line["speedup_vs_fp64_matfree"] = paths["matfree"]["ms_per_step"] / paths[head]["ms_per_step"]
     line["roofline"] = paths[head]["roofline"]
     line["paths"] = {k: {kk: vv for kk, vv in v.items() if kk != "kernels"} for k, v in paths.items()}
     line["kernels"] = {k: v["kernels"] for k, v in paths.items()}
