@@ -131,6 +131,36 @@ int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, co
                  int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out, int sync_every,
                  void* stream);
 
+/* Solver flags (mpa_bicgstab_ex, mpa_system.flags). */
+#define MPA_SOLVE_REF_LOOP 1 /* the reference's hand-written loop (cpp/src/interactions.cpp:98-209,
+                                230-288): x0 = b, stop at ||r|| < tol (absolute) or after maxiter + 1
+                                iterations, no restart on rho breakdown */
+
+/* Bytes of device workspace mpa_bicgstab_ex needs; preconditioned != 0 when
+ * minv is passed (two more vectors per RHS). */
+size_t mpa_bicgstab_ex_workspace_bytes(int n, int nrhs, int dtype, int preconditioned);
+
+/* mpa_bicgstab with the options of the reference's other BiCGSTAB entry points:
+ *   flags       0: Eigen's BiCGSTAB, as mpa_bicgstab; MPA_SOLVE_REF_LOOP: the
+ *               reference's own loop, bound as interactions.bicgstab /
+ *               bicgstab_profiled (cpp/src/interactions_pyb.cpp:39-55)
+ *   minv        NULL, or [n, block_size] of dtype (device): the stacked inverses
+ *               of the diagonal blocks, applied as a right preconditioner
+ *               (y = M^-1 p, z = M^-1 s, x += alpha y + w z): with flags 0 the
+ *               BlockDiagonalPreconditioner solve of
+ *               solve_linear_system_preconditioned (interactions.cpp:291-345,
+ *               interactions_pyb.cpp:71-87 with M_inv_blocks), with
+ *               MPA_SOLVE_REF_LOOP bicgstab_preconditioned_profiled
+ *               (interactions.cpp:230-288, interactions_pyb.cpp:57-69)
+ *   block_size  rows per diagonal block (divides n); ignored without minv
+ *   workspace   mpa_bicgstab_ex_workspace_bytes(n, nrhs, dtype, minv != NULL)
+ * resid_out is ||r||/||b|| as for mpa_bicgstab (the reference's loop returns
+ * ||r||); MPA_NOT_CONVERGED marks an iteration cap the reference's loop
+ * reaches silently. */
+int mpa_bicgstab_ex(int n, int nrhs, int dtype, const void* A, int add_identity, const void* b, void* x, double tol,
+                    int maxiter, int flags, const void* minv, int block_size, void* workspace, int* iters_out,
+                    double* resid_out, int* status_out, int sync_every, void* stream);
+
 /* ---- matrix-free operator ----------------------------------------------------
  * The aggregate interaction operator of a sphere cluster applied without
  * storing it: the translation blocks of interactions.cpp:370-415 /
@@ -156,10 +186,11 @@ size_t mpa_matfree_workspace_bytes(int N, int lmax, int nrhs);
 int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,
                        void* stream);
 
-/* The same product in complex64 on the FP32 matrix cores (translation
- * special functions in FP64, products and sums in FP32), x and y [nrhs, n]
- * c64: the operator of mixed-precision inner solves without a stored
- * matrix.  Spheres with lmax <= 3. */
+/* The same product in complex64 on the matrix cores (the separation and
+ * argument reduction in FP64, special functions and sums in FP32, products
+ * on the FP16 matrix cores as hi/lo splits of scaled operands), x and y
+ * [nrhs, n] c64: the operator of mixed-precision inner solves without a
+ * stored matrix.  Spheres with lmax <= 4. */
 int mpa_matfree_matvec_c64(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,
                            void* stream);
 
@@ -192,17 +223,20 @@ int mpa_matfree_variant(int dtype, int set_to, int* count_out);
  * set, A must be NULL and the operator is applied matrix-free (dtype
  * MPA_C128 only; op->workspace sized for nrhs). */
 typedef struct {
-    int n, nrhs, add_identity, pad;
+    int n, nrhs, add_identity, flags; /* flags: MPA_SOLVE_* bits */
     const void* A; /* [n, n] of dtype (T, or I + T with add_identity = 0) */
     const void* b; /* [nrhs, n] */
     void* x;       /* [nrhs, n], overwritten */
     void* state;
-    double tol;    /* relative: ||r|| <= tol ||b|| */
-    int maxiter, pad2;
+    double tol;    /* relative: ||r|| <= tol ||b|| (absolute with MPA_SOLVE_REF_LOOP) */
+    int maxiter, block_size;
     const mpa_cluster_operator* op; /* host pointer or NULL */
+    const void* minv; /* NULL, or the [n, block_size] right preconditioner of mpa_bicgstab_ex (dtype) */
 } mpa_system;
 
 size_t mpa_system_state_bytes(int n, int nrhs, int dtype);
+/* State bytes of a system with (preconditioned != 0) or without minv. */
+size_t mpa_system_state_bytes_ex(int n, int nrhs, int dtype, int preconditioned);
 /* Device scratch for the descriptors and work lists of one batch. */
 size_t mpa_batch_scratch_bytes(int nsys, const mpa_system* systems, int dtype);
 

@@ -23,6 +23,7 @@ MPA_C64 = 1
 MPA_OPT_GRAPHS = 0
 MPA_OPT_MATFREE_FP64_KERNEL = 1
 MPA_OPT_MATFREE_C64_KERNEL = 2
+MPA_SOLVE_REF_LOOP = 1
 
 STATUS_NAMES = {
     MPA_OK: "ok",
@@ -57,6 +58,8 @@ SIGNATURES = {
     "mpa_matvec": (_i, [_i, _i, _i, _v, _i, _v, _v, _v]),
     "mpa_bicgstab_workspace_bytes": (_sz, [_i, _i, _i]),
     "mpa_bicgstab": (_i, [_i, _i, _i, _v, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
+    "mpa_bicgstab_ex_workspace_bytes": (_sz, [_i, _i, _i, _i]),
+    "mpa_bicgstab_ex": (_i, [_i, _i, _i, _v, _i, _v, _v, _d, _i, _i, _v, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_refine_workspace_bytes": (_sz, [_i, _i]),
     "mpa_solve_refined": (_i, [_i, _i, _v, _i, _v, _i, _v, _v, _d, _d, _i, _i, _v, _pi, _pi, _pd, _pi, _v]),
     "mpa_cluster_observables": (_i, [_i, _i, _i, _v, _v, _v, _v, _d, _d, _v, _v, _v, _v, _v, _v, _v]),
@@ -78,10 +81,10 @@ SIGNATURES = {
 
 class MpaSystem(ctypes.Structure):
     """mpa_system (include/miepy_amd.h)"""
-    _fields_ = [("n", ctypes.c_int), ("nrhs", ctypes.c_int), ("add_identity", ctypes.c_int), ("pad", ctypes.c_int),
+    _fields_ = [("n", ctypes.c_int), ("nrhs", ctypes.c_int), ("add_identity", ctypes.c_int), ("flags", ctypes.c_int),
                 ("A", ctypes.c_void_p), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p), ("state", ctypes.c_void_p),
-                ("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("pad2", ctypes.c_int),
-                ("op", ctypes.c_void_p)]
+                ("tol", ctypes.c_double), ("maxiter", ctypes.c_int), ("block_size", ctypes.c_int),
+                ("op", ctypes.c_void_p), ("minv", ctypes.c_void_p)]
 
 
 class MpaClusterOperator(ctypes.Structure):
@@ -114,6 +117,7 @@ SIGNATURES.update({
     "mpa_refine_systems_advance": (_i, [_i, _prs, _v, _i, _pi, _pi, _pi, _pd, _pi, _v]),
     "mpa_bicgstab_matfree": (_i, [_pop, _i, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_system_state_bytes": (_sz, [_i, _i, _i]),
+    "mpa_system_state_bytes_ex": (_sz, [_i, _i, _i, _i]),
     "mpa_batch_scratch_bytes": (_sz, [_i, _ps, _i]),
     "mpa_bicgstab_batched": (_i, [_i, _ps, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_systems_start": (_i, [_i, _ps, _i, _v, _v]),

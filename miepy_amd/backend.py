@@ -5,7 +5,11 @@ call when the 'mi355x' backend is selected (the slots the JAX 'gpu' backend
 takes in src/miepy/interactions.py:81-83,141-143 and src/miepy/flux.py:21-26):
 
   sphere_aggregate_tmatrix(positions, mie, k)       interactions.py:132-150
-  solve_linear_system(tmatrix, p_src, method, ...)  interactions.py:71-92
+  solve_linear_system(tmatrix, p_src, method, M_inv_blocks, block_size)
+                                                    interactions.py:71-92
+  build_block_preconditioner_sphere / _particle     interactions.py:9-68
+  bicgstab, bicgstab_profiled, bicgstab_preconditioned_profiled
+                                                    cpp/src/interactions_pyb.cpp:39-69
   particle_cross_sections(p_scat, p_inc, p_src, k)  flux.py:12-32
   force_all / torque_all(p_scat, p_inc, k, eps_b, mu_b)  forces.py:44-81
 
@@ -88,14 +92,27 @@ def _device_matrix(tmatrix, n):
     return D.to_device(np.asarray(tmatrix, dtype=complex).reshape(n, n), D.C128)
 
 
+def _device_blocks(M_inv_blocks, n, block_size, dtype):
+    M = M_inv_blocks.data if isinstance(M_inv_blocks, DeviceTMatrix) else M_inv_blocks
+    if isinstance(M, torch.Tensor):
+        M = M.to(dtype)
+    else:
+        M = D.to_device(np.asarray(M, dtype=complex), dtype)
+    if block_size < 1 or n % block_size or M.numel() != n * block_size:
+        raise ValueError(f"M_inv_blocks must be [{n}, block_size] with block_size dividing {n}")
+    return M.reshape(n, block_size).contiguous()
+
+
 def solve_linear_system(tmatrix, p_src, method=solver.bicgstab, M_inv_blocks=None, block_size=0, tol=None,
                         maxiter=None):
     """Solve (I + T) p_inc = p_src.  p_src[N,2,rmax] -> p_inc[N,2,rmax] (host numpy).
 
-    method=bicgstab runs BiCGSTAB (default tolerance 1e-5 as the reference);
-    method=exact runs the same solver to a tolerance of 1e-14."""
-    if M_inv_blocks is not None:
-        raise NotImplementedError("block-diagonal preconditioning is not available on the mi355x backend")
+    method=bicgstab runs BiCGSTAB (default tolerance 1e-5 as the reference),
+    right-preconditioned by the block-diagonal M_inv_blocks [n, block_size]
+    when it is given (solve_linear_system_preconditioned,
+    cpp/src/interactions.cpp:291-345); method=exact runs the same solver to a
+    tolerance of 1e-14 and, like the reference's LU branch, ignores the
+    preconditioner."""
     p_src = np.asarray(p_src)
     shape = p_src.shape
     n = p_src.size
@@ -104,9 +121,79 @@ def solve_linear_system(tmatrix, p_src, method=solver.bicgstab, M_inv_blocks=Non
     t = options["exact_tol"] if exact else (options["tol"] if tol is None else tol)
     mi = max(options["maxiter"], 10 * n) if exact else (options["maxiter"] if maxiter is None else maxiter)
     b = D.to_device(p_src.reshape(1, n), A.dtype)
-    x, report = D.bicgstab(A, b, tol=t, maxiter=mi, add_identity=True)
+    M = None if (M_inv_blocks is None or exact) else _device_blocks(M_inv_blocks, n, int(block_size), A.dtype)
+    x, report = D.bicgstab(A, b, tol=t, maxiter=mi, add_identity=True, M_inv_blocks=M,
+                           block_size=int(block_size) if M is not None else 0)
     solve_linear_system.last_report = report
     return x.cpu().numpy().reshape(shape)
+
+
+def build_block_preconditioner_sphere(mie_scat, lmax):
+    """Block-diagonal M^-1 of a sphere cluster (src/miepy/interactions.py:9-40):
+    each particle's block is diag(1 / (1 + mie_scat[i, pol, n-1])), 1 where
+    |1 + mie_scat| < 1e-12.  Returns (M_inv_blocks [N*bs, bs], bs), bs = 2 rmax."""
+    mie_scat = np.asarray(mie_scat, dtype=complex)
+    N = mie_scat.shape[0]
+    rmax = lmax * (lmax + 2)
+    bs = 2 * rmax
+    nidx = np.concatenate([np.full(2 * n + 1, n - 1) for n in range(1, lmax + 1)])
+    d = 1.0 + mie_scat[:, :, nidx].reshape(N, bs)
+    small = np.abs(d) < 1e-12
+    inv = np.where(small, 1.0, 1.0 / np.where(small, 1.0, d))
+    M = np.zeros((N, bs, bs), dtype=complex)
+    M[:, np.arange(bs), np.arange(bs)] = inv
+    return M.reshape(N * bs, bs), bs
+
+
+def build_block_preconditioner_particle(tmatrix):
+    """Block-diagonal M^-1 = inv(I + T_i) per particle (src/miepy/interactions.py:43-68);
+    a singular block falls back to the identity.  Returns (M_inv_blocks [N*bs, bs], bs)."""
+    tmatrix = np.asarray(tmatrix, dtype=complex)
+    N = tmatrix.shape[0]
+    bs = 2 * tmatrix.shape[-1]
+    blocks = np.eye(bs) + tmatrix.reshape(N, bs, bs)
+    M = np.empty_like(blocks)
+    for i in range(N):
+        try:
+            M[i] = np.linalg.inv(blocks[i])
+        except np.linalg.LinAlgError:
+            M[i] = np.eye(bs)
+    return M.reshape(N * bs, bs), bs
+
+
+def _ref_loop(A, b, maxiter, tolerance, M_inv_blocks=None, block_size=0):
+    """The reference's own BiCGSTAB loop on the device (x0 = b, absolute
+    tolerance, at most maxiter + 1 iterations; cpp/src/interactions.cpp:98-209
+    and, preconditioned, :230-288).  A is the full system matrix."""
+    b = np.asarray(b, dtype=complex).reshape(-1)
+    n = b.size
+    Ad = A.data.reshape(n, n) if isinstance(A, DeviceTMatrix) else (
+        A.reshape(n, n) if isinstance(A, torch.Tensor) else D.to_device(np.asarray(A, dtype=complex).reshape(n, n),
+                                                                         D.C128))
+    M = None if M_inv_blocks is None else _device_blocks(M_inv_blocks, n, int(block_size), Ad.dtype)
+    x, rep = D.bicgstab(Ad, D.to_device(b.reshape(1, n), Ad.dtype), tol=float(tolerance), maxiter=int(maxiter),
+                        add_identity=False, M_inv_blocks=M, block_size=int(block_size) if M is not None else 0,
+                        ref_loop=True, check=False)
+    residual = rep.residual[0] * float(np.linalg.norm(b))  # ||r|| (the engine reports ||r||/||b||)
+    return x.cpu().numpy().reshape(n), int(rep.iterations[0]), residual
+
+
+def bicgstab(A, b, maxiter, tolerance):
+    """interactions.bicgstab(A, b, maxiter, tolerance) -> x
+    (cpp/src/interactions_pyb.cpp:39-44, interactions.cpp:98-150)."""
+    return _ref_loop(A, b, maxiter, tolerance)[0]
+
+
+def bicgstab_profiled(A, b, maxiter=1000, tolerance=1e-5):
+    """interactions.bicgstab_profiled -> (x, iterations, ||r||)
+    (cpp/src/interactions_pyb.cpp:46-55, interactions.cpp:152-209)."""
+    return _ref_loop(A, b, maxiter, tolerance)
+
+
+def bicgstab_preconditioned_profiled(A, b, M_inv_blocks, block_size, maxiter=1000, tolerance=1e-5):
+    """interactions.bicgstab_preconditioned_profiled -> (x, iterations, ||r||)
+    (cpp/src/interactions_pyb.cpp:57-69, interactions.cpp:230-288)."""
+    return _ref_loop(A, b, maxiter, tolerance, M_inv_blocks, block_size)
 
 
 solve_linear_system.last_report = None

@@ -6,6 +6,13 @@
 // zero initial guess, stop at ||r||^2 <= tol^2 ||b||^2 or after maxiter
 // iterations, restart when |rho| < eps^2 ||r0||^2.  The reference hard-codes
 // tol 1e-5 and maxiter 1000 (:362-363); here both are per-system inputs.
+// Two per-system options cover the reference's other BiCGSTAB entry points:
+// a block-diagonal right preconditioner (y = M^-1 p, z = M^-1 s, x += alpha y
+// + w z; solve_linear_system_preconditioned, :291-345, and
+// bicgstab_preconditioned_profiled, :230-288), and MPA_SOLVE_REF_LOOP, the
+// reference's hand-written loop (bicgstab / bicgstab_profiled, :98-209):
+// x0 = b, stop at ||r|| < tol absolute or after maxiter + 1 iterations, no
+// restart.
 //
 // Every system keeps its matrix, Krylov vectors and per-RHS scalars in HBM.
 // One launch of each kernel covers all systems of the batch through a work
@@ -44,6 +51,7 @@ __global__ __launch_bounds__(kThreads) void k_init(const SysDev* __restrict__ sy
     VEC_PROLOGUE
     const C* b = (const C*)d.b;
     C *x = (C*)d.x, *r = (C*)d.r, *r0 = (C*)d.r0, *p = (C*)d.p, *v = (C*)d.v;
+    const bool x0_b = (d.flags & MPA_SOLVE_REF_LOOP) != 0;  // r = b - A b is formed by the restart kernels
     double2 acc[1] = {make_double2(0, 0)};
     for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
         const size_t k = base + i;
@@ -53,7 +61,7 @@ __global__ __launch_bounds__(kThreads) void k_init(const SysDev* __restrict__ sy
         const C z = cmake<T>(0, 0);
         p[k] = z;
         v[k] = z;
-        x[k] = z;
+        x[k] = x0_b ? bv : z;
         const double2 bd = to_d(bv);
         acc[0].x += bd.x * bd.x + bd.y * bd.y;
     }
@@ -68,21 +76,31 @@ __global__ __launch_bounds__(kThreads) void k_init_scalars(const SysDev* __restr
     const double2 bb = reduce_partials(d.part, d.G, s, 3);
     if (threadIdx.x == 0) {
         Scal& c = d.scal[s];
+        const bool ref = (d.flags & MPA_SOLVE_REF_LOOP) != 0;
         c.rho = c.alpha = c.w = make_double2(1, 0);
         c.alpha_next = c.w_next = make_double2(1, 0);
         c.rhs_sq = bb.x;
         c.r0_sq = bb.x;
         c.rnorm2 = bb.x;
-        c.tol2 = d.tol * d.tol * bb.x;
+        c.tol2 = ref ? d.tol * d.tol : d.tol * d.tol * bb.x;
         c.it = 0;
         c.restarts = 0;
-        c.maxiter = d.maxiter;
+        c.maxiter = ref ? d.maxiter + 1 : d.maxiter;
         c.status = MPA_OK;
         c.restart_pending = 0;
+        c.init_pending = 0;
         c.rho_next = make_double2(bb.x, 0);  // <r0, r> = ||b||^2 with x0 = 0
-        c.done = (bb.x == 0.0 || !(bb.x > c.tol2) || d.maxiter <= 0) ? 1 : 0;
+        if (ref) {
+            // x0 = b: the restart kernels form r = b - A b, r0 = r and make the
+            // loop's initial test ||r|| < tol (b = 0 gives x = 0, r = 0 at once)
+            c.done = bb.x == 0.0 ? 1 : 0;
+            c.restart_pending = c.init_pending = c.done ? 0 : 1;
+        } else {
+            c.done = (bb.x == 0.0 || !(bb.x > c.tol2) || d.maxiter <= 0) ? 1 : 0;
+        }
         if (bb.x != bb.x) {
             c.done = 1;
+            c.restart_pending = c.init_pending = 0;
             c.status = MPA_NONFINITE;
         }
     }
@@ -101,6 +119,32 @@ __global__ __launch_bounds__(kThreads) void k_update_p(const SysDev* __restrict_
     for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
         const size_t k = base + i;
         p[k] = cadd(r[k], cmul(bt, csub(p[k], cmul(wt, v[k]))));
+    }
+}
+
+// Right preconditioner: y = M^-1 p (which = 0) or z = M^-1 s (which = 1),
+// M^-1 the block-diagonal [n, bs] stack of the system's block inverses
+// (apply_block_preconditioner, cpp/src/interactions.cpp:211-228).  One
+// thread per output row; the bs entries of the row's input block are shared
+// by the block's rows through the cache.  FP64 accumulation.
+template <typename T>
+__global__ __launch_bounds__(kThreads) void k_precond(const SysDev* __restrict__ sys, const int3* __restrict__ work,
+                                                      int which) {
+    using C = cplx_t<T>;
+    VEC_PROLOGUE
+    if (!d.minv) return;
+    const Scal& c = d.scal[s];
+    if (c.done || c.restart_pending) return;
+    const C* src = (const C*)(which ? d.s : d.p) + base;
+    C* dst = (C*)(which ? d.z : d.y) + base;
+    const C* M = (const C*)d.minv;
+    const int bs = d.bs;
+    for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
+        const C* row = M + (size_t)i * bs;
+        const C* in = src + (i - i % bs);
+        double2 acc = make_double2(0, 0);
+        for (int k = 0; k < bs; k++) acc = cadd(acc, cmul(to_d(row[k]), to_d(in[k])));
+        dst[i] = from_d<T>(acc);
     }
 }
 
@@ -161,12 +205,14 @@ __global__ __launch_bounds__(kThreads) void k_update_xr(const SysDev* __restrict
     const double2 tt = reduce_partials(d.part, d.G, s, 2);
     const double2 om = tt.x > 0 ? cscale(ts, 1.0 / tt.x) : make_double2(0, 0);
     const C at = from_d<T>(d.scal[s].alpha_next), wt = from_d<T>(om);
-    C *x = (C*)d.x, *p = (C*)d.p, *sv = (C*)d.s, *tv = (C*)d.t, *r = (C*)d.r, *r0 = (C*)d.r0;
+    // y, z alias p, s without a preconditioner
+    C *x = (C*)d.x, *y = (C*)d.y, *z = (C*)d.z, *sv = (C*)d.s, *tv = (C*)d.t, *r = (C*)d.r, *r0 = (C*)d.r0;
+    const bool pc = d.minv != nullptr;
     double2 acc[2] = {make_double2(0, 0), make_double2(0, 0)};
     for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
         const size_t k = base + i;
         const C s_ = sv[k];
-        x[k] = cadd(x[k], cadd(cmul(at, p[k]), cmul(wt, s_)));
+        x[k] = cadd(x[k], cadd(cmul(at, y[k]), cmul(wt, pc ? z[k] : s_)));
         const C rv = csub(s_, cmul(wt, tv[k]));
         r[k] = rv;
         const double2 rd = to_d(rv);
@@ -198,15 +244,16 @@ __global__ __launch_bounds__(kThreads) void k_commit(const SysDev* __restrict__ 
         c.it += 1;
         c.rho_next = r0r;
         const double eps = 2.220446049250313e-16;  // NumTraits<double>::epsilon()
+        const bool ref = (d.flags & MPA_SOLVE_REF_LOOP) != 0;
         if (!(rr.x == rr.x) || !(r0r.x == r0r.x) || !(r0r.y == r0r.y)) {
             c.done = 1;
             c.status = MPA_NONFINITE;
-        } else if (!(rr.x > c.tol2)) {
+        } else if (ref ? rr.x < c.tol2 : !(rr.x > c.tol2)) {  // ||r|| < tol (reference loop), Eigen's <=
             c.done = 1;
         } else if (c.it >= c.maxiter) {
             c.done = 1;
             c.status = MPA_NOT_CONVERGED;
-        } else if (sqrt(r0r.x * r0r.x + r0r.y * r0r.y) < eps * eps * c.r0_sq) {
+        } else if (!ref && sqrt(r0r.x * r0r.x + r0r.y * r0r.y) < eps * eps * c.r0_sq) {
             c.restart_pending = 1;
         }
     }
@@ -245,8 +292,19 @@ __global__ __launch_bounds__(kThreads) void k_restart_commit(const SysDev* __res
         // committed rho stays as rho_old for beta = (rho/rho_old)(alpha/w).
         c.r0_sq = rr.x;
         c.rho_next = make_double2(rr.x, 0);
-        if (c.restarts++ == 0) c.it = 0;
         c.restart_pending = 0;
+        if (c.init_pending) {  // x0 = b: the reference loop's initial test on ||b - A b||
+            c.init_pending = 0;
+            c.rnorm2 = rr.x;
+            if (!(rr.x == rr.x)) {
+                c.done = 1;
+                c.status = MPA_NONFINITE;
+            } else if (rr.x < c.tol2) {
+                c.done = 1;
+            }
+        } else if (c.restarts++ == 0) {
+            c.it = 0;
+        }
     }
 }
 
@@ -261,9 +319,9 @@ __global__ __launch_bounds__(64) void k_gather_scalars(const SysDev* __restrict_
 
 size_t vec_bytes(int n, int S, int esize) { return align256((size_t)esize * 2 * n * S); }
 
-size_t state_bytes(int n, int S, int dtype) {
+size_t state_bytes(int n, int S, int dtype, bool precond = false) {
     const int es = dtype == MPA_C128 ? 8 : 4;
-    return 6 * vec_bytes(n, S, es) + align256(sizeof(Scal) * S) +
+    return (precond ? 8 : 6) * vec_bytes(n, S, es) + align256(sizeof(Scal) * S) +
            align256(sizeof(double2) * (size_t)S * groups_for(n) * kSlots);
 }
 
@@ -275,6 +333,7 @@ struct Plan {
     std::vector<MvDev> mv_ps, mv_st, mv_xt;
     std::vector<int2> mv_work[kTileClasses];  // (job, row block) per tile class; same for every job set
     std::vector<MfJob> mf_ps, mf_st, mf_xt;   // matrix-free systems (launched by value)
+    bool any_precond = false;
     // device copies
     SysDev* d_sys = nullptr;
     int3* d_vec = nullptr;
@@ -308,6 +367,7 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
     P.mf_ps.clear();
     P.mf_st.clear();
     P.mf_xt.clear();
+    P.any_precond = false;
     const int es = dtype == MPA_C128 ? 8 : 4;
     for (int i = 0; i < nsys; i++) {
         const mpa_system& q = S_[i];
@@ -325,20 +385,31 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
         d.s = p; p += vb;
         d.t = p; p += vb;
         d.scal = (Scal*)p; p += align256(sizeof(Scal) * S);
-        d.part = (double2*)p;
+        d.part = (double2*)p; p += align256(sizeof(double2) * (size_t)S * groups_for(n) * kSlots);
+        d.minv = q.minv;
+        d.bs = q.minv ? q.block_size : 0;
+        if (q.minv) {  // after the unpreconditioned layout: state_bytes(n, S, dtype, true)
+            d.y = p; p += vb;
+            d.z = p; p += vb;
+            P.any_precond = true;
+        } else {
+            d.y = d.p;
+            d.z = d.s;
+        }
         d.tol = q.tol;
         d.n = n;
         d.S = S;
         d.G = groups_for(n);
         d.add_identity = q.add_identity;
         d.maxiter = q.maxiter;
+        d.flags = q.flags;
         d.pad = 0;
         for (int s = 0; s < S; s++) {
             P.scal_work.push_back(make_int2(i, s));
             for (int g = 0; g < d.G; g++) P.vec_work.push_back(make_int3(i, s, g));
         }
-        P.mv_ps[i] = MvDev{q.A, d.p, d.v, n, S, q.add_identity, i};
-        P.mv_st[i] = MvDev{q.A, d.s, d.t, n, S, q.add_identity, i};
+        P.mv_ps[i] = MvDev{q.A, d.y, d.v, n, S, q.add_identity, i};
+        P.mv_st[i] = MvDev{q.A, d.z, d.t, n, S, q.add_identity, i};
         P.mv_xt[i] = MvDev{q.A, q.x, d.t, n, S, q.add_identity, -1};
         if (q.op) {
             const mpa_cluster_operator& o = *q.op;
@@ -347,10 +418,10 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
                              nullptr, nullptr, (double2*)o.workspace, i, 0, (const double2*)o.tmatrix, nullptr,
                              dtype == MPA_C64 ? 1 : 0, 0};
             MfJob j = base;
-            j.x = (const double2*)d.p;
+            j.x = (const double2*)d.y;
             j.y = (double2*)d.v;
             P.mf_ps.push_back(j);
-            j.x = (const double2*)d.s;
+            j.x = (const double2*)d.z;
             j.y = (double2*)d.t;
             P.mf_st.push_back(j);
             j.x = (const double2*)q.x;
@@ -420,10 +491,12 @@ template <typename T>
 int launch_iteration(const Plan& P, hipStream_t st) {
     const int nv = (int)P.vec_work.size(), ns = (int)P.scal_work.size();
     hipLaunchKernelGGL(k_update_p<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    if (P.any_precond) hipLaunchKernelGGL(k_precond<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec, 0);
     int rc = matvec_phase<T>(P, P.d_mv_ps, P.mf_ps, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_dot_r0v<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
     hipLaunchKernelGGL(k_update_s<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
+    if (P.any_precond) hipLaunchKernelGGL(k_precond<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec, 1);
     rc = matvec_phase<T>(P, P.d_mv_st, P.mf_st, st);
     if (rc) return rc;
     hipLaunchKernelGGL(k_dot_ts<T>, dim3(nv), dim3(kThreads), 0, st, P.d_sys, P.d_vec);
@@ -592,8 +665,9 @@ bool valid_systems(int nsys, const mpa_system* s, int dtype) {
     if (nsys < 1 || !s || (dtype != MPA_C128 && dtype != MPA_C64)) return false;
     for (int i = 0; i < nsys; i++) {
         if (s[i].n < 2 || (s[i].n & 1) || s[i].nrhs < 1 || !s[i].b || !s[i].x || !s[i].state || !(s[i].tol >= 0) ||
-            s[i].maxiter < 0)
+            s[i].maxiter < 0 || (s[i].flags & ~MPA_SOLVE_REF_LOOP) != 0)
             return false;
+        if (s[i].minv && (s[i].block_size < 1 || s[i].n % s[i].block_size != 0)) return false;
         if (s[i].op) {
             const mpa_cluster_operator& o = *s[i].op;
             if (s[i].A || o.N < 1 || o.lmax < 1 || o.lmax > MPA_LMAX_MAX || !o.positions ||
@@ -621,6 +695,10 @@ int mpa_internal_solve_batch(int nsys, const mpa_system* systems, int dtype, voi
 extern "C" {
 
 size_t mpa_system_state_bytes(int n, int nrhs, int dtype) { return state_bytes(n, nrhs, dtype); }
+
+size_t mpa_system_state_bytes_ex(int n, int nrhs, int dtype, int preconditioned) {
+    return state_bytes(n, nrhs, dtype, preconditioned != 0);
+}
 
 size_t mpa_batch_scratch_bytes(int nsys, const mpa_system* systems, int dtype) {
     return plan_bytes(nsys, systems, dtype);
@@ -708,6 +786,38 @@ int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, co
     s.tol = tol;
     s.maxiter = maxiter;
     void* scratch = (char*)workspace + state_bytes(n, nrhs, dtype);
+    return mpa_internal_solve_batch(1, &s, dtype, scratch, iters_out, resid_out, status_out, sync_every,
+                                    (hipStream_t)stream);
+}
+
+size_t mpa_bicgstab_ex_workspace_bytes(int n, int nrhs, int dtype, int preconditioned) {
+    mpa_system s{};
+    s.n = n;
+    s.nrhs = nrhs;
+    return state_bytes(n, nrhs, dtype, preconditioned != 0) + plan_bytes(1, &s, dtype);
+}
+
+int mpa_bicgstab_ex(int n, int nrhs, int dtype, const void* A, int add_identity, const void* b, void* x, double tol,
+                    int maxiter, int flags, const void* minv, int block_size, void* workspace, int* iters_out,
+                    double* resid_out, int* status_out, int sync_every, void* stream) {
+    mpa_system s{};
+    s.n = n;
+    s.nrhs = nrhs;
+    s.add_identity = add_identity;
+    s.flags = flags;
+    s.A = A;
+    s.b = b;
+    s.x = x;
+    s.state = workspace;
+    s.tol = tol;
+    s.maxiter = maxiter;
+    s.minv = minv;
+    s.block_size = block_size;
+    if (!A || !workspace || !valid_systems(1, &s, dtype)) {
+        mpa_set_error("bicgstab_ex: bad argument (n even and positive, block_size dividing n, known flags)");
+        return MPA_ERR_ARG;
+    }
+    void* scratch = (char*)workspace + state_bytes(n, nrhs, dtype, minv != nullptr);
     return mpa_internal_solve_batch(1, &s, dtype, scratch, iters_out, resid_out, status_out, sync_every,
                                     (hipStream_t)stream);
 }

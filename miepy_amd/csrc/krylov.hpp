@@ -17,7 +17,7 @@ struct Scal {
     double2 rho_next, alpha_next, w_next;
     double r0_sq, rhs_sq, rnorm2, tol2;
     int it, done, status, restarts;
-    int maxiter, restart_pending, pad0, pad1;
+    int maxiter, restart_pending, init_pending, pad1;  // init_pending: x0 = b, r = b - A b not yet formed
 };
 
 // One linear system (I + T) x = b with S right-hand sides, as the kernels see it.
@@ -33,9 +33,13 @@ struct SysDev {
     void* t;        // [S, n] each
     Scal* scal;     // [S]
     double2* part;  // [S][G][kSlots]
+    void* y;        // M^-1 p (the matvec operand of the first half); aliases p without a preconditioner
+    void* z;        // M^-1 s (second half); aliases s without a preconditioner
+    const void* minv;  // [n, bs] stack of block inverses of the right preconditioner, or null
     double tol;
     int n, S, G, add_identity;
-    int maxiter, pad;
+    int maxiter, flags;  // MPA_SOLVE_* bits
+    int bs, pad;
 };
 
 // One y = (I +) A x job of a batched matvec.

@@ -185,20 +185,36 @@ class SolveReport:
         return f"SolveReport(iterations={self.iterations}, residual={self.residual}, status={self.status})"
 
 
-def bicgstab(A, b, tol=1e-5, maxiter=1000, add_identity=False, x=None, workspace=None, sync_every=1, check=True):
+def bicgstab(A, b, tol=1e-5, maxiter=1000, add_identity=False, x=None, workspace=None, sync_every=1, check=True,
+             M_inv_blocks=None, block_size=0, ref_loop=False):
     """BiCGSTAB (Eigen's algorithm) for (I +) A x = b on the device.
 
-    b: [n] or [S, n] with A's dtype.  Returns (x, SolveReport)."""
+    b: [n] or [S, n] with A's dtype.  Returns (x, SolveReport).
+    M_inv_blocks: [n, block_size] device tensor of A's dtype, the stacked
+    inverses of the diagonal blocks, applied as a right preconditioner
+    (cpp/src/interactions.cpp:291-345).  ref_loop: the reference's own loop
+    (interactions.cpp:98-209, :230-288): x0 = b, tol absolute on ||r||, at
+    most maxiter + 1 iterations, no restart; the report's residuals stay
+    ||r||/||b||."""
     lib = L.load()
     n = A.shape[0]
     b2 = b.reshape(-1, n).contiguous()
     S = b2.shape[0]
     if isinstance(A, ClusterOperator):
+        if M_inv_blocks is not None or ref_loop:
+            raise NotImplementedError("preconditioned and reference-loop solves take a stored matrix")
         return _bicgstab_matfree(lib, A, b, b2, tol, maxiter, add_identity, x, workspace, sync_every, check)
     code = DTYPE_CODE[A.dtype]
     if b2.dtype != A.dtype:
         raise ValueError("b must have the matrix dtype")
-    nbytes = lib.mpa_bicgstab_workspace_bytes(n, S, code)
+    if M_inv_blocks is not None:
+        if M_inv_blocks.dtype != A.dtype or not M_inv_blocks.is_cuda or M_inv_blocks.numel() != n * block_size \
+                or block_size < 1 or n % block_size:
+            raise ValueError("M_inv_blocks must be a device [n, block_size] tensor of the matrix dtype, "
+                             "block_size dividing n")
+        M_inv_blocks = M_inv_blocks.contiguous()
+    pc = 1 if M_inv_blocks is not None else 0
+    nbytes = lib.mpa_bicgstab_ex_workspace_bytes(n, S, code, pc)
     if workspace is None or workspace.numel() < nbytes:
         workspace = torch.empty(nbytes, dtype=torch.uint8, device=A.device)
     if x is None:
@@ -206,8 +222,10 @@ def bicgstab(A, b, tol=1e-5, maxiter=1000, add_identity=False, x=None, workspace
     it = (ctypes.c_int * S)()
     res = (ctypes.c_double * S)()
     st = (ctypes.c_int * S)()
-    rc = lib.mpa_bicgstab(n, S, code, _ptr(A), 1 if add_identity else 0, _ptr(b2), _ptr(x), float(tol), int(maxiter),
-                          _ptr(workspace), it, res, st, int(sync_every), _stream())
+    rc = lib.mpa_bicgstab_ex(n, S, code, _ptr(A), 1 if add_identity else 0, _ptr(b2), _ptr(x), float(tol),
+                             int(maxiter), L.MPA_SOLVE_REF_LOOP if ref_loop else 0,
+                             _ptr(M_inv_blocks) if pc else None, int(block_size) if pc else 0,
+                             _ptr(workspace), it, res, st, int(sync_every), _stream())
     report = SolveReport(list(it), list(res), list(st))
     if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED, L.MPA_NONFINITE) or (check and rc == L.MPA_NONFINITE):
         L.check(rc, "bicgstab")
@@ -458,29 +476,45 @@ def _sys_struct(A, b, x, state, tol, maxiter, add_identity):
     return s
 
 
-def bicgstab_batched(matrices, rhs, tol=1e-5, maxiter=1000, add_identity=False, sync_every=1):
+def bicgstab_batched(matrices, rhs, tol=1e-5, maxiter=1000, add_identity=False, sync_every=1,
+                     preconditioners=None, ref_loop=False):
     """Solve several independent systems in one batched BiCGSTAB.
 
     matrices: list of [n_i, n_i] tensors (one dtype) or ClusterOperator
     (matrix-free, complex128); rhs: list of [S_i, n_i].
-    tol / maxiter: scalars or per-system lists.  Returns (list of x, list of SolveReport)."""
+    tol / maxiter: scalars or per-system lists.  preconditioners: None or a
+    per-system list of None / (M_inv_blocks [n_i, bs] device tensor, bs), the
+    right preconditioner of `bicgstab`; ref_loop: scalar or per-system list
+    (the reference's own loop, as in `bicgstab`).
+    Returns (list of x, list of SolveReport)."""
     lib = L.load()
     k = len(matrices)
     dtype = matrices[0].dtype
     code = DTYPE_CODE[dtype]
     tols = tol if isinstance(tol, (list, tuple)) else [tol] * k
     maxits = maxiter if isinstance(maxiter, (list, tuple)) else [maxiter] * k
-    xs, states, structs = [], [], (L.MpaSystem * k)()
+    pcs = preconditioners if preconditioners is not None else [None] * k
+    refs = ref_loop if isinstance(ref_loop, (list, tuple)) else [ref_loop] * k
+    xs, states, structs, keep = [], [], (L.MpaSystem * k)(), []
     for i, (A, b) in enumerate(zip(matrices, rhs)):
         n = A.shape[0]
         b2 = b.reshape(-1, n)
         if A.dtype != dtype or b2.dtype != dtype or not b2.is_contiguous():
             raise ValueError("all matrices and right-hand sides need one dtype and contiguous storage")
         x = torch.empty_like(b2)
-        st = torch.empty(lib.mpa_system_state_bytes(n, b2.shape[0], code), dtype=torch.uint8, device=A.device)
+        st = torch.empty(lib.mpa_system_state_bytes_ex(n, b2.shape[0], code, 1 if pcs[i] is not None else 0),
+                         dtype=torch.uint8, device=A.device)
         xs.append(x)
         states.append(st)
         structs[i] = _sys_struct(A, b2, x, st, tols[i], maxits[i], add_identity)
+        structs[i].flags = L.MPA_SOLVE_REF_LOOP if refs[i] else 0
+        if pcs[i] is not None:
+            M, bs = pcs[i]
+            if M.dtype != dtype or not M.is_contiguous() or M.numel() != n * bs:
+                raise ValueError("each preconditioner is a contiguous [n, bs] tensor of the matrices' dtype")
+            keep.append(M)
+            structs[i].minv = M.data_ptr()
+            structs[i].block_size = int(bs)
     scratch = torch.empty(lib.mpa_batch_scratch_bytes(k, structs, code), dtype=torch.uint8, device=matrices[0].device)
     total = sum(x.shape[0] for x in xs)
     it = (ctypes.c_int * total)()

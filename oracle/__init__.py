@@ -47,6 +47,8 @@ _SIGS = {
     "orc_particle_aggregate_tmatrix": (None, [_pd, _pd, _i, _i, _d, _pd]),
     "orc_bicgstab_eigen": (_i, [_pd, _pd, _i, _d, _i, _pd, _pd]),
     "orc_bicgstab_custom": (_i, [_pd, _pd, _i, _i, _d, _pd, _pd]),
+    "orc_bicgstab_eigen_pc": (_i, [_pd, _pd, _i, _pd, _i, _d, _i, _pd, _pd]),
+    "orc_bicgstab_custom_pc": (_i, [_pd, _pd, _i, _pd, _i, _i, _d, _pd, _pd]),
     "orc_lu_solve": (_i, [_pd, _pd, _i, _pd]),
     "orc_solve_linear_system": (_i, [_pd, _pd, _i, _i, _d, _i, _pd, _pd]),
     "orc_gemv": (None, [_pd, _pd, _pd, _i]),
@@ -210,24 +212,72 @@ def particle_aggregate_tmatrix(positions, tmatrix, k):
     return out
 
 
-def bicgstab_eigen(A, b, tol=1e-5, maxiter=1000):
-    """Eigen-style BiCGSTAB on the full matrix A.  Returns (x, iters, rel_err)."""
+def bicgstab_eigen(A, b, tol=1e-5, maxiter=1000, M_inv_blocks=None, block_size=0):
+    """Eigen-style BiCGSTAB on the full matrix A, optionally right-preconditioned
+    by the block-diagonal M_inv_blocks [n, block_size].  Returns (x, iters, rel_err)."""
     A = _c128(A)
     b = _c128(b).reshape(-1)
     x = np.zeros_like(b)
     err = np.zeros(1)
-    it = lib().orc_bicgstab_eigen(_p(A), _p(b), b.size, float(tol), int(maxiter), _p(x), _p(err))
+    if M_inv_blocks is None:
+        it = lib().orc_bicgstab_eigen(_p(A), _p(b), b.size, float(tol), int(maxiter), _p(x), _p(err))
+    else:
+        M = _c128(M_inv_blocks).reshape(b.size, block_size)
+        it = lib().orc_bicgstab_eigen_pc(_p(A), _p(b), b.size, _p(M), int(block_size), float(tol), int(maxiter),
+                                         _p(x), _p(err))
     return x, it, float(err[0])
 
 
-def bicgstab(A, b, maxiter, tolerance):
-    """Custom BiCGSTAB (x0 = b, absolute tol).  Returns (x, iters, residual)."""
+def bicgstab(A, b, maxiter, tolerance, M_inv_blocks=None, block_size=0):
+    """Custom BiCGSTAB (x0 = b, absolute tol), optionally right-preconditioned
+    (cpp/src/interactions.cpp:230-289).  Returns (x, iters, residual)."""
     A = _c128(A)
     b = _c128(b).reshape(-1)
     x = np.zeros_like(b)
     res = np.zeros(1)
-    it = lib().orc_bicgstab_custom(_p(A), _p(b), b.size, int(maxiter), float(tolerance), _p(x), _p(res))
+    if M_inv_blocks is None:
+        it = lib().orc_bicgstab_custom(_p(A), _p(b), b.size, int(maxiter), float(tolerance), _p(x), _p(res))
+    else:
+        M = _c128(M_inv_blocks).reshape(b.size, block_size)
+        it = lib().orc_bicgstab_custom_pc(_p(A), _p(b), b.size, _p(M), int(block_size), int(maxiter),
+                                          float(tolerance), _p(x), _p(res))
     return x, it, float(res[0])
+
+
+def build_block_preconditioner_sphere(mie_scat, lmax):
+    """src/miepy/interactions.py:9-40: blocks diag(1 / (1 + mie_scat[i, pol, n-1])),
+    identity where |1 + a| < 1e-12.  Returns (M_inv_blocks [N*bs, bs], bs)."""
+    mie_scat = np.asarray(mie_scat, dtype=complex)
+    N = mie_scat.shape[0]
+    rmax = lmax_to_rmax(lmax)
+    bs = 2 * rmax
+    M = np.zeros((N * bs, bs), dtype=complex)
+    for i in range(N):
+        for pol in range(2):
+            r = 0
+            for n in range(1, lmax + 1):
+                for _m in range(-n, n + 1):
+                    idx = pol * rmax + r
+                    d = 1.0 + mie_scat[i, pol, n - 1]
+                    M[i * bs + idx, idx] = 1.0 if abs(d) < 1e-12 else 1.0 / d
+                    r += 1
+    return M, bs
+
+
+def build_block_preconditioner_particle(tmatrix):
+    """src/miepy/interactions.py:43-68: blocks inv(I + T_i), identity for a
+    singular block.  Returns (M_inv_blocks [N*bs, bs], bs)."""
+    tmatrix = np.asarray(tmatrix, dtype=complex)
+    N = tmatrix.shape[0]
+    bs = 2 * tmatrix.shape[-1]
+    M = np.zeros((N * bs, bs), dtype=complex)
+    for i in range(N):
+        blk = np.eye(bs) + tmatrix[i].reshape(bs, bs)
+        try:
+            M[i * bs:(i + 1) * bs] = np.linalg.inv(blk)
+        except np.linalg.LinAlgError:
+            M[i * bs:(i + 1) * bs] = np.eye(bs)
+    return M, bs
 
 
 def lu_solve(A, b):

@@ -677,23 +677,41 @@ void orc_particle_aggregate_tmatrix(const double* positions, const double* tmatr
 
 // ---- solvers ---------------------------------------------------------------
 
-// Restates Eigen::BiCGSTAB (Eigen 3.x, IterativeLinearSolvers/BiCGSTAB.h) with
-// the identity preconditioner, as driven at cpp/src/interactions.cpp:359-365:
-// zero initial guess, stop when ||r||^2 <= tol^2 ||b||^2 or at maxiter,
-// restart on rho breakdown.  A is the full system matrix (I + T).
-// Returns iterations; *err = ||r|| / ||b||.
-int orc_bicgstab_eigen(const double* A_, const double* b_, int n, double tol, int maxiter, double* x_,
-                       double* err) {
-    const cd* A = reinterpret_cast<const cd*>(A_);
-    const cd* bp = reinterpret_cast<const cd*>(b_);
+// Block-diagonal preconditioner apply (cpp/src/interactions.cpp:211-228):
+// y[blk] = Minv[blk rows, 0:bs] x[blk] for each bs-long block; Minv is the
+// [n, bs] row-major stack of the N block inverses.  Minv == null is the
+// identity (a copy).
+static void apply_block_precond(const cd* Minv, int bs, const std::vector<cd>& x, std::vector<cd>& y) {
+    const int n = (int)x.size();
+    if (!Minv) {
+        y = x;
+        return;
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < n; i++) {
+        const int b0 = (i / bs) * bs;
+        cd acc = 0;
+        for (int k = 0; k < bs; k++) acc += Minv[(size_t)i * bs + k] * x[b0 + k];
+        y[i] = acc;
+    }
+}
+
+// Restates Eigen::BiCGSTAB (Eigen 3.x, IterativeLinearSolvers/BiCGSTAB.h) as
+// driven at cpp/src/interactions.cpp:359-365 (identity preconditioner) and
+// :324-333 (BlockDiagonalPreconditioner, Minv != null): zero initial guess,
+// stop when ||r||^2 <= tol^2 ||b||^2 or at maxiter, restart on rho
+// breakdown; right preconditioning y = M^-1 p, z = M^-1 s, x += alpha y + w z.
+// A is the full system matrix (I + T).  Returns iterations; *err = ||r||/||b||.
+static int bicgstab_eigen(const cd* A, const cd* bp, int n, const cd* Minv, int bs, double tol, int maxiter, cd* xout,
+                   double* err) {
     std::vector<cd> b(bp, bp + n), x(n, cd(0, 0));
-    std::vector<cd> r(n), r0, v(n, 0.0), p(n, 0.0), s(n), t(n), tmp(n);
+    std::vector<cd> r(n), r0, v(n, 0.0), p(n, 0.0), s(n), t(n), tmp(n), y(n), z(n);
     gemv(A, x.data(), tmp.data(), n);
     for (int i = 0; i < n; i++) r[i] = b[i] - tmp[i];
     r0 = r;
     double r0_sq = sqnorm(r0), rhs_sq = sqnorm(b);
     if (rhs_sq == 0) {
-        std::fill(x_, x_ + 2 * n, 0.0);
+        std::fill(xout, xout + n, cd(0, 0));
         *err = 0;
         return 0;
     }
@@ -715,52 +733,71 @@ int orc_bicgstab_eigen(const double* A_, const double* b_, int n, double tol, in
         }
         cd beta = (rho / rho_old) * (alpha / w);
         for (int i = 0; i < n; i++) p[i] = r[i] + beta * (p[i] - w * v[i]);
-        gemv(A, p.data(), v.data(), n);
+        apply_block_precond(Minv, bs, p, y);
+        gemv(A, y.data(), v.data(), n);
         alpha = rho / dotc(r0, v);
         for (int i = 0; i < n; i++) s[i] = r[i] - alpha * v[i];
-        gemv(A, s.data(), t.data(), n);
+        apply_block_precond(Minv, bs, s, z);
+        gemv(A, z.data(), t.data(), n);
         double tt = sqnorm(t);
         w = (tt > 0) ? dotc(t, s) / tt : cd(0, 0);
         for (int i = 0; i < n; i++) {
-            x[i] += alpha * p[i] + w * s[i];
+            x[i] += alpha * y[i] + w * z[i];
             r[i] = s[i] - w * t[i];
         }
         ++it;
     }
     *err = std::sqrt(sqnorm(r) / rhs_sq);
-    std::memcpy(x_, x.data(), sizeof(cd) * n);
+    std::memcpy(xout, x.data(), sizeof(cd) * n);
     return it;
 }
 
+int orc_bicgstab_eigen(const double* A_, const double* b_, int n, double tol, int maxiter, double* x_,
+                       double* err) {
+    return bicgstab_eigen(reinterpret_cast<const cd*>(A_), reinterpret_cast<const cd*>(b_), n, nullptr, 1, tol,
+                          maxiter, reinterpret_cast<cd*>(x_), err);
+}
+
+// The same with the block-diagonal preconditioner Minv [n, bs]
+// (cpp/src/interactions.cpp:305-345, solve_linear_system_preconditioned).
+int orc_bicgstab_eigen_pc(const double* A_, const double* b_, int n, const double* Minv, int bs, double tol,
+                          int maxiter, double* x_, double* err) {
+    return bicgstab_eigen(reinterpret_cast<const cd*>(A_), reinterpret_cast<const cd*>(b_), n,
+                          reinterpret_cast<const cd*>(Minv), bs, tol, maxiter, reinterpret_cast<cd*>(x_), err);
+}
+
 // cpp/src/interactions.cpp:152-209 (bicgstab_profiled; bicgstab :98-150 is
-// the same loop): x0 = b, absolute tolerance on ||r||.  Returns iterations.
-int orc_bicgstab_custom(const double* A_, const double* b_, int n, int maxiter, double tol, double* x_,
-                        double* residual) {
-    const cd* A = reinterpret_cast<const cd*>(A_);
-    const cd* bp = reinterpret_cast<const cd*>(b_);
+// the same loop) and, with Minv != null, :230-289
+// (bicgstab_preconditioned_profiled: p_hat = M^-1 p, s_hat = M^-1 s,
+// x = x + alpha p_hat + w s_hat): x0 = b, absolute tolerance on ||r||, no
+// restart.  Returns iterations.
+static int bicgstab_custom(const cd* A, const cd* bp, int n, const cd* Minv, int bs, int maxiter, double tol, cd* xout,
+                    double* residual) {
     std::vector<cd> b(bp, bp + n), x = b, r(n), tmp(n);
     gemv(A, x.data(), tmp.data(), n);
     for (int i = 0; i < n; i++) r[i] = b[i] - tmp[i];
     double error = std::sqrt(sqnorm(r));
     if (error < tol) {
-        std::memcpy(x_, x.data(), sizeof(cd) * n);
+        std::memcpy(xout, x.data(), sizeof(cd) * n);
         *residual = error;
         return 0;
     }
-    std::vector<cd> rhat = r, v(n, 0.0), p(n, 0.0), s(n), t(n), pi(n), vi(n);
+    std::vector<cd> rhat = r, v(n, 0.0), p(n, 0.0), s(n), t(n), pi(n), vi(n), ph(n), sh(n);
     cd rho_prev = 1, alpha = 1, w_prev = 1;
     int it = 1;
     while (true) {
         cd rho = dotc(rhat, r);
         cd beta = (rho / rho_prev) * (alpha / w_prev);
         for (int i = 0; i < n; i++) pi[i] = r[i] + beta * (p[i] - w_prev * v[i]);
-        gemv(A, pi.data(), vi.data(), n);
+        apply_block_precond(Minv, bs, pi, ph);
+        gemv(A, ph.data(), vi.data(), n);
         alpha = rho / dotc(rhat, vi);
         for (int i = 0; i < n; i++) s[i] = r[i] - alpha * vi[i];
-        gemv(A, s.data(), t.data(), n);
+        apply_block_precond(Minv, bs, s, sh);
+        gemv(A, sh.data(), t.data(), n);
         cd w = dotc(t, s) / dotc(t, t);
         for (int i = 0; i < n; i++) {
-            x[i] = x[i] + alpha * pi[i] + w * s[i];
+            x[i] = x[i] + alpha * ph[i] + w * sh[i];
             r[i] = s[i] - w * t[i];
         }
         error = std::sqrt(sqnorm(r));
@@ -771,9 +808,21 @@ int orc_bicgstab_custom(const double* A_, const double* b_, int n, int maxiter, 
         w_prev = w;
         it++;
     }
-    std::memcpy(x_, x.data(), sizeof(cd) * n);
+    std::memcpy(xout, x.data(), sizeof(cd) * n);
     *residual = error;
     return it;
+}
+
+int orc_bicgstab_custom(const double* A_, const double* b_, int n, int maxiter, double tol, double* x_,
+                        double* residual) {
+    return bicgstab_custom(reinterpret_cast<const cd*>(A_), reinterpret_cast<const cd*>(b_), n, nullptr, 1, maxiter,
+                           tol, reinterpret_cast<cd*>(x_), residual);
+}
+
+int orc_bicgstab_custom_pc(const double* A_, const double* b_, int n, const double* Minv, int bs, int maxiter,
+                           double tol, double* x_, double* residual) {
+    return bicgstab_custom(reinterpret_cast<const cd*>(A_), reinterpret_cast<const cd*>(b_), n,
+                           reinterpret_cast<const cd*>(Minv), bs, maxiter, tol, reinterpret_cast<cd*>(x_), residual);
 }
 
 // Partial-pivot LU solve (Eigen partialPivLu at cpp/src/interactions.cpp:357).
