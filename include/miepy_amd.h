@@ -161,6 +161,26 @@ int mpa_bicgstab_ex(int n, int nrhs, int dtype, const void* A, int add_identity,
                     int maxiter, int flags, const void* minv, int block_size, void* workspace, int* iters_out,
                     double* resid_out, int* status_out, int sync_every, void* stream);
 
+/* ---- dense LU: the exact method ---------------------------------------------- */
+
+/* Device workspace of one factorization (pivots, row permutation); the same
+ * workspace goes to mpa_lu_factor and mpa_lu_solve. */
+size_t mpa_lu_workspace_bytes(int n);
+
+/* In-place LU with partial pivoting of (add_identity ? I + A : A), A [n, n]
+ * c128 row-major (device): L (unit diagonal) and U overwrite A, the row
+ * permutation goes to the workspace.  Replaces the partialPivLu() of the
+ * exact branch of solve_linear_system (cpp/src/interactions.cpp:356-357 and
+ * :338-339).  Pivot: the first row of largest modulus in the column.
+ * *info_out (may be NULL): 0, or 1 + the first column without a nonzero
+ * pivot, returned as MPA_BREAKDOWN (Eigen divides by the zero pivot
+ * silently).  Synchronizes the stream. */
+int mpa_lu_factor(int n, void* A, int add_identity, void* workspace, int* info_out, void* stream);
+
+/* x = A^-1 b from mpa_lu_factor's factors and workspace; b, x [nrhs, n] c128
+ * (device, distinct buffers). */
+int mpa_lu_solve(int n, int nrhs, const void* LU, void* workspace, const void* b, void* x, void* stream);
+
 /* ---- matrix-free operator ----------------------------------------------------
  * The aggregate interaction operator of a sphere cluster applied without
  * storing it: the translation blocks of interactions.cpp:370-415 /

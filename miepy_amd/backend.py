@@ -35,7 +35,7 @@ class solver(enum.Enum):
 
 
 # Solver knobs the reference hard-codes (cpp/src/interactions.cpp:362-363).
-options = dict(tol=1e-5, maxiter=1000, exact_tol=1e-14)
+options = dict(tol=1e-5, maxiter=1000)
 
 
 class DeviceTMatrix:
@@ -110,18 +110,23 @@ def solve_linear_system(tmatrix, p_src, method=solver.bicgstab, M_inv_blocks=Non
     method=bicgstab runs BiCGSTAB (default tolerance 1e-5 as the reference),
     right-preconditioned by the block-diagonal M_inv_blocks [n, block_size]
     when it is given (solve_linear_system_preconditioned,
-    cpp/src/interactions.cpp:291-345); method=exact runs the same solver to a
-    tolerance of 1e-14 and, like the reference's LU branch, ignores the
-    preconditioner."""
+    cpp/src/interactions.cpp:291-345); method=exact factors I + T by LU with
+    partial pivoting on the device (interactions.cpp:356-357) and, like the
+    reference, ignores the preconditioner."""
     p_src = np.asarray(p_src)
     shape = p_src.shape
     n = p_src.size
     A = _device_matrix(tmatrix, n)
     exact = (method == solver.exact) or (getattr(method, "name", None) == "exact")
-    t = options["exact_tol"] if exact else (options["tol"] if tol is None else tol)
-    mi = max(options["maxiter"], 10 * n) if exact else (options["maxiter"] if maxiter is None else maxiter)
+    if exact:
+        A64 = A if A.dtype == D.C128 else A.to(D.C128)
+        x = D.lu_solve(A64, D.to_device(p_src.reshape(1, n), D.C128), add_identity=True)
+        solve_linear_system.last_report = None
+        return x.cpu().numpy().reshape(shape)
+    t = options["tol"] if tol is None else tol
+    mi = options["maxiter"] if maxiter is None else maxiter
     b = D.to_device(p_src.reshape(1, n), A.dtype)
-    M = None if (M_inv_blocks is None or exact) else _device_blocks(M_inv_blocks, n, int(block_size), A.dtype)
+    M = None if M_inv_blocks is None else _device_blocks(M_inv_blocks, n, int(block_size), A.dtype)
     x, report = D.bicgstab(A, b, tol=t, maxiter=mi, add_identity=True, M_inv_blocks=M,
                            block_size=int(block_size) if M is not None else 0)
     solve_linear_system.last_report = report

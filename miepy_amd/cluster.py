@@ -123,9 +123,16 @@ class cluster:
         n = 2 * rmax * N
         if self.interactions and N > 1:
             exact = self.method == solver.exact or getattr(self.method, "name", None) == "exact"
+            # exact without a stored matrix (matrix-free mode): BiCGSTAB to 1e-14
             tol = 1e-14 if exact else (1e-5 if self.tol is None else self.tol)
             maxiter = max(self.maxiter, 10 * n) if exact else self.maxiter
-            if self.solver_mode == "stored":
+            if exact and self.solver_mode == "stored":
+                # the reference's exact branch: LU of I + T (interactions.cpp:356-357)
+                A = D.assemble_tmatrix(self.position, tm_d, k, add_identity=True)
+                x = D.lu_solve(A, p_src_d, overwrite=True)
+                self.report = None
+                del A
+            elif self.solver_mode == "stored":
                 A = D.assemble_tmatrix(self.position, tm_d, k, add_identity=True)
                 x, self.report = D.bicgstab(A, p_src_d, tol=tol, maxiter=maxiter)
                 del A

@@ -232,6 +232,28 @@ def bicgstab(A, b, tol=1e-5, maxiter=1000, add_identity=False, x=None, workspace
     return x.reshape(b.shape), report
 
 
+def lu_solve(A, b, add_identity=False, overwrite=False):
+    """Dense LU with partial pivoting on the device (the `exact` method):
+    x = (I +) A^-1 b.  A: [n, n] complex128; b: [n] or [S, n].  A is copied
+    unless overwrite (then it holds the factors afterwards).  Raises
+    numpy.linalg.LinAlgError for a singular matrix.  Returns x like b."""
+    lib = L.load()
+    if A.dtype != C128 or b.dtype != C128:
+        raise ValueError("lu_solve is complex128")
+    n = A.shape[0]
+    F = A if overwrite else A.clone()
+    b2 = b.reshape(-1, n).contiguous()
+    ws = torch.empty(lib.mpa_lu_workspace_bytes(n), dtype=torch.uint8, device=A.device)
+    info = ctypes.c_int(0)
+    rc = lib.mpa_lu_factor(n, _ptr(F), 1 if add_identity else 0, _ptr(ws), ctypes.byref(info), _stream())
+    if rc == L.MPA_BREAKDOWN:
+        raise np.linalg.LinAlgError(f"singular matrix: no nonzero pivot in column {info.value - 1}")
+    L.check(rc, "lu_factor")
+    x = torch.empty_like(b2)
+    L.check(lib.mpa_lu_solve(n, b2.shape[0], _ptr(F), _ptr(ws), _ptr(b2), _ptr(x), _stream()), "lu_solve")
+    return x.reshape(b.shape)
+
+
 def observables(p_inc, p_src, mie_scat, mie_int, k, eps_b, want_int=True, want_torque=True):
     """Per-source observables of a solved cluster (all FP64, on the device).
 

@@ -122,10 +122,17 @@ class sphere_cluster:
         n = 2 * rmax * N
         if self.interactions and N > 1:
             exact = self.method == solver.exact or getattr(self.method, "name", None) == "exact"
+            # exact without a stored matrix (matrix-free precisions): BiCGSTAB to 1e-14
             tol = 1e-14 if exact else (1e-5 if self.tol is None else self.tol)
             maxiter = max(self.maxiter, 10 * n) if exact else self.maxiter
             b = p_src_d.reshape(S, n)
-            if self.precision == "fp64":
+            if exact and self.precision in ("fp64", "mixed"):
+                # the reference's exact branch: LU of I + T (interactions.cpp:356-357), in FP64
+                A = D.assemble(self.position, mie_scat_d, k, dtype=D.C128, add_identity=True)
+                x = D.lu_solve(A, b, overwrite=True)
+                self.report = None
+                del A
+            elif self.precision == "fp64":
                 A = D.assemble(self.position, mie_scat_d, k, dtype=D.C128, add_identity=True)
                 x, self.report = D.bicgstab(A, b, tol=tol, maxiter=maxiter)
                 del A
