@@ -161,6 +161,32 @@ int mpa_bicgstab_ex(int n, int nrhs, int dtype, const void* A, int add_identity,
                     int maxiter, int flags, const void* minv, int block_size, void* workspace, int* iters_out,
                     double* resid_out, int* status_out, int sync_every, void* stream);
 
+/* ---- cluster coefficients ------------------------------------------------------ */
+
+/* Device workspace of mpa_cluster_coefficients (per-chunk partial sums). */
+size_t mpa_cluster_coefficients_workspace_bytes(int N, int lmax_out, int nrhs);
+
+/* The cluster's scattered-field coefficients about one origin: every
+ * particle's p_scat re-expanded with the incident-mode translation of
+ * origin - r_i, summed over particles (a particle exactly at the origin adds
+ * its coefficients).  Replaces miepy.cluster_coefficients
+ * (src/miepy/vsh/cluster_coefficients.py:14-53), which loops over the scalar
+ * vsh_translation (cpp/src/vsh_translation.cpp:428-470) per particle and mode.
+ *   positions  [N, 3] f64 (device); origin (ox, oy, oz)
+ *   p_scat     [nrhs, N, 2, rmax(lmax_in)] c128 (device)
+ *   p_cluster  [nrhs, 2, rmax(lmax_out)] c128 (device), overwritten
+ *   1 <= lmax_in, lmax_out <= 6; k the medium wavenumber. */
+int mpa_cluster_coefficients(int N, int lmax_in, int lmax_out, int nrhs, const double* positions, double ox,
+                             double oy, double oz, const void* p_scat, double k, void* p_cluster, void* workspace,
+                             void* stream);
+
+/* Cluster cross-sections per multipole (cpp/src/flux.cpp:32-57,
+ * src/miepy/flux.py:35-56): p_cluster and p_src (the source's coefficients at
+ * the origin) [nrhs, 2, rmax] c128 -> C [nrhs, 3 (scattering, absorption,
+ * extinction), 2, lmax] f64 (all device). */
+int mpa_cluster_cross_sections(int lmax, int nrhs, const void* p_cluster, const void* p_src, double k, double* C,
+                               void* stream);
+
 /* ---- dense LU: the exact method ---------------------------------------------- */
 
 /* Device workspace of one factorization (pivots, row permutation); the same

@@ -12,6 +12,8 @@ takes in src/miepy/interactions.py:81-83,141-143 and src/miepy/flux.py:21-26):
                                                     cpp/src/interactions_pyb.cpp:39-69
   particle_cross_sections(p_scat, p_inc, p_src, k)  flux.py:12-32
   force_all / torque_all(p_scat, p_inc, k, eps_b, mu_b)  forces.py:44-81
+  cluster_coefficients(positions, p_scat, k, origin, lmax)  vsh/cluster_coefficients.py:14-53
+  cluster_cross_sections(p_cluster, p_src, k)       flux.py:35-56
 
 Like the JAX backend, the assembled matrix stays in HBM: it is returned as a
 `DeviceTMatrix`, which has the reference's [N,2,rmax,N,2,rmax] shape,
@@ -202,6 +204,28 @@ def bicgstab_preconditioned_profiled(A, b, M_inv_blocks, block_size, maxiter=100
 
 
 solve_linear_system.last_report = None
+
+
+def cluster_coefficients(positions, p_scat, k, origin, lmax=None):
+    """miepy.cluster_coefficients (src/miepy/vsh/cluster_coefficients.py:14-53):
+    p_scat[N,2,rmax] of the particles -> p_cluster[2,rmax(lmax)] about origin
+    (host numpy), computed on the device."""
+    p_scat = np.asarray(p_scat, dtype=complex)
+    rmax_in = p_scat.shape[-1]
+    lmax_in = int(round((1 + rmax_in) ** 0.5)) - 1
+    lmax = lmax_in if lmax is None else int(lmax)
+    pc = D.cluster_coefficients(positions, p_scat[None], float(np.real(k)), origin, lmax)
+    return pc[0].cpu().numpy()
+
+
+def cluster_cross_sections(p_cluster, p_src, k):
+    """miepy.flux.cluster_cross_sections (src/miepy/flux.py:35-56,
+    cpp/src/flux.cpp:32-57): (scattering, absorption, extinction), each [2, lmax]."""
+    from .flux import cross_sections
+    pc = np.asarray(p_cluster, dtype=complex).reshape(1, 2, -1)
+    C = D.cluster_cross_sections(D.to_device(pc, D.C128), np.asarray(p_src, dtype=complex).reshape(pc.shape),
+                                 float(np.real(k))).cpu().numpy()[0]
+    return cross_sections(C[0], C[1], C[2])
 
 
 def _rows(a):

@@ -161,6 +161,25 @@ class cluster:
 
     # ---- observables (src/miepy/cluster.py:379-510) ---------------------------------
 
+    def solve_cluster_coefficients(self, lmax=None):
+        """p_cluster [2, rmax]: the cluster's scattered field expanded about
+        self.origin up to lmax (src/miepy/cluster.py:616-627)."""
+        lmax = self.lmax if lmax is None else lmax
+        k = float(np.real(self.material_data.k_b))
+        self.p_cluster = D.cluster_coefficients(self.position, self.p_scat[None], k, self.origin,
+                                                lmax).cpu().numpy()[0]
+        return self.p_cluster
+
+    def cross_sections_per_multipole(self, lmax=None):
+        """Cluster (scattering, absorption, extinction) per multipole, each
+        [2, lmax] (src/miepy/cluster.py:379-391)."""
+        lmax = self.lmax if lmax is None else lmax
+        self.solve_cluster_coefficients(lmax)
+        k = float(np.real(self.material_data.k_b))
+        p0 = self.source.structure(np.array([self.origin]), k, lmax)[0]
+        C = D.cluster_cross_sections(D.to_device(self.p_cluster[None], D.C128), p0[None], k).cpu().numpy()[0]
+        return _cs(C[0], C[1], C[2])
+
     def cross_sections_per_multipole_of_particle(self, i):
         from .backend import particle_cross_sections
         return particle_cross_sections(self.p_scat[i], self.p_inc[i], self.p_src[i], self.material_data.k_b)

@@ -254,6 +254,47 @@ def lu_solve(A, b, add_identity=False, overwrite=False):
     return x.reshape(b.shape)
 
 
+def cluster_coefficients(positions, p_scat, k, origin, lmax):
+    """Scattered-field coefficients of the whole cluster about `origin`
+    (src/miepy/vsh/cluster_coefficients.py:14-53) on the device.
+
+    positions [N, 3]; p_scat [S, N, 2, rmax_in] complex128 (device or host);
+    lmax: the output order (<= 6).  Returns p_cluster [S, 2, rmax] (device)."""
+    lib = L.load()
+    require_gpu()
+    pos = to_device(np.asarray(positions, dtype=float).reshape(-1, 3), torch.float64)
+    ps = p_scat if isinstance(p_scat, torch.Tensor) else to_device(np.asarray(p_scat, dtype=complex), C128)
+    ps = ps.to(C128).contiguous()
+    S, N, _, rmax_in = ps.shape
+    if N != pos.shape[0]:
+        raise ValueError("p_scat and positions disagree on the number of particles")
+    lmax_in = int(round((1 + rmax_in) ** 0.5)) - 1
+    rmax = lmax_to_rmax(lmax)
+    out = torch.empty((S, 2, rmax), dtype=C128, device=ps.device)
+    ws = torch.empty(lib.mpa_cluster_coefficients_workspace_bytes(N, lmax, S), dtype=torch.uint8, device=ps.device)
+    o = np.asarray(origin, dtype=float).reshape(3)
+    L.check(lib.mpa_cluster_coefficients(N, lmax_in, int(lmax), S, _ptr(pos), float(o[0]), float(o[1]), float(o[2]),
+                                         _ptr(ps), float(k), _ptr(out), _ptr(ws), _stream()), "cluster_coefficients")
+    return out
+
+
+def cluster_cross_sections(p_cluster, p_src, k):
+    """Per-multipole (scattering, absorption, extinction) of the cluster
+    (cpp/src/flux.cpp:32-57) on the device: p_cluster, p_src [S, 2, rmax]
+    complex128 (p_src: the source's coefficients at the origin).  Returns
+    C [S, 3, 2, lmax] float64 (device)."""
+    lib = L.load()
+    pc = p_cluster.to(C128).contiguous()
+    p0 = p_src if isinstance(p_src, torch.Tensor) else to_device(np.asarray(p_src, dtype=complex), C128)
+    p0 = p0.to(C128).reshape(pc.shape).contiguous()
+    S, _, rmax = pc.shape
+    lmax = int(round((1 + rmax) ** 0.5)) - 1
+    C = torch.empty((S, 3, 2, lmax), dtype=torch.float64, device=pc.device)
+    L.check(lib.mpa_cluster_cross_sections(lmax, S, _ptr(pc), _ptr(p0), float(k), _ptr(C), _stream()),
+            "cluster_cross_sections")
+    return C
+
+
 def observables(p_inc, p_src, mie_scat, mie_int, k, eps_b, want_int=True, want_torque=True):
     """Per-source observables of a solved cluster (all FP64, on the device).
 

@@ -179,6 +179,29 @@ class sphere_cluster:
         C = self._C_total
         return _cs(C[..., 0], C[..., 1], C[..., 2])
 
+    def solve_cluster_coefficients(self, lmax=None):
+        """p_cluster [(S,) 2, rmax]: the cluster's scattered field expanded about
+        self.origin up to lmax (src/miepy/sphere_cluster.py:651-662)."""
+        lmax = self.lmax if lmax is None else lmax
+        k = float(np.real(self.material_data.k_b))
+        p_scat = self.p_scat if self._multi else self.p_scat[None]
+        pc = D.cluster_coefficients(self.position, p_scat, k, self.origin, lmax).cpu().numpy()
+        self.p_cluster = pc if self._multi else pc[0]
+        return self.p_cluster
+
+    def cross_sections_per_multipole(self, lmax=None):
+        """Cluster (scattering, absorption, extinction) per multipole, each
+        [(S,) 2, lmax], from the cluster coefficients about self.origin
+        (src/miepy/sphere_cluster.py:428-440)."""
+        lmax = self.lmax if lmax is None else lmax
+        self.solve_cluster_coefficients(lmax)
+        k = float(np.real(self.material_data.k_b))
+        p0 = np.stack([s.structure(np.array([self.origin]), k, lmax)[0] for s in self._sources()])
+        pc = self.p_cluster if self._multi else self.p_cluster[None]
+        C = D.cluster_cross_sections(D.to_device(pc, D.C128), p0, k).cpu().numpy()
+        C = C if self._multi else C[0]
+        return _cs(C[..., 0, :, :], C[..., 1, :, :], C[..., 2, :, :])
+
     def cross_sections_per_multipole_of_particle(self, i):
         C = self._C_part[..., i, :, :, :]
         return _cs(C[..., 0, :, :], C[..., 1, :, :], C[..., 2, :, :])

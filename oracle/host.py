@@ -175,3 +175,38 @@ def solve_cluster(position, radius, eps, wavelength, lmax, polarization=(1, 0), 
     Tq = O.torque_all(p_scat, p_inc, k, np.real(eps_b), np.real(mu_b))
     return dict(k=k, mie_scat=mie_scat, mie_int=mie_int, p_src=p_src, p_inc=p_inc, p_scat=p_scat, p_int=p_int,
                 T=T, iters=iters, err=err, cross_sections=Cs, per_particle=per_particle, force=F, torque=Tq)
+
+
+# ---- cluster coefficients ------------------------------------------------------------
+
+def cluster_coefficients(positions, p_scat, k, origin, lmax=None):
+    """src/miepy/vsh/cluster_coefficients.py:14-53: every particle's outgoing
+    expansion re-expanded about `origin` with the incident-mode translation
+    coefficients of rij = origin - r_i (the scalar vsh_translation,
+    cpp/src/vsh_translation.cpp:428-470, through the oracle); a particle at the
+    origin adds its coefficients directly.  Returns p_cluster [2, rmax]."""
+    positions = np.asarray(positions, dtype=float).reshape(-1, 3)
+    p_scat = np.asarray(p_scat, dtype=complex)
+    origin = np.asarray(origin, dtype=float)
+    rmax_in = p_scat.shape[-1]
+    lmax_in = O.rmax_to_lmax(rmax_in)
+    lmax = lmax_in if lmax is None else lmax
+    rmax = O.lmax_to_rmax(lmax)
+    modes = lambda L: [(n, m) for n in range(1, L + 1) for m in range(-n, n + 1)]
+    p_cluster = np.zeros((2, rmax), dtype=complex)
+    for i in range(positions.shape[0]):
+        if np.all(positions[i] == origin):
+            p_cluster[0, :rmax_in] += p_scat[i, 0, :rmax_in]
+            p_cluster[1, :rmax_in] += p_scat[i, 1, :rmax_in]
+            continue
+        d = origin - positions[i]
+        rad = np.linalg.norm(d)
+        theta = np.arccos(d[2] / rad)
+        phi = np.arctan2(d[1], d[0])
+        for r, (n, m) in enumerate(modes(lmax)):
+            for rp, (v, u) in enumerate(modes(lmax_in)):
+                a, b = p_scat[i, 0, rp], p_scat[i, 1, rp]
+                A, B = O.vsh_translation(m, n, u, v, rad, theta, phi, k, mode="incident")
+                p_cluster[0, r] += a * A + b * B
+                p_cluster[1, r] += a * B + b * A
+    return p_cluster

@@ -15,7 +15,11 @@ Pins:
   and index of the assembly independently of the oracle's formulas;
 - assembled blocks vs the scalar vsh_translation entry point;
 - the general T-matrix assembly reduces to the sphere assembly for diagonal
-  (Mie) T-matrices.
+  (Mie) T-matrices;
+- cluster coefficients (src/miepy/vsh/cluster_coefficients.py:14-53, the
+  incident-mode translation): the outgoing field of the particles equals the
+  outgoing expansion of p_cluster about the origin, converging with
+  lmax_out; a particle at the origin contributes its coefficients directly.
 """
 
 import numpy as np
@@ -188,3 +192,36 @@ def test_particle_tmatrix_is_bare_blocks_times_tmatrix(oracle):
             blk = T[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs]
             ref = X[i * bs:(i + 1) * bs, j * bs:(j + 1) * bs] @ tm[j]
             assert np.max(np.abs(blk - ref)) <= 1e-12 * max(np.max(np.abs(ref)), 1e-300)
+
+
+@pytest.mark.parametrize("lmax_in,lmax_out,tol", [(1, 4, 1e-2), (1, 8, 3e-6), (2, 8, 3e-5)])
+def test_cluster_coefficients_addition_theorem(oracle, lmax_in, lmax_out, tol):
+    """The outgoing field of every particle's expansion equals the outgoing
+    expansion of p_cluster about the origin outside a sphere holding the
+    particles (the incident-mode translation re-expands outgoing waves); the
+    truncation at lmax_out leaves an error falling like (kd)^n / (2n+1)!!.
+    kd is about 1.2 here: the reference's upward recurrence for j_p
+    (cpp/src/special.cpp:20-44) loses accuracy when kd is much below p, so
+    smaller separations would test that recurrence rather than the sums."""
+    import oracle.host as H
+    rng = np.random.default_rng(20 + lmax_out)
+    k = 2 * np.pi / 1.0
+    origin = np.array([0.02, -0.01, 0.03])
+    pos = origin + np.array([[0.15, 0.06, -0.09], [-0.12, 0.09, 0.15], [0.0, -0.18, 0.03]])
+    rmax = lmax_in * (lmax_in + 2)
+    p = rng.normal(size=(3, 2, rmax)) + 1j * rng.normal(size=(3, 2, rmax))
+    pc = H.cluster_coefficients(pos, p, k, origin, lmax=lmax_out)
+    dirs = rng.normal(size=(8, 3))
+    dirs /= np.linalg.norm(dirs, axis=1)[:, None]
+    pts = origin + 1.5 * dirs
+    E_parts = sum(_expand(p[i], lmax_in, "outgoing", pts, pos[i], k) for i in range(3))
+    E_cluster = _expand(pc, lmax_out, "outgoing", pts, origin, k)
+    assert np.linalg.norm(E_cluster - E_parts) < tol * np.linalg.norm(E_parts)
+
+
+def test_cluster_coefficients_particle_at_origin(oracle):
+    import oracle.host as H
+    rng = np.random.default_rng(3)
+    p = rng.normal(size=(1, 2, 8)) + 1j * rng.normal(size=(1, 2, 8))
+    pc = H.cluster_coefficients(np.zeros((1, 3)), p, 3.0, np.zeros(3), lmax=3)
+    assert np.array_equal(pc[:, :8], p[0]) and not np.any(pc[:, 8:])
