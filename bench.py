@@ -224,17 +224,15 @@ class Problem:
 
     def __init__(self, w):
         import miepy_amd.device as D
-        from miepy_amd.mie import mie_coefficients
         self.w = w
         k = w.k
-        self.mie_scat, self.mie_int = mie_coefficients(w.radius, w.eps, np.ones(w.N), w.medium_eps, 1.0, k, w.lmax)
-        self.p_src = np.stack([s.structure(w.position, k, w.lmax) for s in w.sources])
+        self.pos_d = D.to_device(w.position, None)
+        self.ms_d, self.mi_d = D.mie_coefficients(w.radius, w.eps, np.ones(w.N), k, w.medium_eps, 1.0, w.lmax)
+        self.src_d = D.plane_wave_structure(list(w.sources), self.pos_d, k, w.lmax)
+        self.mie_scat, self.mie_int = self.ms_d.cpu().numpy(), self.mi_d.cpu().numpy()
+        self.p_src = self.src_d.cpu().numpy()
         self.S = len(w.sources)
         self.n = w.n
-        self.pos_d = D.to_device(w.position, None)
-        self.ms_d = D.to_device(self.mie_scat, D.C128)
-        self.mi_d = D.to_device(self.mie_int, D.C128)
-        self.src_d = D.to_device(self.p_src, D.C128)
 
 
 class Solver:

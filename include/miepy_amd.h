@@ -161,6 +161,29 @@ int mpa_bicgstab_ex(int n, int nrhs, int dtype, const void* A, int add_identity,
                     int maxiter, int flags, const void* minv, int block_size, void* workspace, int* iters_out,
                     double* resid_out, int* status_out, int sync_every, void* stream);
 
+/* ---- per-wavelength inputs ------------------------------------------------------ */
+
+/* Mie coefficients of N spheres: mie_scat[i] = (a_n, b_n), mie_int[i] = (c_n,
+ * d_n), each [N, 2, lmax] c128 (device).  Replaces the per-particle, per-order
+ * calls of mie_sphere_scattering_coefficients / _interior_coefficients
+ * (src/miepy/mie_single/mie_sphere.py:16-76, driven by
+ * src/miepy/sphere_cluster.py:99-125).  Per particle (device arrays [N]):
+ * radius, k (the medium wavenumber) f64; eps, mu, eps_b, mu_b c128;
+ * conducting int (nonzero: perfect conductor) or NULL.  1 <= lmax <= 20. */
+int mpa_mie_coefficients(int N, int lmax, const double* radius, const void* eps, const void* mu, const double* k,
+                         const void* eps_b, const void* mu_b, const int* conducting, void* mie_scat, void* mie_int,
+                         void* stream);
+
+/* Plane-wave source coefficients p_src [nsrc, N, 2, rmax] c128 (device) at
+ * the particle positions [N, 3] (device): source s is the sum of the waves
+ * w in [wave_begin[s], wave_begin[s+1]), p_src[s][i] += e^{i (k r_i . khat_w
+ * + phase_w)} table_w; table [waves, 2, rmax] c128 holds each wave's
+ * amplitude |E_mn| e^{-i m phi} (tau pol0 - i pi pol1, pi pol0 - i tau pol1)
+ * (plane_wave.structure, src/miepy/sources/plane_waves.py:126-156).
+ * wave_begin [nsrc + 1] int, khat [waves, 3] and phase [waves] f64 (device). */
+int mpa_plane_wave_structure(int N, int lmax, int nsrc, const double* positions, double k, const int* wave_begin,
+                             const double* khat, const double* phase, const void* table, void* p_src, void* stream);
+
 /* ---- cluster coefficients ------------------------------------------------------ */
 
 /* Device workspace of mpa_cluster_coefficients (per-chunk partial sums). */

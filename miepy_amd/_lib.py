@@ -60,6 +60,8 @@ SIGNATURES = {
     "mpa_bicgstab": (_i, [_i, _i, _i, _v, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_bicgstab_ex_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "mpa_lu_workspace_bytes": (_sz, [_i]),
+    "mpa_mie_coefficients": (_i, [_i, _i, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v]),
+    "mpa_plane_wave_structure": (_i, [_i, _i, _i, _v, _d, _v, _v, _v, _v, _v, _v]),
     "mpa_cluster_coefficients_workspace_bytes": (_sz, [_i, _i, _i]),
     "mpa_cluster_coefficients": (_i, [_i, _i, _i, _i, _v, _d, _d, _d, _v, _d, _v, _v, _v]),
     "mpa_cluster_cross_sections": (_i, [_i, _i, _v, _v, _d, _v, _v]),

@@ -18,7 +18,6 @@ import numpy as np
 import torch
 
 from . import device as D
-from .mie import mie_coefficients
 
 
 class PreparedCluster:
@@ -27,17 +26,15 @@ class PreparedCluster:
     def __init__(self, w):
         self.w = w
         k = w.k
-        conducting = np.zeros(w.N, dtype=bool) if w.conducting is None else np.asarray(w.conducting)
-        ms, mi = mie_coefficients(w.radius, w.eps, np.ones(w.N), w.medium_eps, 1.0, k, w.lmax, conducting)
-        p_src = np.stack([s.structure(w.position, k, w.lmax) for s in w.sources])
-        self.S = p_src.shape[0]
+        conducting = None if w.conducting is None else np.asarray(w.conducting)
+        self.pos = D.to_device(w.position, torch.float64)
+        self.mie_scat, self.mie_int = D.mie_coefficients(w.radius, w.eps, np.ones(w.N), k, w.medium_eps, 1.0, w.lmax,
+                                                         conducting)
+        self.p_src = D.plane_wave_structure(list(w.sources), self.pos, k, w.lmax)
+        self.S = self.p_src.shape[0]
         self.n = w.n
         self.k = k
         self.eps_b = float(np.real(w.medium_eps))
-        self.pos = D.to_device(w.position, torch.float64)
-        self.mie_scat = D.to_device(ms, D.C128)
-        self.mie_int = D.to_device(mi, D.C128)
-        self.p_src = D.to_device(p_src, D.C128)
         self.A64 = None
         self.A32 = None
         self.op = None

@@ -254,6 +254,61 @@ def lu_solve(A, b, add_identity=False, overwrite=False):
     return x.reshape(b.shape)
 
 
+def mie_coefficients(radius, eps, mu, k, eps_b, mu_b, lmax, conducting=None):
+    """Mie coefficients of N spheres on the device (src/miepy/mie_single/mie_sphere.py:16-76):
+    returns (mie_scat, mie_int), each [N, 2, lmax] complex128 (device).
+    radius, eps, mu: [N]; k, eps_b, mu_b: scalars or [N] (one value per
+    particle); conducting: [N] bool (perfect-conductor limit) or None."""
+    lib = L.load()
+    dev = require_gpu()
+    radius = np.atleast_1d(np.asarray(radius, dtype=float))
+    N = radius.shape[0]
+    c = lambda a: np.broadcast_to(np.asarray(a, dtype=complex), (N,))
+    r = lambda a: np.broadcast_to(np.asarray(np.real(a), dtype=float), (N,))
+    reals = to_device(np.stack([radius, r(k)]), torch.float64)
+    cplx = to_device(np.stack([c(eps), c(mu), c(eps_b), c(mu_b)]), C128)
+    cond = None
+    if conducting is not None and np.any(conducting):
+        cond = torch.from_numpy(np.broadcast_to(np.asarray(conducting, dtype=np.int32), (N,)).copy()).to(dev)
+    ms = torch.empty((N, 2, lmax), dtype=C128, device=dev)
+    mi = torch.empty((N, 2, lmax), dtype=C128, device=dev)
+    L.check(lib.mpa_mie_coefficients(N, int(lmax), _ptr(reals[0]), _ptr(cplx[0]), _ptr(cplx[1]), _ptr(reals[1]),
+                                     _ptr(cplx[2]), _ptr(cplx[3]), _ptr(cond) if cond is not None else None,
+                                     _ptr(ms), _ptr(mi), _stream()), "mie_coefficients")
+    return ms, mi
+
+
+def plane_wave_structure(sources, positions, k, lmax):
+    """p_src [S, N, 2, rmax] complex128 (device) of the given sources (plane
+    waves or sums of them) at the particle positions
+    (src/miepy/sources/plane_waves.py:126-156).  The per-wave mode tables are
+    built on the host (rmax values each); the per-particle phases and
+    products run on the device."""
+    lib = L.load()
+    dev = require_gpu()
+    pos = positions if isinstance(positions, torch.Tensor) else to_device(
+        np.asarray(positions, dtype=float).reshape(-1, 3), torch.float64)
+    pos = pos.to(torch.float64).reshape(-1, 3).contiguous()
+    N = pos.shape[0]
+    rmax = lmax_to_rmax(lmax)
+    waves, begin = [], [0]
+    for src in sources:
+        waves += src.waves()
+        begin.append(len(waves))
+    W = len(waves)
+    table = np.stack([w.mode_table(lmax) for w in waves])
+    khat = np.stack([np.asarray(w.k_hat, dtype=float) for w in waves])
+    phase = np.array([float(np.real(w.phase)) for w in waves])
+    packed = to_device(np.concatenate([khat.reshape(-1), phase]), torch.float64)
+    tab = to_device(table, C128)
+    wb = torch.tensor(begin, dtype=torch.int32).to(dev)
+    out = torch.empty((len(sources), N, 2, rmax), dtype=C128, device=dev)
+    L.check(lib.mpa_plane_wave_structure(N, int(lmax), len(sources), _ptr(pos), float(np.real(k)), _ptr(wb),
+                                         _ptr(packed), _ptr(packed[3 * W:]), _ptr(tab), _ptr(out), _stream()),
+            "plane_wave_structure")
+    return out
+
+
 def cluster_coefficients(positions, p_scat, k, origin, lmax):
     """Scattered-field coefficients of the whole cluster about `origin`
     (src/miepy/vsh/cluster_coefficients.py:14-53) on the device.

@@ -44,7 +44,6 @@ import torch
 
 from . import _lib as L
 from . import device as D
-from .mie import mie_coefficients
 
 
 class _System:
@@ -198,10 +197,10 @@ class ServingEngine:
             self.rejected.append(request)
 
     def _host_inputs(self, systems):
-        """Mie coefficients of every particle of `systems` in one vectorized call
-        (per-particle wavenumber and medium), the plane-wave p_src of each, and
-        one upload of all of it (positions as FP64, the rest as c128); each
-        system gets views of the uploaded buffers."""
+        """Inputs of every system in `systems`: one upload of the per-particle
+        data (positions, radius, k, eps, medium), the Mie coefficients of all
+        their particles in one device call, and each system's plane-wave p_src
+        on the device; each system gets views of the device buffers."""
         lmax = max(s.w.lmax for s in systems)
         rad = np.concatenate([np.broadcast_to(np.asarray(s.w.radius, dtype=float), (s.w.N,)) for s in systems])
         eps = np.concatenate([np.broadcast_to(np.asarray(s.w.eps, dtype=complex), (s.w.N,)) for s in systems])
@@ -209,30 +208,18 @@ class ServingEngine:
         epsb = np.concatenate([np.full(s.w.N, complex(s.w.medium_eps)) for s in systems])
         cond = np.concatenate([np.zeros(s.w.N, dtype=bool) if s.w.conducting is None else
                                np.broadcast_to(np.asarray(s.w.conducting, dtype=bool), (s.w.N,)) for s in systems])
-        ms_all, mi_all = mie_coefficients(rad, eps, np.ones(rad.shape[0]), epsb, 1.0, kk, lmax, cond)
-        pos_parts, c_parts, o = [], [], 0
+        ms_all, mi_all = D.mie_coefficients(rad, eps, np.ones(rad.shape[0]), kk, epsb, 1.0, lmax, cond)
+        pos_d = D.to_device(np.concatenate([np.asarray(s.w.position, dtype=float).reshape(-1) for s in systems]),
+                            torch.float64)
+        o = 0
         for s in systems:
             w = s.w
-            ms = ms_all[o:o + w.N, :, :w.lmax]
-            mi = mi_all[o:o + w.N, :, :w.lmax]
-            o += w.N
-            p_src = np.stack([src.structure(w.position, w.k, w.lmax) for src in w.sources])
-            pos_parts.append(np.asarray(w.position, dtype=float).reshape(-1))
-            c_parts += [ms.reshape(-1), mi.reshape(-1), p_src.reshape(-1)]
-        pos_d = D.to_device(np.concatenate(pos_parts), torch.float64)
-        c_d = D.to_device(np.concatenate(c_parts), D.C128)
-        po, co = 0, 0
-        for s in systems:
-            w = s.w
-            N, L, S = w.N, w.lmax, len(w.sources)
-            s.pos = pos_d[po:po + 3 * N].view(N, 3)
-            po += 3 * N
-            s.mie_scat = c_d[co:co + 2 * N * L].view(N, 2, L)
-            co += 2 * N * L
-            s.mie_int = c_d[co:co + 2 * N * L].view(N, 2, L)
-            co += 2 * N * L
-            s.p_src = c_d[co:co + S * 2 * N * w.rmax].view(S, N, 2, w.rmax)
-            co += S * 2 * N * w.rmax
+            N = w.N
+            s.pos = pos_d[3 * o:3 * (o + N)].view(N, 3)
+            s.mie_scat = ms_all[o:o + N, :, :w.lmax].contiguous()
+            s.mie_int = mi_all[o:o + N, :, :w.lmax].contiguous()
+            s.p_src = D.plane_wave_structure(list(w.sources), s.pos, w.k, w.lmax)
+            o += N
 
     def _prepare(self, s):
         w = s.w

@@ -5,7 +5,8 @@ Mirrors src/miepy/sources/plane_waves.py (from_string :35-90, structure
 and the k-hat/TM/TE basis of src/miepy/coordinates.py:58-68.  The angular
 functions pi_n^m, tau_n^m (cpp/src/special.cpp:209-270, including the
 theta = 0 / pi limits) and |E_mn| (cpp/src/vsh_functions.cpp:56-58) are
-evaluated here in numpy.
+tabulated here per wave (rmax values); the per-particle coefficients are
+formed on the device (k_plane_wave).
 """
 
 import math
@@ -125,18 +126,25 @@ class plane_wave:
             raise ValueError(f"'{polarization}' is not a valid polarization. Use one of ['x', 'y', 'z', 'rhc', 'lhc']")
         return cls(polarization=pol, theta=theta, phi=phi, amplitude=amplitude, phase=phase, standing=standing)
 
-    def structure(self, position, k, lmax):
-        """p_src[N, 2, rmax] of this wave at the particle positions."""
-        position = np.asarray(position, dtype=float).reshape(-1, 3)
+    def waves(self):
+        return [self]
+
+    def mode_table(self, lmax):
+        """T[2, rmax]: amplitude |E_mn| e^{-i m phi} (tau pol0 - i pi pol1, pi pol0 - i tau pol1),
+        so that p_src[i] = e^{i (k r_i . khat + phase)} T (plane_waves.py:126-156)."""
         pi_vals, tau_vals = pi_tau(lmax, float(self.theta))
         E = emn_abs(lmax)
         m_vals = np.concatenate([np.arange(-n, n + 1) for n in range(1, lmax + 1)])
-        phases = k * (position @ self.k_hat) + self.phase
-        ef = self.amplitude * np.exp(1j * (phases[:, None] - m_vals * self.phi)) * E
-        p = np.empty((position.shape[0], 2, pi_vals.size), dtype=complex)
-        p[:, 0, :] = ef * (tau_vals * self.polarization[0] - 1j * pi_vals * self.polarization[1])
-        p[:, 1, :] = ef * (pi_vals * self.polarization[0] - 1j * tau_vals * self.polarization[1])
-        return p
+        ef = self.amplitude * np.exp(-1j * m_vals * self.phi) * E
+        T = np.empty((2, pi_vals.size), dtype=complex)
+        T[0] = ef * (tau_vals * self.polarization[0] - 1j * pi_vals * self.polarization[1])
+        T[1] = ef * (pi_vals * self.polarization[0] - 1j * tau_vals * self.polarization[1])
+        return T
+
+    def structure(self, position, k, lmax):
+        """p_src[N, 2, rmax] of this wave at the particle positions (computed on the device)."""
+        from . import device as D
+        return D.plane_wave_structure([self], position, k, lmax)[0].cpu().numpy()
 
     def __add__(self, other):
         return combined_source(self, other)
@@ -150,8 +158,12 @@ class combined_source:
         for s in sources:
             self.sources += s.sources if isinstance(s, combined_source) else [s]
 
+    def waves(self):
+        return list(self.sources)
+
     def structure(self, position, k, lmax):
-        return sum(s.structure(position, k, lmax) for s in self.sources)
+        from . import device as D
+        return D.plane_wave_structure([self], position, k, lmax)[0].cpu().numpy()
 
     def __add__(self, other):
         return combined_source(self, other)

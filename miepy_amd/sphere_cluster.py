@@ -28,7 +28,6 @@ from . import device as D
 from .backend import solver
 from .flux import cross_sections as _cs
 from .materials import constant_material, material_struct
-from .mie import mie_coefficients
 
 
 def _atleast(array, dim, length, dtype=None):
@@ -96,8 +95,9 @@ class sphere_cluster:
         self.material_data = material_struct(self.material, self.medium, wavelength=self.wavelength)
         md = self.material_data
         conducting = np.array([getattr(m, "name", None) == "metal" for m in self.material])
-        self.mie_scat, self.mie_int = mie_coefficients(self.radius, md.eps, md.mu, md.eps_b, md.mu_b,
-                                                       float(np.real(md.k_b)), lmax, conducting)
+        self._mie_d = D.mie_coefficients(self.radius, md.eps, md.mu, float(np.real(md.k_b)), md.eps_b, md.mu_b,
+                                         lmax, conducting)
+        self.mie_scat, self.mie_int = (a.cpu().numpy() for a in self._mie_d)
         self._multi = isinstance(source, (list, tuple))
         self.p_cluster = None
         self.report = None
@@ -114,11 +114,10 @@ class sphere_cluster:
         N, rmax, lmax = self.Nparticles, self.rmax, self.lmax
         srcs = self._sources()
         S = len(srcs)
-        p_src = np.stack([s.structure(self.position, k, lmax) for s in srcs])  # [S, N, 2, rmax]
-        dev = D.require_gpu()
-        p_src_d = D.to_device(p_src, D.C128)
-        mie_scat_d = D.to_device(self.mie_scat, D.C128)
-        mie_int_d = D.to_device(self.mie_int, D.C128)
+        D.require_gpu()
+        p_src_d = D.plane_wave_structure(srcs, self.position, k, lmax)  # [S, N, 2, rmax]
+        p_src = p_src_d.cpu().numpy()
+        mie_scat_d, mie_int_d = self._mie_d
         n = 2 * rmax * N
         if self.interactions and N > 1:
             exact = self.method == solver.exact or getattr(self.method, "name", None) == "exact"

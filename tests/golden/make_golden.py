@@ -11,6 +11,12 @@ Fixtures (all small .npz files, inputs + expected outputs):
                       cpp/src/special.cpp:276-296 with sympy's exact Wigner 3j
                       (sympy.physics.wigner), plus the known 3j cases of
                       tests/test_jax_special.py:289-312.
+- mie_mpmath.npz      Mie coefficients a_n, b_n, c_n, d_n of 36 spheres
+                      (size parameters 0.01-40; dielectric, absorbing,
+                      plasmonic, metallic, magnetic spheres; three media) by
+                      the formulas of src/miepy/mie_single/mie_sphere.py:16-76
+                      evaluated in 30-digit arithmetic (mpmath Bessel
+                      functions; psi' from psi_{n-1} - (n/z) psi_n).
 - fdtd_au_dimer.npz   the reference's own FDTD data for the Au dimer
                       (tests/test_fdtd_comparison.py): wavelengths, inline eps,
                       FDTD cross-sections and forces.  The arrays are read from
@@ -84,6 +90,49 @@ def make_gaunt(lmax=5):
     np.savez(os.path.join(HERE, "gaunt_sympy.npz"), rows=rows, w3j_cases=np.array(cases), w3j_values=w3)
 
 
+def make_mie(lmax=8):
+    import mpmath as mp
+    mp.mp.dps = 30
+    rng = np.random.default_rng(11)
+    eps_set = [2.56, 12.0 + 0.3j, 4.0 + 0.05j, -20 + 1.2j, -1.2 + 0.2j, -50 + 5j, 0.1 + 3j, 16.0]
+    media = [(1.0, 1.0), (1.33 ** 2, 1.0), (2.25, 1.2)]
+    N = 36
+    x = np.exp(np.linspace(np.log(0.01), np.log(40.0), N))
+    rng.shuffle(x)
+    eps = np.array([eps_set[i % len(eps_set)] for i in range(N)], dtype=complex)
+    mu = np.where(np.arange(N) % 5 == 0, 1.5 + 0.1j, 1.0 + 0j)
+    eps_b = np.array([media[i % 3][0] for i in range(N)], dtype=complex)
+    mu_b = np.array([media[i % 3][1] for i in range(N)], dtype=complex)
+    k = 2 * np.pi * np.sqrt(np.real(eps_b * mu_b)) / 600e-9
+    radius = x / k
+
+    def psi(n, t):  # t j_n(t)
+        return t * mp.sqrt(mp.pi / (2 * t)) * mp.besselj(n + mp.mpf(1) / 2, t)
+
+    def chi(n, t):  # t y_n(t)
+        return t * mp.sqrt(mp.pi / (2 * t)) * mp.bessely(n + mp.mpf(1) / 2, t)
+
+    a = np.zeros((N, 2, lmax), dtype=complex)
+    c = np.zeros((N, 2, lmax), dtype=complex)
+    for i in range(N):
+        xx = mp.mpf(float(k[i] * radius[i]))
+        m = mp.sqrt(mp.mpc(eps[i]) / mp.mpc(eps_b[i]))
+        mt = m * mp.mpc(mu_b[i]) / mp.mpc(mu[i])
+        z = m * xx
+        for n in range(1, lmax + 1):
+            pz, pz1 = psi(n, z), psi(n - 1, z)
+            px, px1 = psi(n, xx), psi(n - 1, xx)
+            xi, xi1 = px + 1j * chi(n, xx), px1 + 1j * chi(n - 1, xx)
+            dpz, dpx, dxi = pz1 - n / z * pz, px1 - n / xx * px, xi1 - n / xx * xi
+            a[i, 0, n - 1] = complex((mt * pz * dpx - px * dpz) / (mt * pz * dxi - xi * dpz))
+            a[i, 1, n - 1] = complex((pz * dpx - mt * px * dpz) / (pz * dxi - mt * xi * dpz))
+            w = m * (px * dxi - xi * dpx)
+            c[i, 0, n - 1] = complex(w / (pz * dxi - mt * xi * dpz))
+            c[i, 1, n - 1] = complex(w / (mt * pz * dxi - xi * dpz))
+    np.savez(os.path.join(HERE, "mie_mpmath.npz"), radius=radius, eps=eps, mu=mu, eps_b=eps_b, mu_b=mu_b, k=k,
+             lmax=lmax, mie_scat=a, mie_int=c)
+
+
 def _np_array_literals(path):
     """Return {name: list} for module-level `name = np.array([...])` assignments."""
     tree = ast.parse(open(path).read())
@@ -114,13 +163,15 @@ def make_fdtd(reference):
 if __name__ == "__main__":
     ap = argparse.ArgumentParser()
     ap.add_argument("--reference", default="/root/reference")
-    ap.add_argument("--only", default="special,gaunt,fdtd")
+    ap.add_argument("--only", default="special,gaunt,mie,fdtd")
     args = ap.parse_args()
     todo = args.only.split(",")
     if "special" in todo:
         make_special()
     if "gaunt" in todo:
         make_gaunt()
+    if "mie" in todo:
+        make_mie()
     if "fdtd" in todo:
         make_fdtd(args.reference)
     print("ok")

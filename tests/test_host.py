@@ -1,6 +1,7 @@
 """Host-side pieces of the package vs the oracle's restatement of the
-reference's Python host (Mie coefficients, plane-wave p_src, workloads,
-backend registry).  CPU only."""
+reference's Python host (plane-wave mode tables, workloads, backend
+registry, serving arena).  CPU only; the Mie coefficients and per-particle
+p_src run on the device (tests/test_gpu_inputs.py)."""
 
 import numpy as np
 import pytest
@@ -14,55 +15,14 @@ def H(oracle):
     return H
 
 
-def test_mie_coefficients_match_reference_loop(H):
-    from miepy_amd.mie import mie_coefficients
-    rng = np.random.default_rng(0)
-    N = 40
-    radius = rng.uniform(50, 150, N) * nm
-    eps = rng.uniform(2, 6, N) + 1j * rng.uniform(0, 0.1, N)
-    eps[::3] = -20 + 1.2j
-    k = 2 * np.pi * 1.33 / (640 * nm)
-    a, c = mie_coefficients(radius, eps, np.ones(N), 1.33 ** 2, 1.0, k, 5)
-    ra, rc = H.mie_coefficients(radius, eps, np.ones(N), 1.33 ** 2, 1.0, k, 5)
-    np.testing.assert_allclose(a, ra, rtol=1e-12)
-    np.testing.assert_allclose(c, rc, rtol=1e-12)
-
-
-def test_mie_conducting_limit(H):
-    from miepy_amd.mie import mie_coefficients
-    k = 2 * np.pi / (600 * nm)
-    a, c = mie_coefficients([80 * nm], [1 + 1e40j], [1.0], 1.0, 1.0, k, 3, conducting=[True])
-    ra, rc = H.mie_coefficients([80 * nm], [1 + 1e40j], [1.0], 1.0, 1.0, k, 3, conducting=[True])
-    np.testing.assert_allclose(a, ra, rtol=1e-13)
-    assert np.all(c == 0)
-
-
-def test_mie_rayleigh_limit():
-    """a_1 -> -(2i/3) x^3 (m^2-1)/(m^2+2) for x -> 0 (Bohren & Huffman 5.1)."""
-    from miepy_amd.mie import mie_coefficients
-    eps = 3.0 + 0.5j
-    x = 1e-3
-    a, _ = mie_coefficients([x], [eps], [1.0], 1.0, 1.0, 1.0, 1)
-    expect = -2j / 3 * x ** 3 * (eps - 1) / (eps + 2)
-    assert abs(a[0, 0, 0] - expect) < 1e-5 * abs(expect)
-
-
-def test_mie_lossless_energy_conservation():
-    from miepy_amd.mie import mie_coefficients
-    a, _ = mie_coefficients([120 * nm], [2.56], [1.0], 1.0, 1.0, 2 * np.pi / (500 * nm), 6)
-    np.testing.assert_allclose(a.real, np.abs(a) ** 2, atol=1e-15)
-
-
-@pytest.mark.parametrize("pol,theta,phi", [((1, 0), 0.0, 0.0), ((0, 1), np.pi, 0.0), ((1, 1j), 0.7, 2.1),
-                                           ((0.3, -0.8j), 2.9, 5.5), ((1, 0), np.pi / 2, np.pi / 2)])
-def test_plane_wave_structure(H, pol, theta, phi):
+def test_plane_wave_mode_table_at_origin(H):
+    """The host-side per-wave table equals the oracle's p_src of a particle
+    at the origin (phase 0): the device applies only e^{i k r . khat}."""
     import miepy_amd as M
-    rng = np.random.default_rng(1)
-    pos = rng.normal(size=(9, 3)) * 300 * nm
-    k = 2 * np.pi / (700 * nm)
-    got = M.plane_wave(pol, theta=theta, phi=phi).structure(pos, k, 4)
-    ref = H.plane_wave_structure(pos, k, 4, pol, theta, phi)
-    np.testing.assert_allclose(got, ref, rtol=1e-12, atol=1e-14)
+    for pol, theta, phi in [((1, 0), 0.0, 0.0), ((1, 1j), 0.7, 2.1), ((0.3, -0.8j), np.pi, 5.5)]:
+        w = M.plane_wave(pol, theta=theta, phi=phi, amplitude=1.3)
+        ref = H.plane_wave_structure(np.zeros((1, 3)), 1e7, 4, pol, theta, phi, amplitude=1.3)[0]
+        np.testing.assert_allclose(w.mode_table(4), ref, rtol=1e-13, atol=1e-15)
 
 
 def test_plane_wave_from_string():
