@@ -288,9 +288,11 @@ constexpr int kFacMaxLmax = 3;
 // c64: 0 one accumulator chain per tile, 1 two (K steps alternate), 2 and 3
 // one chain with the next column's special functions interleaved with the
 // MFMAs (8 or 12 VALU instructions per MFMA; FP16-split kernel; the FP32
-// kernel treats 2, 3 as 0); c128: 0 two chains, 1 one.
+// kernel treats 2, 3 as 0); c128: 0 and 1 the slot-mapped kernel (k_matfree_mf64s) with one or two
+// accumulator chains (one is the faster at rand1000: 0.34 ms per product against 0.43 ms for the
+// best per-lane-select variant), 2 and 3 the per-lane-select kernel (k_matfree_mf64), two or one.
 int g_mf_variant[2] = {0, 0};
-constexpr int kMfVariants[2] = {4, 2};
+constexpr int kMfVariants[2] = {4, 4};
 constexpr int kFacMinN = 128;  // smaller clusters keep the LDS kernel (too few lanes)
 
 constexpr int fac_n_of(int r) {
@@ -1331,6 +1333,196 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64(MfJob J, const SysDev
         }
 }
 
+// Slot-mapped FP64 variant.  The four K positions of MFMA step t hold one
+// (l, |m|) combination for all four 16-lane groups g: for m > 0, g = 0 and
+// 1 the real and imaginary parts of h_l P_l^m e^{+i m phi} (HLE index
+// l(l+1)+m), g = 2 and 3 those of h_l P_l^m e^{-i m phi} (index l(l+1)-m);
+// the m = 0 orders go in pairs, g = 0, 1 order l and g = 2, 3 order l+1.
+// Every lane therefore evaluates the same combinations -- the A value is
+// P (h.x a + h.y b) with per-m lane constants (a, b) -- instead of all of
+// its pair's values with a lane select per value; the B value is the real or
+// imaginary part of W[h][c] picked by the LDS address, with a lane-constant
+// sign (the -Im W of the product's real part).
+template <int L>
+struct Mf64Slots {
+    static constexpr int PM = 2 * L;
+    static constexpr int NM0 = (PM + 2) / 2;            // m = 0 pair steps
+    static constexpr int KT = NM0 + PM * (PM + 1) / 2;  // = (2 NH + 3) / 4
+    int ha[KT], hb[KT];                                 // HLE index of lane groups 0-1 and 2-3
+    constexpr Mf64Slots() : ha(), hb() {
+        int t = 0;
+        for (int l = 0; l <= PM; l += 2) {
+            ha[t] = l * (l + 1);
+            hb[t] = l + 1 <= PM ? (l + 1) * (l + 2) : l * (l + 1);
+            t++;
+        }
+        for (int m = 1; m <= PM; m++)
+            for (int l = m; l <= PM; l++) {
+                ha[t] = l * (l + 1) + m;
+                hb[t] = l * (l + 1) - m;
+                t++;
+            }
+    }
+};
+
+template <int L, int KS>
+__global__ __launch_bounds__(kThreads) void k_matfree_mf64s(MfJob J, const SysDev* __restrict__ sys, int nchunks,
+                                                            int chunk, const double2* __restrict__ Wm) {
+    constexpr int R = L * (L + 2);
+    constexpr int NC = 2 * R;
+    constexpr int NT = (NC + 7) / 8;
+    constexpr int PM = 2 * L;
+    constexpr int NH = (PM + 1) * (PM + 1);
+    using SM = Mf64Slots<L>;
+    constexpr int KT = SM::KT, NM0 = SM::NM0;
+    static_assert(KT == (2 * NH + 3) / 4, "one K step per combination");
+    constexpr SM slots{};
+    __shared__ double2 Ws[2][NH][NT * 8 + 1];
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int N = J.N;
+    const int s = blockIdx.y;
+    const int groups = (N + 63) / 64;
+    const int v = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int c = v / groups;
+    const int group = v - c * groups;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, col = lane & 15, part = col & 1;
+    const int odd = g & 1, hi = g >> 1;
+    const int comp = odd ^ part;  // B: Re W (0) or Im W (1)
+    const unsigned long long bsign = (odd && !part) ? 0x8000000000000000ull : 0ull;
+    const double sgn = hi ? 1.0 : -1.0;  // sign of Im e seen by this lane group
+    const int i = group * 64 + w * 16 + col;
+    const bool live = i < N;
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const double2* Wall = Wm + (size_t)s * N * NH * NC;
+    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
+    d4v acc[NT][KS];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int k = 0; k < KS; k++)
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[t][k][q] = 0.0;
+    // W_j staging, software-pipelined: the next column's block is loaded into
+    // registers at the top of an iteration and written to the other LDS
+    // buffer after the MFMAs, so the load latency hides behind this column's work
+    constexpr int PF = (NH * NT * 8 + kThreads - 1) / kThreads;  // elements per thread
+    double2 pf[PF];
+    auto fetch = [&](int j) {
+        const double2* src = Wall + (size_t)j * NH * NC;
+#pragma unroll
+        for (int e = 0; e < PF; e++) {
+            const int u = e * kThreads + tid;
+            const int h = u / (NT * 8), cc = u - h * (NT * 8);
+            pf[e] = (u < NH * NT * 8 && cc < NC) ? src[h * NC + cc] : make_double2(0.0, 0.0);
+        }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < PF; e++) {
+            const int u = e * kThreads + tid;
+            if (u < NH * NT * 8) Ws[buf][u / (NT * 8)][u % (NT * 8)] = pf[e];
+        }
+    };
+    if (jbeg < jend) {
+        fetch(jbeg);
+        put(0);
+    }
+    __syncthreads();
+    for (int j = jbeg; j < jend; j++) {
+        const int buf = (j - jbeg) & 1;
+        const bool more = j + 1 < jend;
+        if (more) fetch(j + 1);
+        double af[KT];
+        {
+            const bool on = live && j != ii;
+            double dx = xi - J.pos[3 * j], dy = yi - J.pos[3 * j + 1], dz = zi - J.pos[3 * j + 2];
+            dx = on ? dx : 1.0;  // a harmless geometry for the zero entries
+            dy = on ? dy : 0.0;
+            dz = on ? dz : 0.0;
+            const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+            const double ct = dz / rad, st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
+            const double zz = J.k * rad;
+            const double rz = on ? 1.0 / zz : 0.0;  // zero Hankel values off the pairs
+            double sz, cz;
+            sincos(zz, &sz, &cz);
+            const double rho = sqrt(dx * dx + dy * dy);
+            const double2 e1 = rho > 0 ? make_double2(dx / rho, dy / rho) : make_double2(1.0, 0.0);
+            double2 hh[PM + 1];
+            hh[0] = make_double2(sz * rz, -cz * rz);
+            hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
+#pragma unroll
+            for (int l = 2; l <= PM; l++)
+                hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
+                                     (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
+            // m = 0: P_l^0 by the recurrence in l; steps of two orders
+            double P0[PM + 1];
+            P0[0] = 1.0;
+            P0[1] = ct;
+#pragma unroll
+            for (int l = 2; l <= PM; l++) P0[l] = ((2 * l - 1) * ct * P0[l - 1] - (l - 1) * P0[l - 2]) * (1.0 / l);
+            // (a0, b0) = (1, 0) or (0, 1) picks Re or Im exactly; fh = 0 or 1
+            // picks the step's first or second order exactly
+            const double a0 = odd ? 0.0 : 1.0, b0 = odd ? 1.0 : 0.0, fh = hi ? 1.0 : 0.0;
+#pragma unroll
+            for (int q = 0; q < NM0; q++) {
+                const int l1 = 2 * q, l2 = 2 * q + 1;
+                const double v1 = P0[l1] * fma(hh[l1].x, a0, hh[l1].y * b0);
+                const double v2 = l2 <= PM ? P0[l2] * fma(hh[l2].x, a0, hh[l2].y * b0) : 0.0;
+                af[q] = fma(v2, fh, v1 * (1.0 - fh));
+            }
+            // m > 0: one step per (l, m), P_l^m by the recurrence in l
+            double2 e = make_double2(1.0, 0.0);
+            double pmm = 1.0;
+            int t = NM0;
+#pragma unroll
+            for (int m = 1; m <= PM; m++) {
+                e = cmul(e, e1);
+                pmm *= (2 * m - 1) * st;
+                const double sy = sgn * e.y;
+                const double a = odd ? -sy : e.x, b = odd ? e.x : sy;
+                double p2 = 0.0, p1 = 0.0;
+#pragma unroll
+                for (int l = m; l <= PM; l++) {
+                    const double P = (l == m)       ? pmm
+                                     : (l == m + 1) ? ct * (2 * m + 1) * pmm
+                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.0 / (l - m));
+                    p2 = p1;
+                    p1 = P;
+                    af[t++] = P * fma(hh[l].x, a, hh[l].y * b);
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < KT; t++) {
+            const int h = hi ? slots.hb[t] : slots.ha[t];
+#pragma unroll
+            for (int tile = 0; tile < NT; tile++) {
+                const double wv = ((const double*)&Ws[buf][h][tile * 8 + (col >> 1)])[comp];
+                const double bv = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, wv) ^ bsign);
+                acc[tile][t % KS] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], bv, acc[tile][t % KS], 0, 0, 0);
+            }
+        }
+        if (more) put(buf ^ 1);
+        __syncthreads();
+    }
+    const size_t n = (size_t)NC * N;
+#pragma unroll
+    for (int tile = 0; tile < NT; tile++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int ip = group * 64 + w * 16 + 4 * q + g;
+            const int cc = tile * 8 + (col >> 1);
+            if (ip < N && cc < NC) {
+                double v = acc[tile][0][q];
+#pragma unroll
+                for (int k = 1; k < KS; k++) v += acc[tile][k][q];
+                ((double*)J.part)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = v;
+            }
+        }
+}
+
 // Column chunks of the FP64 matrix-core kernel: about 1024 workgroups in total.
 inline int mf64_chunks(int N) {
     const int groups = (N + 63) / 64;
@@ -1551,10 +1743,12 @@ int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
                            dim3((unsigned)std::min<size_t>(4096, (wtot + kThreads - 1) / kThreads)), dim3(kThreads),
                            0, st, J, sys, nh, T->hmap, T->wmeta, T->wsrc, T->wcoef, Wm);
         const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
-        const bool ks2 = g_mf_variant[1] == 0;
+        const int var = g_mf_variant[1];
 #define MPA_MF64(L_)                                                                                          \
     do {                                                                                                      \
-        if (ks2) hipLaunchKernelGGL((k_matfree_mf64<L_, 2>), grid, block, 0, st, J, sys, nch, chunk, Wm);     \
+        if (var == 0) hipLaunchKernelGGL((k_matfree_mf64s<L_, 1>), grid, block, 0, st, J, sys, nch, chunk, Wm); \
+        else if (var == 1) hipLaunchKernelGGL((k_matfree_mf64s<L_, 2>), grid, block, 0, st, J, sys, nch, chunk, Wm); \
+        else if (var == 2) hipLaunchKernelGGL((k_matfree_mf64<L_, 2>), grid, block, 0, st, J, sys, nch, chunk, Wm); \
         else hipLaunchKernelGGL((k_matfree_mf64<L_, 1>), grid, block, 0, st, J, sys, nch, chunk, Wm);         \
     } while (0)
         if (J.lmax == 1) MPA_MF64(1);

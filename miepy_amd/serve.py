@@ -196,7 +196,7 @@ class ServingEngine:
             request.finished = self.clock()
             self.rejected.append(request)
 
-    def _host_inputs(self, systems):
+    def _inputs(self, systems):
         """Inputs of every system in `systems`: one upload of the per-particle
         data (positions, radius, k, eps, medium), the Mie coefficients of all
         their particles in one device call, and each system's plane-wave p_src
@@ -291,7 +291,7 @@ class ServingEngine:
             s.block = off
             new.append(s)
         if new:
-            self._host_inputs(new)
+            self._inputs(new)
             for s in new:
                 self._prepare(s)
                 if s.req.admitted is None:

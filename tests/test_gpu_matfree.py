@@ -271,14 +271,16 @@ def test_c64_operator_extreme_separations(dev, c64_kernel, kd_near, kd_far, lmax
     assert _c64_vs_c128(dev, pos, mie, k, 1, 5, False) <= 5e-6
 
 
-def test_matfree_every_variant(dev):
+@pytest.mark.parametrize("lmax", [1, 2, 3, 4])
+def test_matfree_every_variant(dev, lmax):
     """Each matrix-core kernel variant of the matrix-free product (accumulator
-    chains, interleaved special functions) against the variant-0 product:
-    c64 to 2e-6 of the largest entry, c128 to 1e-13."""
+    chains, interleaved special functions; FP64: the slot-mapped and the
+    per-lane-select kernels) against the variant-0 product: c64 to 2e-6 of
+    the largest entry, c128 to 1e-13."""
     import ctypes
     import miepy_amd._lib as L
     lib = L.load()
-    pos, mie = _cluster(170, 3, seed=23, spacing=300 * nm)
+    pos, mie = _cluster(170, lmax, seed=23, spacing=300 * nm)
     k = 2 * np.pi / (600 * nm)
     op = dev.ClusterOperator(pos, mie, k, max_rhs=2)
     rng = np.random.default_rng(8)
