@@ -203,6 +203,17 @@ int mpa_cluster_coefficients(int N, int lmax_in, int lmax_out, int nrhs, const d
                              double oy, double oz, const void* p_scat, double k, void* p_cluster, void* workspace,
                              void* stream);
 
+/* Translation coefficients (A, B) of `count` tuples (m, n, u, v) (mnuv
+ * [count][4] int, device; 1 <= n, v <= lmax, |m| <= n, |u| <= v) for nvec
+ * translation vectors in spherical coordinates rad, theta, phi [nvec] f64
+ * (device, rad > 0), wavenumber k and VSH mode 0 outgoing, 1 ingoing,
+ * 2 incident, 3 interior: the scalar vsh_translation and its numpy form
+ * (cpp/src/vsh_translation.cpp:366-470, bound in
+ * cpp/src/vsh_translation_pyb.cpp:14-31).  out [nvec][count][2] c128
+ * (device). */
+int mpa_vsh_translation(int count, const int* mnuv, int lmax, int nvec, const double* rad, const double* theta,
+                        const double* phi, double k, int mode, void* out, void* stream);
+
 /* Cluster cross-sections per multipole (cpp/src/flux.cpp:32-57,
  * src/miepy/flux.py:35-56): p_cluster and p_src (the source's coefficients at
  * the origin) [nrhs, 2, rmax] c128 -> C [nrhs, 3 (scattering, absorption,

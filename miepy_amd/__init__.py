@@ -10,7 +10,7 @@ lib/libmiepy_amd.so behind the C ABI of include/miepy_amd.h.
 
 from . import backends, materials as _materials_mod
 from . import backend as _backend
-from .backend import solver, DeviceTMatrix
+from .backend import solver, DeviceTMatrix, cluster_coefficients, vsh_mode, vsh_translation
 from .flux import cross_sections
 from .materials import (constant_material, data_material, dielectric, function_material, material_struct,
                         materials)
@@ -40,4 +40,5 @@ def mode_indices(lmax):
 
 __all__ = ["sphere_cluster", "cluster", "sphere", "tmatrix_particle", "plane_wave", "combined_source", "constant_material", "data_material", "dielectric",
            "function_material", "material_struct", "materials", "cross_sections", "solver", "DeviceTMatrix",
-           "backends", "lmax_to_rmax", "rmax_to_lmax", "mode_indices"]
+           "backends", "lmax_to_rmax", "rmax_to_lmax", "mode_indices", "cluster_coefficients", "vsh_mode",
+           "vsh_translation"]

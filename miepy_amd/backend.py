@@ -14,6 +14,7 @@ takes in src/miepy/interactions.py:81-83,141-143 and src/miepy/flux.py:21-26):
   force_all / torque_all(p_scat, p_inc, k, eps_b, mu_b)  forces.py:44-81
   cluster_coefficients(positions, p_scat, k, origin, lmax)  vsh/cluster_coefficients.py:14-53
   cluster_cross_sections(p_cluster, p_src, k)       flux.py:35-56
+  vsh_translation, vsh_translation_numpy, vsh_mode  cpp/src/vsh_translation_pyb.cpp:14-31
 
 Like the JAX backend, the assembled matrix stays in HBM: it is returned as a
 `DeviceTMatrix`, which has the reference's [N,2,rmax,N,2,rmax] shape,
@@ -204,6 +205,34 @@ def bicgstab_preconditioned_profiled(A, b, M_inv_blocks, block_size, maxiter=100
 
 
 solve_linear_system.last_report = None
+
+
+class vsh_mode(enum.Enum):
+    """Mirror of the pybind enum miepy.vsh_mode (cpp/src/vsh_functions.hpp:10-15)."""
+
+    outgoing = 0
+    ingoing = 1
+    incident = 2
+    interior = 3
+
+
+def _mode_value(mode):
+    return mode.value if isinstance(mode, enum.Enum) else int(getattr(mode, "value", mode))
+
+
+def vsh_translation(m, n, u, v, rad, theta, phi, k, mode):
+    """miepy.cpp.vsh_translation.vsh_translation: the translation coefficients
+    (A, B) of one (m, n, u, v) for one vector (cpp/src/vsh_translation.cpp:428-470),
+    evaluated on the device."""
+    out = D.vsh_translation([[m, n, u, v]], rad, theta, phi, float(np.real(k)), _mode_value(mode))
+    return complex(out[0, 0, 0]), complex(out[0, 0, 1])
+
+
+def vsh_translation_numpy(m, n, u, v, rad, theta, phi, k, mode):
+    """miepy.cpp.vsh_translation.vsh_translation_numpy: one (m, n, u, v) over
+    arrays of vectors -> [2, len] (row 0 A, row 1 B) (vsh_translation.cpp:366-426)."""
+    out = D.vsh_translation([[m, n, u, v]], rad, theta, phi, float(np.real(k)), _mode_value(mode))
+    return np.ascontiguousarray(out[:, 0, :].T)
 
 
 def cluster_coefficients(positions, p_scat, k, origin, lmax=None):

@@ -50,7 +50,7 @@ __global__ __launch_bounds__(kCT) void k_cluster_coef(int N, int L, int lmax_in,
             const double dx = ox - pos[3 * i], dy = oy - pos[3 * i + 1], dz = oz - pos[3 * i + 2];
             const bool at_origin = dx == 0.0 && dy == 0.0 && dz == 0.0;
             __syncthreads();  // the previous particle's HL is no longer read
-            if (!at_origin && (int)threadIdx.x <= pmax) pair_hl(HL, eph, threadIdx.x, L, dx, dy, dz, k, true);
+            if (!at_origin && (int)threadIdx.x <= pmax) pair_hl(HL, eph, threadIdx.x, L, dx, dy, dz, k, 2);
             __syncthreads();
             if (!mine) continue;
             const double2* ps = p_scat + ((size_t)s * N + i) * 2 * rmax_in;
@@ -74,6 +74,35 @@ __global__ __launch_bounds__(kCT) void k_cluster_coef(int N, int L, int lmax_in,
             out[r] = acc0;
             out[rmax_out + r] = acc1;
         }
+    }
+}
+
+// (A, B) = the translation coefficients of tuples (m, n, u, v) for the
+// translation vectors (rad, theta, phi)[g] and one VSH mode: the scalar
+// vsh_translation of cpp/src/vsh_translation.cpp:428-470 (and its numpy
+// form over arrays of vectors, :366-426; the row order m enters negated
+// there, as in the table).  One workgroup per vector: threads 0..pmax form
+// the products of the radial function and P_p^mu in LDS, then one thread per
+// tuple sums its table terms.  mnuv [count][4]; out [g][count][2].
+__global__ __launch_bounds__(kCT) void k_vsh_translation(int count, const int* __restrict__ mnuv, int L,
+                                                         const double* __restrict__ rad,
+                                                         const double* __restrict__ theta,
+                                                         const double* __restrict__ phi, double k, int zkind,
+                                                         TabDev tab, double2* __restrict__ out) {
+    __shared__ double2 HL[(2 * MPA_LMAX_MAX + 2) * (2 * MPA_LMAX_MAX + 3) / 2];
+    __shared__ double2 eph[2 * MPA_LMAX_MAX + 1];
+    const int g = blockIdx.x;
+    const int rmaxL = lmax_to_rmax(L);
+    const double st = sin(theta[g]);
+    const double dx = rad[g] * st * cos(phi[g]), dy = rad[g] * st * sin(phi[g]), dz = rad[g] * cos(theta[g]);
+    if ((int)threadIdx.x <= 2 * L + 1) pair_hl(HL, eph, threadIdx.x, L, dx, dy, dz, k, zkind);
+    __syncthreads();
+    for (int q = threadIdx.x; q < count; q += kCT) {
+        const int m = mnuv[4 * q], n = mnuv[4 * q + 1], u = mnuv[4 * q + 2], v = mnuv[4 * q + 3];
+        double2 xa, xb;
+        tuple_xab(tab, mode_r(n, m) * rmaxL + mode_r(v, u), HL, eph, xa, xb);
+        out[2 * ((size_t)g * count + q)] = xa;
+        out[2 * ((size_t)g * count + q) + 1] = xb;
     }
 }
 
@@ -136,6 +165,23 @@ int mpa_cluster_coefficients(int N, int lmax_in, int lmax_out, int nrhs, const d
     const int count = nrhs * 2 * lmax_to_rmax(lmax_out);
     hipLaunchKernelGGL(k_cluster_sum, dim3((count + kCT - 1) / kCT), dim3(kCT), 0, st, (const double2*)workspace,
                        chunks, count, (double2*)p_cluster);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+int mpa_vsh_translation(int count, const int* mnuv, int lmax, int nvec, const double* rad, const double* theta,
+                        const double* phi, double k, int mode, void* out, void* stream) {
+    if (count < 1 || nvec < 1 || nvec > 65535 || !mnuv || !rad || !theta || !phi || !out || lmax < 1 ||
+        lmax > MPA_LMAX_MAX || !(k > 0) || mode < 0 || mode > 3) {
+        mpa_set_error("vsh_translation: bad argument (1 <= lmax <= %d, mode 0-3)", MPA_LMAX_MAX);
+        return MPA_ERR_ARG;
+    }
+    const TransTable* T = mpa_get_table(lmax);
+    if (!T) return MPA_ERR_HIP;
+    const TabDev tab{T->cmeta, T->ccoef, T->clidx};
+    const int zkind = mode == 0 ? 0 : (mode == 1 ? 1 : 2);  // outgoing, ingoing, incident / interior
+    hipLaunchKernelGGL(k_vsh_translation, dim3(nvec), dim3(kCT), 0, (hipStream_t)stream, count, mnuv, lmax, rad,
+                       theta, phi, k, zkind, tab, (double2*)out);
     MPA_LAUNCH_CHECK();
     return MPA_OK;
 }

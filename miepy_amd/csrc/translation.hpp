@@ -26,10 +26,12 @@ __host__ __device__ inline int hl_count(int lmax) {
 // Hankel h_l = j_l + i y_l by upward recurrence from j_0 = sin z / z,
 // y_0 = -cos z / z (cpp/src/special.cpp:90-112); P_l^mu without the
 // Condon-Shortley phase by the standard three-term recurrence in l
-// (cpp/src/special.cpp:180-204, GSL_SF_LEGENDRE_NONE).  regular: j_l alone
-// (the incident mode, spherical_jn's upward recurrence, special.cpp:20-44).
+// (cpp/src/special.cpp:180-204, GSL_SF_LEGENDRE_NONE).  zkind selects the
+// radial function of the VSH mode (cpp/src/vsh_functions.cpp:42-54): 0 h_l^(1)
+// = j_l + i y_l (outgoing), 1 h_l^(2) = j_l - i y_l (ingoing), 2 j_l alone
+// (incident and interior; spherical_jn's upward recurrence, special.cpp:20-44).
 __device__ inline void pair_hl(double2* HL, double2* eph, int mu, int lmax, double dx, double dy, double dz,
-                               double k, bool regular = false) {
+                               double k, int zkind = 0) {
     const int pmax = 2 * lmax + 1;
     const double rad = sqrt(dx * dx + dy * dy + dz * dz);
     const double x = dz / rad;
@@ -68,7 +70,8 @@ __device__ inline void pair_hl(double2* HL, double2* eph, int mu, int lmax, doub
             P = ((2 * l - 1) * x * p1 - (l + mu - 1) * p2) / (l - mu);
         p2 = p1;
         p1 = P;
-        HL[leg_index(l, mu)] = regular ? make_double2(jl * P, 0.0) : cscale(make_double2(jl, yl), P);
+        HL[leg_index(l, mu)] = zkind == 2 ? make_double2(jl * P, 0.0)
+                                          : cscale(make_double2(jl, zkind == 1 ? -yl : yl), P);
     }
     if (mu <= 2 * lmax) {
         double sn, cs;

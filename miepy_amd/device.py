@@ -333,6 +333,28 @@ def cluster_coefficients(positions, p_scat, k, origin, lmax):
     return out
 
 
+def vsh_translation(mnuv, rad, theta, phi, k, mode=0):
+    """Translation coefficients on the device: mnuv [count, 4] ints (m, n, u,
+    v); rad, theta, phi [nvec] (the translation vectors); mode 0 outgoing,
+    1 ingoing, 2 incident, 3 interior.  Returns [nvec, count, 2] complex128
+    (A, B) on the host (cpp/src/vsh_translation.cpp:366-470)."""
+    lib = L.load()
+    dev = require_gpu()
+    mnuv = np.ascontiguousarray(np.asarray(mnuv, dtype=np.int32).reshape(-1, 4))
+    n, v = mnuv[:, 1], mnuv[:, 3]
+    if np.any(n < 1) or np.any(v < 1) or np.any(np.abs(mnuv[:, 0]) > n) or np.any(np.abs(mnuv[:, 2]) > v):
+        raise ValueError("vsh_translation: need 1 <= n, v and |m| <= n, |u| <= v")
+    lmax = int(max(n.max(), v.max()))
+    geo = np.stack([np.atleast_1d(np.asarray(a, dtype=float)) for a in np.broadcast_arrays(rad, theta, phi)])
+    nvec = geo.shape[1]
+    g = to_device(geo, torch.float64)
+    idx = torch.from_numpy(mnuv).to(dev)
+    out = torch.empty((nvec, mnuv.shape[0], 2), dtype=C128, device=dev)
+    L.check(lib.mpa_vsh_translation(mnuv.shape[0], _ptr(idx), lmax, nvec, _ptr(g[0]), _ptr(g[1]), _ptr(g[2]),
+                                    float(k), int(mode), _ptr(out), _stream()), "vsh_translation")
+    return out.cpu().numpy()
+
+
 def cluster_cross_sections(p_cluster, p_src, k):
     """Per-multipole (scattering, absorption, extinction) of the cluster
     (cpp/src/flux.cpp:32-57) on the device: p_cluster, p_src [S, 2, rmax]

@@ -136,3 +136,31 @@ def test_multi_source_and_cluster_class(dev):
     C1 = one.cross_sections_per_multipole(lmax=4)
     for q in range(3):
         assert np.allclose(Cc[q], C1[q], rtol=1e-9, atol=1e-12 * np.abs(C1.extinction).max())
+
+
+@pytest.mark.parametrize("mode", ["outgoing", "ingoing", "incident", "interior"])
+def test_vsh_translation_matches_oracle(dev, oracle, mode):
+    """The scalar vsh_translation and its numpy form (cpp/src/vsh_translation.cpp:366-470)
+    on the device vs the oracle's restatement, every (m, n, u, v) up to lmax 4,
+    at two translation vectors (kd about 4 and 9), 1e-12 of the largest entry."""
+    import miepy_amd as M
+    from miepy_amd import backend as B
+    lmax = 4
+    tuples = [(m, n, u, v) for n in range(1, lmax + 1) for m in range(-n, n + 1)
+              for v in range(1, lmax + 1) for u in range(-v, v + 1)]
+    k = 2 * np.pi / 1.0
+    geos = [(0.65, 0.7, 0.3), (1.4, 2.4, -2.0)]
+    md = getattr(M.vsh_mode, mode)
+    for rad, th, ph in geos:
+        got = np.array([M.vsh_translation(m, n, u, v, rad, th, ph, k, md) for (m, n, u, v) in tuples[::23]])
+        ref = np.array([oracle.vsh_translation(m, n, u, v, rad, th, ph, k, mode) for (m, n, u, v) in tuples[::23]])
+        assert np.max(np.abs(got - ref)) <= 1e-12 * np.abs(ref).max()
+        allv = dev.vsh_translation(tuples, rad, th, ph, k, md.value)[0]
+        refv = np.array([oracle.vsh_translation(*t, rad, th, ph, k, mode) for t in tuples])
+        assert np.max(np.abs(allv - refv)) <= 1e-12 * np.abs(refv).max()
+    rads, ths, phs = (np.array(a) for a in zip(*geos))
+    got = B.vsh_translation_numpy(2, 3, -1, 2, rads, ths, phs, k, md)
+    assert got.shape == (2, 2)
+    for g, (rad, th, ph) in enumerate(geos):
+        A, Bv = oracle.vsh_translation(2, 3, -1, 2, rad, th, ph, k, mode)
+        assert abs(got[0, g] - A) <= 1e-12 * abs(A) + 1e-15 and abs(got[1, g] - Bv) <= 1e-12 * abs(Bv) + 1e-15
