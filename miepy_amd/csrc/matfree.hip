@@ -1531,37 +1531,56 @@ inline int mf64_chunks(int N) {
     return c < 1 ? 1 : c;
 }
 
-// y32 = (add_identity ? x32 : 0) + sum of the complex64 chunk partials in chunk order (FP64 sum)
+// y32 = (add_identity ? x32 : 0) + sum of the complex64 chunk partials (FP64
+// sums).  A workgroup covers 64 consecutive elements: wave w sums its quarter
+// of the chunks (coalesced loads, 8 in flight), and the four partial sums are
+// added in wave order through LDS (fixed order: deterministic).
 __global__ __launch_bounds__(kThreads) void k_matfree_sum32(MfJob J, const SysDev* __restrict__ sys, int nchunks,
                                                             const float2* __restrict__ part32) {
+    __shared__ double2 red[kThreads / 64 - 1][64];
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
     const size_t total = (size_t)J.S * 2 * lmax_to_rmax(J.lmax) * J.N;
     const float2* X = (const float2*)J.x;
     float2* Y = (float2*)J.y;
-    for (size_t idx = (size_t)blockIdx.x * kThreads + threadIdx.x; idx < total; idx += (size_t)gridDim.x * kThreads) {
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int per = (nchunks + kThreads / 64 - 1) / (kThreads / 64);
+    const int c0 = w * per, c1 = min(nchunks, c0 + per);
+    for (size_t base = (size_t)blockIdx.x * 64; base < total; base += (size_t)gridDim.x * 64) {
+        const size_t idx = base + lane;
         double ax = 0, ay = 0;
-        if (J.add_identity) {
-            ax = X[idx].x;
-            ay = X[idx].y;
-        }
-        // loads issued 8 at a time (latency overlap), added in chunk order
-        int cc = 0;
-        for (; cc + 8 <= nchunks; cc += 8) {
-            float2 v[8];
+        if (idx < total) {
+            int cc = c0;
+            for (; cc + 8 <= c1; cc += 8) {
+                float2 v[8];
 #pragma unroll
-            for (int q = 0; q < 8; q++) v[q] = part32[(size_t)(cc + q) * total + idx];
+                for (int q = 0; q < 8; q++) v[q] = part32[(size_t)(cc + q) * total + idx];
 #pragma unroll
-            for (int q = 0; q < 8; q++) {
-                ax += v[q].x;
-                ay += v[q].y;
+                for (int q = 0; q < 8; q++) {
+                    ax += v[q].x;
+                    ay += v[q].y;
+                }
+            }
+            for (; cc < c1; cc++) {
+                const float2 v = part32[(size_t)cc * total + idx];
+                ax += v.x;
+                ay += v.y;
             }
         }
-        for (; cc < nchunks; cc++) {
-            const float2 v = part32[(size_t)cc * total + idx];
-            ax += v.x;
-            ay += v.y;
+        if (w > 0) red[w - 1][lane] = make_double2(ax, ay);
+        __syncthreads();
+        if (w == 0 && idx < total) {
+#pragma unroll
+            for (int q = 0; q < kThreads / 64 - 1; q++) {
+                ax += red[q][lane].x;
+                ay += red[q][lane].y;
+            }
+            if (J.add_identity) {
+                ax += X[idx].x;
+                ay += X[idx].y;
+            }
+            Y[idx] = make_float2((float)ax, (float)ay);
         }
-        Y[idx] = make_float2((float)ax, (float)ay);
+        __syncthreads();
     }
 }
 
@@ -1684,7 +1703,7 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
         else if (J.lmax == 3) MPA_MF16(3);
         else MPA_MF16(4);
 #undef MPA_MF16
-        hipLaunchKernelGGL(k_matfree_sum32, dim3((unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads)),
+        hipLaunchKernelGGL(k_matfree_sum32, dim3((unsigned)std::min<size_t>(4096, (total + 63) / 64)),
                            dim3(kThreads), 0, st, J, sys, nch, part32);
         MPA_LAUNCH_CHECK();
         return MPA_OK;
@@ -1703,7 +1722,7 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     else if (J.lmax == 2) MPA_MF32(2);
     else MPA_MF32(3);
 #undef MPA_MF32
-    hipLaunchKernelGGL(k_matfree_sum32, dim3((unsigned)std::min<size_t>(2048, (total + kThreads - 1) / kThreads)),
+    hipLaunchKernelGGL(k_matfree_sum32, dim3((unsigned)std::min<size_t>(4096, (total + 63) / 64)),
                        dim3(kThreads), 0, st, J, sys, nch, part32);
     MPA_LAUNCH_CHECK();
     return MPA_OK;
