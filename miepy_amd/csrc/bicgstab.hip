@@ -51,6 +51,7 @@ __global__ __launch_bounds__(kThreads) void k_init(const SysDev* __restrict__ sy
     VEC_PROLOGUE
     const C* b = (const C*)d.b;
     C *x = (C*)d.x, *r = (C*)d.r, *r0 = (C*)d.r0, *p = (C*)d.p, *v = (C*)d.v;
+    C *y = (C*)d.y, *zz = (C*)d.z;  // the preconditioned operands (alias p, s without a preconditioner)
     const bool x0_b = (d.flags & MPA_SOLVE_REF_LOOP) != 0;  // r = b - A b is formed by the restart kernels
     double2 acc[1] = {make_double2(0, 0)};
     for (int i = g * kThreads + threadIdx.x; i < n; i += stride) {
@@ -61,6 +62,10 @@ __global__ __launch_bounds__(kThreads) void k_init(const SysDev* __restrict__ sy
         const C z = cmake<T>(0, 0);
         p[k] = z;
         v[k] = z;
+        if (d.minv) {  // defined before any product reads them (a product may run while a restart is pending)
+            y[k] = z;
+            zz[k] = z;
+        }
         x[k] = x0_b ? bv : z;
         const double2 bd = to_d(bv);
         acc[0].x += bd.x * bd.x + bd.y * bd.y;
@@ -592,14 +597,33 @@ int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, doubl
         rc = capture_iterations<T>(P, k_sync, graph);
         if (rc) return rc;
     }
-    // Rounds of k_sync iterations between host checks: after the first
-    // rounds, as many rounds as the slowest RHS's observed residual decay
-    // says it still needs (70 % of the estimate, 1..8 rounds), so a solve of
+    // Rounds of k_sync iterations between host checks.  The first check
+    // comes after min(4, maxiter) iterations (every solve runs a few; a
+    // system done earlier turns the rest into early-exit launches); from
+    // then on, as many rounds as the slowest RHS's observed residual decay
+    // says it still needs (70 % of the estimate, 1..8 rounds; the first
+    // estimate measures the decay from ||b|| at iteration 0), so a solve of
     // a few dozen iterations reads its scalars back a handful of times.
-    // Iterations past convergence cost only early-exit launches (done flags).
     std::vector<double> prev_r2;
     std::vector<int> prev_it;
-    int rounds = 1;
+    auto launch_rounds = [&](int nr) -> int {
+        for (int round = 0; round < nr; round++) {
+            if (use_graph) {
+                MPA_HIP_CHECK(hipGraphLaunch(graph.exec, st));
+            } else {
+                for (int k = 0; k < k_sync; k++) {
+                    const int r2 = launch_iteration<T>(P, st);
+                    if (r2) return r2;
+                }
+            }
+        }
+        launched += (long)nr * k_sync;
+        return MPA_OK;
+    };
+    int rounds = std::max(1, std::min(4, maxit) / k_sync);
+    bool from_start = maxit > 0;  // the decay so far is measured from iteration 0
+    for (int i = 0; i < nsys; i++) from_start &= (S_[i].flags & MPA_SOLVE_REF_LOOP) == 0;  // x0 = b: restart first
+    if (from_start && (rc = launch_rounds(rounds))) return rc;
     for (;;) {
         rc = read_scalars(P, host, st);
         if (rc) return rc;
@@ -612,6 +636,7 @@ int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, doubl
             rc = launch_restart<T>(P, st);
             if (rc) return rc;
             prev_r2.clear();
+            from_start = false;
             rounds = 1;
             continue;
         }
@@ -620,36 +645,28 @@ int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, doubl
             mpa_set_error("bicgstab: iteration budget exhausted");
             return MPA_NOT_CONVERGED;
         }
-        if (prev_r2.size() == host.size()) {
+        if (from_start || prev_r2.size() == host.size()) {
             double need = 1.0;
             for (size_t k = 0; k < host.size(); k++) {
                 const Scal& c = host[k];
                 if (c.done) continue;
-                const int di = c.it - prev_it[k];
-                if (di <= 0 || !(prev_r2[k] > 0) || !(c.rnorm2 > 0) || !(c.tol2 > 0)) continue;
-                const double rate = std::log(c.rnorm2 / prev_r2[k]) / di;  // per iteration
-                if (!(rate < -1e-3)) continue;                            // no measurable decay: one round
+                const double p2 = from_start ? c.rhs_sq : prev_r2[k];
+                const int di = c.it - (from_start ? 0 : prev_it[k]);
+                if (di <= 0 || !(p2 > 0) || !(c.rnorm2 > 0) || !(c.tol2 > 0)) continue;
+                const double rate = std::log(c.rnorm2 / p2) / di;  // per iteration
+                if (!(rate < -1e-3)) continue;                    // no measurable decay: one round
                 need = std::max(need, 0.7 * std::log(c.tol2 / c.rnorm2) / rate / k_sync);
             }
             rounds = (int)std::min(8.0, std::max(1.0, std::floor(need)));
         }
+        from_start = false;
         prev_r2.resize(host.size());
         prev_it.resize(host.size());
         for (size_t k = 0; k < host.size(); k++) {
             prev_r2[k] = host[k].rnorm2;
             prev_it[k] = host[k].it;
         }
-        for (int round = 0; round < rounds; round++) {
-            if (use_graph) {
-                MPA_HIP_CHECK(hipGraphLaunch(graph.exec, st));
-            } else {
-                for (int k = 0; k < k_sync; k++) {
-                    rc = launch_iteration<T>(P, st);
-                    if (rc) return rc;
-                }
-            }
-        }
-        launched += (long)rounds * k_sync;
+        if ((rc = launch_rounds(rounds))) return rc;
     }
     int worst = MPA_OK;
     for (size_t k = 0; k < host.size(); k++) {
