@@ -29,7 +29,11 @@ at the iteration level (`mpa_refine_systems_advance`): complex64 BiCGSTAB
 corrections on a stored complex64 matrix or, for lmax <= 4, the FP16-split
 matrix-free operator, FP64 residuals on the stored complex128 matrix or the
 FP64 matrix-free operator; a system leaves the batch once every source
-reached the tolerance in FP64.
+reached the tolerance in FP64.  A system with lmax > 4 that is to run
+matrix-free (matrix_free=True, or "auto" with matrices above a quarter of
+the budget) has no complex64 operator: it joins a second set, solved by
+FP64 BiCGSTAB on the FP64 matrix-free operator (`mpa_systems_advance`),
+which every engine step advances beside the refined set.
 
 A system whose footprint alone exceeds the budget is rejected with status
 "too_large" instead of blocking the queue.
@@ -48,7 +52,7 @@ from . import device as D
 
 class _System:
     __slots__ = ("req", "index", "w", "n", "S", "k", "eps_b", "pos", "mie_scat", "mie_int", "p_src", "A", "x",
-                 "state", "struct", "done", "iters", "footprint", "block", "matrix_free")
+                 "state", "struct", "done", "iters", "footprint", "block", "matrix_free", "refine")
 
 
 class Arena:
@@ -169,23 +173,27 @@ class ServingEngine:
 
     # ---- intake --------------------------------------------------------------------
 
-    def _use_matrix_free(self, w):
+    def _route(self, w):
+        """(matrix_free, refined) for one system."""
         if self.precision == "mixed":
-            # the complex64 matrix-free operator covers lmax <= 4; larger lmax keep stored matrices
-            if w.lmax > 4 or self.matrix_free is False:
-                return False
-            return self.matrix_free is True or w.n >= 6000 or 24.0 * w.n * w.n > 0.25 * self.budget
+            if self.matrix_free is False:
+                return False, True
+            if w.lmax > 4:
+                # no complex64 matrix-free operator above lmax 4: stored matrices, or FP64 matrix-free
+                mf = self.matrix_free is True or 24.0 * w.n * w.n > 0.25 * self.budget
+                return mf, not mf
+            return self.matrix_free is True or w.n >= 6000 or 24.0 * w.n * w.n > 0.25 * self.budget, True
         if self.matrix_free != "auto":
-            return bool(self.matrix_free)
+            return bool(self.matrix_free), False
         # per-system matrix-free launches cost more than one batched product over many small matrices
-        return (w.lmax <= 3 and w.n >= 6000) or 16.0 * w.n * w.n > 0.25 * self.budget
+        return (w.lmax <= 3 and w.n >= 6000) or 16.0 * w.n * w.n > 0.25 * self.budget, False
 
     def submit(self, request):
         for i, w in enumerate(request.workloads):
             s = _System()
             s.req, s.index, s.w = request, i, w
-            s.matrix_free = self._use_matrix_free(w)
-            s.footprint = system_footprint(w, s.matrix_free, self.precision)
+            s.matrix_free, s.refine = self._route(w)
+            s.footprint = system_footprint(w, s.matrix_free, "mixed" if s.refine else "fp64")
             if s.footprint > self.budget:
                 request.status = "too_large"
                 request.results[i] = {"status": "too_large"}
@@ -227,7 +235,7 @@ class ServingEngine:
         s.k = w.k
         s.eps_b = float(np.real(w.medium_eps))
         off = s.block
-        if self.precision == "mixed":
+        if s.refine:
             self._prepare_mixed(s, off)
             return
         nst = _aligned(self.lib.mpa_system_state_bytes(s.n, s.S, L.MPA_C128))
@@ -297,21 +305,24 @@ class ServingEngine:
                 if s.req.admitted is None:
                     s.req.admitted = self.clock()
                     s.req.status = "running"
-            if self.precision == "mixed":
-                arr = (L.MpaRefineSystem * len(new))(*[s.struct for s in new])
-                scratch = self._scratch_for(arr, len(new))
-                L.check(self.lib.mpa_refine_systems_start(len(new), arr, D._ptr(scratch), D._stream()),
+            ref = [s for s in new if s.refine]
+            f64 = [s for s in new if not s.refine]
+            if ref:
+                arr = (L.MpaRefineSystem * len(ref))(*[s.struct for s in ref])
+                scratch = self._scratch_for(arr, len(ref), True)
+                L.check(self.lib.mpa_refine_systems_start(len(ref), arr, D._ptr(scratch), D._stream()),
                         "refine_systems_start")
-            else:
-                arr = (L.MpaSystem * len(new))(*[s.struct for s in new])
-                scratch = self._scratch_for(arr, len(new))
-                L.check(self.lib.mpa_systems_start(len(new), arr, L.MPA_C128, D._ptr(scratch), D._stream()),
+            if f64:
+                arr = (L.MpaSystem * len(f64))(*[s.struct for s in f64])
+                scratch = self._scratch_for(arr, len(f64), False)
+                L.check(self.lib.mpa_systems_start(len(f64), arr, L.MPA_C128, D._ptr(scratch), D._stream()),
                         "systems_start")
             self.active.extend(new)
         return len(new)
 
-    def _scratch_for(self, arr, count):
-        if self.precision == "mixed":
+    def _scratch_for(self, arr, count, refine):
+        """One scratch buffer serves both sets: their launches are ordered on one stream."""
+        if refine:
             need = self.lib.mpa_refine_scratch_bytes(count, arr)
         else:
             need = self.lib.mpa_batch_scratch_bytes(count, arr, L.MPA_C128)
@@ -326,33 +337,38 @@ class ServingEngine:
         self._admit()
         if not self.active:
             return False
-        count = len(self.active)
-        total = sum(s.S for s in self.active)
-        done = (ctypes.c_int * total)()
-        its = (ctypes.c_int * total)()
-        res = (ctypes.c_double * total)()
-        stt = (ctypes.c_int * total)()
-        if self.precision == "mixed":
-            arr = (L.MpaRefineSystem * count)(*[s.struct for s in self.active])
-            outer = (ctypes.c_int * total)()
-            scratch = self._scratch_for(arr, count)
-            L.check(self.lib.mpa_refine_systems_advance(count, arr, D._ptr(scratch), self.k, done, outer, its, res,
-                                                        stt, D._stream()), "refine_systems_advance")
-        else:
-            arr = (L.MpaSystem * count)(*[s.struct for s in self.active])
-            scratch = self._scratch_for(arr, count)
-            L.check(self.lib.mpa_systems_advance(count, arr, L.MPA_C128, D._ptr(scratch), self.k, done, its, res,
-                                                 stt, D._stream()), "systems_advance")
-        self.steps += 1
-        keep, o = [], 0
-        for s in self.active:
-            d = all(done[o + j] for j in range(s.S))
-            if d:
-                self._retire(s, list(its[o:o + s.S]), list(res[o:o + s.S]), list(stt[o:o + s.S]))
+        keep = []
+        for refine in (True, False):
+            group = [s for s in self.active if s.refine == refine]
+            if not group:
+                continue
+            count = len(group)
+            total = sum(s.S for s in group)
+            done = (ctypes.c_int * total)()
+            its = (ctypes.c_int * total)()
+            res = (ctypes.c_double * total)()
+            stt = (ctypes.c_int * total)()
+            if refine:
+                arr = (L.MpaRefineSystem * count)(*[s.struct for s in group])
+                outer = (ctypes.c_int * total)()
+                scratch = self._scratch_for(arr, count, True)
+                L.check(self.lib.mpa_refine_systems_advance(count, arr, D._ptr(scratch), self.k, done, outer, its,
+                                                            res, stt, D._stream()), "refine_systems_advance")
             else:
-                keep.append(s)
-            o += s.S
-        self.active = keep
+                arr = (L.MpaSystem * count)(*[s.struct for s in group])
+                scratch = self._scratch_for(arr, count, False)
+                L.check(self.lib.mpa_systems_advance(count, arr, L.MPA_C128, D._ptr(scratch), self.k, done, its, res,
+                                                     stt, D._stream()), "systems_advance")
+            o = 0
+            for s in group:
+                if all(done[o + j] for j in range(s.S)):
+                    self._retire(s, list(its[o:o + s.S]), list(res[o:o + s.S]), list(stt[o:o + s.S]))
+                else:
+                    keep.append(s)
+                o += s.S
+        self.steps += 1
+        kept = {id(s) for s in keep}
+        self.active = [s for s in self.active if id(s) in kept]
         return True
 
     def _retire(self, s, iters, resid, status):
@@ -364,6 +380,7 @@ class ServingEngine:
             "cross_sections": obs["C_total"].cpu().numpy(),  # [S, 3]
             "force": obs["F"].cpu().numpy(),                  # [S, N, 3]
             "iterations": iters, "residual": resid, "status": status, "matrix_free": s.matrix_free,
+            "refined": s.refine,
         }
         self.solves_done += 1
         s.A = s.x = s.state = s.pos = s.mie_scat = s.mie_int = s.p_src = None

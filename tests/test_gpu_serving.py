@@ -156,3 +156,48 @@ def test_mixed_precision_serving(serve):
                 assert np.array_equal(ca, cm) and np.array_equal(fa, fm)
                 np.testing.assert_allclose(cm, c64, rtol=1e-9, atol=1e-9 * np.abs(c64).max())
                 assert np.max(np.abs(fm - f64)) <= 1e-8 * np.abs(f64).max()
+
+
+def test_mixed_engine_runs_large_lmax5_matrix_free_in_fp64(serve):
+    """precision='mixed' with a budget below an lmax-5 system's two matrices
+    (24 n^2 = 0.42 GB for n = 4200): the system has no complex64 matrix-free
+    operator, so it runs FP64 BiCGSTAB on the FP64 matrix-free operator in
+    the same engine steps as refined small systems; its result equals the
+    FP64 engine's matrix-free solve bit for bit (the same kernels and
+    iterations) and the small systems' results equal theirs alone."""
+    from miepy_amd import workloads as W
+    big = W.scaled_random(60, 5, seed=3)
+    big.tol = 1e-9
+    small = [ws[0] for _, ws in _requests(17, 3)]
+    for w in small:
+        w.tol = 1e-10
+    budget = 1.2e9
+    eng = serve.ServingEngine(hbm_budget_bytes=budget, iterations_per_step=3, precision="mixed")
+    assert eng._route(big) == (True, False) and 24 * big.n ** 2 > 0.25 * eng.budget
+    assert all(eng._route(w)[1] for w in small)
+    rb = serve.Request(0, [big], 0.0)
+    eng.submit(rb)
+    rs = []
+    for i, w in enumerate(small):
+        eng.step()
+        r = serve.Request(i + 1, [w], 0.0)
+        eng.submit(r)
+        rs.append(r)
+    eng.drain()
+    assert rb.status == "done" and all(r.status == "done" for r in rs)
+    assert rb.results[0]["matrix_free"] and not rb.results[0]["refined"]
+    e64 = serve.ServingEngine(hbm_budget_bytes=budget, iterations_per_step=3, matrix_free=True)
+    r64 = serve.Request(0, [big], 0.0)
+    e64.submit(r64)
+    e64.drain()
+    assert np.array_equal(rb.results[0]["cross_sections"], r64.results[0]["cross_sections"])
+    assert np.array_equal(rb.results[0]["force"], r64.results[0]["force"])
+    assert rb.results[0]["iterations"] == r64.results[0]["iterations"]
+    for w, r in zip(small, rs):
+        alone = serve.ServingEngine(hbm_budget_bytes=budget, iterations_per_step=2, precision="mixed")
+        ra = serve.Request(0, [w], 0.0)
+        alone.submit(ra)
+        alone.drain()
+        assert r.results[0]["refined"]
+        assert np.array_equal(_results(ra)[0][0], _results(r)[0][0])
+        assert np.array_equal(_results(ra)[0][1], _results(r)[0][1])
