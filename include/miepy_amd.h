@@ -203,6 +203,31 @@ int mpa_cluster_coefficients(int N, int lmax_in, int lmax_out, int nrhs, const d
                              double oy, double oz, const void* p_scat, double k, void* p_cluster, void* workspace,
                              void* stream);
 
+/* The source matrix of a sphere cluster's T-matrix (the S of
+ * tmatrix_sphere_cluster, src/miepy/tmatrix/common.py:199-247, built there by
+ * loops over the scalar vsh_translation): origin mode (b, n, m), n <= lmax_out,
+ * expanded about each particle in modes (a, v, u), v <= lmax_p, with the
+ * incident-mode translation of r_i - origin (a particle within 1e-15 of the
+ * origin takes the identity).  Stored transposed, one origin mode per row:
+ *   out [2 rmax(lmax_out), N, 2, rmax(lmax_p)] c128 (device), overwritten;
+ * each row is one right-hand side of (I + T) X = S.  N <= 65535. */
+int mpa_cluster_source(int N, int lmax_p, int lmax_out, const double* positions, double ox, double oy, double oz,
+                       double k, void* out, void* stream);
+
+/* The cluster T-matrix from the solved cluster system: T = R diag(mie) X,
+ * R the re-expansion of every particle's scattered modes about the origin
+ * (src/miepy/tmatrix/common.py:250-295; steps 5-6 of tmatrix_sphere_cluster,
+ * :352-364).
+ *   X        [2 rmax(lmax_out), N, 2, rmax(lmax_p)] c128 (device): the
+ *            solutions of (I + T) X = S, one origin mode per row
+ *   mie      [N, 2, lmax_p] c128 (device): a_n, b_n
+ *   tmatrix  [2, rmax(lmax_out), 2, rmax(lmax_out)] c128 (device),
+ *            row-major, overwritten
+ *   workspace: mpa_cluster_coefficients_workspace_bytes(N, lmax_out,
+ *              2 rmax(lmax_out)) bytes. */
+int mpa_cluster_tmatrix(int N, int lmax_p, int lmax_out, const double* positions, double ox, double oy, double oz,
+                        const void* mie, const void* X, double k, void* tmatrix, void* workspace, void* stream);
+
 /* Translation coefficients (A, B) of `count` tuples (m, n, u, v) (mnuv
  * [count][4] int, device; 1 <= n, v <= lmax, |m| <= n, |u| <= v) for nvec
  * translation vectors in spherical coordinates rad, theta, phi [nvec] f64

@@ -16,7 +16,8 @@ from .materials import (constant_material, data_material, dielectric, function_m
                         materials)
 from .sources import plane_wave, combined_source
 from .sphere_cluster import sphere_cluster
-from .cluster import cluster, sphere, tmatrix_particle
+from .cluster import cluster, sphere, sphere_cluster_particle, tmatrix_particle
+from . import tmatrix
 
 backends.register_backend("mi355x", _backend)
 

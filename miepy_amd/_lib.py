@@ -64,6 +64,8 @@ SIGNATURES = {
     "mpa_plane_wave_structure": (_i, [_i, _i, _i, _v, _d, _v, _v, _v, _v, _v, _v]),
     "mpa_cluster_coefficients_workspace_bytes": (_sz, [_i, _i, _i]),
     "mpa_cluster_coefficients": (_i, [_i, _i, _i, _i, _v, _d, _d, _d, _v, _d, _v, _v, _v]),
+    "mpa_cluster_source": (_i, [_i, _i, _i, _v, _d, _d, _d, _d, _v, _v]),
+    "mpa_cluster_tmatrix": (_i, [_i, _i, _i, _v, _d, _d, _d, _v, _v, _d, _v, _v, _v]),
     "mpa_cluster_cross_sections": (_i, [_i, _i, _v, _v, _d, _v, _v]),
     "mpa_vsh_translation": (_i, [_i, _v, _i, _i, _v, _v, _v, _d, _i, _v, _v]),
     "mpa_lu_factor": (_i, [_i, _v, _i, _v, _pi, _v]),

@@ -12,9 +12,11 @@ torque() :492, update() :588), with the solve of :629-657 on the device:
           p_scat = T p_inc (einsum "naibj,nbj->nai", :650,657) ->
           per-particle cross-sections, force, torque.
 
-Particles: `sphere` (T-matrix = diag(a_n, b_n), src/miepy/tmatrix/common.py:6-28)
-and `tmatrix_particle` (a T-matrix computed elsewhere; the EBCM solvers for
-non-spherical shapes are not part of this library).  Extra keyword
+Particles: `sphere` (T-matrix = diag(a_n, b_n), src/miepy/tmatrix/common.py:6-28),
+`sphere_cluster_particle` (the T-matrix of a rigid sphere cluster,
+src/miepy/tmatrix/common.py:298-364) and `tmatrix_particle` (a T-matrix
+computed elsewhere; the EBCM solvers for non-spherical shapes are not part
+of this library).  Extra keyword
 arguments (the reference hard-codes these): solver_mode ('stored' |
 'matfree'), tol, maxiter.
 """
@@ -66,6 +68,46 @@ class tmatrix_particle:
         if self.tmatrix.shape != (2, rmax, 2, rmax):
             raise ValueError(f"tmatrix must be [2, {rmax}, 2, {rmax}] for lmax={lmax}")
         return self.tmatrix
+
+
+class sphere_cluster_particle:
+    """A rigid collection of non-overlapping spheres acting as one particle
+    (src/miepy/particles/sphere_cluster_particle.py:9-70): its T-matrix is
+    tmatrix_sphere_cluster of the spheres at their given coordinates, about
+    the coordinate origin as in the reference, reduced to the cluster's lmax;
+    the particle sits at the spheres' centre of mass.  Fixed orientation only
+    (the T-matrix rotation of the reference's particle base is not part of
+    this library)."""
+
+    def __init__(self, position, radius, material, lmax, orientation=None):
+        self.p_position = np.asarray(position, dtype=float)
+        self.Nparticles = len(self.p_position)
+        if self.p_position.ndim != 2 or self.Nparticles == 1:
+            raise ValueError("a sphere_cluster_particle must have at least 2 particles")
+        if orientation is not None:
+            raise NotImplementedError("sphere_cluster_particle: only the fixed orientation is supported")
+        self.com = np.average(self.p_position, axis=0)
+        self.position = self.com
+        self.p_radii = np.empty(self.Nparticles, dtype=float)
+        self.p_radii[...] = radius
+        self.p_material = np.empty(self.Nparticles, dtype=object)
+        self.p_material[...] = material
+        self.p_lmax = np.empty(self.Nparticles, dtype=int)
+        self.p_lmax[...] = lmax
+        self.lmax_cluster = int(np.max(self.p_lmax))
+        self.material = self.p_material[0]
+        self.tmatrix = None
+
+    def compute_tmatrix(self, lmax, wavelength, eps_m, **kwargs):
+        from .tmatrix import tmatrix_reduce_lmax, tmatrix_sphere_cluster
+        eps = np.array([complex(np.squeeze(m.eps(wavelength))) for m in self.p_material])
+        T = tmatrix_sphere_cluster(self.p_position, self.p_radii, self.p_lmax, self.lmax_cluster, wavelength, eps,
+                                   eps_m)
+        self.tmatrix = tmatrix_reduce_lmax(T, lmax)
+        return self.tmatrix
+
+    def enclosed_radius(self):
+        return np.max(np.linalg.norm(self.p_position - self.position[np.newaxis], axis=1)) + np.max(self.p_radii)
 
 
 class cluster:

@@ -32,10 +32,14 @@ constexpr int kMaxChunks = 128;  // workgroups over particles, one partial sum e
 
 int chunks_for(int N) { return N < kMaxChunks ? N : kMaxChunks; }
 
+// mie (optional, [N][2][lmax_in]) scales the input first: p_scat = mie (.) x per
+// (particle, polarization, n).  A particle is at the origin when the
+// translation vector is exactly zero or shorter than origin_tol.
 __global__ __launch_bounds__(kCT) void k_cluster_coef(int N, int L, int lmax_in, int lmax_out, int S,
                                                       const double* __restrict__ pos, double ox, double oy, double oz,
-                                                      const double2* __restrict__ p_scat, double k, TabDev tab,
-                                                      double2* __restrict__ part) {
+                                                      const double2* __restrict__ p_scat,
+                                                      const double2* __restrict__ mie, double origin_tol, double k,
+                                                      TabDev tab, double2* __restrict__ part) {
     __shared__ double2 HL[(2 * MPA_LMAX_MAX + 2) * (2 * MPA_LMAX_MAX + 3) / 2];
     __shared__ double2 eph[2 * MPA_LMAX_MAX + 1];
     const int rmax_in = lmax_to_rmax(lmax_in), rmax_out = lmax_to_rmax(lmax_out), rmaxL = lmax_to_rmax(L);
@@ -48,23 +52,31 @@ __global__ __launch_bounds__(kCT) void k_cluster_coef(int N, int L, int lmax_in,
         double2 acc0 = make_double2(0, 0), acc1 = make_double2(0, 0);
         for (int i = blockIdx.x; i < N; i += gridDim.x) {
             const double dx = ox - pos[3 * i], dy = oy - pos[3 * i + 1], dz = oz - pos[3 * i + 2];
-            const bool at_origin = dx == 0.0 && dy == 0.0 && dz == 0.0;
+            const bool at_origin = (dx == 0.0 && dy == 0.0 && dz == 0.0) ||
+                                   sqrt(dx * dx + dy * dy + dz * dz) < origin_tol;
             __syncthreads();  // the previous particle's HL is no longer read
             if (!at_origin && (int)threadIdx.x <= pmax) pair_hl(HL, eph, threadIdx.x, L, dx, dy, dz, k, 2);
             __syncthreads();
             if (!mine) continue;
             const double2* ps = p_scat + ((size_t)s * N + i) * 2 * rmax_in;
+            const double2* mi = mie ? mie + (size_t)i * 2 * lmax_in : nullptr;
             if (at_origin) {
                 if (r < rmax_in) {
-                    acc0 = cadd(acc0, ps[r]);
-                    acc1 = cadd(acc1, ps[rmax_in + r]);
+                    const int v = mode_n(r);
+                    acc0 = cadd(acc0, mi ? cmul(mi[v - 1], ps[r]) : ps[r]);
+                    acc1 = cadd(acc1, mi ? cmul(mi[lmax_in + v - 1], ps[rmax_in + r]) : ps[rmax_in + r]);
                 }
                 continue;
             }
             for (int rp = 0; rp < rmax_in; rp++) {
                 double2 xa, xb;
                 tuple_xab(tab, r * rmaxL + rp, HL, eph, xa, xb);
-                const double2 a = ps[rp], b = ps[rmax_in + rp];
+                double2 a = ps[rp], b = ps[rmax_in + rp];
+                if (mi) {
+                    const int v = mode_n(rp);
+                    a = cmul(mi[v - 1], a);
+                    b = cmul(mi[lmax_in + v - 1], b);
+                }
                 acc0 = cadd(acc0, cadd(cmul(a, xa), cmul(b, xb)));
                 acc1 = cadd(acc1, cadd(cmul(a, xb), cmul(b, xa)));
             }
@@ -106,14 +118,57 @@ __global__ __launch_bounds__(kCT) void k_vsh_translation(int count, const int* _
     }
 }
 
-// out[e] = sum over chunks c (in order) of part[c][e], e < count
+// out[e] = sum over chunks c (in order) of part[c][e], e < count; with
+// rows > 0 the [count / rows][rows] result is stored transposed, [rows][count / rows]
 __global__ __launch_bounds__(kCT) void k_cluster_sum(const double2* __restrict__ part, int chunks, int count,
-                                                     double2* __restrict__ out) {
+                                                     int rows, double2* __restrict__ out) {
     const int e = blockIdx.x * kCT + threadIdx.x;
     if (e >= count) return;
     double2 acc = part[e];
     for (int c = 1; c < chunks; c++) acc = cadd(acc, part[(size_t)c * count + e]);
-    out[e] = acc;
+    if (rows > 0) {
+        const int col = e / rows, row = e - col * rows;
+        out[(size_t)row * (count / rows) + col] = acc;
+    } else {
+        out[e] = acc;
+    }
+}
+
+// The source matrix of a sphere cluster's T-matrix: origin mode (b, n, m)
+// (n <= lmax_out) expanded about particle i in modes (a, v, u) (v <= lmax_p)
+// with the incident-mode translation coefficients (A, B) of the tuple
+// (u, v, m, n) and d = r_i - origin: A where a == b, B otherwise; a particle
+// within 1e-15 of the origin takes the identity on the modes both sides hold
+// (src/miepy/tmatrix/common.py:199-247).  Stored one origin mode per row,
+// out[b * rmax_out + r][i][a][s] (each row a right-hand side of the cluster
+// system).  One workgroup per particle; threads over (r, s).
+__global__ __launch_bounds__(kCT) void k_cluster_source(int N, int L, int lmax_p, int lmax_out,
+                                                        const double* __restrict__ pos, double ox, double oy,
+                                                        double oz, double k, TabDev tab, double2* __restrict__ out) {
+    __shared__ double2 HL[(2 * MPA_LMAX_MAX + 2) * (2 * MPA_LMAX_MAX + 3) / 2];
+    __shared__ double2 eph[2 * MPA_LMAX_MAX + 1];
+    const int i = blockIdx.x;
+    const int rmax_p = lmax_to_rmax(lmax_p), rmax_out = lmax_to_rmax(lmax_out), rmaxL = lmax_to_rmax(L);
+    const size_t n = (size_t)N * 2 * rmax_p;
+    const double dx = pos[3 * i] - ox, dy = pos[3 * i + 1] - oy, dz = pos[3 * i + 2] - oz;
+    const bool at_origin = sqrt(dx * dx + dy * dy + dz * dz) < 1e-15;
+    if (!at_origin && (int)threadIdx.x <= 2 * L + 1) pair_hl(HL, eph, threadIdx.x, L, dx, dy, dz, k, 2);
+    __syncthreads();
+    for (int q = threadIdx.x; q < rmax_out * rmax_p; q += kCT) {
+        const int r = q / rmax_p, s = q - r * rmax_p;
+        double2 xa = make_double2(0, 0), xb = make_double2(0, 0);
+        if (at_origin) {
+            if (r == s) xa.x = 1.0;
+        } else {
+            tuple_xab(tab, s * rmaxL + r, HL, eph, xa, xb);
+        }
+        double2* o0 = out + (size_t)r * n + (size_t)i * 2 * rmax_p;               // origin polarization 0
+        double2* o1 = out + (size_t)(rmax_out + r) * n + (size_t)i * 2 * rmax_p;  // origin polarization 1
+        o0[s] = xa;
+        o0[rmax_p + s] = xb;
+        o1[s] = xb;
+        o1[rmax_p + s] = xa;
+    }
 }
 
 // C[s] = (Cscat, Cabs, Cext) [3][2][lmax]; one thread per (s, a, n)
@@ -161,10 +216,50 @@ int mpa_cluster_coefficients(int N, int lmax_in, int lmax_out, int nrhs, const d
     const int chunks = chunks_for(N);
     const TabDev tab{T->cmeta, T->ccoef, T->clidx};
     hipLaunchKernelGGL(k_cluster_coef, dim3(chunks), dim3(kCT), 0, st, N, L, lmax_in, lmax_out, nrhs, positions, ox,
-                       oy, oz, (const double2*)p_scat, k, tab, (double2*)workspace);
+                       oy, oz, (const double2*)p_scat, (const double2*)nullptr, 0.0, k, tab, (double2*)workspace);
     const int count = nrhs * 2 * lmax_to_rmax(lmax_out);
     hipLaunchKernelGGL(k_cluster_sum, dim3((count + kCT - 1) / kCT), dim3(kCT), 0, st, (const double2*)workspace,
-                       chunks, count, (double2*)p_cluster);
+                       chunks, count, 0, (double2*)p_cluster);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+int mpa_cluster_source(int N, int lmax_p, int lmax_out, const double* positions, double ox, double oy, double oz,
+                       double k, void* out, void* stream) {
+    if (N < 1 || N > 65535 || lmax_p < 1 || lmax_out < 1 || lmax_p > MPA_LMAX_MAX || lmax_out > MPA_LMAX_MAX ||
+        !positions || !out || !(k > 0)) {
+        mpa_set_error("cluster_source: bad argument (1 <= N <= 65535, 1 <= lmax_p, lmax_out <= %d)", MPA_LMAX_MAX);
+        return MPA_ERR_ARG;
+    }
+    const int L = lmax_p > lmax_out ? lmax_p : lmax_out;
+    const TransTable* T = mpa_get_table(L);
+    if (!T) return MPA_ERR_HIP;
+    const TabDev tab{T->cmeta, T->ccoef, T->clidx};
+    hipLaunchKernelGGL(k_cluster_source, dim3(N), dim3(kCT), 0, (hipStream_t)stream, N, L, lmax_p, lmax_out,
+                       positions, ox, oy, oz, k, tab, (double2*)out);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+int mpa_cluster_tmatrix(int N, int lmax_p, int lmax_out, const double* positions, double ox, double oy, double oz,
+                        const void* mie, const void* X, double k, void* tmatrix, void* workspace, void* stream) {
+    if (N < 1 || lmax_p < 1 || lmax_out < 1 || lmax_p > MPA_LMAX_MAX || lmax_out > MPA_LMAX_MAX || !positions ||
+        !mie || !X || !tmatrix || !workspace || !(k > 0)) {
+        mpa_set_error("cluster_tmatrix: bad argument (1 <= lmax_p, lmax_out <= %d)", MPA_LMAX_MAX);
+        return MPA_ERR_ARG;
+    }
+    const int L = lmax_p > lmax_out ? lmax_p : lmax_out;
+    const TransTable* T = mpa_get_table(L);
+    if (!T) return MPA_ERR_HIP;
+    hipStream_t st = (hipStream_t)stream;
+    const int chunks = chunks_for(N);
+    const int rows = 2 * lmax_to_rmax(lmax_out);  // one right-hand side per origin mode
+    const TabDev tab{T->cmeta, T->ccoef, T->clidx};
+    hipLaunchKernelGGL(k_cluster_coef, dim3(chunks), dim3(kCT), 0, st, N, L, lmax_p, lmax_out, rows, positions, ox,
+                       oy, oz, (const double2*)X, (const double2*)mie, 1e-15, k, tab, (double2*)workspace);
+    const int count = rows * rows;
+    hipLaunchKernelGGL(k_cluster_sum, dim3((count + kCT - 1) / kCT), dim3(kCT), 0, st, (const double2*)workspace,
+                       chunks, count, rows, (double2*)tmatrix);
     MPA_LAUNCH_CHECK();
     return MPA_OK;
 }

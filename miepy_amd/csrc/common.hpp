@@ -67,6 +67,13 @@ __host__ __device__ inline double2 cdiv(double2 a, double2 b) {
 __host__ __device__ inline int lmax_to_rmax(int lmax) { return lmax * (lmax + 2); }
 // r(n, m) = n(n+2) - n + m - 1
 __host__ __device__ inline int mode_r(int n, int m) { return n * (n + 2) - n + m - 1; }
+// the degree n of mode index r: n^2 <= r + 1 < (n + 1)^2
+__host__ __device__ inline int mode_n(int r) {
+    int n = (int)sqrt((double)(r + 1));
+    while (n * n > r + 1) n--;
+    while ((n + 1) * (n + 1) <= r + 1) n++;
+    return n;
+}
 
 // Largest lmax the kernels are compiled for (tables, per-pair scratch).
 #define MPA_LMAX_MAX 6

@@ -333,6 +333,44 @@ def cluster_coefficients(positions, p_scat, k, origin, lmax):
     return out
 
 
+def cluster_source(positions, k, origin, lmax_p, lmax_out):
+    """The source matrix of tmatrix_sphere_cluster (src/miepy/tmatrix/common.py:199-247)
+    on the device, stored one origin mode per row: [2 rmax_out, N * 2 rmax_p]
+    complex128 (device), the right-hand sides of (I + T) X = S."""
+    lib = L.load()
+    require_gpu()
+    pos = to_device(np.asarray(positions, dtype=float).reshape(-1, 3), torch.float64)
+    N = pos.shape[0]
+    out = torch.empty((2 * lmax_to_rmax(lmax_out), N * 2 * lmax_to_rmax(lmax_p)), dtype=C128, device=pos.device)
+    o = np.asarray(origin, dtype=float).reshape(3)
+    L.check(lib.mpa_cluster_source(N, int(lmax_p), int(lmax_out), _ptr(pos), float(o[0]), float(o[1]), float(o[2]),
+                                   float(k), _ptr(out), _stream()), "cluster_source")
+    return out
+
+
+def cluster_tmatrix(positions, mie, X, k, origin, lmax_out):
+    """T = R diag(mie) X (src/miepy/tmatrix/common.py:250-295, 352-364): mie
+    [N, 2, lmax_p] and X [2 rmax_out, N * 2 rmax_p] complex128 (device).
+    Returns T [2, rmax_out, 2, rmax_out] (device)."""
+    lib = L.load()
+    require_gpu()
+    pos = to_device(np.asarray(positions, dtype=float).reshape(-1, 3), torch.float64)
+    N = pos.shape[0]
+    mie = mie.to(C128).contiguous()
+    lmax_p = mie.shape[2]
+    rows = 2 * lmax_to_rmax(lmax_out)
+    if mie.shape[:2] != (N, 2) or X.shape != (rows, N * 2 * lmax_to_rmax(lmax_p)) or X.dtype != C128:
+        raise ValueError("cluster_tmatrix: mie must be [N, 2, lmax_p] and X [2 rmax_out, N * 2 rmax_p] complex128")
+    X = X.contiguous()
+    out = torch.empty((2, rows // 2, 2, rows // 2), dtype=C128, device=pos.device)
+    ws = torch.empty(lib.mpa_cluster_coefficients_workspace_bytes(N, int(lmax_out), rows), dtype=torch.uint8,
+                     device=pos.device)
+    o = np.asarray(origin, dtype=float).reshape(3)
+    L.check(lib.mpa_cluster_tmatrix(N, int(lmax_p), int(lmax_out), _ptr(pos), float(o[0]), float(o[1]), float(o[2]),
+                                    _ptr(mie), _ptr(X), float(k), _ptr(out), _ptr(ws), _stream()), "cluster_tmatrix")
+    return out
+
+
 def vsh_translation(mnuv, rad, theta, phi, k, mode=0):
     """Translation coefficients on the device: mnuv [count, 4] ints (m, n, u,
     v); rad, theta, phi [nvec] (the translation vectors); mode 0 outgoing,

@@ -11,7 +11,10 @@ Scalar, loop-per-particle code that follows the reference line by line:
 - sphere_cluster construction + solve: src/miepy/sphere_cluster.py:27-137,
   664-717 (p_scat = p_inc * mie_scat[:, :, n_idx], p_int uses the swapped
   polarization mie_int[:, ::-1, n_idx]);
-- cross-sections (sphere_cluster.py:442-471) and force/torque (:521-559).
+- cross-sections (sphere_cluster.py:442-471) and force/torque (:521-559);
+- cluster coefficients (src/miepy/vsh/cluster_coefficients.py:14-53);
+- the sphere-cluster T-matrix with its source and re-expansion matrices
+  (src/miepy/tmatrix/common.py:199-364).
 """
 
 import numpy as np
@@ -210,3 +213,97 @@ def cluster_coefficients(positions, p_scat, k, origin, lmax=None):
                 p_cluster[0, r] += a * A + b * B
                 p_cluster[1, r] += a * B + b * A
     return p_cluster
+
+
+# ---- T-matrix of a sphere cluster ------------------------------------------------------
+
+def _modes(lmax):
+    return [(n, m) for n in range(1, lmax + 1) for m in range(-n, n + 1)]
+
+
+def _sph(d):
+    rad = np.linalg.norm(d)
+    return rad, np.arccos(d[2] / rad), np.arctan2(d[1], d[0])
+
+
+def cluster_source_matrix(positions, origin, lmax_p, lmax_out, k):
+    """src/miepy/tmatrix/common.py:199-247: S [N 2 rmax_p, 2 rmax_out], origin
+    mode (b, n, m) expanded about particle i with the incident-mode
+    vsh_translation(u, v, m, n) of d = r_i - origin; a particle within 1e-15
+    of the origin takes the identity on the common modes."""
+    positions = np.asarray(positions, dtype=float).reshape(-1, 3)
+    rmax_p, rmax_out = O.lmax_to_rmax(lmax_p), O.lmax_to_rmax(lmax_out)
+    S = np.zeros((positions.shape[0] * 2 * rmax_p, 2 * rmax_out), dtype=complex)
+    for i in range(positions.shape[0]):
+        d = positions[i] - origin
+        row = i * 2 * rmax_p
+        if np.linalg.norm(d) < 1e-15:
+            rmin = O.lmax_to_rmax(min(lmax_p, lmax_out))
+            for a in range(2):
+                for r in range(rmin):
+                    S[row + a * rmax_p + r, a * rmax_out + r] = 1.0
+            continue
+        rad, theta, phi = _sph(d)
+        for s, (v, u) in enumerate(_modes(lmax_p)):
+            for r, (n, m) in enumerate(_modes(lmax_out)):
+                A, B = O.vsh_translation(u, v, m, n, rad, theta, phi, k, mode="incident")
+                S[row + s, r] += A
+                S[row + s, rmax_out + r] += B
+                S[row + rmax_p + s, r] += B
+                S[row + rmax_p + s, rmax_out + r] += A
+    return S
+
+
+def cluster_reexpansion_matrix(positions, origin, lmax_p, lmax_out, k):
+    """src/miepy/tmatrix/common.py:250-295: R [2 rmax_out, N 2 rmax_p], particle
+    i's scattered mode (b, v, u) re-expanded about the origin with the
+    incident-mode vsh_translation(m, n, u, v) of d = origin - r_i."""
+    positions = np.asarray(positions, dtype=float).reshape(-1, 3)
+    rmax_p, rmax_out = O.lmax_to_rmax(lmax_p), O.lmax_to_rmax(lmax_out)
+    R = np.zeros((2 * rmax_out, positions.shape[0] * 2 * rmax_p), dtype=complex)
+    for i in range(positions.shape[0]):
+        d = origin - positions[i]
+        col = i * 2 * rmax_p
+        if np.linalg.norm(d) < 1e-15:
+            rmin = O.lmax_to_rmax(min(lmax_p, lmax_out))
+            for a in range(2):
+                for r in range(rmin):
+                    R[a * rmax_out + r, col + a * rmax_p + r] = 1.0
+            continue
+        rad, theta, phi = _sph(d)
+        for r, (n, m) in enumerate(_modes(lmax_out)):
+            for s, (v, u) in enumerate(_modes(lmax_p)):
+                A, B = O.vsh_translation(m, n, u, v, rad, theta, phi, k, mode="incident")
+                R[r, col + s] += A
+                R[r, col + rmax_p + s] += B
+                R[rmax_out + r, col + s] += B
+                R[rmax_out + r, col + rmax_p + s] += A
+    return R
+
+
+def tmatrix_sphere_cluster(pos, radii, lmax, lmax_cluster, wavelength, eps, eps_m):
+    """src/miepy/tmatrix/common.py:298-364: T = R diag(mie) solve(I + T_agg, S),
+    [2, rmax_out, 2, rmax_out]; per-sphere lmax (Mie coefficients above it
+    zero), the origin at (0, 0, 0)."""
+    pos = np.asarray(pos, dtype=float).reshape(-1, 3)
+    N = pos.shape[0]
+    radii = np.broadcast_to(np.atleast_1d(np.asarray(radii, dtype=float)), (N,))
+    lmax_arr = np.broadcast_to(np.atleast_1d(np.asarray(lmax, dtype=int)), (N,))
+    eps_arr = np.broadcast_to(np.atleast_1d(np.asarray(eps, dtype=complex)), (N,))
+    lmax_p = int(np.max(lmax_arr))
+    k = 2 * np.pi * eps_m ** 0.5 / wavelength
+    origin = np.zeros(3)
+    rmax_p, rmax_out = O.lmax_to_rmax(lmax_p), O.lmax_to_rmax(lmax_cluster)
+    mie = np.zeros((N, 2, lmax_p), dtype=complex)
+    for i in range(N):
+        for n in range(1, int(lmax_arr[i]) + 1):
+            mie[i, :, n - 1] = mie_sphere_scattering_coefficients(radii[i], n, eps_arr[i], 1, eps_m, 1, k)
+    size = N * 2 * rmax_p
+    A = np.eye(size, dtype=complex) + O.sphere_aggregate_tmatrix(pos, mie, k).reshape(size, size)
+    S = cluster_source_matrix(pos, origin, lmax_p, lmax_cluster, k)
+    X = np.linalg.solve(A, S)
+    n_idx = n_index(lmax_p)
+    mie_diag = np.concatenate([np.concatenate([mie[i, 0, n_idx], mie[i, 1, n_idx]]) for i in range(N)])
+    Y = mie_diag[:, None] * X
+    R = cluster_reexpansion_matrix(pos, origin, lmax_p, lmax_cluster, k)
+    return (R @ Y).reshape(2, rmax_out, 2, rmax_out)
