@@ -158,12 +158,8 @@ def bench_batch(args):
         for i in range(max(args.warmup, 1)):
             B.solve()
             if i == 0 and not args.no_autotune and prec in ("fp64", "mixed"):
-                import miepy_amd.device as D
-                big = max(B.clusters, key=lambda c: c.n)
-                tuned = {"matvec_c128": D.autotune_matvec(big.A64, big.S)}
-                if prec == "mixed":
-                    tuned["matvec_c64"] = D.autotune_matvec(big.A32, big.S)
-                log(f"{prec}: autotuned on n={big.n}: {tuned}")
+                tuned = {f"matvec_{dt}_S{S}": v for (dt, S), v in B.autotune().items()}
+                log(f"{prec}: autotuned on the batched launch over {len(B.clusters)} clusters: {tuned}")
         torch.cuda.synchronize()
         lat = []
         t0 = time.perf_counter()

@@ -107,6 +107,14 @@ int mpa_matvec(int n, int nrhs, int dtype, const void* A, int add_identity, cons
  * average launch time in ms. */
 int mpa_autotune_matvec(int n, int nrhs, int dtype, const void* A, const void* x, void* y, int reps,
                         int* variant_out, double* ms_out, void* stream);
+/* The same over a batch (no counterpart in the reference): every variant of
+ * the class of nrhs[0] timed on one batched launch over the `count` systems
+ * of that class (n[i], nrhs[i], device pointers A[i], x[i], y[i]; systems of
+ * other classes are left out), as the batched solvers launch it.  Allocates
+ * its job and work lists for the duration of the call. */
+int mpa_autotune_matvec_batched(int count, const int* n, const int* nrhs, int dtype, const void* const* A,
+                                const void* const* x, void* const* y, int reps, int* variant_out, double* ms_out,
+                                void* stream);
 /* Variant in use for (dtype, class of nrhs); set_to >= 0 selects one. */
 int mpa_matvec_variant(int dtype, int nrhs, int set_to);
 const char* mpa_matvec_variant_name(int dtype, int nrhs, int variant);

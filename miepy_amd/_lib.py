@@ -48,6 +48,7 @@ _v = ctypes.c_void_p
 _sz = ctypes.c_size_t
 _pi = ctypes.POINTER(ctypes.c_int)
 _pd = ctypes.POINTER(ctypes.c_double)
+_pv = ctypes.POINTER(ctypes.c_void_p)
 
 # name -> (restype, argtypes); kept in the order of include/miepy_amd.h
 SIGNATURES = {
@@ -78,6 +79,7 @@ SIGNATURES = {
     "mpa_particle_cross_sections": (_i, [_i, _i, _v, _v, _v, _d, _v, _v]),
     "mpa_stream_copy": (_i, [_v, _v, _sz, _v]),
     "mpa_autotune_matvec": (_i, [_i, _i, _i, _v, _v, _v, _i, _pi, _pd, _v]),
+    "mpa_autotune_matvec_batched": (_i, [_i, _pi, _pi, _i, _pv, _pv, _pv, _i, _pi, _pd, _v]),
     "mpa_autotune_assembly": (_i, [_i, _i, _v, _v, _d, _i, _v, _i, _pi, _pd, _v]),
     "mpa_assembly_variant": (_i, [_i, _i, _pi]),
     "mpa_matvec_variant": (_i, [_i, _i, _i]),

@@ -68,6 +68,18 @@ class ClusterBatch:
         self.reports = None
         self.results = None
 
+    def autotune(self, reps=3):
+        """Pick the matvec tile of every (dtype, RHS-count class) present on
+        one batched launch over this batch's stored matrices (their contents
+        need not be assembled yet).  Returns {(dtype, S): (variant, ms)}."""
+        out = {}
+        for attr, dt in (("A64", "c128"), ("A32", "c64")):
+            mats = [c for c in self.clusters if getattr(c, attr) is not None]
+            for S in sorted({c.S for c in mats}):
+                group = [getattr(c, attr) for c in mats if c.S == S]
+                out[(dt, S)] = D.autotune_matvec_batched(group, S, reps=reps)
+        return out
+
     def solve(self):
         """Assemble, solve and reduce every cluster; results stay on the device."""
         cs = self.clusters

@@ -254,8 +254,102 @@ __device__ inline void gemm_rows_c64(const MvDev& J, int rb) {
     }
 }
 
+// Eight right-hand sides per pass (16 real columns, one 16x16x4 tile wide):
+// v_mfma_f32_16x16x4f32, a wave owns 32 rows as two 16-row tiles sharing
+// each B operand; half the matrix-core work per byte of A of the 16-RHS
+// form, whose upper 16 columns would be idle at S <= 8.  Each lane reads a
+// float4 of its row (real K 16q + 4g .. +3, g = lane / 16) and feeds its
+// four components to four MFMAs, so MFMA (q, e) covers real K 16q + 4g + e
+// over g; the B operands follow the same order.  D[i] of 16x16x4f32 ->
+// (row 4 (l/16) + i, col l%16).
+constexpr int kGemmS8 = 8;
+
+__device__ inline void gemm_rows_c64_s8(const MvDev& J, int rb) {
+    constexpr int RW = 128, RS = 2 * kGemmKB + 2 * kPadC64;  // rows per workgroup, row stride (floats)
+    __shared__ float As[RW * RS];
+    __shared__ fvec4 xs4[kGemmS8][kGemmKB / 2 + 1];  // two complex per float4; one float4 of padding
+    const int n = J.n;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, col = lane & 15, s_of = col >> 1, part = col & 1;
+    const float2* A = (const float2*)J.A;
+    const float2* X = (const float2*)J.x;
+    const int nkb = (n + kGemmKB - 1) / kGemmKB;
+    for (int s0 = 0; s0 < J.S; s0 += kGemmS8) {
+        // A: 128 rows x 16 chunks of 16 B, 8 per thread (a wave covers 4 full rows per it)
+        auto load_a = [&](int kb, fvec4 (&ra)[8]) {
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int id = it * kThreads + tid;
+                const int r = id >> 4, c16 = id & 15;
+                const int row = min(rb * RW + r, n - 1);
+                const int j = kb * kGemmKB + 2 * c16;
+                fvec4 z = {0.f, 0.f, 0.f, 0.f};
+                ra[it] = j < n ? __builtin_nontemporal_load((const fvec4*)(A + (size_t)row * n + j)) : z;
+            }
+        };
+        // X: 8 right-hand sides x 32 complex, one per thread
+        auto load_x = [&](int kb, float2& rx) {
+            const int s = tid / kGemmKB, jj = tid % kGemmKB, j = kb * kGemmKB + jj;
+            rx = (s0 + s < J.S && j < n) ? X[(size_t)(s0 + s) * n + j] : make_float2(0.f, 0.f);
+        };
+        auto store_lds = [&](const fvec4 (&ra)[8], const float2& rx) {
+#pragma unroll
+            for (int it = 0; it < 8; it++) {
+                const int id = it * kThreads + tid;
+                *(fvec4*)(As + (id >> 4) * RS + (id & 15) * 4) = ra[it];
+            }
+            ((float2*)&xs4[tid / kGemmKB][0])[tid % kGemmKB] = rx;
+        };
+        fvec4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+        fvec4 ra[8];
+        float2 rx;
+        load_a(0, ra);
+        load_x(0, rx);
+        for (int kb = 0; kb < nkb; kb++) {
+            __syncthreads();  // the previous step's reads of As / xs4 are done
+            store_lds(ra, rx);
+            __syncthreads();
+            if (kb + 1 < nkb) {
+                load_a(kb + 1, ra);
+                load_x(kb + 1, rx);
+            }
+            const float* arow0 = As + (w * 32 + col) * RS + 4 * g;
+            const float* arow1 = arow0 + 16 * RS;
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const fvec4 a0 = *(const fvec4*)(arow0 + 16 * q);
+                const fvec4 a1 = *(const fvec4*)(arow1 + 16 * q);
+                const fvec4 xv = xs4[s_of][4 * q + g];  // complex columns 8q + 2g and 8q + 2g + 1
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    const float xr = (e >> 1) ? xv.z : xv.x, xi = (e >> 1) ? xv.w : xv.y;
+                    const float b = xprime<float>(xr, xi, e & 1, part);
+                    acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[e], b, acc0, 0, 0, 0);
+                    acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[e], b, acc1, 0, 0, 0);
+                }
+            }
+        }
+        const int s = s0 + s_of;
+#pragma unroll
+        for (int t = 0; t < 2; t++)
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const int r = rb * RW + w * 32 + 16 * t + 4 * g + i;
+                if (r < n && s < J.S) {
+                    const size_t o = 2 * ((size_t)s * n + r) + part;
+                    float v = t ? acc1[i] : acc0[i];
+                    if (J.add_identity) v += ((const float*)J.x)[o];
+                    ((float*)J.y)[o] = v;
+                }
+            }
+    }
+}
+
 constexpr int kPadC128 = 2;  // doubles of padding per row (c128: 64 + 2 doubles -> stride 132 dwords)
 
+// NT tiles of 8 right-hand sides per pass (NT = 2: 16 RHS; NT = 1: 8 RHS at
+// half the matrix-core work)
+template <int NT>
 __device__ inline void gemm_rows_c128(const MvDev& J, int rb) {
     constexpr int RW = 64, RS = 2 * kGemmKB + kPadC128;  // rows per workgroup, row stride (doubles)
     __shared__ double As[RW * RS];
@@ -268,7 +362,7 @@ __device__ inline void gemm_rows_c128(const MvDev& J, int rb) {
     const int nkb = (n + kGemmKB - 1) / kGemmKB;
     // 64 rows x 32 complex = 2048 chunks of 16 B (one complex each): thread loads 8
     // (chunk id = it * 256 + tid: 32 chunks per row, a wave covers 2 full rows)
-    for (int s0 = 0; s0 < J.S; s0 += kGemmS) {
+    for (int s0 = 0; s0 < J.S; s0 += 8 * NT) {
         auto load_a = [&](int kb, dvec2 (&ra)[8]) {
 #pragma unroll
             for (int it = 0; it < 8; it++) {
@@ -323,11 +417,13 @@ __device__ inline void gemm_rows_c128(const MvDev& J, int rb) {
                 for (int e = 0; e < 2; e++) {
                     const int t = 2 * q + e;
                     const double2 x0 = xs[s_of][8 * g + (t >> 1)];
-                    const double2 x1 = xs[8 + s_of][8 * g + (t >> 1)];
                     acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xprime<double>(x0.x, x0.y, t & 1, part), acc0, 0,
                                                                 0, 0);
-                    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xprime<double>(x1.x, x1.y, t & 1, part), acc1, 0,
-                                                                0, 0);
+                    if (NT == 2) {
+                        const double2 x1 = xs[8 + s_of][8 * g + (t >> 1)];
+                        acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[e], xprime<double>(x1.x, x1.y, t & 1, part),
+                                                                    acc1, 0, 0, 0);
+                    }
                 }
             }
         }
@@ -335,7 +431,7 @@ __device__ inline void gemm_rows_c128(const MvDev& J, int rb) {
         for (int i = 0; i < 4; i++) {
             const int r = rb * RW + w * 16 + 4 * i + g;
 #pragma unroll
-            for (int c = 0; c < 2; c++) {
+            for (int c = 0; c < NT; c++) {
                 const int s = s0 + 8 * c + s_of;
                 if (r < n && s < J.S) {
                     const size_t o = 2 * ((size_t)s * n + r) + part;
@@ -348,29 +444,38 @@ __device__ inline void gemm_rows_c128(const MvDev& J, int rb) {
     }
 }
 
-template <typename T>
+// SP right-hand sides per pass: 16 or 8
+template <typename T, int SP>
 __device__ inline void gemm_rows(const MvDev& J, int rb);
 template <>
-__device__ inline void gemm_rows<float>(const MvDev& J, int rb) {
+__device__ inline void gemm_rows<float, 16>(const MvDev& J, int rb) {
     gemm_rows_c64(J, rb);
 }
 template <>
-__device__ inline void gemm_rows<double>(const MvDev& J, int rb) {
-    gemm_rows_c128(J, rb);
+__device__ inline void gemm_rows<float, 8>(const MvDev& J, int rb) {
+    gemm_rows_c64_s8(J, rb);
+}
+template <>
+__device__ inline void gemm_rows<double, 16>(const MvDev& J, int rb) {
+    gemm_rows_c128<2>(J, rb);
+}
+template <>
+__device__ inline void gemm_rows<double, 8>(const MvDev& J, int rb) {
+    gemm_rows_c128<1>(J, rb);
 }
 
-template <typename T>
+template <typename T, int SP>
 __global__ __launch_bounds__(kThreads) void k_gemm(const MvDev* __restrict__ jobs, const SysDev* __restrict__ sys,
                                                    const int2* __restrict__ work) {
     const int2 wk = work[blockIdx.x];
     const MvDev J = jobs[wk.x];
     if (J.skip >= 0 && all_done(sys[J.skip])) return;
-    gemm_rows<T>(J, wk.y);
+    gemm_rows<T, SP>(J, wk.y);
 }
 
-template <typename T>
+template <typename T, int SP>
 __global__ __launch_bounds__(kThreads) void k_gemm1(MvDev J) {
-    gemm_rows<T>(J, blockIdx.x);
+    gemm_rows<T, SP>(J, blockIdx.x);
 }
 
 // batched: one workgroup per (job, row block) of the work list
@@ -401,7 +506,7 @@ const Variant kVariants[kTileClasses][kMaxVariants] = {
     {{8, 1, 2, "r8u2"}, {4, 1, 2, "r4u2"}, {8, 1, 4, "r8u4"}, {16, 1, 1, "r16u1"}, {4, 1, 4, "r4u4"}},
     {{4, 2, 2, "r4u2"}, {2, 2, 2, "r2u2"}, {4, 2, 4, "r4u4"}, {8, 2, 1, "r8u1"}, {0, 0, 0, nullptr}},
     {{2, 4, 2, "r2u2"}, {1, 4, 2, "r1u2"}, {2, 4, 4, "r2u4"}, {0, 16, 0, "mfma"}, {0, 0, 0, nullptr}},
-    {{0, 16, 0, "mfma"}, {1, 8, 2, "r1s8u2"}, {2, 8, 1, "r2s8u1"}, {0, 0, 0, nullptr}, {0, 0, 0, nullptr}},
+    {{0, 16, 0, "mfma"}, {1, 8, 2, "r1s8u2"}, {2, 8, 1, "r2s8u1"}, {0, 8, 0, "mfma8"}, {0, 0, 0, nullptr}},
 };
 int g_variant[2][kTileClasses] = {{0, 0, 0, 0}, {0, 0, 0, 0}};  // [es == 8][class]
 
@@ -422,9 +527,12 @@ int launch_variant(int c, int v, const MvDev* jobs, const SysDev* sys, const int
         else hipLaunchKernelGGL((k_gemv<T, R_, SG_, U_>), grid, block, 0, st, jobs, sys, work);      \
     } while (0)
     const Variant& V = kVariants[c][v];
-    if (V.R == 0) {
-        if (one) hipLaunchKernelGGL(k_gemm1<T>, grid, block, 0, st, *one);
-        else hipLaunchKernelGGL(k_gemm<T>, grid, block, 0, st, jobs, sys, work);
+    if (V.R == 0 && V.SG == 8) {
+        if (one) hipLaunchKernelGGL((k_gemm1<T, 8>), grid, block, 0, st, *one);
+        else hipLaunchKernelGGL((k_gemm<T, 8>), grid, block, 0, st, jobs, sys, work);
+    } else if (V.R == 0) {
+        if (one) hipLaunchKernelGGL((k_gemm1<T, 16>), grid, block, 0, st, *one);
+        else hipLaunchKernelGGL((k_gemm<T, 16>), grid, block, 0, st, jobs, sys, work);
     } else if (c == 0) {
         switch (v) {
             case 0: MPA_VAR(8, 1, 2); break;
@@ -549,6 +657,91 @@ extern "C" int mpa_autotune_matvec(int n, int nrhs, int dtype, const void* A, co
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    g_variant[es == 8][c] = rc == MPA_OK ? best : saved;
+    if (rc) return rc;
+    if (variant_out) *variant_out = best;
+    if (ms_out) *ms_out = best_ms;
+    return MPA_OK;
+}
+
+// Batched autotuner: every variant of the class of nrhs[0] timed on the
+// batched launch over all systems of that class (the launch shape of the
+// solvers: one workgroup per (system, row block)), the fastest kept.
+extern "C" int mpa_autotune_matvec_batched(int count, const int* n, const int* nrhs, int dtype, const void* const* A,
+                                           const void* const* x, void* const* y, int reps, int* variant_out,
+                                           double* ms_out, void* stream) {
+    if (count < 1 || !n || !nrhs || !A || !x || !y || reps < 1 || (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("autotune_matvec_batched: bad argument");
+        return MPA_ERR_ARG;
+    }
+    const int es = dtype == MPA_C128 ? 8 : 4;
+    const int c = gemv_class(nrhs[0] < 1 ? 1 : nrhs[0]);
+    std::vector<MvDev> jobs;
+    for (int i = 0; i < count; i++) {
+        if (n[i] < 2 || (n[i] & 1) || nrhs[i] < 1 || !A[i] || !x[i] || !y[i]) {
+            mpa_set_error("autotune_matvec_batched: bad system %d", i);
+            return MPA_ERR_ARG;
+        }
+        if (gemv_class(nrhs[i]) == c) jobs.push_back(MvDev{A[i], x[i], y[i], n[i], nrhs[i], 0, -1});
+    }
+    const int nv = nvariants(c);
+    size_t max_work = 0;
+    for (int v = 0; v < nv; v++) {
+        size_t w = 0;
+        for (const MvDev& J : jobs) w += (J.n + rows_of_variant(c, v, es) - 1) / rows_of_variant(c, v, es);
+        max_work = w > max_work ? w : max_work;
+    }
+    hipStream_t st = (hipStream_t)stream;
+    MvDev* d_jobs = nullptr;
+    int2* d_work = nullptr;
+    MPA_HIP_CHECK(hipMalloc(&d_jobs, sizeof(MvDev) * jobs.size()));
+    if (hipMalloc(&d_work, sizeof(int2) * max_work) != hipSuccess) {
+        (void)hipFree(d_jobs);
+        mpa_set_error("autotune_matvec_batched: out of device memory");
+        return MPA_ERR_OOM;
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = MPA_OK;
+    if (hipMemcpyAsync(d_jobs, jobs.data(), sizeof(MvDev) * jobs.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+        hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+        rc = MPA_ERR_HIP;
+    const int saved = g_variant[es == 8][c];
+    int best = saved;
+    double best_ms = 1e30;
+    std::vector<int2> work;
+    for (int v = 0; v < nv && rc == MPA_OK; v++) {
+        const int rows = rows_of_variant(c, v, es);
+        work.clear();
+        for (int j = 0; j < (int)jobs.size(); j++)
+            for (int b = 0; b < (jobs[j].n + rows - 1) / rows; b++) work.push_back(make_int2(j, b));
+        if (hipMemcpyAsync(d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            rc = MPA_ERR_HIP;
+            break;
+        }
+        g_variant[es == 8][c] = v;
+        int2* wl[kTileClasses] = {nullptr, nullptr, nullptr, nullptr};
+        int nw[kTileClasses] = {0, 0, 0, 0};
+        wl[c] = d_work;
+        nw[c] = (int)work.size();
+        rc = gemv_launch(es, d_jobs, nullptr, wl, nw, st);  // warm-up
+        if (rc) break;
+        (void)hipEventRecord(e0, st);
+        for (int r = 0; r < reps && rc == MPA_OK; r++) rc = gemv_launch(es, d_jobs, nullptr, wl, nw, st);
+        (void)hipEventRecord(e1, st);
+        if (hipEventSynchronize(e1) != hipSuccess) rc = MPA_ERR_HIP;
+        float ms = 0;
+        if (rc == MPA_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = MPA_ERR_HIP;
+        if (rc == MPA_OK && ms / reps < best_ms) {
+            best_ms = ms / reps;
+            best = v;
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(d_work);
+    (void)hipFree(d_jobs);
     g_variant[es == 8][c] = rc == MPA_OK ? best : saved;
     if (rc) return rc;
     if (variant_out) *variant_out = best;

@@ -155,6 +155,30 @@ def autotune_matvec(A, nrhs=1, reps=3):
     return lib.mpa_matvec_variant_name(code, nrhs, v.value).decode(), ms.value
 
 
+def autotune_matvec_batched(matrices, nrhs, reps=3):
+    """Time every matvec tile variant of the RHS-count class of `nrhs` on one
+    batched launch over `matrices` (the launch shape of the batched solvers)
+    and keep the fastest for the process.  Returns (variant name, ms per
+    batched launch)."""
+    lib = L.load()
+    mats = [A for A in matrices]
+    if not mats or any(A.dtype != mats[0].dtype for A in mats):
+        raise ValueError("autotune_matvec_batched: one dtype for every matrix")
+    code = DTYPE_CODE[mats[0].dtype]
+    g = torch.Generator(device=mats[0].device).manual_seed(0)
+    xs = [torch.randn((nrhs, A.shape[0]), dtype=A.dtype, device=A.device, generator=g) for A in mats]
+    ys = [torch.empty_like(x) for x in xs]
+    cnt = len(mats)
+    arr = lambda ts: (ctypes.c_void_p * cnt)(*[t.data_ptr() for t in ts])
+    v = ctypes.c_int(0)
+    ms = ctypes.c_double(0)
+    L.check(lib.mpa_autotune_matvec_batched(cnt, (ctypes.c_int * cnt)(*[A.shape[0] for A in mats]),
+                                            (ctypes.c_int * cnt)(*([int(nrhs)] * cnt)), code, arr(mats), arr(xs),
+                                            arr(ys), int(reps), ctypes.byref(v), ctypes.byref(ms), _stream()),
+            "autotune_matvec_batched")
+    return lib.mpa_matvec_variant_name(code, nrhs, v.value).decode(), ms.value
+
+
 def autotune_assembly(positions, mie, k, out, reps=2):
     """Time every assembly tile variant on this cluster (`out` [n, n] is
     overwritten) and keep the fastest for the process.  Returns

@@ -106,6 +106,9 @@ def test_cluster_batch_matches_single_clusters(dev):
     assert {w.lmax for w in ws} >= {4} and min(w.lmax for w in ws) <= 3
     for precision, rtol in (("fp64", 1e-9), ("mixed", 1e-8), ("matfree", 1e-9), ("mixed_matfree", 1e-8)):
         B = ClusterBatch(ws, precision=precision, matfree_min_n=1000)  # the larger clusters run matrix-free
+        if precision in ("fp64", "mixed"):
+            tuned = B.autotune(reps=1)  # the batched autotuner's pick serves the solve below
+            assert tuned and all(name and ms > 0 for name, ms in tuned.values())
         B.solve()
         assert B.converged()
         res = B.host_results()

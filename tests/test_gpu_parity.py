@@ -303,7 +303,7 @@ def test_every_matvec_variant_and_autotune(dev):
     A = rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n))
     for dtype, code, tol in ((dev.C128, L.MPA_C128, 1e-12), (dev.C64, L.MPA_C64, 1e-5)):
         Ad = dev.to_device(A, dtype)
-        for S in (1, 2, 4, 9):
+        for S in (1, 2, 4, 8, 9, 16):
             x = rng.normal(size=(S, n)) + 1j * rng.normal(size=(S, n))
             ref = x @ A.T
             xd = dev.to_device(x, dtype)
@@ -321,6 +321,42 @@ def test_every_matvec_variant_and_autotune(dev):
             assert name and ms > 0
             y = dev.matvec(Ad, xd).cpu().numpy()
             assert np.max(np.abs(y - ref)) <= tol * np.abs(ref).max()
+
+
+def test_batched_matvec_autotune(dev):
+    """The batched autotuner times the variants of the class of nrhs[0] on one
+    launch over several systems (one of another class left out) and leaves a
+    variant in place whose products are right."""
+    import ctypes
+    import miepy_amd._lib as L
+    lib = L.load()
+    rng = np.random.default_rng(8)
+    for dtype, code, tol in ((dev.C128, L.MPA_C128, 1e-12), (dev.C64, L.MPA_C64, 1e-5)):
+        sizes = (64, 250, 610)
+        A = [rng.normal(size=(n, n)) + 1j * rng.normal(size=(n, n)) for n in sizes]
+        Ad = [dev.to_device(a, dtype) for a in A]
+        name, ms = dev.autotune_matvec_batched(Ad, 8, reps=2)
+        assert name == lib.mpa_matvec_variant_name(code, 8, lib.mpa_matvec_variant(code, 8, -1)).decode() and ms > 0
+        for a, ad in zip(A, Ad):
+            x = rng.normal(size=(8, a.shape[0])) + 1j * rng.normal(size=(8, a.shape[0]))
+            y = dev.matvec(ad, dev.to_device(x, dtype)).cpu().numpy()
+            ref = x @ a.T
+            assert np.max(np.abs(y - ref)) <= tol * np.abs(ref).max()
+        # a system of another RHS class is left out of the timing
+        x1 = [torch_like(ad, 8) for ad in Ad[:2]] + [torch_like(Ad[2], 1)]
+        y1 = [t.clone() for t in x1]
+        v, t = ctypes.c_int(-1), ctypes.c_double(0)
+        arr = lambda ts: (ctypes.c_void_p * 3)(*[q.data_ptr() for q in ts])
+        L.check(lib.mpa_autotune_matvec_batched(3, (ctypes.c_int * 3)(*sizes), (ctypes.c_int * 3)(8, 8, 1), code,
+                                                arr(Ad), arr(x1), arr(y1), 1, ctypes.byref(v), ctypes.byref(t), None),
+                "autotune_matvec_batched")
+        assert v.value >= 0 and t.value > 0
+        assert np.array_equal(y1[2].cpu().numpy(), x1[2].cpu().numpy())  # untouched
+
+
+def torch_like(A, S):
+    import torch
+    return torch.randn((S, A.shape[0]), dtype=A.dtype, device=A.device)
 
 
 def test_assembly_autotune_variants_match_oracle(dev, oracle):
