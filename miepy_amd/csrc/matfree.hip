@@ -291,8 +291,8 @@ constexpr int kFacMaxLmax = 3;
 // kernel treats 2, 3 as 0); c128: 0 and 1 the slot-mapped kernel (k_matfree_mf64s) with one or two
 // accumulator chains (one is the faster at rand1000: 0.34 ms per product against 0.43 ms for the
 // best per-lane-select variant), 2 and 3 the per-lane-select kernel (k_matfree_mf64), two or one.
-int g_mf_variant[2] = {0, 0};
-constexpr int kMfVariants[2] = {4, 4};
+int g_mf_variant[2] = {4, 0};  // complex64: column pairs (lmax 4 runs variant 0); FP64: slot-mapped KS 1
+constexpr int kMfVariants[2] = {6, 4};  // complex64: 4, 5 = column pairs (lmax <= 3), KS 1 and 2
 constexpr int kFacMinN = 128;  // smaller clusters keep the LDS kernel (too few lanes)
 
 constexpr int fac_n_of(int r) {
@@ -1090,6 +1090,249 @@ __device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const d
         }
 }
 
+// The B operands of a column pair (j, j + 1) with the special functions of
+// each pair computed once: lane half 0 evaluates its row particle against
+// column j, half 1 against column j + 1 (`jown`), each forming both the +m
+// slots (half 0's share of a column) and the -m slots (half 1's share, zero
+// for m = 0) of its own pair; one v_permlane32_swap per dword then gives the
+// lanes of each half their share of both columns: (P, M) -> (column j:
+// P of half 0 | M of half 0, column j + 1: P of half 1 | M of half 1).  The
+// same for g_ij.  Same values as hle_f16 for each column.
+typedef unsigned u4v __attribute__((ext_vector_type(4)));
+
+template <int L>
+__device__ __forceinline__ void hle_f16_pair(double xi, double yi, double zi, const double* __restrict__ pos, int jown,
+                                             double k, float kinv, bool on, h8v (&bh0)[F16Shape<L>::NKB],
+                                             h8v (&bl0)[F16Shape<L>::NKB], h8v (&bh1)[F16Shape<L>::NKB],
+                                             h8v (&bl1)[F16Shape<L>::NKB], float& ginv0, float& ginv1) {
+    using Sh = F16Shape<L>;
+    constexpr int PM = Sh::PM, NKB = Sh::NKB;
+    double dx = xi - pos[3 * jown], dy = yi - pos[3 * jown + 1], dz = zi - pos[3 * jown + 2];
+    dx = on ? dx : 1.0;
+    dy = on ? dy : 0.0;
+    dz = on ? dz : 0.0;
+    const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+    const double zz = k * rad;
+    const double qd = rint(zz * 0.63661977236758134308);
+    double rr = fma(-qd, 1.5707963267948966, zz);
+    rr = fma(-qd, 6.123233995736766e-17, rr);
+    const float r = (float)rr, r2 = r * r;
+    const float S = r + r * r2 * (-1.f / 6.f + r2 * (1.f / 120.f + r2 * (-1.f / 5040.f + r2 * (1.f / 362880.f))));
+    const float C = 1.f + r2 * (-0.5f + r2 * (1.f / 24.f + r2 * (-1.f / 720.f + r2 * (1.f / 40320.f -
+                                                                                       r2 * (1.f / 3628800.f)))));
+    const int quad = (int)qd & 3;
+    const float s0 = (quad & 1) ? C : S, c0 = (quad & 1) ? S : C;
+    const float sz = (quad & 2) ? -s0 : s0, cz = ((quad + 1) & 2) ? -c0 : c0;
+    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+    const float rinv = __builtin_amdgcn_rcpf((float)rad);
+    const float rz = rinv * kinv;
+    float hx[PM + 1], hy[PM + 1];
+    hx[0] = sz * rz;
+    hy[0] = -cz * rz;
+    hx[1] = hx[0] * rz + hy[0];
+    hy[1] = hy[0] * rz - hx[0];
+#pragma unroll
+    for (int l = 2; l <= PM; l++) {
+        hx[l] = (2 * l - 1) * rz * hx[l - 1] - hx[l - 2];
+        hy[l] = (2 * l - 1) * rz * hy[l - 1] - hy[l - 2];
+    }
+    const float hmax = fmaxf(fabsf(hx[PM]), fabsf(hy[PM]));
+    const float g = on ? __builtin_amdgcn_rcpf(hmax) : 0.f;
+    const float ginv = on ? hmax : 0.f;
+#pragma unroll
+    for (int l = 0; l <= PM; l++) {
+        hx[l] *= g;
+        hy[l] *= g;
+    }
+    const float rho2 = fx * fx + fy * fy;
+    const float rhoinv = rho2 > 0.f ? __builtin_amdgcn_rsqf(rho2) : 0.f;
+    const float ct = fz * rinv;
+    const float st = rho2 * rhoinv * rinv;
+    const float e1x = rho2 > 0.f ? fx * rhoinv : 1.f;
+    const float e1y = fy * rhoinv;
+    float vp[NKB * 8], vm[NKB * 8];
+#pragma unroll
+    for (int t = 0; t < NKB * 8; t++) vp[t] = vm[t] = 0.f;
+    float ex = 1.f, ey = 0.f;
+    float pmm = 1.f;
+    int u = 0;
+#pragma unroll
+    for (int m = 0; m <= PM; m++) {
+        if (m > 0) {
+            const float bx = m == 1 ? 1.f : ex, by = m == 1 ? 0.f : ey;
+            ex = bx * e1x - by * e1y;
+            ey = bx * e1y + by * e1x;
+            pmm *= st * (float)__builtin_sqrt((2.0 * m - 1) / (2.0 * m));
+        }
+        float p2 = 0.f, p1 = 0.f;
+#pragma unroll
+        for (int l = 0; l <= PM; l++) {
+            if (l < m) continue;
+            float P;
+            if (l == m) {
+                P = pmm;
+            } else if (l == m + 1) {
+                P = ct * (float)__builtin_sqrt(2.0 * m + 1) * pmm;
+            } else {
+                const float A = (float)((2.0 * l - 1) / __builtin_sqrt((double)(l - m) * (l + m)));
+                const float B = (float)__builtin_sqrt((double)(l + m - 1) * (l - m - 1) / ((double)(l - m) * (l + m)));
+                P = A * (ct * p1) - B * p2;
+            }
+            p2 = p1;
+            p1 = P;
+            const float ar = hx[l] * P, ai = hy[l] * P;
+            vp[2 * u] = ar * ex - ai * ey;
+            vp[2 * u + 1] = ar * ey + ai * ex;
+            if (m > 0) {  // e^{-i m phi} = conj(e^{i m phi})
+                vm[2 * u] = ar * ex + ai * ey;
+                vm[2 * u + 1] = ai * ex - ar * ey;
+            }
+            u++;
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < NKB; kb++) {
+#pragma unroll
+        for (int e2 = 0; e2 < 4; e2++) {
+            const f2v a = {vp[kb * 8 + 2 * e2], vp[kb * 8 + 2 * e2 + 1]};
+            const f2v b = {vm[kb * 8 + 2 * e2], vm[kb * 8 + 2 * e2 + 1]};
+            h2v ahi, alo, bhi, blo;
+            split_hilo(a, ahi, alo);
+            split_hilo(b, bhi, blo);
+            bh0[kb][2 * e2] = ahi.x;
+            bh0[kb][2 * e2 + 1] = ahi.y;
+            bl0[kb][2 * e2] = alo.x;
+            bl0[kb][2 * e2 + 1] = alo.y;
+            bh1[kb][2 * e2] = bhi.x;
+            bh1[kb][2 * e2 + 1] = bhi.y;
+            bl1[kb][2 * e2] = blo.x;
+            bl1[kb][2 * e2 + 1] = blo.y;
+        }
+        u4v ph = __builtin_bit_cast(u4v, bh0[kb]), mh = __builtin_bit_cast(u4v, bh1[kb]);
+        u4v pl = __builtin_bit_cast(u4v, bl0[kb]), ml = __builtin_bit_cast(u4v, bl1[kb]);
+#pragma unroll
+        for (int d = 0; d < 4; d++) {
+            const auto sh = __builtin_amdgcn_permlane32_swap(ph[d], mh[d], false, false);
+            ph[d] = sh[0];
+            mh[d] = sh[1];
+            const auto sl = __builtin_amdgcn_permlane32_swap(pl[d], ml[d], false, false);
+            pl[d] = sl[0];
+            ml[d] = sl[1];
+        }
+        bh0[kb] = __builtin_bit_cast(h8v, ph);
+        bh1[kb] = __builtin_bit_cast(h8v, mh);
+        bl0[kb] = __builtin_bit_cast(h8v, pl);
+        bl1[kb] = __builtin_bit_cast(h8v, ml);
+    }
+    const unsigned gb = __builtin_bit_cast(unsigned, ginv);
+    const auto sg = __builtin_amdgcn_permlane32_swap(gb, gb, false, false);
+    ginv0 = __builtin_bit_cast(float, (unsigned)sg[0]);
+    ginv1 = __builtin_bit_cast(float, (unsigned)sg[1]);
+}
+
+// Column pairs: as k_matfree_mf16 with KS chains and no interleaving, the B
+// operands of columns j and j + 1 formed together at even steps (the second
+// kept in registers for the odd step), so each pair's special functions are
+// evaluated once instead of once per lane half.
+template <int L, int KS>
+__global__ __launch_bounds__(kThreads, 2) void k_matfree_mf16p(MfJob J, const SysDev* __restrict__ sys, int nchunks,
+                                                            int chunk, const f4v* __restrict__ Wh,
+                                                            const float* __restrict__ tau,
+                                                            float2* __restrict__ part32) {
+    using Sh = F16Shape<L>;
+    constexpr int NC = Sh::NC, NT = Sh::NT, NKB = Sh::NKB, PER = Sh::PER;
+    constexpr int PFN = (PER + kThreads - 1) / kThreads;
+    __shared__ f4v Ws[2][PER];
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int N = J.N;
+    const int s = blockIdx.y;
+    const int groups = (N + 127) / 128;
+    const int v = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int c = v / groups;
+    const int group = v - c * groups;
+    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);
+    if (jbeg >= jend) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int kh = lane >> 5, col = lane & 31;
+    const int i = group * 128 + w * 32 + col;
+    const bool live = i < N;
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const f4v* Wall = Wh + (size_t)s * N * PER;
+    const float* tau_s = tau + (size_t)s * N;
+    const float kinv = (float)(1.0 / J.k);
+    f16v tot[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int q = 0; q < 16; q++) tot[t][q] = 0.f;
+    f4v pf[PFN];
+    wx_fetch<PER, PFN>(pf, Wall + (size_t)jbeg * PER, tid);
+    wx_put<PER, PFN>(pf, Ws[0], tid);
+    __syncthreads();
+    h8v bh[NKB], bl[NKB], bh1[NKB], bl1[NKB];
+    float ginv = 0.f, ginv1 = 0.f;
+    for (int j = jbeg; j < jend; j++) {
+        const int buf = (j - jbeg) & 1;
+        const int jn = min(j + 1, jend - 1);
+        wx_fetch<PER, PFN>(pf, Wall + (size_t)jn * PER, tid);
+        if (buf == 0) {
+            const int jo = j + kh;  // this half's column; past the chunk at an odd tail
+            const bool has = jo < jend;
+            const int jc = has ? jo : j;
+            hle_f16_pair<L>(xi, yi, zi, J.pos, jc, J.k, kinv, live && has && jc != ii, bh, bl, bh1, bl1, ginv, ginv1);
+        } else {
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++) {
+                bh[kb] = bh1[kb];
+                bl[kb] = bl1[kb];
+            }
+            ginv = ginv1;
+        }
+        f16v acc[NT][KS];
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int k = 0; k < KS; k++)
+#pragma unroll
+                for (int q = 0; q < 16; q++) acc[t][k][q] = 0.f;
+#pragma unroll
+        for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+            for (int t = 0; t < NT; t++) {
+                const h8v ah = __builtin_bit_cast(h8v, Ws[buf][(t * NKB + kb) * 64 + lane]);
+                const h8v al = __builtin_bit_cast(h8v, Ws[buf][((NT + t) * NKB + kb) * 64 + lane]);
+                f16v& a = acc[t][kb % KS];
+                a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[kb], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[kb], a, 0, 0, 0);
+                a = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[kb], a, 0, 0, 0);
+            }
+        const float sc = tau_s[j] * ginv;
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                float a = acc[t][0][q];
+#pragma unroll
+                for (int k = 1; k < KS; k++) a += acc[t][k][q];
+                tot[t][q] = fmaf(sc, a, tot[t][q]);
+            }
+        wx_put<PER, PFN>(pf, Ws[buf ^ 1], tid);
+        __syncthreads();
+    }
+    if (live) {
+        const size_t n = (size_t)NC * N;
+        float* out = (float*)part32 + 2 * (((size_t)c * J.S + s) * n + (size_t)i * NC);
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int q = 0; q < 16; q++) {
+                const int Rr = 32 * t + 8 * (q >> 2) + 4 * kh + (q & 3);  // real output column 2 cc + part
+                if (Rr < 2 * NC) out[Rr] = tot[t][q];
+            }
+    }
+}
+
 // Partial rows part32[chunk][s][row] (complex64); blockIdx.y is the RHS.  Per
 // column j: prefetch of the next Wh block (global -> registers), the pair's
 // B operands, 3 NKB NT FP16 MFMAs into KS chains per tile, the block sum
@@ -1694,6 +1937,18 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
                                        (const f4v*)Whp, taup, part32); break;                                       \
             case 3: hipLaunchKernelGGL((k_matfree_mf16<L_, 1, 12>), grid, block, 0, st, J, sys, nch, chunk,          \
                                        (const f4v*)Whp, taup, part32); break;                                       \
+            case 4:                                                                                         \
+            case 5:                                                                                         \
+                if (L_ <= 3 && g_mf_variant[0] == 4)                                                        \
+                    hipLaunchKernelGGL((k_matfree_mf16p<(L_ < 3 ? L_ : 3), 1>), grid, block, 0, st, J, sys, nch,     \
+                                       chunk, (const f4v*)Whp, taup, part32);                                       \
+                else if (L_ <= 3)                                                                           \
+                    hipLaunchKernelGGL((k_matfree_mf16p<(L_ < 3 ? L_ : 3), 2>), grid, block, 0, st, J, sys, nch,     \
+                                       chunk, (const f4v*)Whp, taup, part32);                                       \
+                else /* lmax 4: the pair kernel's operands would not fit the registers */                   \
+                    hipLaunchKernelGGL((k_matfree_mf16<L_, 1, 0>), grid, block, 0, st, J, sys, nch, chunk,   \
+                                       (const f4v*)Whp, taup, part32);                                              \
+                break;                                                                                      \
             default: hipLaunchKernelGGL((k_matfree_mf16<L_, 1, 0>), grid, block, 0, st, J, sys, nch, chunk,          \
                                         (const f4v*)Whp, taup, part32);                                             \
         }                                                                                                           \
