@@ -347,6 +347,7 @@ struct Plan {
     int2* d_mv_work[kTileClasses] = {};
     int n_mv_work[kTileClasses] = {};
     Scal* d_scal_out = nullptr;  // gathered scalars, [sum(nrhs)]
+    std::vector<char> img;       // host image of the descriptor region, uploaded in one copy
 };
 
 size_t plan_bytes(int nsys, const mpa_system* s, int dtype) {
@@ -452,17 +453,22 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
         p += align256(sizeof(int2) * P.mv_work[c].size());
     }
     P.d_scal_out = (Scal*)p;
-    MPA_HIP_CHECK(hipMemcpyAsync(P.d_sys, P.sys.data(), sizeof(SysDev) * nsys, hipMemcpyHostToDevice, st));
-    MPA_HIP_CHECK(hipMemcpyAsync(P.d_vec, P.vec_work.data(), sizeof(int3) * P.vec_work.size(), hipMemcpyHostToDevice, st));
-    MPA_HIP_CHECK(hipMemcpyAsync(P.d_scal, P.scal_work.data(), sizeof(int2) * P.scal_work.size(), hipMemcpyHostToDevice, st));
-    MPA_HIP_CHECK(hipMemcpyAsync(P.d_mv_ps, P.mv_ps.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
-    MPA_HIP_CHECK(hipMemcpyAsync(P.d_mv_st, P.mv_st.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
-    MPA_HIP_CHECK(hipMemcpyAsync(P.d_mv_xt, P.mv_xt.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
-    for (int c = 0; c < kTileClasses; c++)
-        if (P.n_mv_work[c])
-            MPA_HIP_CHECK(hipMemcpyAsync(P.d_mv_work[c], P.mv_work[c].data(), sizeof(int2) * P.n_mv_work[c],
-                                         hipMemcpyHostToDevice, st));
-    // the host vectors are staged by the runtime before the calls return
+    // every descriptor array and work list in one host image with the device
+    // layout, sent in one copy (one pageable transfer instead of one per array)
+    const size_t nbytes = (size_t)(p - (char*)scratch);
+    P.img.assign(nbytes, 0);
+    auto put = [&](const void* dst, const void* src, size_t bytes) {
+        if (bytes) std::memcpy(P.img.data() + ((const char*)dst - (const char*)scratch), src, bytes);
+    };
+    put(P.d_sys, P.sys.data(), sizeof(SysDev) * nsys);
+    put(P.d_vec, P.vec_work.data(), sizeof(int3) * P.vec_work.size());
+    put(P.d_scal, P.scal_work.data(), sizeof(int2) * P.scal_work.size());
+    put(P.d_mv_ps, P.mv_ps.data(), sizeof(MvDev) * nsys);
+    put(P.d_mv_st, P.mv_st.data(), sizeof(MvDev) * nsys);
+    put(P.d_mv_xt, P.mv_xt.data(), sizeof(MvDev) * nsys);
+    for (int c = 0; c < kTileClasses; c++) put(P.d_mv_work[c], P.mv_work[c].data(), sizeof(int2) * P.n_mv_work[c]);
+    // the host image is staged by the runtime before the call returns
+    MPA_HIP_CHECK(hipMemcpyAsync(scratch, P.img.data(), nbytes, hipMemcpyHostToDevice, st));
     return MPA_OK;
 }
 

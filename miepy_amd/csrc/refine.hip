@@ -11,6 +11,7 @@
 // the FP64 residual matvecs of all systems run as one batched c128 matvec.
 
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "gemv.hpp"
@@ -219,12 +220,20 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
         p += align256(sizeof(int2) * mw[c].size());
     }
     void* inner_scratch = p;
-    MPA_HIP_CHECK(hipMemcpyAsync(d_ref, ref.data(), sizeof(RefDev) * nsys, hipMemcpyHostToDevice, st));
-    MPA_HIP_CHECK(hipMemcpyAsync(d_mv, mv.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
-    MPA_HIP_CHECK(hipMemcpyAsync(d_vw, vw.data(), sizeof(int3) * vw.size(), hipMemcpyHostToDevice, st));
-    MPA_HIP_CHECK(hipMemcpyAsync(d_cw, cw.data(), sizeof(int2) * cw.size(), hipMemcpyHostToDevice, st));
-    for (int c = 0; c < kTileClasses; c++)
-        if (n_mw[c]) MPA_HIP_CHECK(hipMemcpyAsync(d_mw[c], mw[c].data(), sizeof(int2) * n_mw[c], hipMemcpyHostToDevice, st));
+    {  // descriptors and work lists in one host image with the device layout, one copy
+        const size_t nbytes = (size_t)(p - (char*)scratch);
+        std::vector<char> img(nbytes, 0);
+        auto put = [&](const void* dst, const void* src, size_t bytes) {
+            if (bytes) std::memcpy(img.data() + ((const char*)dst - (const char*)scratch), src, bytes);
+        };
+        put(d_ref, ref.data(), sizeof(RefDev) * nsys);
+        put(d_mv, mv.data(), sizeof(MvDev) * nsys);
+        put(d_vw, vw.data(), sizeof(int3) * vw.size());
+        put(d_cw, cw.data(), sizeof(int2) * cw.size());
+        for (int c = 0; c < kTileClasses; c++) put(d_mw[c], mw[c].data(), sizeof(int2) * n_mw[c]);
+        // pageable source: staged by the runtime before the call returns
+        MPA_HIP_CHECK(hipMemcpyAsync(scratch, img.data(), nbytes, hipMemcpyHostToDevice, st));
+    }
 
     const int nv = (int)vw.size(), nc = (int)cw.size();
     hipLaunchKernelGGL(k_ref_start, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
