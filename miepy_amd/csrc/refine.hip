@@ -10,6 +10,7 @@
 // inner solves of all systems run as one batch of the complex64 engine;
 // the FP64 residual matvecs of all systems run as one batched c128 matvec.
 
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include <vector>
@@ -76,6 +77,25 @@ __global__ __launch_bounds__(kThreads) void k_ref_norms(const RefDev* __restrict
         __syncthreads();
     }
     if (threadIdx.x == 0) d.norms[s] = red[0];
+}
+
+// Per-RHS values between the systems' records and one flat array in
+// (system, RHS) order (the order of the cw list): mode 0 gathers norms into
+// flat[0, total); mode 1 scatters scale = flat[total + k], unscale =
+// flat[2 total + k]; mode 2 scatters unscale only.  One copy of the flat
+// array then replaces one copy per system.
+__global__ __launch_bounds__(kThreads) void k_ref_flat(const RefDev* __restrict__ sys, const int2* __restrict__ cw,
+                                                       int total, double* __restrict__ flat, int mode) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= total) return;
+    const int2 c2 = cw[k];
+    const RefDev& d = sys[c2.x];
+    if (mode == 0) {
+        flat[k] = d.norms[c2.y];
+    } else {
+        if (mode == 1) d.scale[c2.y] = flat[total + k];
+        d.unscale[c2.y] = flat[2 * total + k];
+    }
 }
 
 // r32 = r * scale, rounded to complex64
@@ -219,6 +239,8 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
         n_mw[c] = (int)mw[c].size();
         p += align256(sizeof(int2) * mw[c].size());
     }
+    double* d_flat = (double*)p;  // [3 total]: gathered norms, scale, unscale
+    p += align256(sizeof(double) * 3 * total);
     void* inner_scratch = p;
     {  // descriptors and work lists in one host image with the device layout, one copy
         const size_t nbytes = (size_t)(p - (char*)scratch);
@@ -244,15 +266,15 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
     std::vector<int> outer(total, 0), inner_it(total, 0), status(total, MPA_NOT_CONVERGED);
     std::vector<int> it(total), ist(total);
     std::vector<double> ires(total);
+    const unsigned flat_grid = (unsigned)((total + kThreads - 1) / kThreads);
     auto read_norms = [&](std::vector<double>& out) -> int {
-        int k = 0;
-        for (int i = 0; i < nsys; i++) {
-            MPA_HIP_CHECK(hipMemcpyAsync(&out[k], ref[i].norms, sizeof(double) * ref[i].S, hipMemcpyDeviceToHost, st));
-            k += ref[i].S;
-        }
+        hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 0);
+        MPA_LAUNCH_CHECK();
+        MPA_HIP_CHECK(hipMemcpyAsync(out.data(), d_flat, sizeof(double) * total, hipMemcpyDeviceToHost, st));
         MPA_HIP_CHECK(hipStreamSynchronize(st));
         return MPA_OK;
     };
+    std::vector<double> flat_up(2 * (size_t)total);
     int rc = read_norms(bn2);
     if (rc) return rc;
     rn2 = bn2;
@@ -286,12 +308,12 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             inner[i].tol = std::isfinite(need) ? std::max(R[i].inner_tol, need) : R[i].inner_tol;
             k += ref[i].S;
         }
-        k = 0;
-        for (int i = 0; i < nsys; i++) {
-            MPA_HIP_CHECK(hipMemcpyAsync(ref[i].scale, &sc[k], sizeof(double) * ref[i].S, hipMemcpyHostToDevice, st));
-            MPA_HIP_CHECK(hipMemcpyAsync(ref[i].unscale, &usc[k], sizeof(double) * ref[i].S, hipMemcpyHostToDevice, st));
-            k += ref[i].S;
-        }
+        std::copy(sc.begin(), sc.end(), flat_up.begin());
+        std::copy(usc.begin(), usc.end(), flat_up.begin() + total);
+        // pageable source: staged by the runtime before the call returns
+        MPA_HIP_CHECK(hipMemcpyAsync(d_flat + total, flat_up.data(), sizeof(double) * 2 * total, hipMemcpyHostToDevice,
+                                     st));
+        hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 1);
         hipLaunchKernelGGL(k_ref_to32, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         MPA_LAUNCH_CHECK();
         rc = mpa_internal_solve_batch(nsys, inner.data(), MPA_C64, inner_scratch, it.data(), ires.data(), ist.data(), 1,
@@ -332,12 +354,9 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             }
         }
         if (any_revert) {
-            k = 0;
-            for (int i = 0; i < nsys; i++) {
-                MPA_HIP_CHECK(hipMemcpyAsync(ref[i].unscale, &usc[k], sizeof(double) * ref[i].S,
-                                             hipMemcpyHostToDevice, st));
-                k += ref[i].S;
-            }
+            MPA_HIP_CHECK(hipMemcpyAsync(d_flat + 2 * total, usc.data(), sizeof(double) * total, hipMemcpyHostToDevice,
+                                         st));
+            hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 2);
             hipLaunchKernelGGL(k_ref_accum, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
             MPA_LAUNCH_CHECK();
             MPA_HIP_CHECK(hipStreamSynchronize(st));  // usc is read by the copies above
@@ -680,7 +699,7 @@ size_t refine_scratch_bytes(int nsys, const mpa_refine_system* R) {
     }
     return align256(sizeof(RefDev) * nsys) + align256(sizeof(MvDev) * nsys) + align256(sizeof(int3) * vw) +
            align256(sizeof(int2) * cw) + kTileClasses * 256 + align256(sizeof(int2) * mw) +
-           mpa_batch_scratch_bytes(nsys, inner.data(), MPA_C64);
+           align256(sizeof(double) * 3 * cw) + mpa_batch_scratch_bytes(nsys, inner.data(), MPA_C64);
 }
 
 }  // namespace
