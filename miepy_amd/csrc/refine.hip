@@ -267,10 +267,15 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
     std::vector<int> it(total), ist(total);
     std::vector<double> ires(total);
     const unsigned flat_grid = (unsigned)((total + kThreads - 1) / kThreads);
+    // one system: its own arrays directly; several: one flat array through the gather/scatter kernel
+    const bool flat = nsys > 1;
     auto read_norms = [&](std::vector<double>& out) -> int {
-        hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 0);
-        MPA_LAUNCH_CHECK();
-        MPA_HIP_CHECK(hipMemcpyAsync(out.data(), d_flat, sizeof(double) * total, hipMemcpyDeviceToHost, st));
+        if (flat) {
+            hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 0);
+            MPA_LAUNCH_CHECK();
+        }
+        MPA_HIP_CHECK(hipMemcpyAsync(out.data(), flat ? d_flat : ref[0].norms, sizeof(double) * total,
+                                     hipMemcpyDeviceToHost, st));
         MPA_HIP_CHECK(hipStreamSynchronize(st));
         return MPA_OK;
     };
@@ -308,12 +313,17 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             inner[i].tol = std::isfinite(need) ? std::max(R[i].inner_tol, need) : R[i].inner_tol;
             k += ref[i].S;
         }
-        std::copy(sc.begin(), sc.end(), flat_up.begin());
-        std::copy(usc.begin(), usc.end(), flat_up.begin() + total);
-        // pageable source: staged by the runtime before the call returns
-        MPA_HIP_CHECK(hipMemcpyAsync(d_flat + total, flat_up.data(), sizeof(double) * 2 * total, hipMemcpyHostToDevice,
-                                     st));
-        hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 1);
+        if (flat) {
+            std::copy(sc.begin(), sc.end(), flat_up.begin());
+            std::copy(usc.begin(), usc.end(), flat_up.begin() + total);
+            // pageable source: staged by the runtime before the call returns
+            MPA_HIP_CHECK(hipMemcpyAsync(d_flat + total, flat_up.data(), sizeof(double) * 2 * total,
+                                         hipMemcpyHostToDevice, st));
+            hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 1);
+        } else {
+            MPA_HIP_CHECK(hipMemcpyAsync(ref[0].scale, sc.data(), sizeof(double) * total, hipMemcpyHostToDevice, st));
+            MPA_HIP_CHECK(hipMemcpyAsync(ref[0].unscale, usc.data(), sizeof(double) * total, hipMemcpyHostToDevice, st));
+        }
         hipLaunchKernelGGL(k_ref_to32, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         MPA_LAUNCH_CHECK();
         rc = mpa_internal_solve_batch(nsys, inner.data(), MPA_C64, inner_scratch, it.data(), ires.data(), ist.data(), 1,
@@ -354,9 +364,10 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             }
         }
         if (any_revert) {
-            MPA_HIP_CHECK(hipMemcpyAsync(d_flat + 2 * total, usc.data(), sizeof(double) * total, hipMemcpyHostToDevice,
-                                         st));
-            hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 2);
+            MPA_HIP_CHECK(hipMemcpyAsync(flat ? d_flat + 2 * total : ref[0].unscale, usc.data(), sizeof(double) * total,
+                                         hipMemcpyHostToDevice, st));
+            if (flat)
+                hipLaunchKernelGGL(k_ref_flat, dim3(flat_grid), dim3(kThreads), 0, st, d_ref, d_cw, total, d_flat, 2);
             hipLaunchKernelGGL(k_ref_accum, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
             MPA_LAUNCH_CHECK();
             MPA_HIP_CHECK(hipStreamSynchronize(st));  // usc is read by the copies above
