@@ -146,6 +146,19 @@ struct RefCtl {
     int outer, inner_it, status, phase;  // phase: 0 decide, 1 correction running, 2 finished
 };
 
+// Control records of every (system, RHS) between the systems' states and one
+// flat array (cw order): to_flat 1 gathers, 0 scatters; ctlp[i] is system
+// i's record array.  One copy of the flat array replaces one per system.
+__global__ __launch_bounds__(kThreads) void k_ref_ctl_flat(RefCtl* const* __restrict__ ctlp,
+                                                           const int2* __restrict__ cw, int total,
+                                                           RefCtl* __restrict__ flat, int to_flat) {
+    const int k = blockIdx.x * kThreads + threadIdx.x;
+    if (k >= total) return;
+    const int2 c2 = cw[k];
+    if (to_flat) flat[k] = ctlp[c2.x][c2.y];
+    else ctlp[c2.x][c2.y] = flat[k];
+}
+
 size_t own_bytes(int n, int S) {
     return 2 * v64(n, S) + 2 * v32(n, S) + align256(sizeof(double2) * (size_t)S * groups_for(n)) +
            3 * align256(sizeof(double) * S) + align256(sizeof(RefCtl) * S);
@@ -491,9 +504,26 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
     int3* d_vw = (int3*)p; p += align256(sizeof(int3) * nvw);
     int2* d_cw = (int2*)p; p += align256(sizeof(int2) * total);
     int2* d_mwbase = (int2*)p; p += kTileClasses * 256 + align256(sizeof(int2) * nmw);
+    // control records: per-system pointers and the full (system, RHS) list, then the flat records
+    char* ctl_region = p;
+    RefCtl** d_ctlp = (RefCtl**)p; p += align256(sizeof(void*) * nsys);
+    int2* d_cwall = (int2*)p; p += align256(sizeof(int2) * total);
+    const size_t ctl_head = (size_t)(p - ctl_region);
+    RefCtl* d_ctlflat = (RefCtl*)p; p += align256(sizeof(RefCtl) * total);
     void* inner_scratch = p;
     MPA_HIP_CHECK(hipMemcpyAsync(d_ref, ref.data(), sizeof(RefDev) * nsys, hipMemcpyHostToDevice, st));
     MPA_HIP_CHECK(hipMemcpyAsync(d_mv, mv.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
+    std::vector<char> ctl_img(ctl_head, 0);
+    for (int i = 0; i < nsys; i++) {
+        std::memcpy(ctl_img.data() + sizeof(void*) * i, &V[i].ctl, sizeof(void*));
+        for (int s = 0; s < R[i].nrhs; s++) {
+            const int2 e = make_int2(i, s);
+            std::memcpy(ctl_img.data() + ((char*)d_cwall - ctl_region) + sizeof(int2) * (first[i] + s), &e, sizeof(int2));
+        }
+    }
+    // pageable source: staged by the runtime before the call returns
+    MPA_HIP_CHECK(hipMemcpyAsync(ctl_region, ctl_img.data(), ctl_head, hipMemcpyHostToDevice, st));
+    const unsigned ctl_grid = (unsigned)((total + kThreads - 1) / kThreads);
     auto upload = [&](const SubLists& L, int2** d_mw, int* n_mw) -> int {
         if (!L.vw.empty())
             MPA_HIP_CHECK(hipMemcpyAsync(d_vw, L.vw.data(), sizeof(int3) * L.vw.size(), hipMemcpyHostToDevice, st));
@@ -514,8 +544,9 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
     std::vector<RefCtl> ctl(total);
     std::vector<double> nrm(total);
     auto read_ctl = [&]() -> int {
-        for (int i = 0; i < nsys; i++)
-            MPA_HIP_CHECK(hipMemcpyAsync(&ctl[first[i]], V[i].ctl, sizeof(RefCtl) * R[i].nrhs, hipMemcpyDeviceToHost, st));
+        hipLaunchKernelGGL(k_ref_ctl_flat, dim3(ctl_grid), dim3(kThreads), 0, st, d_ctlp, d_cwall, total, d_ctlflat, 1);
+        MPA_LAUNCH_CHECK();
+        MPA_HIP_CHECK(hipMemcpyAsync(ctl.data(), d_ctlflat, sizeof(RefCtl) * total, hipMemcpyDeviceToHost, st));
         MPA_HIP_CHECK(hipStreamSynchronize(st));
         return MPA_OK;
     };
@@ -681,8 +712,9 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
             if ((rc = mpa_systems_start((int)in.size(), in.data(), MPA_C64, inner_scratch, st))) return rc;
         }
     }
-    for (int i = 0; i < nsys; i++)
-        MPA_HIP_CHECK(hipMemcpyAsync(V[i].ctl, &ctl[first[i]], sizeof(RefCtl) * R[i].nrhs, hipMemcpyHostToDevice, st));
+    MPA_HIP_CHECK(hipMemcpyAsync(d_ctlflat, ctl.data(), sizeof(RefCtl) * total, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_ref_ctl_flat, dim3(ctl_grid), dim3(kThreads), 0, st, d_ctlp, d_cwall, total, d_ctlflat, 0);
+    MPA_LAUNCH_CHECK();
     MPA_HIP_CHECK(hipStreamSynchronize(st));
     for (int i = 0; i < nsys; i++)
         for (int s = 0; s < R[i].nrhs; s++) {
@@ -710,7 +742,8 @@ size_t refine_scratch_bytes(int nsys, const mpa_refine_system* R) {
     }
     return align256(sizeof(RefDev) * nsys) + align256(sizeof(MvDev) * nsys) + align256(sizeof(int3) * vw) +
            align256(sizeof(int2) * cw) + kTileClasses * 256 + align256(sizeof(int2) * mw) +
-           align256(sizeof(double) * 3 * cw) + mpa_batch_scratch_bytes(nsys, inner.data(), MPA_C64);
+           align256(sizeof(double) * 3 * cw) + align256(sizeof(void*) * nsys) + align256(sizeof(int2) * cw) +
+           align256(sizeof(RefCtl) * cw) + mpa_batch_scratch_bytes(nsys, inner.data(), MPA_C64);
 }
 
 }  // namespace
