@@ -193,9 +193,10 @@ def test_torque_zero_for_linear_pol_single_sphere(H):
     rc = H.solve_cluster([0, 0, 0], 90 * nm, 4.0 + 0.2j, 650 * nm, 3, polarization=(1, 1j))
     # circular light twists an absorbing sphere about k; linear light does not.
     # Angular momentum balance: T_z = P_abs / omega = I * C_abs / omega with
-    # I = eps0 c |E|^2 / 2 for a unit-amplitude wave.
+    # I = eps0 c |E|^2 / 2 for a unit-amplitude wave; (1, i) along +z is
+    # x + i y, whose spin Im(E* x E) points along +z.
     c = 299792458.0
     omega = 2 * np.pi * c / (650 * nm)
     expect = 8.854187817e-12 * c / 2 * rc["cross_sections"][1] / omega
-    assert abs(rc["torque"][0, 2]) == pytest.approx(expect, rel=1e-9)
+    assert rc["torque"][0, 2] == pytest.approx(expect, rel=1e-9)
     assert np.all(np.abs(r["torque"]) < 1e-10 * abs(expect))

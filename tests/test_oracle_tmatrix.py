@@ -24,7 +24,7 @@ Pins, each independent of the oracle's formulas:
 import numpy as np
 import pytest
 
-from test_oracle_translation import _expand
+from vsh_fields import expand as _expand
 
 K = 2 * np.pi
 POS = np.array([[0.19, 0.02, -0.03], [-0.17, 0.05, 0.06]])  # k |r_i| about 1.2

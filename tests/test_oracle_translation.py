@@ -24,12 +24,10 @@ Pins:
 
 import numpy as np
 import pytest
-from math import factorial
-from scipy import special
 
+import vsh_fields as V
 
-def _modes(lmax):
-    return [(n, m) for n in range(1, lmax + 1) for m in range(-n, n + 1)]
+_modes = V.modes
 
 
 # ---- reference symmetry identities -------------------------------------------------
@@ -47,62 +45,9 @@ def test_reference_symmetries(oracle):
             assert np.allclose(A2, (-1) ** (m + u + n + v) * A) and np.allclose(B2, (-1) ** (m + u + n + v) * B)
 
 
-# ---- independent VSH evaluator (numpy/scipy) ----------------------------------------
+# ---- independent VSH evaluator (numpy/scipy): tests/vsh_fields.py ----------------
 
-def _P(n, m, x):
-    """Associated Legendre without Condon-Shortley phase (GSL NONE)."""
-    if abs(m) > n:
-        return 0.0 * x
-    val = special.lpmv(abs(m), n, x) * (-1) ** abs(m)
-    if m < 0:
-        val = val * (-1) ** (-m) * factorial(n + m) / factorial(n - m)
-    return val
-
-
-def _dP_dtheta(n, m, theta, h=1e-3):
-    f = lambda t: _P(n, m, np.cos(t))
-    d1 = (f(theta + h) - f(theta - h)) / (2 * h)
-    d2 = (f(theta + 2 * h) - f(theta - 2 * h)) / (4 * h)
-    return (4 * d1 - d2) / 3  # Richardson: O(h^4)
-
-
-def _zn(n, x, mode, derivative=False):
-    j = special.spherical_jn(n, x, derivative)
-    if mode == "incident":
-        return j
-    return j + 1j * special.spherical_yn(n, x, derivative)
-
-
-def _vsh_cart(n, m, mode, pts, center, k):
-    d = pts - center
-    r = np.linalg.norm(d, axis=1)
-    theta = np.arccos(d[:, 2] / r)
-    phi = np.arctan2(d[:, 1], d[:, 0])
-    kr = k * r
-    H, Hp = _zn(n, kr, mode), _zn(n, kr, mode, True)
-    e = np.exp(1j * m * phi)
-    P = _P(n, m, np.cos(theta))
-    pi = m / np.sin(theta) * P
-    tau = _dP_dtheta(n, m, theta)
-    fac = (H + kr * Hp) * e / kr
-    N = np.array([n * (n + 1) * P * H / kr * e, tau * fac, 1j * pi * fac])
-    M = np.array([0 * kr, 1j * H * e * pi, -H * e * tau])
-    rh = np.array([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)])
-    th = np.array([np.cos(theta) * np.cos(phi), np.cos(theta) * np.sin(phi), -np.sin(theta)])
-    ph = np.array([-np.sin(phi), np.cos(phi), 0 * phi])
-    to_cart = lambda S: S[0] * rh + S[1] * th + S[2] * ph
-    return to_cart(N), to_cart(M)
-
-
-def _expand(p, lmax, mode, pts, center, k):
-    """E = factor * sum Emn (p_e N + p_m M); factor = i (outgoing), -i (incident)."""
-    factor = 1j if mode == "outgoing" else -1j
-    E = 0
-    for r, (n, m) in enumerate(_modes(lmax)):
-        Emn = 1j ** n * np.sqrt((2 * n + 1) * factorial(n - m) / (n * (n + 1) * factorial(n + m)))
-        N, M = _vsh_cart(n, m, mode, pts, center, k)
-        E = E + factor * Emn * (p[0, r] * N + p[1, r] * M)
-    return E
+_expand = V.expand
 
 
 @pytest.mark.parametrize("lmax,tol", [(2, 1e-5), (3, 3e-8), (4, 1e-10)])
