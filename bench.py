@@ -91,6 +91,7 @@ def bench_serving(args):
     torch.cuda.synchronize()
     lat = latency_summary(reqs)
     done_sys = eng.solves_done
+    failed_sys, fallbacks = eng.failed_systems, eng.fp64_fallbacks
     n_mf = sum(1 for r in reqs for res in r.results if isinstance(res, dict) and res.get("matrix_free"))
     steps = eng.steps
     # one-at-a-time FP64 on stored matrices (the reference's mode: one cluster per solve), on the
@@ -129,9 +130,12 @@ def bench_serving(args):
                    "Nmax": args.serve_nmax, "hbm_budget_gb": args.serve_budget_gb, "wall_cap_s": args.serve_wall},
         "latency_ms": {k: v for k, v in lat.items() if k.endswith("_ms")},
         "requests": {"arrived": len(reqs), "completed": lat.get("completed", 0),
+                     "partial_too_large": sum(r.status == "partial" for r in reqs),
+                     "failed": sum(r.status == "failed" for r in reqs),
                      "rejected_too_large": sum(r.status == "too_large" for r in reqs),
                      "unfinished": sum(r.status in ("queued", "running") for r in reqs)},
-        "systems": {"done": done_sys, "matrix_free": n_mf, "policy": args.serve_matrix_free},
+        "systems": {"done": done_sys, "failed": failed_sys, "fp64_fallbacks": fallbacks, "matrix_free": n_mf,
+                    "policy": args.serve_matrix_free},
         "wall_s": wall,
         "one_at_a_time_fp64": {"systems": base_done, "not_servable": base_skip, "seconds": base_t,
                                "solves_per_s": base_done / base_t if base_t > 0 else None},

@@ -510,6 +510,7 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
     int2* d_cwall = (int2*)p; p += align256(sizeof(int2) * total);
     const size_t ctl_head = (size_t)(p - ctl_region);
     RefCtl* d_ctlflat = (RefCtl*)p; p += align256(sizeof(RefCtl) * total);
+    double* d_flat = (double*)p; p += align256(sizeof(double) * 3 * total);  // per-RHS values of a subset
     void* inner_scratch = p;
     MPA_HIP_CHECK(hipMemcpyAsync(d_ref, ref.data(), sizeof(RefDev) * nsys, hipMemcpyHostToDevice, st));
     MPA_HIP_CHECK(hipMemcpyAsync(d_mv, mv.data(), sizeof(MvDev) * nsys, hipMemcpyHostToDevice, st));
@@ -550,11 +551,35 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
         MPA_HIP_CHECK(hipStreamSynchronize(st));
         return MPA_OK;
     };
+    // Per-RHS values of a subset of systems move through d_flat in the order of
+    // the subset's uploaded (system, RHS) list d_cw: one gather kernel and one
+    // copy, or one copy and one scatter kernel, however many systems take part.
+    // two host images (revert values, decisions): the final synchronization of this call ends every copy
+    // from them before they go out of scope
+    std::vector<double> flat_host(3 * (size_t)total), flat_dec(3 * (size_t)total);
     auto read_norms = [&](const std::vector<int>& idx) -> int {
-        for (int i : idx)
-            MPA_HIP_CHECK(hipMemcpyAsync(&nrm[first[i]], V[i].d.norms, sizeof(double) * R[i].nrhs,
-                                         hipMemcpyDeviceToHost, st));
+        int cnt = 0;
+        for (int i : idx) cnt += R[i].nrhs;
+        hipLaunchKernelGGL(k_ref_flat, dim3((unsigned)((cnt + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, d_ref,
+                           d_cw, cnt, d_flat, 0);
+        MPA_LAUNCH_CHECK();
+        MPA_HIP_CHECK(hipMemcpyAsync(flat_host.data(), d_flat, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
         MPA_HIP_CHECK(hipStreamSynchronize(st));
+        int o = 0;
+        for (int i : idx)
+            for (int s = 0; s < R[i].nrhs; s++) nrm[first[i] + s] = flat_host[o++];
+        return MPA_OK;
+    };
+    // scale (mode 1: img[cnt + k]) and unscale (img[2 cnt + k]) of the subset in d_cw
+    auto write_scales = [&](const std::vector<double>& img, int cnt, int mode) -> int {
+        const size_t off = mode == 1 ? (size_t)cnt : 2 * (size_t)cnt;
+        const size_t len = mode == 1 ? 2 * (size_t)cnt : (size_t)cnt;
+        // pageable source: staged by the runtime before the call returns
+        MPA_HIP_CHECK(hipMemcpyAsync(d_flat + off, img.data() + off, sizeof(double) * len,
+                                     hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_ref_flat, dim3((unsigned)((cnt + kThreads - 1) / kThreads)), dim3(kThreads), 0, st, d_ref,
+                           d_cw, cnt, d_flat, mode);
+        MPA_LAUNCH_CHECK();
         return MPA_OK;
     };
     int rc;
@@ -622,15 +647,17 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
         MPA_LAUNCH_CHECK();
         if ((rc = read_norms(finished_inner))) return rc;
         std::vector<int> revert_sys;
+        const int cnt = (int)L.cw.size();
+        int o = 0;
         for (int i : finished_inner) {
             bool any = false;
-            std::vector<double> neg(R[i].nrhs, 0.0);
-            for (int s = 0; s < R[i].nrhs; s++) {
+            for (int s = 0; s < R[i].nrhs; s++, o++) {
                 RefCtl& c = ctl[first[i] + s];
                 const double now = nrm[first[i] + s];
+                flat_host[2 * (size_t)cnt + o] = 0.0;
                 if (c.status == -1 && !(now <= c.rn2)) {
                     // the correction left the residual larger (a failed inner solve): take it back
-                    neg[s] = -std::sqrt(c.rn2);
+                    flat_host[2 * (size_t)cnt + o] = -std::sqrt(c.rn2);
                     c.prev = c.rn2;  // residual unchanged: the decision below stops this RHS
                     any = true;
                 } else {
@@ -638,20 +665,17 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
                     c.rn2 = now;
                 }
             }
-            if (any) {
-                MPA_HIP_CHECK(hipMemcpyAsync(V[i].d.unscale, neg.data(), sizeof(double) * R[i].nrhs,
-                                             hipMemcpyHostToDevice, st));
-                MPA_HIP_CHECK(hipStreamSynchronize(st));
-                revert_sys.push_back(i);
-            }
+            if (any) revert_sys.push_back(i);
             ctl[first[i]].phase = 0;
         }
         if (!revert_sys.empty()) {
+            // unscale = -||r|| for the reverting RHS, 0 for the others of the finished set (every one of
+            // them is decided afresh below, which sets its scales again before any further use)
+            if ((rc = write_scales(flat_host, cnt, 2))) return rc;
             SubLists Lr = sub_lists(revert_sys, R, V);
             if ((rc = upload(Lr, d_mw, n_mw))) return rc;
             hipLaunchKernelGGL(k_ref_accum, dim3((unsigned)Lr.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
             MPA_LAUNCH_CHECK();
-            MPA_HIP_CHECK(hipStreamSynchronize(st));
         }
     }
     // C. decisions: per RHS converged / stalled / out of corrections, or the next correction
@@ -683,19 +707,7 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
             }
             if (any) {
                 const double itol = std::isfinite(need) ? std::max(R[i].inner_tol, need) : R[i].inner_tol;
-                std::vector<double> sc(R[i].nrhs), usc(R[i].nrhs);
-                for (int s = 0; s < R[i].nrhs; s++) {
-                    RefCtl& c = ctl[first[i] + s];
-                    const bool active = c.status == -1;
-                    sc[s] = active ? 1.0 / std::sqrt(c.rn2) : 0.0;
-                    usc[s] = active ? std::sqrt(c.rn2) : 0.0;
-                    c.inner_tol = itol;
-                }
-                MPA_HIP_CHECK(hipMemcpyAsync(V[i].d.scale, sc.data(), sizeof(double) * R[i].nrhs,
-                                             hipMemcpyHostToDevice, st));
-                MPA_HIP_CHECK(hipMemcpyAsync(V[i].d.unscale, usc.data(), sizeof(double) * R[i].nrhs,
-                                             hipMemcpyHostToDevice, st));
-                MPA_HIP_CHECK(hipStreamSynchronize(st));  // sc / usc are host locals
+                for (int s = 0; s < R[i].nrhs; s++) ctl[first[i] + s].inner_tol = itol;
                 ctl[first[i]].phase = 1;
                 starting.push_back(i);
             } else {
@@ -705,6 +717,18 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
         if (!starting.empty()) {
             SubLists L = sub_lists(starting, R, V);
             if ((rc = upload(L, d_mw, n_mw))) return rc;
+            // scale = 1 / ||r||, unscale = ||r|| of every RHS that starts a correction (0 for the others),
+            // in the order of the starting set's (system, RHS) list
+            const int cnt = (int)L.cw.size();
+            int o = 0;
+            for (int i : starting)
+                for (int s = 0; s < R[i].nrhs; s++, o++) {
+                    const RefCtl& c = ctl[first[i] + s];
+                    const bool active = c.status == -1;
+                    flat_dec[(size_t)cnt + o] = active ? 1.0 / std::sqrt(c.rn2) : 0.0;
+                    flat_dec[2 * (size_t)cnt + o] = active ? std::sqrt(c.rn2) : 0.0;
+                }
+            if ((rc = write_scales(flat_dec, cnt, 1))) return rc;
             hipLaunchKernelGGL(k_ref_to32, dim3((unsigned)L.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
             MPA_LAUNCH_CHECK();
             std::vector<mpa_system> in;

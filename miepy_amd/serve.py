@@ -37,6 +37,18 @@ which every engine step advances beside the refined set.
 
 A system whose footprint alone exceeds the budget is rejected with status
 "too_large" instead of blocking the queue.
+
+A system retires when every source has stopped.  A source that stopped
+without reaching the tolerance (maxiter, stalled or out of refinement steps,
+non-finite) is not a result: a refined system is then re-solved by FP64
+BiCGSTAB on its FP64 operator (the stored complex128 matrix or the FP64
+matrix-free operator), the same safety net as `device.solve_refined`, and a
+system that still does not converge, or an FP64 system that did not, is
+recorded as failed.  Only converged systems count in `solves_done`.  A
+request ends "done" when all its systems converged, "partial" when some
+wavelengths were too large for the budget and the rest converged, "failed"
+when any system failed, and "too_large" when none fit; only "done" requests
+enter the latency figures.
 """
 
 import collections
@@ -120,6 +132,19 @@ class Request:
         self.status = "queued"
         self.results = [None] * len(self.workloads)
         self.pending = len(self.workloads)
+        self.rejected = 0  # systems too large for the budget
+        self.failed = 0    # systems that did not converge (after the FP64 safety net)
+
+    def _finish(self, when):
+        self.finished = when
+        if self.rejected == len(self.workloads):
+            self.status = "too_large"
+        elif self.failed:
+            self.status = "failed"
+        elif self.rejected:
+            self.status = "partial"
+        else:
+            self.status = "done"
 
     @property
     def latency(self):
@@ -167,7 +192,9 @@ class ServingEngine:
         self.finished = []
         self.rejected = []
         self.steps = 0
-        self.solves_done = 0
+        self.solves_done = 0     # converged systems
+        self.failed_systems = 0  # systems that stopped without converging
+        self.fp64_fallbacks = 0  # refined systems finished by the FP64 safety net
         self._scratch = torch.empty(0, dtype=torch.uint8, device="cuda")
         self.lib = L.load()
 
@@ -195,13 +222,13 @@ class ServingEngine:
             s.matrix_free, s.refine = self._route(w)
             s.footprint = system_footprint(w, s.matrix_free, "mixed" if s.refine else "fp64")
             if s.footprint > self.budget:
-                request.status = "too_large"
                 request.results[i] = {"status": "too_large"}
+                request.rejected += 1
                 request.pending -= 1
                 continue
             self.queue.append(s)
         if request.pending == 0:
-            request.finished = self.clock()
+            request._finish(self.clock())
             self.rejected.append(request)
 
     def _inputs(self, systems):
@@ -304,6 +331,7 @@ class ServingEngine:
                 self._prepare(s)
                 if s.req.admitted is None:
                     s.req.admitted = self.clock()
+                if s.req.status == "queued":  # status only moves forward
                     s.req.status = "running"
             ref = [s for s in new if s.refine]
             f64 = [s for s in new if not s.refine]
@@ -371,7 +399,25 @@ class ServingEngine:
         self.active = [s for s in self.active if id(s) in kept]
         return True
 
+    def _fp64_safety_net(self, s, iters):
+        """Re-solve a refined system whose refinement stopped short with FP64
+        BiCGSTAB on its FP64 operator (stored I + T or the matrix-free T)."""
+        b = s.p_src.reshape(s.S, s.n)
+        if s.matrix_free:
+            x, rep = D.bicgstab(s.A, b, tol=s.w.tol, maxiter=max(self.maxiter, 10 * s.n), add_identity=True,
+                                check=False)
+        else:
+            x, rep = D.bicgstab(s.A[0], b, tol=s.w.tol, maxiter=max(self.maxiter, 10 * s.n), check=False)
+        s.x.copy_(x.reshape(s.x.shape))
+        self.fp64_fallbacks += 1
+        return [a + b for a, b in zip(iters, rep.iterations)], list(rep.residual), list(rep.status)
+
     def _retire(self, s, iters, resid, status):
+        fallback = False
+        if any(v != L.MPA_OK for v in status) and s.refine:
+            iters, resid, status = self._fp64_safety_net(s, iters)
+            fallback = True
+        ok = all(v == L.MPA_OK for v in status)
         p_inc = s.x.reshape(s.S, s.w.N, 2, s.w.rmax)
         obs = D.observables(p_inc, s.p_src, s.mie_scat, s.mie_int, s.k, s.eps_b, want_int=False,
                             want_torque=False)
@@ -380,15 +426,18 @@ class ServingEngine:
             "cross_sections": obs["C_total"].cpu().numpy(),  # [S, 3]
             "force": obs["F"].cpu().numpy(),                  # [S, N, 3]
             "iterations": iters, "residual": resid, "status": status, "matrix_free": s.matrix_free,
-            "refined": s.refine,
+            "refined": s.refine, "fp64_fallback": fallback, "converged": ok,
         }
-        self.solves_done += 1
+        if ok:
+            self.solves_done += 1
+        else:
+            self.failed_systems += 1
+            s.req.failed += 1
         s.A = s.x = s.state = s.pos = s.mie_scat = s.mie_int = s.p_src = None
         self.arena.free(s.block)
         s.req.pending -= 1
         if s.req.pending == 0:
-            s.req.finished = self.clock()
-            s.req.status = "done" if s.req.status == "running" else s.req.status
+            s.req._finish(self.clock())
             self.finished.append(s.req)
 
     # ---- drivers ---------------------------------------------------------------------------

@@ -30,16 +30,23 @@ from .flux import cross_sections as _cs
 from .materials import constant_material, material_struct
 
 
-def _atleast(array, dim, length, dtype=None):
-    """src/miepy/utils.py:6-21"""
-    ret = np.asarray(np.atleast_1d(array), dtype=dtype)
-    if dim not in (ret.ndim, ret.ndim + 1):
-        raise ValueError(f"dim = {dim} is invalid with input dim = {ret.ndim}")
-    if len(ret.shape) == dim and ret.shape[0] != length:
-        ret = np.repeat(ret, length)
-    elif len(ret.shape) != dim:
-        ret = np.array([ret] * length)
-    return ret
+def _per_particle(value, count, dtype=None):
+    """One entry per particle: a scalar (or one object) is broadcast to
+    `count` entries, a sequence of `count` entries is kept; anything else is a
+    shape error.  Same accepted inputs as the reference's `atleast(value, dim=1,
+    length=count)` (src/miepy/utils.py:6-21) for 1-d per-particle data."""
+    arr = np.asarray(value, dtype=dtype)
+    if arr.ndim == 0:
+        out = np.empty(count, dtype=arr.dtype)
+        out[...] = arr.item() if dtype is object else arr
+        return out
+    if arr.ndim != 1:
+        raise ValueError(f"per-particle data must be a scalar or 1-d, got shape {arr.shape}")
+    if arr.shape[0] == count:
+        return arr
+    if arr.shape[0] == 1:
+        return np.repeat(arr, count)
+    raise ValueError("The shapes of position, radius, and material do not match")
 
 
 def n_index(lmax):
@@ -54,10 +61,8 @@ class sphere_cluster:
         if symmetry is not None or interface is not None:
             raise NotImplementedError("periodic symmetry and interfaces are not part of the mi355x cluster solve")
         self.position = np.asarray(np.atleast_2d(position), dtype=float)
-        self.radius = _atleast(radius, dim=1, length=self.position.shape[0], dtype=float)
-        self.material = _atleast(material, dim=1, length=self.position.shape[0], dtype=object)
-        if not (self.position.shape[0] == self.radius.shape[0] == self.material.shape[0]):
-            raise ValueError("The shapes of position, radius, and material do not match")
+        self.radius = _per_particle(radius, self.position.shape[0], dtype=float)
+        self.material = _per_particle(material, self.position.shape[0], dtype=object)
         self.Nparticles = self.radius.shape[0]
         self.symmetry = symmetry
         self.interface = interface
@@ -92,12 +97,7 @@ class sphere_cluster:
             if (np.imag(self.medium.eps(self.wavelength)) != 0) or (np.imag(self.medium.mu(self.wavelength)) != 0):
                 raise ValueError("medium must be non-absorbing")
 
-        self.material_data = material_struct(self.material, self.medium, wavelength=self.wavelength)
-        md = self.material_data
-        conducting = np.array([getattr(m, "name", None) == "metal" for m in self.material])
-        self._mie_d = D.mie_coefficients(self.radius, md.eps, md.mu, float(np.real(md.k_b)), md.eps_b, md.mu_b,
-                                         lmax, conducting)
-        self.mie_scat, self.mie_int = (a.cpu().numpy() for a in self._mie_d)
+        self._set_wavelength(wavelength)
         self._multi = isinstance(source, (list, tuple))
         self.p_cluster = None
         self.report = None
@@ -105,10 +105,35 @@ class sphere_cluster:
 
     # ---- solve ---------------------------------------------------------------------
 
+    def _set_wavelength(self, wavelength):
+        """Material data at `wavelength` and every sphere's Mie coefficients
+        (computed on the device)."""
+        self.wavelength = wavelength
+        self.material_data = material_struct(self.material, self.medium, wavelength=self.wavelength)
+        md = self.material_data
+        conducting = np.array([getattr(m, "name", None) == "metal" for m in self.material])
+        self._mie_d = D.mie_coefficients(self.radius, md.eps, md.mu, float(np.real(md.k_b)), md.eps_b, md.mu_b,
+                                         self.lmax, conducting)
+        self.mie_scat, self.mie_int = (a.cpu().numpy() for a in self._mie_d)
+
     def _sources(self):
         return list(self.source) if self._multi else [self.source]
 
     def solve(self, wavelength=None, source=None):
+        """Solve for p_inc, p_scat, p_int and the observables
+        (src/miepy/sphere_cluster.py:664-677).  A `wavelength` or `source`
+        given here replaces the current one first (the material data and Mie
+        coefficients follow the wavelength); the reference's signature accepts
+        both and solves with its current state."""
+        if source is not None:
+            self.source = source
+            self._multi = isinstance(source, (list, tuple))
+            self.p_cluster = None
+        if wavelength is not None and wavelength != self.wavelength:
+            if np.imag(self.medium.eps(wavelength)) != 0 or np.imag(self.medium.mu(wavelength)) != 0:
+                raise ValueError("medium must be non-absorbing")
+            self._set_wavelength(wavelength)
+            self.p_cluster = None
         md = self.material_data
         k = float(np.real(md.k_b))
         N, rmax, lmax = self.Nparticles, self.rmax, self.lmax

@@ -52,6 +52,13 @@ def test_workloads():
 def test_backend_registry():
     import miepy_amd as M
     assert M.backends.get_backend() == "mi355x"
+    # the reference's own name stays valid (src/miepy/backends/__init__.py:16-28)
+    assert "cpu" in M.backends.available_backends()
+    with M.backends.backend("cpu"):
+        assert M.backends.get_backend() == "cpu"
+        with pytest.raises(NotImplementedError):
+            M.backends.get_backend_module()
+    assert M.backends.get_backend() == "mi355x"
     with pytest.raises(ValueError):
         M.backends.set_backend("nope")
     sentinel = object()

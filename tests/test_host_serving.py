@@ -59,3 +59,24 @@ def test_tmatrix_reduce_lmax():
     assert np.array_equal(tmatrix_reduce_lmax(T, 4), T)
     with pytest.raises(ValueError):
         tmatrix_reduce_lmax(T, 5)
+
+
+def test_request_final_status():
+    """A request is "done" only when every system converged; wavelengths too
+    large for the budget make it "partial", a failed system "failed", and a
+    request none of whose systems fit "too_large"."""
+    from miepy_amd.serve import Request
+    r = Request(0, [None, None, None], 0.0)
+    r._finish(1.0)
+    assert r.status == "done" and r.latency == 1.0
+    r = Request(0, [None, None], 0.0)
+    r.rejected = 1
+    r._finish(1.0)
+    assert r.status == "partial"
+    r.failed = 1
+    r._finish(1.0)
+    assert r.status == "failed"
+    r = Request(0, [None, None], 0.0)
+    r.rejected = 2
+    r._finish(1.0)
+    assert r.status == "too_large"
