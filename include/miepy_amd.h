@@ -139,6 +139,19 @@ int mpa_bicgstab(int n, int nrhs, int dtype, const void* A, int add_identity, co
                  int maxiter, void* workspace, int* iters_out, double* resid_out, int* status_out, int sync_every,
                  void* stream);
 
+/* mpa_bicgstab without a workspace argument: no counterpart in the reference;
+ * for callers that cannot allocate device memory (code beside HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * solve and results as mpa_bicgstab (sync_every = 1), with the workspace in a
+ * static 64 MiB device array of this library (one per GPU, held until the
+ * process ends): allocates nothing, launches eagerly (no graph), reads the
+ * per-RHS scalars into pageable host memory.  Returns MPA_ERR_ARG when
+ * mpa_bicgstab_workspace_bytes(n, nrhs, dtype) exceeds 64 MiB (96 bytes
+ * per unknown and RHS for MPA_C128: n * nrhs up to about 6.9e5).  Calls on one GPU share the array: calls
+ * that overlap on one GPU race. */
+int mpa_bicgstab_noalloc(int n, int nrhs, int dtype, const void* A, int add_identity, const void* b, void* x,
+                         double tol, int maxiter, int* iters_out, double* resid_out, int* status_out, void* stream);
+
 /* Solver flags (mpa_bicgstab_ex, mpa_system.flags). */
 #define MPA_SOLVE_REF_LOOP 1 /* the reference's hand-written loop (cpp/src/interactions.cpp:98-209,
                                 230-288): x0 = b, stop at ||r|| < tol (absolute) or after maxiter + 1

@@ -530,13 +530,19 @@ int launch_restart(const Plan& P, hipStream_t st) {
 
 // Read back the scalars of every RHS (synchronizes the stream): one gather
 // kernel and one copy through pinned host memory, however many systems.
-int read_scalars(const Plan& P, std::vector<Scal>& out, hipStream_t st) {
+// noalloc: straight into the caller's pageable vector (no pinned staging, which may allocate).
+int read_scalars(const Plan& P, std::vector<Scal>& out, hipStream_t st, bool noalloc = false) {
     const int total = (int)P.scal_work.size();
     out.resize(total);
     hipLaunchKernelGGL(k_gather_scalars, dim3((total + 63) / 64), dim3(64), 0, st, P.d_sys, P.d_scal, total,
                        P.d_scal_out);
     MPA_LAUNCH_CHECK();
     const size_t bytes = sizeof(Scal) * total;
+    if (noalloc) {
+        MPA_HIP_CHECK(hipMemcpyAsync(out.data(), P.d_scal_out, bytes, hipMemcpyDeviceToHost, st));
+        MPA_HIP_CHECK(hipStreamSynchronize(st));
+        return MPA_OK;
+    }
     PinnedLease pin(bytes);
     void* dst = pin.ptr() ? pin.ptr() : (void*)out.data();
     MPA_HIP_CHECK(hipMemcpyAsync(dst, P.d_scal_out, bytes, hipMemcpyDeviceToHost, st));
@@ -584,8 +590,9 @@ int capture_iterations(const Plan& P, int k, IterGraph& g) {
 }
 
 template <typename T>
+// noalloc: eager launches only (no graph capture or instantiation) and no pinned staging.
 int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, double* resid, int* status, int sync_every,
-                hipStream_t st) {
+                hipStream_t st, bool noalloc = false) {
     Plan P;
     const int dtype = sizeof(T) == 8 ? MPA_C128 : MPA_C64;
     int rc = build_plan(P, nsys, S_, dtype, scratch, st);
@@ -598,7 +605,7 @@ int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, doubl
     std::vector<Scal> host;
     long launched = 0;
     IterGraph graph;
-    const bool use_graph = mpa_option(MPA_OPT_GRAPHS) != 0 && !mpa_profile_active();
+    const bool use_graph = !noalloc && mpa_option(MPA_OPT_GRAPHS) != 0 && !mpa_profile_active();
     if (use_graph) {
         rc = capture_iterations<T>(P, k_sync, graph);
         if (rc) return rc;
@@ -631,7 +638,7 @@ int solve_batch(int nsys, const mpa_system* S_, void* scratch, int* iters, doubl
     for (int i = 0; i < nsys; i++) from_start &= (S_[i].flags & MPA_SOLVE_REF_LOOP) == 0;  // x0 = b: restart first
     if (from_start && (rc = launch_rounds(rounds))) return rc;
     for (;;) {
-        rc = read_scalars(P, host, st);
+        rc = read_scalars(P, host, st, noalloc);
         if (rc) return rc;
         bool all_done = true, any_restart = false;
         for (const Scal& c : host) {
@@ -703,6 +710,55 @@ bool valid_systems(int nsys, const mpa_system* s, int dtype) {
         }
     }
     return true;
+}
+
+}  // namespace
+
+// ---- allocation-free BiCGSTAB ----------------------------------------------------------
+//
+// mpa_bicgstab_noalloc is mpa_bicgstab with its workspace (Krylov vectors,
+// partial sums, per-RHS scalars, descriptors and work lists) in this static
+// device array instead of a caller-provided buffer, for callers that cannot
+// allocate device memory.  Every GPU the library runs on holds one copy of
+// the array until the process ends.  Calls on one GPU share it and it keeps
+// the last call's contents: the solve's start kernels initialize every
+// vector and scalar it reads, and calls that overlap on one GPU (several host
+// threads, or streams not ordered with each other) race on it.  Host code
+// never passes `scratch` itself to a kernel or a copy: the array's name is a
+// host-side stand-in, and its device address (per device) comes from
+// hipGetSymbolAddress.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void* g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// The device address of `scratch` on the current device, looked up on the
+// first call on that device (with it current) and kept.
+int noalloc_scratch(void** out) {
+    int dev = 0;
+    MPA_HIP_CHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) {
+        mpa_set_error("bicgstab_noalloc: device %d is past the %d devices this library tracks", dev,
+                      kNoAllocMaxDevices);
+        return MPA_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            mpa_set_error("bicgstab_noalloc: hipGetSymbolAddress of the static scratch failed on device %d: %s", dev,
+                          hipGetErrorString(e));
+            return MPA_ERR_HIP;
+        }
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return MPA_OK;
 }
 
 }  // namespace
@@ -843,6 +899,39 @@ int mpa_bicgstab_ex(int n, int nrhs, int dtype, const void* A, int add_identity,
     void* scratch = (char*)workspace + state_bytes(n, nrhs, dtype, minv != nullptr);
     return mpa_internal_solve_batch(1, &s, dtype, scratch, iters_out, resid_out, status_out, sync_every,
                                     (hipStream_t)stream);
+}
+
+int mpa_bicgstab_noalloc(int n, int nrhs, int dtype, const void* A, int add_identity, const void* b, void* x,
+                         double tol, int maxiter, int* iters_out, double* resid_out, int* status_out, void* stream) {
+    if (n < 2 || (n & 1) || nrhs < 1 || !A || !b || !x || !(tol >= 0) || maxiter < 0 ||
+        (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("bicgstab_noalloc: bad argument (n must be even and positive)");
+        return MPA_ERR_ARG;
+    }
+    const size_t need = mpa_bicgstab_workspace_bytes(n, nrhs, dtype);
+    if (need > kNoAllocScratchBytes) {
+        mpa_set_error("bicgstab_noalloc: n = %d with %d right-hand sides needs %zu bytes of workspace, more than the "
+                      "%zu-byte static scratch; call mpa_bicgstab with a workspace instead",
+                      n, nrhs, need, kNoAllocScratchBytes);
+        return MPA_ERR_ARG;
+    }
+    void* ws = nullptr;
+    const int rc = noalloc_scratch(&ws);
+    if (rc) return rc;
+    mpa_system s{};
+    s.n = n;
+    s.nrhs = nrhs;
+    s.add_identity = add_identity;
+    s.A = A;
+    s.b = b;
+    s.x = x;
+    s.state = ws;
+    s.tol = tol;
+    s.maxiter = maxiter;
+    void* plan = (char*)ws + state_bytes(n, nrhs, dtype);
+    hipStream_t st = (hipStream_t)stream;
+    if (dtype == MPA_C128) return solve_batch<double>(1, &s, plan, iters_out, resid_out, status_out, 1, st, true);
+    return solve_batch<float>(1, &s, plan, iters_out, resid_out, status_out, 1, st, true);
 }
 
 int mpa_bicgstab_matfree(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* b, void* x,

@@ -43,6 +43,13 @@ def test_library_reports_bad_arguments_without_gpu():
     assert "bad argument" in _lib.last_error()
     rc = lib.mpa_sphere_aggregate_tmatrix(0, 2, None, None, 1.0, 0, 0, None, None)
     assert rc == _lib.MPA_ERR_ARG
+    # the allocation-free solve refuses a workspace larger than its static scratch before touching the device
+    one = ctypes.c_void_p(256)
+    it, res, st = (ctypes.c_int * 1)(), (ctypes.c_double * 1)(), (ctypes.c_int * 1)()
+    assert lib.mpa_bicgstab_workspace_bytes(1000000, 1, _lib.MPA_C128) > 64 << 20
+    rc = lib.mpa_bicgstab_noalloc(1000000, 1, _lib.MPA_C128, one, 0, one, one, 1e-8, 10, it, res, st, None)
+    assert rc == _lib.MPA_ERR_ARG and "static scratch" in _lib.last_error()
+    assert lib.mpa_bicgstab_workspace_bytes(2000, 300, _lib.MPA_C128) <= 64 << 20  # n * nrhs = 6e5 fits
 
 
 def test_package_fails_loudly_without_gpu():
