@@ -320,6 +320,18 @@ int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int add_identit
 int mpa_matfree_matvec_c64(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,
                            void* stream);
 
+/* Batched variant of mpa_matfree_matvec / mpa_matfree_matvec_c64 over a
+ * table of sphere-cluster operators (the batched form the reference's
+ * sphere_aggregate_tmatrix + GEMV would take over many clusters,
+ * cpp/src/interactions.cpp:370-415 and :347-368): y[q] = (I +) T_q x[q] for
+ * every q, nrhs[q] right-hand sides each, all in dtype.  With the complex64
+ * kernel whose sources share each pair's special functions selected
+ * (mpa_matfree_variant(MPA_C64, 6)) the complex64 products of all operators
+ * of one lmax share launches (16 per launch); otherwise one launch sequence
+ * per operator.  Results equal the one-operator calls bit for bit. */
+int mpa_matfree_matvec_batched(int count, const mpa_cluster_operator* const* ops, const int* nrhs, int dtype,
+                               int add_identity, const void* const* x, void* const* y, void* stream);
+
 /* BiCGSTAB (as mpa_bicgstab, FP64) on (add_identity ? I + T : T) x = b with
  * the matrix-free operator; workspace: mpa_bicgstab_workspace_bytes(n,
  * nrhs, MPA_C128) bytes, n = 2 lmax (lmax + 2) N; op->workspace sized for

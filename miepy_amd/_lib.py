@@ -60,6 +60,7 @@ SIGNATURES = {
     "mpa_bicgstab_workspace_bytes": (_sz, [_i, _i, _i]),
     "mpa_bicgstab": (_i, [_i, _i, _i, _v, _i, _v, _v, _d, _i, _v, _pi, _pd, _pi, _i, _v]),
     "mpa_bicgstab_noalloc": (_i, [_i, _i, _i, _v, _i, _v, _v, _d, _i, _pi, _pd, _pi, _v]),
+    "mpa_matfree_matvec_batched": (_i, [_i, _v, _pi, _i, _i, _v, _v, _v]),
     "mpa_bicgstab_ex_workspace_bytes": (_sz, [_i, _i, _i, _i]),
     "mpa_lu_workspace_bytes": (_sz, [_i]),
     "mpa_mie_coefficients": (_i, [_i, _i, _v, _v, _v, _v, _v, _v, _v, _v, _v, _v]),

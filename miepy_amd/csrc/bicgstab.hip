@@ -482,11 +482,7 @@ int matvec_phase(const Plan& P, const MvDev* d_jobs, const std::vector<MfJob>& m
         const int rc = gemv_launch(sizeof(T), d_jobs, P.d_sys, P.d_mv_work, P.n_mv_work, st);
         if (rc) return rc;
     }
-    for (const MfJob& j : mf) {
-        const int rc = matfree_launch(j, P.d_sys, st);
-        if (rc) return rc;
-    }
-    return MPA_OK;
+    return mf.empty() ? MPA_OK : matfree_launch_many(mf, P.d_sys, st);
 }
 
 template <typename T>

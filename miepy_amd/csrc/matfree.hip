@@ -291,8 +291,18 @@ constexpr int kFacMaxLmax = 3;
 // kernel treats 2, 3 as 0); c128: 0 and 1 the slot-mapped kernel (k_matfree_mf64s) with one or two
 // accumulator chains (one is the faster at rand1000: 0.34 ms per product against 0.43 ms for the
 // best per-lane-select variant), 2 and 3 the per-lane-select kernel (k_matfree_mf64), two or one.
-int g_mf_variant[2] = {4, 0};  // complex64: column pairs (lmax 4 runs variant 0); FP64: slot-mapped KS 1
-constexpr int kMfVariants[2] = {6, 4};  // complex64: 4, 5 = column pairs (lmax <= 3), KS 1 and 2
+// complex64: 4, 5 = column pairs (lmax <= 3), KS 1 and 2; 6 = sources sharing the special functions,
+// many jobs per launch (k_matfree_mf16s); 7 (default) = per job: 6 for three or more sources or
+// lmax 4, else 4.  The choice depends on the job alone, never on what shares its launch.
+int g_mf_variant[2] = {7, 0};  // FP64: slot-mapped KS 1
+constexpr int kMfVariants[2] = {8, 4};
+constexpr int kMfShared = 6, kMfAuto = 7;
+
+// Whether a complex64 job runs the shared kernel under the current variant choice.
+bool mf_shared_for(int S, int lmax) {
+    if (g_mf_variant[0] == kMfShared) return true;
+    return g_mf_variant[0] == kMfAuto && (S >= 3 || lmax == 4);
+}
 constexpr int kFacMinN = 128;  // smaller clusters keep the LDS kernel (too few lanes)
 
 constexpr int fac_n_of(int r) {
@@ -798,10 +808,12 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf32(MfJob J, const SysDev
 // undone lane-locally), A = W_j from LDS.  K slots: the (p, m >= 0) combos
 // u in m-major order; lane half 0 supplies HLE[p, +m], half 1 HLE[p, -m]
 // (zero for m = 0), each as (Re, Im) -> 2 K positions; 4 slots per 16-K
-// block.  Scaling keeps every FP16 value within [-1, 1]:
+// block.  Scaling keeps every FP16 value within [-2^12, 2^12]:
 //   B' = HLE c_pm g_ij,  c_pm = sqrt((p-m)!/(p+m)!),  g_ij = 1 / max_p |h_p(k d_ij)|
-//   A' = W / (c_pm tau_j),  tau_j = max |W_j / c_pm|
+//   A' = W / (c_pm tau_j),  tau_j = 2^-12 x the power of two at or above max |W_j / c_pm|
 // and each pair's block sum is scaled by tau_j / g_ij into the FP32 total.
+// (A' reaches up to 2^12 so that its small entries keep a normal lo part;
+// k_matfree_mf16s scales B' the same way.)
 // Each operand is split v = hi + lo (hi = v rounded to FP16, lo = v - hi
 // rounded to FP16); the products hi*hi + hi*lo + lo*hi leave a relative
 // error of about 2^-21 per term.
@@ -857,12 +869,19 @@ struct WhShape {
     static constexpr bool TLDS = L <= 3;
 };
 
+// The smallest power of two >= t (t > 0 and normal): a power-of-two tau_j
+// makes every rescale by tau exact.
+__device__ __forceinline__ float pow2_ceil(float t) {
+    unsigned b = __builtin_bit_cast(unsigned, t);
+    if (b & 0x7fffffu) b = (b & 0xff800000u) + 0x800000u;
+    return __builtin_bit_cast(float, b & 0xff800000u);
+}
+
 template <int L>
-__global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* __restrict__ sys, int nh,
-                                                         int nterms, const int2* __restrict__ hlist,
-                                                         const int* __restrict__ hsrc,
-                                                         const double2* __restrict__ hcoef, h8v* __restrict__ Wh,
-                                                         float* __restrict__ tau) {
+__device__ __forceinline__ void wh_body(const MfJob& J, int j0, int s, int nh, int nterms,
+                                        const int2* __restrict__ hlist, const int* __restrict__ hsrc,
+                                        const double2* __restrict__ hcoef, h8v* __restrict__ Wh,
+                                        float* __restrict__ tau) {
     using Sh = F16Shape<L>;
     constexpr int NHm = (2 * L + 1) * (2 * L + 1);
     constexpr int NC = Sh::NC, JB = WhShape<L>::JB, RM = L * (L + 2);
@@ -875,8 +894,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
     __shared__ int hs_s[TLDS ? kWhMaxTerms : 1];
     __shared__ double2 hc_s[TLDS ? kWhMaxTerms : 1];
     __shared__ float red[JB][kThreads / 64];
-    if (J.skip >= 0 && job_done(sys[J.skip])) return;
-    const int j0 = blockIdx.x * JB, s = blockIdx.y, tid = threadIdx.x;
+    const int tid = threadIdx.x;
     const int lmax = J.lmax;
     const size_t n = (size_t)NC * J.N;
     for (int q = tid; q < RM * NHm; q += kThreads) hl_s[q] = hlist[q];
@@ -953,7 +971,9 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
         if (j >= J.N) break;  // items run in b order: the rest are past N too
         float t = red[b][0];
         for (int q = 1; q < kThreads / 64; q++) t = fmaxf(t, red[b][q]);
-        t = t > 0.f ? t : 1.f;
+        // a power of two 2^12 times smaller than the column's largest |W / c_pm| or larger: the A
+        // operands reach up to 2^12, high in the FP16 range, so the lo part of small entries stays normal
+        t = t > 1e-30f ? pow2_ceil(t) * 0x1p-12f : 1.f;
         if (tk == 0 && lane == 0) tau[(size_t)s * J.N + j] = t;
         const float tinv = 1.f / t;
         const int kb = tk % Sh::NKB, tile = tk / Sh::NKB;
@@ -980,6 +1000,17 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
     }
 }
 
+// Per group of WhShape<L>::JB column particles (blockIdx.x) and RHS (blockIdx.y).
+template <int L>
+__global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* __restrict__ sys, int nh,
+                                                         int nterms, const int2* __restrict__ hlist,
+                                                         const int* __restrict__ hsrc,
+                                                         const double2* __restrict__ hcoef, h8v* __restrict__ Wh,
+                                                         float* __restrict__ tau) {
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    wh_body<L>(J, blockIdx.x * WhShape<L>::JB, blockIdx.y, nh, nterms, hlist, hsrc, hcoef, Wh, tau);
+}
+
 // This lane's B operands (hi, lo) for pair (i, j): slots of lane half kh,
 // and ginv = 1 / g_ij (0 when !on; the geometry is replaced by a unit
 // distance there so nothing non-finite arises).  FP64: the separation, k d
@@ -991,10 +1022,12 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* 
 //            - sqrt((l+m-1)(l-m-1)/((l-m)(l+m))) Pn_{l-2}^m,
 // Pn = c_lm P (the standard recurrence of P scaled by c_lm).  g_ij = 1 / max
 // of |Re|, |Im| of h_PM, the largest order.
-template <int L>
+// POW2: the scale is the power of two g = 2^-E with g max|h_PM| in
+// [2^11, 2^12) (E returned; ginv unset), so that rescaling by it is exact.
+template <int L, bool POW2 = false>
 __device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const double* __restrict__ pos, int j, double k,
                                         float kinv, int kh, bool on, h8v (&bh)[F16Shape<L>::NKB],
-                                        h8v (&bl)[F16Shape<L>::NKB], float& ginv) {
+                                        h8v (&bl)[F16Shape<L>::NKB], float& ginv, int* E = nullptr) {
     using Sh = F16Shape<L>;
     constexpr int PM = Sh::PM, NKB = Sh::NKB;
     double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
@@ -1027,8 +1060,17 @@ __device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const d
         hy[l] = (2 * l - 1) * rz * hy[l - 1] - hy[l - 2];
     }
     const float hmax = fmaxf(fabsf(hx[PM]), fabsf(hy[PM]));
-    const float g = on ? __builtin_amdgcn_rcpf(hmax) : 0.f;
-    ginv = on ? hmax : 0.f;
+    float g;
+    if (POW2) {
+        // g max|h_PM| in [2^11, 2^12): the block's largest values sit high in the FP16 range, so the
+        // lo part of elements down to 2^-15 of the largest is still a normal number
+        const int e = (int)((__builtin_bit_cast(unsigned, hmax) >> 23) & 0xff) - 126 - 12;
+        *E = e;
+        g = on ? __builtin_bit_cast(float, (unsigned)(127 - e) << 23) : 0.f;
+    } else {
+        g = on ? __builtin_amdgcn_rcpf(hmax) : 0.f;
+        ginv = on ? hmax : 0.f;
+    }
 #pragma unroll
     for (int l = 0; l <= PM; l++) {
         hx[l] *= g;
@@ -1040,9 +1082,11 @@ __device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const d
     const float st = rho2 * rhoinv * rinv;
     const float e1x = rho2 > 0.f ? fx * rhoinv : 1.f;
     const float e1y = (kh ? -fy : fy) * rhoinv;  // e^{-i phi} on lane half 1
-    float vals[NKB * 8];
+    // each (Re, Im) pair is split into hi and lo as soon as it is formed (no array of all values)
 #pragma unroll
-    for (int t = 0; t < NKB * 8; t++) vals[t] = 0.f;
+    for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) bh[kb][e] = bl[kb][e] = (_Float16)0.f;
     float ex = kh ? 0.f : 1.f, ey = 0.f;  // m = 0 slots belong to lane half 0
     float pmm = 1.f;
     int u = 0;
@@ -1071,23 +1115,16 @@ __device__ __forceinline__ void hle_f16(double xi, double yi, double zi, const d
             p2 = p1;
             p1 = P;
             const float ar = hx[l] * P, ai = hy[l] * P;
-            vals[2 * u] = ar * ex - ai * ey;
-            vals[2 * u + 1] = ar * ey + ai * ex;
+            const f2v v = {ar * ex - ai * ey, ar * ey + ai * ex};
+            h2v hi, lo;
+            split_hilo(v, hi, lo);
+            bh[u >> 2][2 * (u & 3)] = hi.x;
+            bh[u >> 2][2 * (u & 3) + 1] = hi.y;
+            bl[u >> 2][2 * (u & 3)] = lo.x;
+            bl[u >> 2][2 * (u & 3) + 1] = lo.y;
             u++;
         }
     }
-#pragma unroll
-    for (int kb = 0; kb < NKB; kb++)
-#pragma unroll
-        for (int e2 = 0; e2 < 4; e2++) {
-            const f2v v = {vals[kb * 8 + 2 * e2], vals[kb * 8 + 2 * e2 + 1]};
-            h2v hi, lo;
-            split_hilo(v, hi, lo);
-            bh[kb][2 * e2] = hi.x;
-            bh[kb][2 * e2 + 1] = hi.y;
-            bl[kb][2 * e2] = lo.x;
-            bl[kb][2 * e2 + 1] = lo.y;
-        }
 }
 
 // The B operands of a column pair (j, j + 1) with the special functions of
@@ -1443,6 +1480,244 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf16(MfJob J, const SysDev
                 const int Rr = 32 * t + 8 * (q >> 2) + 4 * kh + (q & 3);  // real output column 2 cc + part
                 if (Rr < 2 * NC) out[Rr] = tot[t][q];
             }
+    }
+}
+
+// ---- complex64 operator: sources sharing each pair's special functions, many jobs per launch
+//
+// k_matfree_mf16s is the contraction of k_matfree_mf16 with SG right-hand
+// sides per workgroup: the B operands (the pair's scaled HLE values, split
+// hi/lo) are formed once per pair and feed the 3 NKB NT MFMAs of each of the
+// SG sources, whose A operands (W_j of that source, from k_matfree_wh) are
+// staged in LDS side by side.  Each source keeps its running total R in the
+// scale of the current pair: the pair's factor F = tau_j 2^E_ij (tau_j a
+// power of two from k_matfree_wh, 2^-E_ij the power-of-two scale of the
+// pair's B operands, 2^-E max|h_PM| in [2^11, 2^12)); the pair's block sum is
+// accumulated from zero in one accumulator set shared by the sources in
+// turn, and R = R 2^(E_prev - E_new) + acc (the scale is exact), so the total
+// needs one register set per source instead of two, with the roundings of
+// k_matfree_mf16; the result is R 2^E at the end.
+//
+// Jobs (one cluster operator product each) travel by value in the kernel
+// arguments, up to kMfBatchMax per launch; each job keeps its own
+// decomposition (chunks of columns x 128-row groups x source groups), the
+// same whatever else shares the launch, so a system's arithmetic does not
+// depend on its batch.  Work ids are dealt to the jobs in order and
+// XCD-swizzled as in the one-job kernels.
+constexpr int kMfBatchMax = 16;
+
+struct Mf16Job {
+    MfJob J;
+    const f4v* Wh;      // [S][N][PER] operands of k_matfree_wh
+    const float* tau;   // [S][N] power-of-two column scales
+    float2* part32;     // [nch][S][n] chunk partial rows
+    int nch, chunk;     // column chunks and columns per chunk
+    int groups, nsg;    // 128-row groups, source groups
+};
+
+struct Mf16Batch {
+    int count;
+    int first[kMfBatchMax + 1];  // prefix of each job's workgroups (contraction, operands or sums)
+    Mf16Job job[kMfBatchMax];
+};
+
+__device__ __forceinline__ int batch_job(const Mf16Batch& B, int v) {
+    int q = 0;
+    while (q + 1 < B.count && v >= B.first[q + 1]) q++;
+    return q;
+}
+
+// Operands of every job: work id = (source, column group) of the job's own wh grid.
+template <int L>
+__global__ __launch_bounds__(kThreads) void k_matfree_wh_b(Mf16Batch B, const SysDev* __restrict__ sys, int nh,
+                                                           int nterms, const int2* __restrict__ hlist,
+                                                           const int* __restrict__ hsrc,
+                                                           const double2* __restrict__ hcoef) {
+    const int v = blockIdx.x;
+    const int q = batch_job(B, v);
+    const Mf16Job& Q = B.job[q];
+    if (Q.J.skip >= 0 && job_done(sys[Q.J.skip])) return;
+    constexpr int JB = WhShape<L>::JB;
+    const int ng = (Q.J.N + JB - 1) / JB;
+    const int w = v - B.first[q];
+    const int s = w / ng, jg = w - s * ng;
+    wh_body<L>(Q.J, jg * JB, s, nh, nterms, hlist, hsrc, hcoef, (h8v*)Q.Wh, (float*)Q.tau);
+}
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
+
+template <int L, int SG>
+__global__ __launch_bounds__(kThreads, 2) void k_matfree_mf16s(Mf16Batch B, const SysDev* __restrict__ sys) {
+    using Sh = F16Shape<L>;
+    constexpr int NC = Sh::NC, NT = Sh::NT, NKB = Sh::NKB, PER = Sh::PER;
+    static_assert(PER % kThreads == 0, "operand blocks are whole workgroup rounds");
+    // one operand buffer (lmax 3, SG 2: 57 KB), so two workgroups fit a CU and one's special functions
+    // overlap the other's MFMAs
+    __shared__ f4v Ws[SG * PER];
+    const int v0 = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int q = batch_job(B, v0);
+    const Mf16Job& Q = B.job[q];
+    const MfJob& J = Q.J;
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    int v = v0 - B.first[q];
+    const int group = v % Q.groups;
+    v /= Q.groups;
+    const int c = v % Q.nch;
+    const int s0 = (v / Q.nch) * SG;
+    const int N = J.N, S = J.S;
+    const int nsg = min(SG, S - s0);
+    const int jbeg = c * Q.chunk, jend = min(N, jbeg + Q.chunk);
+    if (jbeg >= jend) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int kh = lane >> 5, col = lane & 31;
+    const int i = group * 128 + w * 32 + col;
+    const bool live = i < N;
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const float kinv = (float)(1.0 / J.k);
+    const f4v* __restrict__ Wh = Q.Wh;
+    const float* __restrict__ tau = Q.tau;
+    f16v R[SG][NT];
+    int cur[SG];
+#pragma unroll
+    for (int g = 0; g < SG; g++) {
+        cur[g] = 0;
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) R[g][t][e] = 0.f;
+    }
+    // the SG sources' operand blocks of column j, side by side (entry u = g PER + e), loaded straight
+    // into LDS: each wave fills 64 consecutive entries per round
+    auto fetch = [&](f4v* dst, int j) {
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+            if (g >= nsg) break;  // uniform
+            const f4v* src = Wh + ((size_t)(s0 + g) * N + j) * PER + w * 64 + lane;  // wave-uniform base + lane
+#pragma unroll
+            for (int p = 0; p < PER / kThreads; p++)
+                __builtin_amdgcn_global_load_lds((const void*)(src + p * kThreads),
+                                                 (lds_ptr_t)(dst + g * PER + p * kThreads + w * 64), 16, 0, 0);
+        }
+    };
+    h8v bh[NKB], bl[NKB];
+    float unused;
+    // Per column: the pair's B operands (VALU, while the DMA of this column's operands, issued at the
+    // end of the previous column, lands), a wait and a barrier (every wave's DMA is in LDS), the rescale
+    // and the MFMAs, a barrier (every wave has read the buffer), then the DMA of the next column.
+    fetch(Ws, jbeg);
+    for (int j = jbeg; j < jend; j++) {
+        const bool on = live && j != ii;
+        int E = 0;
+        hle_f16<L, true>(xi, yi, zi, J.pos, j, J.k, kinv, kh, on, bh, bl, unused, &E);
+        asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+        __syncthreads();
+        // per source: the pair's block sum in an accumulator that starts at zero (its roundings stay at
+        // the pair's own magnitude), then R = R 2^(E_prev - E_new) + acc in one FMA (the scale is exact)
+#pragma unroll
+        for (int g = 0; g < SG; g++) {
+            if (g >= nsg) continue;  // uniform: the last source group of a job may be short
+            const float tj = tau[(size_t)(s0 + g) * N + j];
+            const int nw = (int)((__builtin_bit_cast(unsigned, tj) >> 23) & 0xff) - 127 + E;
+            const float scale = __builtin_amdgcn_ldexpf(1.f, on ? cur[g] - nw : 0);
+            cur[g] = on ? nw : cur[g];
+            f16v acc[NT];
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc[t][e] = 0.f;
+#pragma unroll
+            for (int kb = 0; kb < NKB; kb++)
+#pragma unroll
+                for (int t = 0; t < NT; t++) {
+                    const h8v ah = __builtin_bit_cast(h8v, Ws[g * PER + (t * NKB + kb) * 64 + lane]);
+                    const h8v al = __builtin_bit_cast(h8v, Ws[g * PER + ((NT + t) * NKB + kb) * 64 + lane]);
+                    f16v& a = acc[t];
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[kb], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[kb], a, 0, 0, 0);
+                    a = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[kb], a, 0, 0, 0);
+                }
+#pragma unroll
+            for (int t = 0; t < NT; t++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) R[g][t][e] = fmaf(R[g][t][e], scale, acc[t][e]);
+        }
+        __syncthreads();
+        if (j + 1 < jend) fetch(Ws, j + 1);
+    }
+    if (!live) return;
+    const size_t n = (size_t)NC * N;
+    const bool direct = Q.nch == 1;  // one chunk: the final rows (identity term included) straight into y
+#pragma unroll
+    for (int g = 0; g < SG; g++) {
+        if (g >= nsg) continue;
+        const size_t row = (size_t)(s0 + g) * n + (size_t)i * NC;
+        float* out = direct ? (float*)J.y + 2 * row : (float*)Q.part32 + 2 * ((size_t)c * S * n + row);
+        const float* xin = (const float*)J.x + 2 * row;
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                const int Rr = 32 * t + 8 * (e >> 2) + 4 * kh + (e & 3);  // real output column 2 cc + part
+                if (Rr < 2 * NC) {
+                    float val = __builtin_amdgcn_ldexpf(R[g][t][e], cur[g]);
+                    if (direct && J.add_identity) val += xin[Rr];
+                    out[Rr] = val;
+                }
+            }
+    }
+}
+
+// y = (add_identity ? x : 0) + the chunk partials in chunk order, for every
+// job with more than one chunk: work id = (job, block of 64 elements x 4
+// waves) as in k_matfree_sum32.
+__global__ __launch_bounds__(kThreads) void k_matfree_sum32_b(Mf16Batch B, const SysDev* __restrict__ sys) {
+    __shared__ double2 red[kThreads / 64 - 1][64];
+    const int q = batch_job(B, blockIdx.x);
+    const Mf16Job& Q = B.job[q];
+    const MfJob& J = Q.J;
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int nchunks = Q.nch;
+    const size_t total = (size_t)J.S * 2 * lmax_to_rmax(J.lmax) * J.N;
+    const float2* X = (const float2*)J.x;
+    float2* Y = (float2*)J.y;
+    const float2* __restrict__ part32 = Q.part32;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int per = (nchunks + kThreads / 64 - 1) / (kThreads / 64);
+    const int c0 = w * per, c1 = min(nchunks, c0 + per);
+    const size_t idx = (size_t)(blockIdx.x - B.first[q]) * 64 + lane;
+    double ax = 0, ay = 0;
+    if (idx < total) {
+        int cc = c0;
+        for (; cc + 8 <= c1; cc += 8) {
+            float2 vv[8];
+#pragma unroll
+            for (int p = 0; p < 8; p++) vv[p] = part32[(size_t)(cc + p) * total + idx];
+#pragma unroll
+            for (int p = 0; p < 8; p++) {
+                ax += vv[p].x;
+                ay += vv[p].y;
+            }
+        }
+        for (; cc < c1; cc++) {
+            const float2 vv = part32[(size_t)cc * total + idx];
+            ax += vv.x;
+            ay += vv.y;
+        }
+    }
+    if (w > 0) red[w - 1][lane] = make_double2(ax, ay);
+    __syncthreads();
+    if (w == 0 && idx < total) {
+#pragma unroll
+        for (int p = 0; p < kThreads / 64 - 1; p++) {
+            ax += red[p][lane].x;
+            ay += red[p][lane].y;
+        }
+        if (J.add_identity) {
+            ax += X[idx].x;
+            ay += X[idx].y;
+        }
+        Y[idx] = make_float2((float)ax, (float)ay);
     }
 }
 
@@ -1901,10 +2176,108 @@ size_t matfree_workspace_bytes(int N, int lmax, int S) {
 constexpr int kC64MaxLmax = 4;
 bool matfree_c64_supported(int N, int lmax, bool tmatrix) { return lmax <= kC64MaxLmax && N >= 1 && !tmatrix; }
 
+// Sources per workgroup of k_matfree_mf16s: its LDS holds two buffers of SG
+// operand blocks (lmax 3: 2 x 57 KB; lmax 4: 2 x 74 KB at SG = 1).
+constexpr int mf16s_sg(int L) { return L <= 2 ? 4 : (L == 3 ? 2 : 1); }
+
+// Column chunks of one job in k_matfree_mf16s: about 2048 workgroups for
+// the job alone (128-row groups x source groups x chunks: four rounds of two
+// workgroups per CU, so the last, partly filled round costs little), but no
+// chunk under 32 columns (a workgroup's set-up and partial rows stay small
+// against its columns), and never more chunks than the workspace holds
+// partial rows for.
+inline int mf16s_chunks(int N, int S, int L) {
+    const int groups = (N + 127) / 128, nsg = (S + mf16s_sg(L) - 1) / mf16s_sg(L);
+    int c = (2048 + groups * nsg - 1) / (groups * nsg);
+    c = std::min(c, std::max(1, N / 32));
+    c = std::min(c, mf32_chunks(N));
+    return c < 1 ? 1 : c;
+}
+
+template <int L>
+int launch_mf16s_pack(const MfJob* const* jobs, int count, const SysDev* sys, hipStream_t st, const TransTable* T) {
+    constexpr int SG = mf16s_sg(L), JB = WhShape<L>::JB;
+    Mf16Batch Bw{}, Bc{}, Bs{};
+    Bw.count = Bc.count = count;
+    for (int k = 0; k < count; k++) {
+        const MfJob& J = *jobs[k];
+        const size_t n = (size_t)2 * lmax_to_rmax(L) * J.N, total = (size_t)J.S * n;
+        Mf16Job Q{};
+        Q.J = J;
+        Q.part32 = (float2*)J.part;
+        Q.Wh = (const f4v*)(Q.part32 + (size_t)mf32_chunks(J.N) * total);
+        Q.tau = (const float*)((const h8v*)Q.Wh + (size_t)J.S * J.N * F16Shape<L>::PER);
+        Q.groups = (J.N + 127) / 128;
+        Q.nsg = (J.S + SG - 1) / SG;
+        Q.chunk = (J.N + mf16s_chunks(J.N, J.S, L) - 1) / mf16s_chunks(J.N, J.S, L);
+        Q.nch = (J.N + Q.chunk - 1) / Q.chunk;  // every chunk non-empty: each writes its partial rows
+        Bw.job[k] = Bc.job[k] = Q;
+        Bw.first[k + 1] = Bw.first[k] + J.S * ((J.N + JB - 1) / JB);
+        Bc.first[k + 1] = Bc.first[k] + Q.groups * Q.nch * Q.nsg;
+        if (Q.nch > 1) {
+            Bs.job[Bs.count] = Q;
+            Bs.first[Bs.count + 1] = Bs.first[Bs.count] + (int)((total + 63) / 64);
+            Bs.count++;
+        }
+    }
+    const dim3 block(kThreads);
+    hipLaunchKernelGGL((k_matfree_wh_b<L>), dim3((unsigned)Bw.first[count]), block, 0, st, Bw, sys, T->nh,
+                       T->nhterms, T->hlist, T->hterm_src, T->hterm_coef);
+    hipLaunchKernelGGL((k_matfree_mf16s<L, SG>), dim3((unsigned)Bc.first[count]), block, 0, st, Bc, sys);
+    if (Bs.count)
+        hipLaunchKernelGGL(k_matfree_sum32_b, dim3((unsigned)Bs.first[Bs.count]), block, 0, st, Bs, sys);
+    MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+// complex64 products of sphere clusters (lmax <= 4) through k_matfree_mf16s:
+// jobs grouped by lmax, kMfBatchMax per launch.
+int matfree_launch_c64_shared(const MfJob* const* jobs, int count, const SysDev* sys, hipStream_t st) {
+    ProfScope prof(5, st);
+    for (int L = 1; L <= kC64MaxLmax; L++) {
+        const TransTable* T = nullptr;
+        std::vector<const MfJob*> sel;
+        for (int k = 0; k < count; k++)
+            if (jobs[k]->lmax == L) sel.push_back(jobs[k]);
+        if (sel.empty()) continue;
+        T = mpa_get_table(L);
+        if (!T) return MPA_ERR_ARG;
+        if (L <= 3 && T->nhterms > kWhMaxTerms) {
+            mpa_set_error("matfree: %d contraction terms exceed the operand kernel's %d", T->nhterms, kWhMaxTerms);
+            return MPA_ERR_ARG;
+        }
+        for (size_t b = 0; b < sel.size(); b += kMfBatchMax) {
+            const int cnt = (int)std::min<size_t>(kMfBatchMax, sel.size() - b);
+            long long wg = 0;
+            for (int k = 0; k < cnt; k++) {
+                const MfJob& J = *sel[b + k];
+                wg += (long long)J.S * J.N * 2 * lmax_to_rmax(L);
+            }
+            if (wg > 0x7fffffffLL) {
+                mpa_set_error("matfree: batch too large for one launch");
+                return MPA_ERR_ARG;
+            }
+            int rc;
+            switch (L) {
+                case 1: rc = launch_mf16s_pack<1>(sel.data() + b, cnt, sys, st, T); break;
+                case 2: rc = launch_mf16s_pack<2>(sel.data() + b, cnt, sys, st, T); break;
+                case 3: rc = launch_mf16s_pack<3>(sel.data() + b, cnt, sys, st, T); break;
+                default: rc = launch_mf16s_pack<4>(sel.data() + b, cnt, sys, st, T); break;
+            }
+            if (rc) return rc;
+        }
+    }
+    return MPA_OK;
+}
+
 int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const TransTable* T) {
     if (!matfree_c64_supported(J.N, J.lmax, J.tmat != nullptr)) {
         mpa_set_error("matfree: the complex64 operator covers spheres with lmax <= %d", kC64MaxLmax);
         return MPA_ERR_ARG;
+    }
+    if (mf_shared_for(J.S, J.lmax) && mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0) {
+        const MfJob* one = &J;
+        return matfree_launch_c64_shared(&one, 1, sys, st);
     }
     const int nh = T->nh;
     const size_t n = (size_t)2 * lmax_to_rmax(J.lmax) * J.N;
@@ -1939,7 +2312,8 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
                                        (const f4v*)Whp, taup, part32); break;                                       \
             case 4:                                                                                         \
             case 5:                                                                                         \
-                if (L_ <= 3 && g_mf_variant[0] == 4)                                                        \
+            case 7:                                                                                         \
+                if (L_ <= 3 && g_mf_variant[0] != 5)                                                        \
                     hipLaunchKernelGGL((k_matfree_mf16p<(L_ < 3 ? L_ : 3), 1>), grid, block, 0, st, J, sys, nch,     \
                                        chunk, (const f4v*)Whp, taup, part32);                                       \
                 else if (L_ <= 3)                                                                           \
@@ -2076,6 +2450,20 @@ int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
     return MPA_OK;
 }
 
+int matfree_launch_many(const std::vector<MfJob>& jobs, const SysDev* sys, hipStream_t st) {
+    std::vector<const MfJob*> shared;
+    for (const MfJob& j : jobs) {
+        if (j.c64 && mf_shared_for(j.S, j.lmax) && mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0 &&
+            matfree_c64_supported(j.N, j.lmax, j.tmat != nullptr)) {
+            shared.push_back(&j);
+            continue;
+        }
+        const int rc = matfree_launch(j, sys, st);
+        if (rc) return rc;
+    }
+    return shared.empty() ? MPA_OK : matfree_launch_c64_shared(shared.data(), (int)shared.size(), sys, st);
+}
+
 }  // namespace mpa
 
 using namespace mpa;
@@ -2095,6 +2483,30 @@ extern "C" int mpa_matfree_matvec_c64(const mpa_cluster_operator* op, int nrhs, 
     MfJob J{op->N, op->lmax, nrhs, add_identity ? 1 : 0, op->positions, (const double2*)op->mie, op->k,
             (const double2*)x, (double2*)y, (double2*)op->workspace, -1, 0, nullptr, nullptr, 1, 0};
     return matfree_launch(J, nullptr, (hipStream_t)stream);
+}
+
+extern "C" int mpa_matfree_matvec_batched(int count, const mpa_cluster_operator* const* ops, const int* nrhs, int dtype,
+                                          int add_identity, const void* const* x, void* const* y, void* stream) {
+    if (count < 1 || !ops || !nrhs || !x || !y || (dtype != MPA_C128 && dtype != MPA_C64)) {
+        mpa_set_error("matfree_matvec_batched: bad argument");
+        return MPA_ERR_ARG;
+    }
+    std::vector<MfJob> jobs;
+    for (int q = 0; q < count; q++) {
+        const mpa_cluster_operator* op = ops[q];
+        if (!op || op->N < 1 || nrhs[q] < 1 || !op->positions || !op->mie || op->tmatrix || !op->workspace || !x[q] ||
+            !y[q] || !(op->k > 0) || op->lmax < 1 || op->lmax > MPA_LMAX_MAX ||
+            (dtype == MPA_C64 && !matfree_c64_supported(op->N, op->lmax, false))) {
+            mpa_set_error("matfree_matvec_batched: bad operator %d (spheres only; complex64: lmax <= %d)", q,
+                          kC64MaxLmax);
+            return MPA_ERR_ARG;
+        }
+        if (!mpa_get_table(op->lmax)) return MPA_ERR_ARG;
+        jobs.push_back(MfJob{op->N, op->lmax, nrhs[q], add_identity ? 1 : 0, op->positions, (const double2*)op->mie,
+                             op->k, (const double2*)x[q], (double2*)y[q], (double2*)op->workspace, -1, 0, nullptr,
+                             nullptr, dtype == MPA_C64 ? 1 : 0, 0});
+    }
+    return matfree_launch_many(jobs, nullptr, (hipStream_t)stream);
 }
 
 extern "C" int mpa_matfree_matvec(const mpa_cluster_operator* op, int nrhs, int add_identity, const void* x, void* y,

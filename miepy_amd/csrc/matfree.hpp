@@ -2,6 +2,8 @@
 // batched Krylov engine.
 #pragma once
 
+#include <vector>
+
 #include "krylov.hpp"
 
 namespace mpa {
@@ -31,5 +33,9 @@ size_t matfree_workspace_bytes(int N, int lmax, int S);
 
 // Launches the two kernels of one job; `sys` may be null when skip < 0.
 int matfree_launch(const MfJob& job, const SysDev* sys, hipStream_t st);
+
+// Launches every job: complex64 sphere jobs share launches (grouped by lmax)
+// when the shared kernel is selected, the others run one by one.
+int matfree_launch_many(const std::vector<MfJob>& jobs, const SysDev* sys, hipStream_t st);
 
 }  // namespace mpa

@@ -355,10 +355,7 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st);
             if (rc) return rc;
         }
-        for (const MfJob& j : mf) {
-            rc = matfree_launch(j, nullptr, st);
-            if (rc) return rc;
-        }
+        if (!mf.empty() && (rc = matfree_launch_many(mf, nullptr, st))) return rc;
         hipLaunchKernelGGL(k_ref_residual, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         hipLaunchKernelGGL(k_ref_norms, dim3(nc), dim3(kThreads), 0, st, d_ref, d_cw);
         MPA_LAUNCH_CHECK();
@@ -640,8 +637,7 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
         bool any_mv = false;
         for (int c = 0; c < kTileClasses; c++) any_mv |= n_mw[c] > 0;
         if (any_mv && (rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st))) return rc;
-        for (const MfJob& j : L.mf)
-            if ((rc = matfree_launch(j, nullptr, st))) return rc;
+        if (!L.mf.empty() && (rc = matfree_launch_many(L.mf, nullptr, st))) return rc;
         hipLaunchKernelGGL(k_ref_residual, dim3((unsigned)L.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
         hipLaunchKernelGGL(k_ref_norms, dim3((unsigned)L.cw.size()), dim3(kThreads), 0, st, d_ref, d_cw);
         MPA_LAUNCH_CHECK();

@@ -601,6 +601,31 @@ def matfree_matvec(op, x, add_identity=False, out=None):
     return out.reshape(x.shape)
 
 
+def matfree_matvec_batched(ops, xs, add_identity=False, outs=None):
+    """y_q = (I +) T_q x_q for every operator q in one batched call
+    (mpa_matfree_matvec_batched); xs [nrhs_q, n_q], all complex128 or all
+    complex64.  Returns the list of outputs."""
+    lib = L.load()
+    ops, xs = list(ops), [x.reshape(-1, op.n) for op, x in zip(ops, xs)]
+    if len(ops) != len(xs) or not ops:
+        raise ValueError("one right-hand-side block per operator")
+    dt = xs[0].dtype
+    if dt not in (C128, C64) or any(x.dtype != dt or not x.is_contiguous() for x in xs):
+        raise ValueError("xs must be contiguous, all complex128 or all complex64")
+    for op, x in zip(ops, xs):
+        op.reserve(x.shape[0])
+    outs = [torch.empty_like(x) for x in xs] if outs is None else [o.reshape(x.shape) for o, x in zip(outs, xs)]
+    cnt = len(ops)
+    structs = [op.struct for op in ops]
+    optr = (ctypes.c_void_p * cnt)(*[ctypes.cast(ctypes.pointer(s), ctypes.c_void_p).value for s in structs])
+    L.check(lib.mpa_matfree_matvec_batched(cnt, optr, (ctypes.c_int * cnt)(*[x.shape[0] for x in xs]), DTYPE_CODE[dt],
+                                           1 if add_identity else 0,
+                                           (ctypes.c_void_p * cnt)(*[x.data_ptr() for x in xs]),
+                                           (ctypes.c_void_p * cnt)(*[o.data_ptr() for o in outs]), _stream()),
+            "matfree_matvec_batched")
+    return outs
+
+
 def autotune_matfree(op, nrhs=1, dtype=C64, reps=3):
     """Time the matrix-core contraction variants of the matrix-free operator
     on `op` and keep the fastest for dtype.  Returns (variant, ms)."""

@@ -218,16 +218,23 @@ def _factorized_vs_oracle(dev, oracle):
     assert _rel_err(got.cpu().numpy()[0], oracle.gemv(T, x)) <= 1e-12
 
 
-@pytest.fixture(params=[0, 1], ids=["fp16split", "fp32"])
+@pytest.fixture(params=[(0, 4), (0, 6), (1, None)], ids=["fp16split", "fp16shared", "fp32"])
 def c64_kernel(request, dev):
     """Selects the complex64 matrix-free kernel (FP16 matrix cores on hi/lo
-    split operands, or FP32 matrix cores) and restores the default after."""
+    split operands: the column-pair kernel, or the kernel whose sources share
+    each pair's special functions, with jobs batched per launch; or FP32
+    matrix cores) and restores the defaults after."""
     import miepy_amd._lib as L
     lib = L.load()
+    option, variant = request.param
     old = lib.mpa_get_option(L.MPA_OPT_MATFREE_C64_KERNEL)
-    lib.mpa_set_option(L.MPA_OPT_MATFREE_C64_KERNEL, request.param)
+    old_v = lib.mpa_matfree_variant(L.MPA_C64, -1, None)
+    lib.mpa_set_option(L.MPA_OPT_MATFREE_C64_KERNEL, option)
+    if variant is not None:
+        assert lib.mpa_matfree_variant(L.MPA_C64, variant, None) == variant
     yield request.param
     lib.mpa_set_option(L.MPA_OPT_MATFREE_C64_KERNEL, old)
+    lib.mpa_matfree_variant(L.MPA_C64, old_v, None)
 
 
 def _c64_vs_c128(dev, pos, mie, k, S, seed, identity):
