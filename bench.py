@@ -55,6 +55,10 @@ def parse():
     ap.add_argument("--matfree-min-n", type=int, default=6000,
                     help="batch256 mixed_matfree: clusters with n >= this (and lmax <= 4) run matrix-free")
     ap.add_argument("--cpu-seconds", type=float, default=20.0, help="oracle solve time of the CPU sample")
+    ap.add_argument("--roofline-only", action="store_true",
+                    help="batch256: build the --precision path's batch, solve once, and time only its dominant "
+                         "kernels over one product of the whole batch (the roofline measurement of the bench line; "
+                         "a short command to run under rocprofv3)")
     ap.add_argument("--matfree-variant", type=int, default=None,
                     help="complex64 matrix-free kernel variant (mpa_matfree_variant); default: the library's")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -357,6 +361,13 @@ def bench_batch(args):
     from miepy_amd.batch import ClusterBatch
     ws = W.batch256(count=args.batch_count)
     S = len(ws[0].sources)
+    if args.roofline_only:
+        prec = "mixed_matfree" if args.precision == "default" else args.precision
+        B = ClusterBatch(ws, precision=prec, matfree_min_n=args.matfree_min_n)
+        B.solve()
+        print(json.dumps({"workload": "batch256", "precision": prec, "rooflines": batch_rooflines(B, S, reps=20)}),
+              flush=True)
+        return
     log(f"batch of {len(ws)} clusters: sum n^2 = {sum(w.n ** 2 for w in ws):.3e}, "
         f"largest n = {max(w.n for w in ws)}")
     out, res, roof = {}, {}, {}
