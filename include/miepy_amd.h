@@ -469,6 +469,33 @@ int mpa_cluster_observables(int N, int lmax, int nsrc, const double* p_inc, cons
                             const double* mie_scat, const double* mie_int, double k, double eps_b, double* p_scat,
                             double* p_int, double* C_part, double* C_total, double* F, double* T, void* stream);
 
+/* One cluster of mpa_cluster_observables_batched: the arguments of
+ * mpa_cluster_observables for that cluster (same shapes and meanings; p_int,
+ * mie_int, C_total and T may be NULL as there). */
+typedef struct {
+    int N, lmax, nsrc, pad;
+    const double* p_inc;
+    const double* p_src;
+    const double* mie_scat;
+    const double* mie_int;
+    double k, eps_b;
+    double* p_scat;
+    double* p_int;
+    double* C_part;
+    double* C_total;
+    double* F;
+    double* T;
+} mpa_obs_cluster;
+
+/* mpa_cluster_observables for `count` clusters of any sizes and lmax in a few
+ * launches (the cluster descriptors travel in the kernel arguments, 32 per
+ * launch); each cluster's results equal those of its own
+ * mpa_cluster_observables call bit for bit.  The reference reduces each
+ * cluster on its own (src/miepy/sphere_cluster.py:442-471,715-717 over
+ * cpp/src/flux.cpp:5-30 and cpp/src/forces.cpp:7-171); this batched form is
+ * for many independent clusters solved together. */
+int mpa_cluster_observables_batched(int count, const mpa_obs_cluster* clusters, void* stream);
+
 /* Force and/or torque on M particles from explicit coefficients.
  * Replaces forces.force_all / torque_all (cpp/src/forces_pyb.cpp:27-43 ->
  * cpp/src/forces.cpp:139-171): p_scat, p_inc [M, 2 rmax] c128 (device);

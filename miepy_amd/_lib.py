@@ -117,9 +117,19 @@ class MpaRefineSystem(ctypes.Structure):
                 ("maxiter", ctypes.c_int), ("max_refine", ctypes.c_int), ("op", ctypes.c_void_p)]
 
 
+class MpaObsCluster(ctypes.Structure):
+    """mpa_obs_cluster (include/miepy_amd.h)"""
+    _fields_ = [("N", ctypes.c_int), ("lmax", ctypes.c_int), ("nsrc", ctypes.c_int), ("pad", ctypes.c_int),
+                ("p_inc", ctypes.c_void_p), ("p_src", ctypes.c_void_p), ("mie_scat", ctypes.c_void_p),
+                ("mie_int", ctypes.c_void_p), ("k", ctypes.c_double), ("eps_b", ctypes.c_double),
+                ("p_scat", ctypes.c_void_p), ("p_int", ctypes.c_void_p), ("C_part", ctypes.c_void_p),
+                ("C_total", ctypes.c_void_p), ("F", ctypes.c_void_p), ("T", ctypes.c_void_p)]
+
+
 _ps = ctypes.POINTER(MpaSystem)
 _pop = ctypes.POINTER(MpaClusterOperator)
 _prs = ctypes.POINTER(MpaRefineSystem)
+_pobs = ctypes.POINTER(MpaObsCluster)
 SIGNATURES.update({
     "mpa_particle_aggregate_tmatrix": (_i, [_i, _i, _v, _v, _d, _i, _i, _v, _v]),
     "mpa_tmatrix_apply": (_i, [_i, _i, _i, _v, _v, _v, _v]),
@@ -140,6 +150,7 @@ SIGNATURES.update({
     "mpa_refine_state_bytes": (_sz, [_i, _i]),
     "mpa_refine_scratch_bytes": (_sz, [_i, _prs]),
     "mpa_solve_refined_batched": (_i, [_i, _prs, _v, _pi, _pi, _pd, _pi, _v]),
+    "mpa_cluster_observables_batched": (_i, [_i, _pobs, _v]),
 })
 
 _lib = None

@@ -106,13 +106,13 @@ class ClusterBatch:
         return self._reduce(xs)
 
     def _reduce(self, xs):
+        """Observables of every cluster in a few batched launches."""
         cs = self.clusters
-        self.results = []
-        for c, x in zip(cs, xs):
-            p_inc = x.reshape(c.S, c.w.N, 2, c.w.rmax)
-            obs = D.observables(p_inc, c.p_src, c.mie_scat, c.mie_int, c.k, c.eps_b)
-            obs["p_inc"] = p_inc
-            self.results.append(obs)
+        p_incs = [x.reshape(c.S, c.w.N, 2, c.w.rmax) for c, x in zip(cs, xs)]
+        self.results = D.observables_batched([(p, c.p_src, c.mie_scat, c.mie_int, c.k, c.eps_b)
+                                              for c, p in zip(cs, p_incs)])
+        for obs, p in zip(self.results, p_incs):
+            obs["p_inc"] = p
         return self.results
 
     def converged(self):

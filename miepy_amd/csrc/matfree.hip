@@ -854,20 +854,26 @@ __device__ __forceinline__ void combo_pm(int u, int PM, int& p, int& m) {
     p = m + (u - base);
 }
 
-// Per group of WhShape<L>::JB column particles j (blockIdx.x) and RHS (blockIdx.y):
-// W_j / c_pm into LDS, tau_j = its largest modulus, then the A operands (hi
-// and lo) in lane order Wh[s][j][hl][tile][kb][lane] (8 halves each) and
-// tau[s][j].  The flattened term lists of every (r, h) are staged in LDS
-// first (a few coalesced loads per thread), so the per-entry lookups are LDS
-// reads instead of chains of dependent global loads.
-constexpr int kWhMaxTerms = 1024;  // LDS term capacity (lmax 3 has 789); lmax 4 reads its terms through the caches
+// Per group of WhShape<L>::JB column particles j (blockIdx.x), every source
+// s in [s0, s1) in turn: W_j / c_pm into LDS, tau_j = its largest modulus,
+// then the A operands (hi and lo) in lane order Wh[s][j][hl][tile][kb][lane]
+// (8 halves each) and tau[s][j].  The flattened term lists of every (r, h)
+// are staged in LDS once per workgroup and serve all of its sources (packed:
+// start | count << 16 per list, 9-bit sources, FP32 coefficients), so each
+// entry's terms are LDS reads rather than chains of dependent global loads.
+// W is formed in FP32 (its terms are few and the operands keep 22 bits).
 
-// column particles per workgroup and whether the term lists are staged in LDS
+// column particles per workgroup, the LDS capacity of the term lists (lmax 3 has 789 terms, lmax 4 2452)
+// and the threads per workgroup (lmax 4: 512, so the 64 KB of LDS per workgroup still leaves 4 waves per SIMD)
 template <int L>
 struct WhShape {
     static constexpr int JB = L <= 3 ? 2 : 1;
-    static constexpr bool TLDS = L <= 3;
+    static constexpr int TERMS = L <= 3 ? 1024 : 2560;
+    static constexpr int TH = L <= 3 ? 256 : 512;
 };
+
+// The operand kernels' term capacity at lmax (for the launch-time check).
+inline int wh_term_capacity(int lmax) { return lmax <= 3 ? WhShape<3>::TERMS : WhShape<4>::TERMS; }
 
 // The smallest power of two >= t (t > 0 and normal): a power-of-two tau_j
 // makes every rescale by tau exact.
@@ -878,137 +884,159 @@ __device__ __forceinline__ float pow2_ceil(float t) {
 }
 
 template <int L>
-__device__ __forceinline__ void wh_body(const MfJob& J, int j0, int s, int nh, int nterms,
+__device__ __forceinline__ void wh_body(const MfJob& J, int j0, int s0, int s1, int nterms,
                                         const int2* __restrict__ hlist, const int* __restrict__ hsrc,
                                         const double2* __restrict__ hcoef, h8v* __restrict__ Wh,
                                         float* __restrict__ tau) {
     using Sh = F16Shape<L>;
     constexpr int NHm = (2 * L + 1) * (2 * L + 1);
-    constexpr int NC = Sh::NC, JB = WhShape<L>::JB, RM = L * (L + 2);
-    constexpr bool TLDS = WhShape<L>::TLDS;
+    constexpr int NC = Sh::NC, NT = Sh::NT, NKB = Sh::NKB, JB = WhShape<L>::JB, RM = L * (L + 2);
+    constexpr int TERMS = WhShape<L>::TERMS, TH = WhShape<L>::TH, NW = TH / 64;
+    static_assert(JB <= 2 && TH % 64 == 0, "the item loop assumes at most two columns and whole waves");
     __shared__ float2 Wn[JB][NHm][NC];
-    __shared__ double2 zs[JB][NC];  // Mie coefficient times x of column j, per (polarization, r)
-    __shared__ double cinv_s[NHm];  // 1 / c_pm = sqrt((p+m)!/(p-m)!) per HLE index
-    __shared__ int2 hl_s[RM * NHm];
-    __shared__ int slot_h[2][Sh::NKB * 4];  // HLE index of K slot u for lane half kh, -1 for the zero slots
-    __shared__ int hs_s[TLDS ? kWhMaxTerms : 1];
-    __shared__ double2 hc_s[TLDS ? kWhMaxTerms : 1];
-    __shared__ float red[JB][kThreads / 64];
+    __shared__ float2 zs[JB][NC];  // Mie coefficient times x of column j, per (polarization, r)
+    __shared__ float cinv_s[NHm];  // 1 / c_pm = sqrt((p+m)!/(p-m)!) per HLE index
+    __shared__ unsigned hl_s[RM * NHm];
+    __shared__ int4 slot_h[2][NKB];  // HLE indices of the 4 K slots of block kb for lane half kh, -1: zero slot
+    __shared__ unsigned short hs_s[TERMS];
+    __shared__ float2 hc_s[TERMS];
+    __shared__ float red[JB][NW];
     const int tid = threadIdx.x;
     const int lmax = J.lmax;
     const size_t n = (size_t)NC * J.N;
-    for (int q = tid; q < RM * NHm; q += kThreads) hl_s[q] = hlist[q];
-    if (TLDS)
-        for (int q = tid; q < nterms; q += kThreads) {
-            hs_s[q] = hsrc[q];
-            hc_s[q] = hcoef[q];
-        }
-    const int* tsrc = TLDS ? hs_s : hsrc;
-    const double2* tcoef = TLDS ? hc_s : hcoef;
-    if (tid < JB * NC) {
-        const int b = tid / NC, q = tid - b * NC, j = j0 + b;
-        double2 z = make_double2(0, 0);
-        if (j < J.N) {
-            const float2 xv = ((const float2*)J.x)[s * n + (size_t)j * NC + q];
-            const int a = q >= RM ? 1 : 0, r = q - a * RM;
-            const int v = (int)sqrtf((float)(r + 1) + 0.5f);
-            z = cmul(J.mie[((size_t)j * 2 + a) * lmax + v - 1], make_double2(xv.x, xv.y));
-        }
-        zs[b][q] = z;
+    for (int q = tid; q < RM * NHm; q += TH) {
+        const int2 se = hlist[q];
+        hl_s[q] = (unsigned)se.x | ((unsigned)se.y << 16);
+    }
+    for (int q = tid; q < nterms; q += TH) {
+        const double2 c = hcoef[q];
+        hs_s[q] = (unsigned short)hsrc[q];
+        hc_s[q] = make_float2((float)c.x, (float)c.y);
     }
     if (tid < NHm) {
         const int p = (int)sqrtf((float)tid + 0.5f), mu = tid - p * (p + 1), m = mu < 0 ? -mu : mu;
         double prod = 1.0;
         for (int q = p - m + 1; q <= p + m; q++) prod *= q;
-        cinv_s[tid] = sqrt(prod);
+        cinv_s[tid] = (float)sqrt(prod);
     }
-    if (tid < 2 * Sh::NKB * 4) {
-        const int kh = tid / (Sh::NKB * 4), u = tid - kh * (Sh::NKB * 4);
+    if (tid < 2 * NKB * 4) {
+        const int kh = tid / (NKB * 4), u = tid - kh * (NKB * 4);
         int h = -1;
         if (u < Sh::NCOMBO) {
             int p, m;
             combo_pm(u, Sh::PM, p, m);
             if (!(kh && m == 0)) h = p * (p + 1) + (kh ? -m : m);
         }
-        slot_h[kh][u] = h;
+        ((int*)&slot_h[kh][0])[u] = h;
     }
-    __syncthreads();
-    float mx[JB];
-#pragma unroll
-    for (int b = 0; b < JB; b++) mx[b] = 0.f;
-    for (int idx = tid; idx < NHm * NC; idx += kThreads) {
-        const int h = idx / NC, cc = idx - h * NC;
-        const int a = cc >= RM ? 1 : 0, r = cc - a * RM;
-        const int2 se = hl_s[r * nh + h];
-        double2 w[JB];
-#pragma unroll
-        for (int b = 0; b < JB; b++) w[b] = make_double2(0, 0);
-        for (int k = se.x; k < se.x + se.y; k++) {
-            const int src = tsrc[k];
-            const int zi = ((src >> 8) ? 1 - a : a) * RM + (src & 0xff);  // B terms couple the other polarization
-            const double2 cf = tcoef[k];
-#pragma unroll
-            for (int b = 0; b < JB; b++) cfma(w[b], cf, zs[b][zi]);
+    // this thread's output lane is fixed (whole waves per round): lane half, real row parity, row pair
+    const int lane = tid & 63, kh = lane >> 5, part = lane & 1, c0 = (lane & 31) >> 1;
+    for (int s = s0; s < s1; s++) {
+        if (tid < JB * NC) {
+            const int b = tid / NC, q = tid - b * NC, j = j0 + b;
+            double2 z = make_double2(0, 0);
+            if (j < J.N) {
+                const float2 xv = ((const float2*)J.x)[s * n + (size_t)j * NC + q];
+                const int a = q >= RM ? 1 : 0, r = q - a * RM;
+                const int v = (int)sqrtf((float)(r + 1) + 0.5f);
+                z = cmul(J.mie[((size_t)j * 2 + a) * lmax + v - 1], make_double2(xv.x, xv.y));
+            }
+            zs[b][q] = make_float2((float)z.x, (float)z.y);
         }
-        const double cinv = cinv_s[h];
+        __syncthreads();  // zs (and the staged tables at the first source)
+        float mx[JB];
+#pragma unroll
+        for (int b = 0; b < JB; b++) mx[b] = 0.f;
+        // entries (h, cc) = idx in steps of TH, the indices advanced without divisions
+        int h = tid / NC, cc = tid - (tid / NC) * NC;
+        for (int idx = tid; idx < NHm * NC; idx += TH) {
+            const int a = cc >= RM ? 1 : 0, r = cc - a * RM;
+            const unsigned se = hl_s[r * NHm + h];
+            const int k0 = (int)(se & 0xffffu), k1 = k0 + (int)(se >> 16);
+            float2 w[JB];
+#pragma unroll
+            for (int b = 0; b < JB; b++) w[b] = make_float2(0.f, 0.f);
+            for (int k = k0; k < k1; k++) {
+                const int src = hs_s[k];
+                const int zi = ((src >> 8) ? 1 - a : a) * RM + (src & 0xff);  // B terms couple the other polarization
+                const float2 cf = hc_s[k];
+#pragma unroll
+                for (int b = 0; b < JB; b++) cfma(w[b], cf, zs[b][zi]);
+            }
+            const float cinv = cinv_s[h];
+#pragma unroll
+            for (int b = 0; b < JB; b++) {
+                const float2 wn = make_float2(w[b].x * cinv, w[b].y * cinv);
+                Wn[b][h][cc] = wn;
+                mx[b] = fmaxf(mx[b], fmaxf(fabsf(wn.x), fabsf(wn.y)));
+            }
+            cc += TH % NC;
+            h += TH / NC;
+            if (cc >= NC) {
+                cc -= NC;
+                h++;
+            }
+        }
 #pragma unroll
         for (int b = 0; b < JB; b++) {
-            const float2 wn = make_float2((float)(w[b].x * cinv), (float)(w[b].y * cinv));
-            Wn[b][h][cc] = wn;
-            mx[b] = fmaxf(mx[b], fmaxf(fabsf(wn.x), fabsf(wn.y)));
+            for (int off = 32; off > 0; off >>= 1) mx[b] = fmaxf(mx[b], __shfl_xor(mx[b], off));
+            if (lane == 0) red[b][tid >> 6] = mx[b];
         }
-    }
+        __syncthreads();
+        // per column: a power of two 2^12 times smaller than its largest |W / c_pm| or larger (the A
+        // operands reach up to 2^12, high in the FP16 range, so the lo part of small entries stays normal)
+        // and its exact reciprocal
+        float tinv[JB];
 #pragma unroll
-    for (int b = 0; b < JB; b++) {
-        for (int off = 32; off > 0; off >>= 1) mx[b] = fmaxf(mx[b], __shfl_xor(mx[b], off));
-        if ((tid & 63) == 0) red[b][tid >> 6] = mx[b];
-    }
-    __syncthreads();
-    for (int item = tid; item < JB * Sh::NT * Sh::NKB * 64; item += kThreads) {
-        const int lane = item & 63, tkb = item >> 6;
-        const int b = tkb / (Sh::NT * Sh::NKB), tk = tkb - b * (Sh::NT * Sh::NKB);
-        const int j = j0 + b;
-        if (j >= J.N) break;  // items run in b order: the rest are past N too
-        float t = red[b][0];
-        for (int q = 1; q < kThreads / 64; q++) t = fmaxf(t, red[b][q]);
-        // a power of two 2^12 times smaller than the column's largest |W / c_pm| or larger: the A
-        // operands reach up to 2^12, high in the FP16 range, so the lo part of small entries stays normal
-        t = t > 1e-30f ? pow2_ceil(t) * 0x1p-12f : 1.f;
-        if (tk == 0 && lane == 0) tau[(size_t)s * J.N + j] = t;
-        const float tinv = 1.f / t;
-        const int kb = tk % Sh::NKB, tile = tk / Sh::NKB;
-        const int kh = lane >> 5, Rr = 32 * tile + (lane & 31), cc = Rr >> 1, part = Rr & 1;
-        h8v hv, lv;
+        for (int b = 0; b < JB; b++) {
+            float t = red[b][0];
 #pragma unroll
-        for (int e2 = 0; e2 < 4; e2++) {
-            const int h = slot_h[kh][4 * kb + e2];
-            const float2 wv = (cc < NC && h >= 0) ? Wn[b][h][cc] : make_float2(0.f, 0.f);
-            // A[col = 2cc + part][K = (Re, Im) of the slot]
-            f2v av;
-            av.x = (part ? wv.y : wv.x) * tinv;
-            av.y = (part ? wv.x : -wv.y) * tinv;
-            h2v hi, lo;
-            split_hilo(av, hi, lo);
-            hv[2 * e2] = hi.x;
-            hv[2 * e2 + 1] = hi.y;
-            lv[2 * e2] = lo.x;
-            lv[2 * e2 + 1] = lo.y;
+            for (int q = 1; q < NW; q++) t = fmaxf(t, red[b][q]);
+            t = t > 1e-30f ? pow2_ceil(t) * 0x1p-12f : 1.f;
+            tinv[b] = __builtin_bit_cast(float, (254u << 23) - __builtin_bit_cast(unsigned, t));
+            if (tid == 0 && j0 + b < J.N) tau[(size_t)s * J.N + j0 + b] = t;
         }
-        h8v* dst = Wh + ((size_t)s * J.N + j) * Sh::PER;
-        dst[tk * 64 + lane] = hv;
-        dst[(Sh::NT * Sh::NKB + tk) * 64 + lane] = lv;
+        for (int tkb = tid >> 6; tkb < JB * NT * NKB; tkb += NW) {
+            const int b = (JB == 2 && tkb >= NT * NKB) ? 1 : 0;
+            const int j = j0 + b;
+            if (j >= J.N) break;  // items run in b order: the rest are past N too
+            const int tk = tkb - b * (NT * NKB);
+            const int tile = tk / NKB, kb = tk - tile * NKB;
+            const int ccl = 16 * tile + c0;  // this lane's real rows 2 ccl + part
+            const float ti = (JB == 2 && b) ? tinv[JB - 1] : tinv[0];
+            const int4 hq = slot_h[kh][kb];
+            const int hh[4] = {hq.x, hq.y, hq.z, hq.w};
+            h8v hv, lv;
+#pragma unroll
+            for (int e2 = 0; e2 < 4; e2++) {
+                const float2 wv = (ccl < NC && hh[e2] >= 0) ? Wn[b][hh[e2]][ccl] : make_float2(0.f, 0.f);
+                // A[col = 2cc + part][K = (Re, Im) of the slot]
+                f2v av;
+                av.x = (part ? wv.y : wv.x) * ti;
+                av.y = (part ? wv.x : -wv.y) * ti;
+                h2v hi, lo;
+                split_hilo(av, hi, lo);
+                hv[2 * e2] = hi.x;
+                hv[2 * e2 + 1] = hi.y;
+                lv[2 * e2] = lo.x;
+                lv[2 * e2 + 1] = lo.y;
+            }
+            h8v* dst = Wh + ((size_t)s * J.N + j) * Sh::PER;
+            dst[tk * 64 + lane] = hv;
+            dst[(NT * NKB + tk) * 64 + lane] = lv;
+        }
+        __syncthreads();  // Wn, zs and red are rewritten for the next source
     }
 }
 
-// Per group of WhShape<L>::JB column particles (blockIdx.x) and RHS (blockIdx.y).
+// Per group of WhShape<L>::JB column particles (blockIdx.x), every source.
 template <int L>
-__global__ __launch_bounds__(kThreads) void k_matfree_wh(MfJob J, const SysDev* __restrict__ sys, int nh,
-                                                         int nterms, const int2* __restrict__ hlist,
+__global__ __launch_bounds__(WhShape<L>::TH) void k_matfree_wh(MfJob J, const SysDev* __restrict__ sys, int nterms, const int2* __restrict__ hlist,
                                                          const int* __restrict__ hsrc,
                                                          const double2* __restrict__ hcoef, h8v* __restrict__ Wh,
                                                          float* __restrict__ tau) {
     if (J.skip >= 0 && job_done(sys[J.skip])) return;
-    wh_body<L>(J, blockIdx.x * WhShape<L>::JB, blockIdx.y, nh, nterms, hlist, hsrc, hcoef, Wh, tau);
+    wh_body<L>(J, blockIdx.x * WhShape<L>::JB, 0, J.S, nterms, hlist, hsrc, hcoef, Wh, tau);
 }
 
 // This lane's B operands (hi, lo) for pair (i, j): slots of lane half kh,
@@ -1527,10 +1555,9 @@ __device__ __forceinline__ int batch_job(const Mf16Batch& B, int v) {
     return q;
 }
 
-// Operands of every job: work id = (source, column group) of the job's own wh grid.
+// Operands of every job: work id = column group of the job, every source in turn.
 template <int L>
-__global__ __launch_bounds__(kThreads) void k_matfree_wh_b(Mf16Batch B, const SysDev* __restrict__ sys, int nh,
-                                                           int nterms, const int2* __restrict__ hlist,
+__global__ __launch_bounds__(WhShape<L>::TH) void k_matfree_wh_b(Mf16Batch B, const SysDev* __restrict__ sys, int nterms, const int2* __restrict__ hlist,
                                                            const int* __restrict__ hsrc,
                                                            const double2* __restrict__ hcoef) {
     const int v = blockIdx.x;
@@ -1538,10 +1565,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_wh_b(Mf16Batch B, const Sy
     const Mf16Job& Q = B.job[q];
     if (Q.J.skip >= 0 && job_done(sys[Q.J.skip])) return;
     constexpr int JB = WhShape<L>::JB;
-    const int ng = (Q.J.N + JB - 1) / JB;
-    const int w = v - B.first[q];
-    const int s = w / ng, jg = w - s * ng;
-    wh_body<L>(Q.J, jg * JB, s, nh, nterms, hlist, hsrc, hcoef, (h8v*)Q.Wh, (float*)Q.tau);
+    const int jg = v - B.first[q];
+    wh_body<L>(Q.J, jg * JB, 0, Q.J.S, nterms, hlist, hsrc, hcoef, (h8v*)Q.Wh, (float*)Q.tau);
 }
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
@@ -1572,6 +1597,9 @@ __global__ __launch_bounds__(kThreads, 2) void k_matfree_mf16s(Mf16Batch B, cons
     const int kh = lane >> 5, col = lane & 31;
     const int i = group * 128 + w * 32 + col;
     const bool live = i < N;
+    // a wave whose 32 rows are all past N only helps load the operands: its SIMD goes to the other
+    // workgroup of the CU (the 128-row groups pad lmax-4 clusters of 125-377 spheres by 19 %)
+    const bool wlive = group * 128 + w * 32 < N;
     const int ii = live ? i : N - 1;
     const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
     const float kinv = (float)(1.0 / J.k);
@@ -1606,17 +1634,28 @@ __global__ __launch_bounds__(kThreads, 2) void k_matfree_mf16s(Mf16Batch B, cons
     // end of the previous column, lands), a wait and a barrier (every wave's DMA is in LDS), the rescale
     // and the MFMAs, a barrier (every wave has read the buffer), then the DMA of the next column.
     fetch(Ws, jbeg);
+    // lmax 4 skips the idle waves' work inside the loop below (measured faster there); lmax <= 3 in a loop
+    // of their own (inside, the condition costs the SG = 2 kernel registers and it spills)
+    if (L <= 3 && !wlive) {  // wave-uniform: the same DMA share and barriers per column as the working waves
+        for (int j = jbeg; j < jend; j++) {
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+            __syncthreads();
+            if (j + 1 < jend) fetch(Ws, j + 1);
+        }
+        return;
+    }
     for (int j = jbeg; j < jend; j++) {
         const bool on = live && j != ii;
         int E = 0;
-        hle_f16<L, true>(xi, yi, zi, J.pos, j, J.k, kinv, kh, on, bh, bl, unused, &E);
+        if (wlive) hle_f16<L, true>(xi, yi, zi, J.pos, j, J.k, kinv, kh, on, bh, bl, unused, &E);
         asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
         __syncthreads();
         // per source: the pair's block sum in an accumulator that starts at zero (its roundings stay at
         // the pair's own magnitude), then R = R 2^(E_prev - E_new) + acc in one FMA (the scale is exact)
 #pragma unroll
         for (int g = 0; g < SG; g++) {
-            if (g >= nsg) continue;  // uniform: the last source group of a job may be short
+            if (g >= nsg || !wlive) continue;  // uniform: the last source group of a job may be short
             const float tj = tau[(size_t)(s0 + g) * N + j];
             const int nw = (int)((__builtin_bit_cast(unsigned, tj) >> 23) & 0xff) - 127 + E;
             const float scale = __builtin_amdgcn_ldexpf(1.f, on ? cur[g] - nw : 0);
@@ -2041,10 +2080,13 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64s(MfJob J, const SysDe
         }
 }
 
-// Column chunks of the FP64 matrix-core kernel: about 1024 workgroups in total.
-inline int mf64_chunks(int N) {
-    const int groups = (N + 63) / 64;
-    int c = (1024 + groups - 1) / groups;
+// Column chunks of the FP64 matrix-core kernel: about 1024 workgroups in total
+// over the 64-row groups and the S sources (the sources are the grid's y), so
+// several sources mean fewer, longer chunks and fewer partial rows to sum.
+// The workspace is sized for S = 1, the most chunks.
+inline int mf64_chunks(int N, int S = 1) {
+    const int groups = (N + 63) / 64, gs = groups * (S < 1 ? 1 : S);
+    int c = (1024 + gs - 1) / gs;
     if (c > N) c = N;
     return c < 1 ? 1 : c;
 }
@@ -2212,7 +2254,7 @@ int launch_mf16s_pack(const MfJob* const* jobs, int count, const SysDev* sys, hi
         Q.chunk = (J.N + mf16s_chunks(J.N, J.S, L) - 1) / mf16s_chunks(J.N, J.S, L);
         Q.nch = (J.N + Q.chunk - 1) / Q.chunk;  // every chunk non-empty: each writes its partial rows
         Bw.job[k] = Bc.job[k] = Q;
-        Bw.first[k + 1] = Bw.first[k] + J.S * ((J.N + JB - 1) / JB);
+        Bw.first[k + 1] = Bw.first[k] + (J.N + JB - 1) / JB;
         Bc.first[k + 1] = Bc.first[k] + Q.groups * Q.nch * Q.nsg;
         if (Q.nch > 1) {
             Bs.job[Bs.count] = Q;
@@ -2221,7 +2263,7 @@ int launch_mf16s_pack(const MfJob* const* jobs, int count, const SysDev* sys, hi
         }
     }
     const dim3 block(kThreads);
-    hipLaunchKernelGGL((k_matfree_wh_b<L>), dim3((unsigned)Bw.first[count]), block, 0, st, Bw, sys, T->nh,
+    hipLaunchKernelGGL((k_matfree_wh_b<L>), dim3((unsigned)Bw.first[count]), dim3(WhShape<L>::TH), 0, st, Bw, sys,
                        T->nhterms, T->hlist, T->hterm_src, T->hterm_coef);
     hipLaunchKernelGGL((k_matfree_mf16s<L, SG>), dim3((unsigned)Bc.first[count]), block, 0, st, Bc, sys);
     if (Bs.count)
@@ -2242,8 +2284,9 @@ int matfree_launch_c64_shared(const MfJob* const* jobs, int count, const SysDev*
         if (sel.empty()) continue;
         T = mpa_get_table(L);
         if (!T) return MPA_ERR_ARG;
-        if (L <= 3 && T->nhterms > kWhMaxTerms) {
-            mpa_set_error("matfree: %d contraction terms exceed the operand kernel's %d", T->nhterms, kWhMaxTerms);
+        if (T->nhterms > wh_term_capacity(L)) {
+            mpa_set_error("matfree: %d contraction terms exceed the operand kernel's %d", T->nhterms,
+                          wh_term_capacity(L));
             return MPA_ERR_ARG;
         }
         for (size_t b = 0; b < sel.size(); b += kMfBatchMax) {
@@ -2291,17 +2334,18 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
     const dim3 grid((unsigned)(groups * nch), (unsigned)J.S), block(kThreads);
     const bool ks2 = g_mf_variant[0] == 1;
     if (mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0 || J.lmax > kFacMaxLmax) {
-        if (J.lmax <= 3 && T->nhterms > kWhMaxTerms) {
-            mpa_set_error("matfree: %d contraction terms exceed the operand kernel's %d", T->nhterms, kWhMaxTerms);
+        if (T->nhterms > wh_term_capacity(J.lmax)) {
+            mpa_set_error("matfree: %d contraction terms exceed the operand kernel's %d", T->nhterms,
+                          wh_term_capacity(J.lmax));
             return MPA_ERR_ARG;
         }
         h8v* Whp = (h8v*)Wx;
         float* taup = (float*)(Whp + (size_t)J.S * J.N * wh_per(J.lmax));
         const int jb = J.lmax <= 3 ? WhShape<3>::JB : WhShape<4>::JB;
-        const dim3 wgrid((unsigned)((J.N + jb - 1) / jb), (unsigned)J.S);
+        const dim3 wgrid((unsigned)((J.N + jb - 1) / jb));
 #define MPA_MF16(L_)                                                                                                \
     do {                                                                                                            \
-        hipLaunchKernelGGL((k_matfree_wh<L_>), wgrid, block, 0, st, J, sys, nh, T->nhterms, T->hlist,               \
+        hipLaunchKernelGGL((k_matfree_wh<L_>), wgrid, dim3(WhShape<L_>::TH), 0, st, J, sys, T->nhterms, T->hlist,  \
                            T->hterm_src, T->hterm_coef, Whp, taup);                                                 \
         switch (g_mf_variant[0]) {                                                                                  \
             case 1: hipLaunchKernelGGL((k_matfree_mf16<L_, 2, 0>), grid, block, 0, st, J, sys, nch, chunk,           \
@@ -2382,7 +2426,7 @@ int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
     if (mf64_path(J.lmax)) {
         const int nh = T->nh;
         double2* Wm = zbuf + total;
-        const int nch = mf64_chunks(J.N);
+        const int nch = mf64_chunks(J.N, J.S);
         const int chunk = (J.N + nch - 1) / nch;
         const int groups = (J.N + 63) / 64;
         const size_t wtot = (size_t)J.S * J.N * nh * (n / J.N);

@@ -127,17 +127,12 @@ __device__ void particle_torque(int lmax, const double2* ps_, const double2* pi_
     To[2] = T2;
 }
 
-// fused: p_scat / p_int from p_inc, then cross-sections, force, torque
-__global__ __launch_bounds__(kThreads) void k_observables(int N, int lmax, int S, const double2* __restrict__ p_inc,
-                                                          const double2* __restrict__ p_src,
-                                                          const double2* __restrict__ mie_scat,
-                                                          const double2* __restrict__ mie_int, double k, double eps_b,
-                                                          double2* __restrict__ p_scat, double2* __restrict__ p_int,
-                                                          double* __restrict__ C_part, double* __restrict__ F,
-                                                          double* __restrict__ Tq) {
-    const int idx = blockIdx.x * kThreads + threadIdx.x;
-    if (idx >= N * S) return;
-    const int s = idx / N, i = idx - s * N;
+// fused, for particle i and source s: p_scat / p_int from p_inc, then cross-sections, force, torque
+__device__ void observables_one(int N, int lmax, int s, int i, const double2* __restrict__ p_inc,
+                                const double2* __restrict__ p_src, const double2* __restrict__ mie_scat,
+                                const double2* __restrict__ mie_int, double k, double eps_b,
+                                double2* __restrict__ p_scat, double2* __restrict__ p_int, double* __restrict__ C_part,
+                                double* __restrict__ F, double* __restrict__ Tq) {
     const int rmax = lmax_to_rmax(lmax);
     const size_t base = ((size_t)s * N + i) * 2 * rmax;
     for (int n = 1; n <= lmax; n++)
@@ -152,6 +147,48 @@ __global__ __launch_bounds__(kThreads) void k_observables(int N, int lmax, int S
     particle_xs(lmax, p_scat + base, p_inc + base, p_src + base, k, C_part + ((size_t)s * N + i) * 6 * lmax);
     particle_force(lmax, p_scat + base, p_inc + base, k, eps_b, F + ((size_t)s * N + i) * 3);
     if (Tq) particle_torque(lmax, p_scat + base, p_inc + base, k, eps_b, Tq + ((size_t)s * N + i) * 3);
+}
+
+__global__ __launch_bounds__(kThreads) void k_observables(int N, int lmax, int S, const double2* __restrict__ p_inc,
+                                                          const double2* __restrict__ p_src,
+                                                          const double2* __restrict__ mie_scat,
+                                                          const double2* __restrict__ mie_int, double k, double eps_b,
+                                                          double2* __restrict__ p_scat, double2* __restrict__ p_int,
+                                                          double* __restrict__ C_part, double* __restrict__ F,
+                                                          double* __restrict__ Tq) {
+    const int idx = blockIdx.x * kThreads + threadIdx.x;
+    if (idx >= N * S) return;
+    const int s = idx / N, i = idx - s * N;
+    observables_one(N, lmax, s, i, p_inc, p_src, mie_scat, mie_int, k, eps_b, p_scat, p_int, C_part, F, Tq);
+}
+
+// ---- many clusters per launch: descriptors travel by value in the kernel
+// arguments (up to kObsBatchMax clusters per launch), work ids (source,
+// particle) dealt to the clusters in order; each cluster's arithmetic is
+// that of its own launch.
+constexpr int kObsBatchMax = 32;
+
+struct ObsBatch {
+    int count;
+    int first[kObsBatchMax + 1];  // prefix of each cluster's workgroups
+    mpa_obs_cluster c[kObsBatchMax];
+};
+
+__device__ __forceinline__ int obs_cluster_of(const ObsBatch& B, int v) {
+    int q = 0;
+    while (q + 1 < B.count && v >= B.first[q + 1]) q++;
+    return q;
+}
+
+__global__ __launch_bounds__(kThreads) void k_observables_b(ObsBatch B) {
+    const int q = obs_cluster_of(B, blockIdx.x);
+    const mpa_obs_cluster& C = B.c[q];
+    const int idx = (blockIdx.x - B.first[q]) * kThreads + threadIdx.x;
+    if (idx >= C.N * C.nsrc) return;
+    const int s = idx / C.N, i = idx - s * C.N;
+    observables_one(C.N, C.lmax, s, i, (const double2*)C.p_inc, (const double2*)C.p_src,
+                    (const double2*)C.mie_scat, (const double2*)C.mie_int, C.k, C.eps_b, (double2*)C.p_scat,
+                    (double2*)C.p_int, C.C_part, C.F, C.T);
 }
 
 // force_all / torque_all with explicit p_scat and p_inc ([M, 2 rmax] rows)
@@ -178,9 +215,7 @@ __global__ __launch_bounds__(kThreads) void k_cross_sections(int M, int lmax, co
 
 // cluster totals per source: sum over particles (fixed order) of the
 // per-particle sums over multipoles; out [S][3] (scat, abs, ext)
-__global__ __launch_bounds__(kThreads) void k_totals(int N, int lmax, const double* __restrict__ C_part,
-                                                     double* __restrict__ C_total) {
-    const int s = blockIdx.x;
+__device__ void totals_one(int N, int lmax, int s, const double* __restrict__ C_part, double* __restrict__ C_total) {
     const int L = 2 * lmax;
     double acc[3] = {0, 0, 0};
     for (int i = threadIdx.x; i < N; i += kThreads) {
@@ -201,6 +236,18 @@ __global__ __launch_bounds__(kThreads) void k_totals(int N, int lmax, const doub
     }
     if (threadIdx.x == 0)
         for (int c = 0; c < 3; c++) C_total[s * 3 + c] = red[c][0];
+}
+
+__global__ __launch_bounds__(kThreads) void k_totals(int N, int lmax, const double* __restrict__ C_part,
+                                                     double* __restrict__ C_total) {
+    totals_one(N, lmax, blockIdx.x, C_part, C_total);
+}
+
+// one workgroup per (cluster, source) of the batch with C_total set; first[] counts sources
+__global__ __launch_bounds__(kThreads) void k_totals_b(ObsBatch B) {
+    const int q = obs_cluster_of(B, blockIdx.x);
+    const mpa_obs_cluster& C = B.c[q];
+    totals_one(C.N, C.lmax, blockIdx.x - B.first[q], C.C_part, C.C_total);
 }
 
 // y_j = Tm_j x_j for every particle and right-hand side (block-diagonal
@@ -260,6 +307,48 @@ extern "C" int mpa_cluster_observables(int N, int lmax, int nsrc, const double* 
                        (const double2*)mie_int, k, eps_b, (double2*)p_scat, (double2*)p_int, C_part, F, T);
     if (C_total) hipLaunchKernelGGL(k_totals, dim3(nsrc), dim3(kThreads), 0, st, N, lmax, C_part, C_total);
     MPA_LAUNCH_CHECK();
+    return MPA_OK;
+}
+
+extern "C" int mpa_cluster_observables_batched(int count, const mpa_obs_cluster* clusters, void* stream) {
+    if (count < 0 || (count > 0 && !clusters)) {
+        mpa_set_error("cluster_observables_batched: bad argument");
+        return MPA_ERR_ARG;
+    }
+    for (int q = 0; q < count; q++) {
+        const mpa_obs_cluster& C = clusters[q];
+        if (C.N < 1 || C.nsrc < 1 || C.lmax < 1 || C.lmax > MPA_LMAX_MAX || !C.p_inc || !C.p_src || !C.mie_scat ||
+            !C.p_scat || !C.C_part || !C.F || !(C.k > 0) || (C.p_int && !C.mie_int) ||
+            (long long)C.N * C.nsrc > 0x7fffffffLL - kThreads) {
+            mpa_set_error("cluster_observables_batched: bad cluster %d", q);
+            return MPA_ERR_ARG;
+        }
+    }
+    hipStream_t st = (hipStream_t)stream;
+    for (int b = 0; b < count; b += kObsBatchMax) {
+        const int cnt = std::min(kObsBatchMax, count - b);
+        ObsBatch Bo{}, Bt{};
+        Bo.count = cnt;
+        long long wg = 0;
+        for (int q = 0; q < cnt; q++) {
+            const mpa_obs_cluster& C = clusters[b + q];
+            Bo.c[q] = C;
+            Bo.first[q + 1] = Bo.first[q] + (C.N * C.nsrc + kThreads - 1) / kThreads;
+            wg += (C.N * C.nsrc + kThreads - 1) / kThreads;
+            if (C.C_total) {
+                Bt.c[Bt.count] = C;
+                Bt.first[Bt.count + 1] = Bt.first[Bt.count] + C.nsrc;
+                Bt.count++;
+            }
+        }
+        if (wg > 0x7fffffffLL) {
+            mpa_set_error("cluster_observables_batched: batch too large for one launch");
+            return MPA_ERR_ARG;
+        }
+        hipLaunchKernelGGL(k_observables_b, dim3((unsigned)Bo.first[cnt]), dim3(kThreads), 0, st, Bo);
+        if (Bt.count) hipLaunchKernelGGL(k_totals_b, dim3((unsigned)Bt.first[Bt.count]), dim3(kThreads), 0, st, Bt);
+        MPA_LAUNCH_CHECK();
+    }
     return MPA_OK;
 }
 

@@ -458,6 +458,51 @@ def observables(p_inc, p_src, mie_scat, mie_int, k, eps_b, want_int=True, want_t
     return dict(p_scat=p_scat, p_int=p_int, C_part=C_part, C_total=C_total, F=F, T=T)
 
 
+def observables_batched(clusters, want_int=True, want_torque=True):
+    """observables() for many clusters in a few launches
+    (mpa_cluster_observables_batched); each cluster's results equal its own
+    observables() call bit for bit.  clusters: [(p_inc, p_src, mie_scat,
+    mie_int, k, eps_b)] with p_inc, p_src [S, N, 2, rmax] c128.  Each output
+    kind is one flat allocation; returns one dict of views per cluster."""
+    lib = L.load()
+    if not clusters:
+        return []
+    dev = clusters[0][0].device
+    shapes = [tuple(c[0].shape) for c in clusters]
+    lm = [c[2].shape[-1] for c in clusters]
+    sizes = {"p": sum(S * N * 2 * r for S, N, _, r in shapes), "C_part": sum(S * N * 6 * l for (S, N, _, _), l in
+                                                                             zip(shapes, lm)),
+             "C_total": sum(S * 3 for S, *_ in shapes), "F": sum(S * N * 3 for S, N, *_ in shapes)}
+    p_scat_all = torch.empty(sizes["p"], dtype=C128, device=dev)
+    p_int_all = torch.empty(sizes["p"], dtype=C128, device=dev) if want_int else None
+    C_part_all = torch.empty(sizes["C_part"], dtype=torch.float64, device=dev)
+    C_total_all = torch.empty(sizes["C_total"], dtype=torch.float64, device=dev)
+    F_all = torch.empty(sizes["F"], dtype=torch.float64, device=dev)
+    T_all = torch.empty(sizes["F"], dtype=torch.float64, device=dev) if want_torque else None
+    descs = (L.MpaObsCluster * len(clusters))()
+    out = []
+    op = oc = ot = of = 0
+    for q, ((p_inc, p_src, mie_scat, mie_int, k, eps_b), (S, N, _, rmax), lmax) in enumerate(zip(clusters, shapes, lm)):
+        np_, nc, nf = S * N * 2 * rmax, S * N * 6 * lmax, S * N * 3
+        r = dict(p_scat=p_scat_all[op:op + np_].view(S, N, 2, rmax),
+                 p_int=p_int_all[op:op + np_].view(S, N, 2, rmax) if want_int else None,
+                 C_part=C_part_all[oc:oc + nc].view(S, N, 3, 2, lmax), C_total=C_total_all[ot:ot + 3 * S].view(S, 3),
+                 F=F_all[of:of + nf].view(S, N, 3), T=T_all[of:of + nf].view(S, N, 3) if want_torque else None)
+        op, oc, ot, of = op + np_, oc + nc, ot + 3 * S, of + nf
+        d = descs[q]
+        d.N, d.lmax, d.nsrc = N, lmax, S
+        d.p_inc, d.p_src, d.mie_scat = p_inc.data_ptr(), p_src.data_ptr(), mie_scat.data_ptr()
+        d.mie_int = mie_int.data_ptr() if want_int else None
+        d.k, d.eps_b = float(k), float(np.real(eps_b))
+        d.p_scat, d.C_part, d.C_total, d.F = (r["p_scat"].data_ptr(), r["C_part"].data_ptr(),
+                                              r["C_total"].data_ptr(), r["F"].data_ptr())
+        d.p_int = r["p_int"].data_ptr() if want_int else None
+        d.T = r["T"].data_ptr() if want_torque else None
+        out.append(r)
+    L.check(lib.mpa_cluster_observables_batched(len(clusters), descs, _stream()), "cluster_observables_batched")
+    return out
+
+
 def translation_table(lmax):
     """Host copy of the device translation table: (meta [rmax^2, 4], coef [rmax^2, 2 lmax + 1])."""
     lib = L.load()

@@ -160,3 +160,26 @@ def test_graph_replay_equals_eager(dev):
     for (xg, ig), (xe, ie) in zip(graphed, eager):
         assert ig == ie
         assert np.array_equal(xg, xe)
+
+
+def test_observables_batched_equals_single(dev):
+    """mpa_cluster_observables_batched over clusters of mixed N, lmax and
+    source counts (more than one launch's worth of descriptors) gives each
+    cluster exactly the observables of its own mpa_cluster_observables call."""
+    import torch
+    rng = np.random.default_rng(3)
+    clusters = []
+    for q in range(37):  # > 32 descriptors: two launches
+        N = int(rng.integers(1, 70))
+        lmax = int(rng.choice([1, 2, 3, 4, 5]))
+        S = int(rng.choice([1, 2, 4]))
+        rmax = lmax * (lmax + 2)
+        cplx = lambda *shape: dev.to_device(rng.normal(size=shape) + 1j * rng.normal(size=shape), dev.C128)
+        clusters.append((cplx(S, N, 2, rmax), cplx(S, N, 2, rmax), cplx(N, 2, lmax), cplx(N, 2, lmax),
+                         float(rng.uniform(5e6, 2e7)), float(rng.uniform(1.0, 2.0))))
+    got = dev.observables_batched(clusters)
+    torch.cuda.synchronize()
+    for q, (c, g) in enumerate(zip(clusters, got)):
+        one = dev.observables(*c)
+        for key in ("p_scat", "p_int", "C_part", "C_total", "F", "T"):
+            assert np.array_equal(g[key].cpu().numpy(), one[key].cpu().numpy()), (q, key)
