@@ -7,7 +7,7 @@ device work (assembly of every matrix, one batched BiCGSTAB or one batched
 complex64 + FP64-refinement solve over all clusters, one batched matrix-free
 FP64 BiCGSTAB with no stored matrices, or one batched refinement whose
 complex64 corrections and FP64 residuals both use the matrix-free operator
-for clusters with lmax <= 4 and n >= matfree_min_n (each matrix-free
+for clusters with lmax <= 5 and n >= matfree_min_n (each matrix-free
 system launches its own operator kernels, so small clusters keep stored
 matrices, whose products run as one batched launch); observables) is
 `ClusterBatch.solve`.  The batched engine covers all clusters with one launch
@@ -40,7 +40,7 @@ class PreparedCluster:
         self.op = None
 
     def matrix_bytes(self, precision, matfree_min_n=6000):
-        if precision == "matfree" or (precision == "mixed_matfree" and self.w.lmax <= 4 and
+        if precision == "matfree" or (precision == "mixed_matfree" and self.w.lmax <= 5 and
                                       self.n >= matfree_min_n):
             return 0
         return 16 * self.n * self.n + (8 * self.n * self.n if precision.startswith("mixed") else 0)
@@ -58,7 +58,7 @@ class ClusterBatch:
         self.tol_scale = tol_scale  # solve to tol_scale x each workload's tolerance (reference solutions)
         self.clusters = [PreparedCluster(w) for w in workloads]
         for c in self.clusters:
-            if precision == "matfree" or (precision == "mixed_matfree" and c.w.lmax <= 4 and
+            if precision == "matfree" or (precision == "mixed_matfree" and c.w.lmax <= 5 and
                                           c.n >= matfree_min_n):
                 c.op = D.ClusterOperator(c.pos, c.mie_scat, c.k, max_rhs=c.S)
                 continue

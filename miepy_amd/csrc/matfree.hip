@@ -864,16 +864,19 @@ __device__ __forceinline__ void combo_pm(int u, int PM, int& p, int& m) {
 // W is formed in FP32 (its terms are few and the operands keep 22 bits).
 
 // column particles per workgroup, the LDS capacity of the term lists (lmax 3 has 789 terms, lmax 4 2452)
-// and the threads per workgroup (lmax 4: 512, so the 64 KB of LDS per workgroup still leaves 4 waves per SIMD)
+// and the threads per workgroup (lmax 4: 512, so the 64 KB of LDS per workgroup still leaves 4 waves per SIMD;
+// lmax 5, 6151 terms: 150 KB of LDS, one workgroup of 1024 threads per CU)
 template <int L>
 struct WhShape {
     static constexpr int JB = L <= 3 ? 2 : 1;
-    static constexpr int TERMS = L <= 3 ? 1024 : 2560;
-    static constexpr int TH = L <= 3 ? 256 : 512;
+    static constexpr int TERMS = L <= 3 ? 1024 : (L == 4 ? 2560 : 6400);
+    static constexpr int TH = L <= 3 ? 256 : (L == 4 ? 512 : 1024);
 };
 
 // The operand kernels' term capacity at lmax (for the launch-time check).
-inline int wh_term_capacity(int lmax) { return lmax <= 3 ? WhShape<3>::TERMS : WhShape<4>::TERMS; }
+inline int wh_term_capacity(int lmax) {
+    return lmax <= 3 ? WhShape<3>::TERMS : (lmax == 4 ? WhShape<4>::TERMS : WhShape<5>::TERMS);
+}
 
 // The smallest power of two >= t (t > 0 and normal): a power-of-two tau_j
 // makes every rescale by tau exact.
@@ -1707,6 +1710,279 @@ __global__ __launch_bounds__(kThreads, 2) void k_matfree_mf16s(Mf16Batch B, cons
     }
 }
 
+// hle_f16<L, true> in two parts, so that a kernel can form the B operands of
+// one K slice at a time (the same values): hle_geom evaluates the pair's
+// geometry and scaled Hankel values; hle_part<L, U0, U1> forms the combos
+// u in [U0, U1) into bh/bl[u / 4 - U0 / 4] (U0 a multiple of 4), resuming the
+// Legendre and azimuth recurrences from `st`, which the slices carry in order.
+struct HleGeom {
+    float ct, st, e1x, e1y;
+};
+struct HleRec {
+    float ex, ey, pmm, p1, p2;
+};
+
+template <int L>
+__device__ __forceinline__ void hle_geom(double xi, double yi, double zi, const double* __restrict__ pos, int j,
+                                         double k, float kinv, int kh, bool on, float (&hx)[2 * L + 1],
+                                         float (&hy)[2 * L + 1], HleGeom& G, HleRec& Rc, int& E) {
+    constexpr int PM = 2 * L;
+    double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+    dx = on ? dx : 1.0;
+    dy = on ? dy : 0.0;
+    dz = on ? dz : 0.0;
+    const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+    const double zz = k * rad;
+    const double qd = rint(zz * 0.63661977236758134308);
+    double rr = fma(-qd, 1.5707963267948966, zz);
+    rr = fma(-qd, 6.123233995736766e-17, rr);
+    const float r = (float)rr, r2 = r * r;
+    const float S = r + r * r2 * (-1.f / 6.f + r2 * (1.f / 120.f + r2 * (-1.f / 5040.f + r2 * (1.f / 362880.f))));
+    const float C = 1.f + r2 * (-0.5f + r2 * (1.f / 24.f + r2 * (-1.f / 720.f + r2 * (1.f / 40320.f -
+                                                                                       r2 * (1.f / 3628800.f)))));
+    const int quad = (int)qd & 3;
+    const float s0 = (quad & 1) ? C : S, c0 = (quad & 1) ? S : C;
+    const float sz = (quad & 2) ? -s0 : s0, cz = ((quad + 1) & 2) ? -c0 : c0;
+    const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+    const float rinv = __builtin_amdgcn_rcpf((float)rad);
+    const float rz = rinv * kinv;
+    hx[0] = sz * rz;
+    hy[0] = -cz * rz;
+    hx[1] = hx[0] * rz + hy[0];
+    hy[1] = hy[0] * rz - hx[0];
+#pragma unroll
+    for (int l = 2; l <= PM; l++) {
+        hx[l] = (2 * l - 1) * rz * hx[l - 1] - hx[l - 2];
+        hy[l] = (2 * l - 1) * rz * hy[l - 1] - hy[l - 2];
+    }
+    // g max|h_PM| in [2^11, 2^12), as hle_f16<L, true>
+    const float hmax = fmaxf(fabsf(hx[PM]), fabsf(hy[PM]));
+    E = (int)((__builtin_bit_cast(unsigned, hmax) >> 23) & 0xff) - 126 - 12;
+    const float g = on ? __builtin_bit_cast(float, (unsigned)(127 - E) << 23) : 0.f;
+#pragma unroll
+    for (int l = 0; l <= PM; l++) {
+        hx[l] *= g;
+        hy[l] *= g;
+    }
+    const float rho2 = fx * fx + fy * fy;
+    const float rhoinv = rho2 > 0.f ? __builtin_amdgcn_rsqf(rho2) : 0.f;
+    G.ct = fz * rinv;
+    G.st = rho2 * rhoinv * rinv;
+    G.e1x = rho2 > 0.f ? fx * rhoinv : 1.f;
+    G.e1y = (kh ? -fy : fy) * rhoinv;  // e^{-i phi} on lane half 1
+    Rc.ex = kh ? 0.f : 1.f;  // m = 0 slots belong to lane half 0
+    Rc.ey = 0.f;
+    Rc.pmm = 1.f;
+    Rc.p1 = Rc.p2 = 0.f;
+}
+
+template <int L, int U0, int U1>
+__device__ __forceinline__ void hle_part(const float (&hx)[2 * L + 1], const float (&hy)[2 * L + 1],
+                                         const HleGeom& G, HleRec& Rc, h8v* __restrict__ bh,
+                                         h8v* __restrict__ bl) {
+    constexpr int PM = 2 * L;
+    static_assert(U0 % 4 == 0, "slices start at a K block");
+#pragma unroll
+    for (int kb = 0; kb < (U1 - U0 + 3) / 4; kb++)
+#pragma unroll
+        for (int e = 0; e < 8; e++) bh[kb][e] = bl[kb][e] = (_Float16)0.f;
+    int u = 0;
+#pragma unroll
+    for (int m = 0; m <= PM; m++) {
+        if (m > 0 && u >= U0 && u < U1) {  // the group of order m starts in this slice
+            const float bx = m == 1 ? 1.f : Rc.ex, by = m == 1 ? 0.f : Rc.ey;
+            Rc.ex = bx * G.e1x - by * G.e1y;
+            Rc.ey = bx * G.e1y + by * G.e1x;
+            Rc.pmm *= G.st * (float)__builtin_sqrt((2.0 * m - 1) / (2.0 * m));
+        }
+#pragma unroll
+        for (int l = 0; l <= PM; l++) {
+            if (l < m) continue;
+            if (u >= U0 && u < U1) {
+                float P;
+                if (l == m) {
+                    P = Rc.pmm;
+                } else if (l == m + 1) {
+                    P = G.ct * (float)__builtin_sqrt(2.0 * m + 1) * Rc.pmm;
+                } else {
+                    const float A = (float)((2.0 * l - 1) / __builtin_sqrt((double)(l - m) * (l + m)));
+                    const float B =
+                        (float)__builtin_sqrt((double)(l + m - 1) * (l - m - 1) / ((double)(l - m) * (l + m)));
+                    P = A * (G.ct * Rc.p1) - B * Rc.p2;
+                }
+                Rc.p2 = Rc.p1;
+                Rc.p1 = P;
+                const float ar = hx[l] * P, ai = hy[l] * P;
+                const f2v v = {ar * Rc.ex - ai * Rc.ey, ar * Rc.ey + ai * Rc.ex};
+                h2v hi, lo;
+                split_hilo(v, hi, lo);
+                const int w = u - U0;
+                bh[w >> 2][2 * (w & 3)] = hi.x;
+                bh[w >> 2][2 * (w & 3) + 1] = hi.y;
+                bl[w >> 2][2 * (w & 3)] = lo.x;
+                bl[w >> 2][2 * (w & 3) + 1] = lo.y;
+            }
+            u++;
+        }
+    }
+}
+
+// Compile-time loop over the K slices: f(std::integral_constant<int, P>) for P = 0 .. KP - 1.
+template <int P, int KP>
+struct SliceLoop {
+    template <class F>
+    __device__ __forceinline__ static void run(F& f) {
+        f(std::integral_constant<int, P>{});
+        SliceLoop<P + 1, KP>::run(f);
+    }
+};
+template <int KP>
+struct SliceLoop<KP, KP> {
+    template <class F>
+    __device__ __forceinline__ static void run(F&) {}
+};
+
+// ---- lmax 5: the operand block of a column (174 KB) exceeds the LDS
+//
+// k_matfree_mf16k is the contraction of k_matfree_mf16s (one source per
+// workgroup, the same per-pair power-of-two scales and running total) with
+// the column's operand block staged in KP slices of K blocks (lmax 5: 6, 6
+// and 5 of the 17) through two LDS buffers: the next slice (of this column or
+// of the next one) is loaded into registers while the MFMAs of the current
+// slice run and written to the other buffer after them, one barrier per
+// slice; the B operands are formed slice by slice (hle_part), so only one
+// slice's stay live.  One workgroup per CU (two 61 KB buffers): each wave has
+// its SIMD's 512 registers for the accumulators and running totals of the NT
+// tiles, the slice in flight and the pair's special functions.
+template <int L, int KP>
+__global__ __launch_bounds__(kThreads, 1) void k_matfree_mf16k(Mf16Batch B, const SysDev* __restrict__ sys) {
+    using Sh = F16Shape<L>;
+    constexpr int NC = Sh::NC, NT = Sh::NT, NKB = Sh::NKB, PER = Sh::PER;
+    constexpr int KBP = (NKB + KP - 1) / KP;             // K blocks per slice
+    constexpr int SL = 2 * NT * KBP * 64;                // f4v entries of a slice buffer: [hl][tile][kbi][lane]
+    constexpr int LDP = (SL + kThreads - 1) / kThreads;  // entries per thread and slice
+    __shared__ f4v Ws[2][SL];
+    const int v0 = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int q = batch_job(B, v0);
+    const Mf16Job& Q = B.job[q];
+    const MfJob& J = Q.J;
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    int v = v0 - B.first[q];
+    const int group = v % Q.groups;
+    v /= Q.groups;
+    const int c = v % Q.nch;
+    const int s = v / Q.nch;  // one source per workgroup
+    const int N = J.N, S = J.S;
+    const int jbeg = c * Q.chunk, jend = min(N, jbeg + Q.chunk);
+    if (jbeg >= jend) return;  // uniform over the workgroup
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int kh = lane >> 5, col = lane & 31;
+    const int i = group * 128 + w * 32 + col;
+    const bool live = i < N;
+    const bool wlive = group * 128 + w * 32 < N;  // a wave past N only loads its share of the slices
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const float kinv = (float)(1.0 / J.k);
+    const f4v* __restrict__ Wsrc = Q.Wh + (size_t)s * N * PER;
+    const float* __restrict__ tau = Q.tau + (size_t)s * N;
+    f16v R[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) R[t][e] = 0.f;
+    int cur = 0;
+    f4v pf[LDP];
+    // slice p of column j: buffer entry e = ((hl NT + t) KBP + kbi) 64 + lane <- the column's block entry
+    // ((hl NT + t) NKB + p KBP + kbi) 64 + lane, zero for kbi past the slice's K blocks
+    auto load = [&](int j, int p) {
+        const int kb0 = p * KBP, nk = min(KBP, NKB - kb0);
+        const f4v* src = Wsrc + (size_t)j * PER;
+#pragma unroll
+        for (int r = 0; r < LDP; r++) {
+            const int e = r * kThreads + tid;
+            const int run = e >> 6, ht = run / KBP, kbi = run - ht * KBP;
+            pf[r] = (e < SL && kbi < nk) ? src[((size_t)ht * NKB + kb0 + kbi) * 64 + (e & 63)] : f4v{0, 0, 0, 0};
+        }
+    };
+    auto put = [&](int buf) {
+#pragma unroll
+        for (int r = 0; r < LDP; r++) {
+            const int e = r * kThreads + tid;
+            if (e < SL) Ws[buf][e] = pf[r];
+        }
+    };
+    load(jbeg, 0);
+    put(0);
+    __syncthreads();
+    int buf = 0;
+    for (int j = jbeg; j < jend; j++) {
+        const bool on = live && j != ii;
+        int E = 0;
+        float hx[2 * L + 1], hy[2 * L + 1];
+        HleGeom G;
+        HleRec Rc;
+        if (wlive) hle_geom<L>(xi, yi, zi, J.pos, j, J.k, kinv, kh, on, hx, hy, G, Rc, E);
+        const float tj = tau[j];
+        const int nw = (int)((__builtin_bit_cast(unsigned, tj) >> 23) & 0xff) - 127 + E;
+        const float scale = __builtin_amdgcn_ldexpf(1.f, on ? cur - nw : 0);
+        cur = on ? nw : cur;
+        f16v acc[NT];
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[t][e] = 0.f;
+        // slice p: load the next slice, form this slice's B operands, its MFMAs, store the next slice
+        auto step = [&](auto pc) {
+            constexpr int p = decltype(pc)::value;
+            constexpr int U0 = 4 * KBP * p;
+            constexpr int U1 = (4 * KBP * (p + 1) < Sh::NCOMBO) ? 4 * KBP * (p + 1) : Sh::NCOMBO;
+            constexpr int NKS = (U1 - U0 + 3) / 4;  // K blocks of this slice
+            const bool more = p + 1 < KP || j + 1 < jend;  // uniform
+            if (more) load(p + 1 < KP ? j : j + 1, p + 1 < KP ? p + 1 : 0);
+            if (wlive) {
+                h8v bh[NKS], bl[NKS];
+                hle_part<L, U0, U1>(hx, hy, G, Rc, bh, bl);
+#pragma unroll
+                for (int kbi = 0; kbi < NKS; kbi++)
+#pragma unroll
+                    for (int t = 0; t < NT; t++) {
+                        const h8v ah = __builtin_bit_cast(h8v, Ws[buf][((0 * NT + t) * KBP + kbi) * 64 + lane]);
+                        const h8v al = __builtin_bit_cast(h8v, Ws[buf][((1 * NT + t) * KBP + kbi) * 64 + lane]);
+                        f16v& a = acc[t];
+                        a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bh[kbi], a, 0, 0, 0);
+                        a = __builtin_amdgcn_mfma_f32_32x32x16_f16(ah, bl[kbi], a, 0, 0, 0);
+                        a = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[kbi], a, 0, 0, 0);
+                    }
+            }
+            if (more) put(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        };
+        SliceLoop<0, KP>::run(step);
+#pragma unroll
+        for (int t = 0; t < NT; t++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) R[t][e] = fmaf(R[t][e], scale, acc[t][e]);
+    }
+    if (!live) return;
+    const size_t n = (size_t)NC * N;
+    const bool direct = Q.nch == 1;  // one chunk: the final rows (identity term included) straight into y
+    const size_t row = (size_t)s * n + (size_t)i * NC;
+    float* out = direct ? (float*)J.y + 2 * row : (float*)Q.part32 + 2 * ((size_t)c * S * n + row);
+    const float* xin = (const float*)J.x + 2 * row;
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            const int Rr = 32 * t + 8 * (e >> 2) + 4 * kh + (e & 3);  // real output column 2 cc + part
+            if (Rr < 2 * NC) {
+                float val = __builtin_amdgcn_ldexpf(R[t][e], cur);
+                if (direct && J.add_identity) val += xin[Rr];
+                out[Rr] = val;
+            }
+        }
+}
+
 // y = (add_identity ? x : 0) + the chunk partials in chunk order, for every
 // job with more than one chunk: work id = (job, block of 64 elements x 4
 // waves) as in k_matfree_sum32.
@@ -2172,7 +2448,8 @@ inline int wh_per(int lmax) {
     return lmax == 1 ? F16Shape<1>::PER
            : lmax == 2 ? F16Shape<2>::PER
            : lmax == 3 ? F16Shape<3>::PER
-                       : F16Shape<4>::PER;
+           : lmax == 4 ? F16Shape<4>::PER
+                       : F16Shape<5>::PER;
 }
 
 inline int fac_entries_of(int lmax) {
@@ -2214,8 +2491,9 @@ size_t matfree_workspace_bytes(int N, int lmax, int S) {
     return b;
 }
 
-// lmax <= 3: FP16-split or FP32 matrix-core kernel; lmax 4: FP16-split kernel (whatever the option says)
-constexpr int kC64MaxLmax = 4;
+// lmax <= 3: FP16-split or FP32 matrix-core kernel; lmax 4: FP16-split kernel (whatever the option says);
+// lmax 5: the FP16-split kernel with the operand block staged in K slices (k_matfree_mf16k)
+constexpr int kC64MaxLmax = 5;
 bool matfree_c64_supported(int N, int lmax, bool tmatrix) { return lmax <= kC64MaxLmax && N >= 1 && !tmatrix; }
 
 // Sources per workgroup of k_matfree_mf16s: its LDS holds two buffers of SG
@@ -2265,7 +2543,10 @@ int launch_mf16s_pack(const MfJob* const* jobs, int count, const SysDev* sys, hi
     const dim3 block(kThreads);
     hipLaunchKernelGGL((k_matfree_wh_b<L>), dim3((unsigned)Bw.first[count]), dim3(WhShape<L>::TH), 0, st, Bw, sys,
                        T->nhterms, T->hlist, T->hterm_src, T->hterm_coef);
-    hipLaunchKernelGGL((k_matfree_mf16s<L, SG>), dim3((unsigned)Bc.first[count]), block, 0, st, Bc, sys);
+    if constexpr (L == 5)
+        hipLaunchKernelGGL((k_matfree_mf16k<5, 3>), dim3((unsigned)Bc.first[count]), block, 0, st, Bc, sys);
+    else
+        hipLaunchKernelGGL((k_matfree_mf16s<L, SG>), dim3((unsigned)Bc.first[count]), block, 0, st, Bc, sys);
     if (Bs.count)
         hipLaunchKernelGGL(k_matfree_sum32_b, dim3((unsigned)Bs.first[Bs.count]), block, 0, st, Bs, sys);
     MPA_LAUNCH_CHECK();
@@ -2305,7 +2586,8 @@ int matfree_launch_c64_shared(const MfJob* const* jobs, int count, const SysDev*
                 case 1: rc = launch_mf16s_pack<1>(sel.data() + b, cnt, sys, st, T); break;
                 case 2: rc = launch_mf16s_pack<2>(sel.data() + b, cnt, sys, st, T); break;
                 case 3: rc = launch_mf16s_pack<3>(sel.data() + b, cnt, sys, st, T); break;
-                default: rc = launch_mf16s_pack<4>(sel.data() + b, cnt, sys, st, T); break;
+                case 4: rc = launch_mf16s_pack<4>(sel.data() + b, cnt, sys, st, T); break;
+                default: rc = launch_mf16s_pack<5>(sel.data() + b, cnt, sys, st, T); break;
             }
             if (rc) return rc;
         }
@@ -2318,8 +2600,8 @@ int matfree_launch_c64(const MfJob& J, const SysDev* sys, hipStream_t st, const 
         mpa_set_error("matfree: the complex64 operator covers spheres with lmax <= %d", kC64MaxLmax);
         return MPA_ERR_ARG;
     }
-    if (mf_shared_for(J.S, J.lmax) && mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0) {
-        const MfJob* one = &J;
+    if (J.lmax == 5 || (mf_shared_for(J.S, J.lmax) && mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0)) {
+        const MfJob* one = &J;  // lmax 5 has only the K-sliced kernel of the shared launches
         return matfree_launch_c64_shared(&one, 1, sys, st);
     }
     const int nh = T->nh;
@@ -2497,7 +2779,7 @@ int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
 int matfree_launch_many(const std::vector<MfJob>& jobs, const SysDev* sys, hipStream_t st) {
     std::vector<const MfJob*> shared;
     for (const MfJob& j : jobs) {
-        if (j.c64 && mf_shared_for(j.S, j.lmax) && mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0 &&
+        if (j.c64 && (j.lmax == 5 || (mf_shared_for(j.S, j.lmax) && mpa_option(MPA_OPT_MATFREE_C64_KERNEL) == 0)) &&
             matfree_c64_supported(j.N, j.lmax, j.tmat != nullptr)) {
             shared.push_back(&j);
             continue;

@@ -12,7 +12,7 @@ budget), assembles them (stored systems) and joins them to the running
 batch.  A system runs matrix-free (the operator recomputed in every
 product, O(n) memory) when `matrix_free` is True, or, with "auto" (the
 default), when it is large enough for the matrix-free kernels to beat
-reading a stored matrix (FP64: lmax <= 3 and n >= 6000; mixed: lmax <= 4
+reading a stored matrix (FP64: lmax <= 3 and n >= 6000; mixed: lmax <= 5
 and n >= 6000) or its matrices would take more than a quarter of the
 budget; stored and matrix-free systems share every batched launch; every engine step advances all active systems by
 a few BiCGSTAB iterations in one batched launch sequence
@@ -26,10 +26,10 @@ request's result is the same alone and in the mix.
 
 With precision="mixed" every system is solved as FP32 + FP64 refinement
 at the iteration level (`mpa_refine_systems_advance`): complex64 BiCGSTAB
-corrections on a stored complex64 matrix or, for lmax <= 4, the FP16-split
+corrections on a stored complex64 matrix or, for lmax <= 5, the FP16-split
 matrix-free operator, FP64 residuals on the stored complex128 matrix or the
 FP64 matrix-free operator; a system leaves the batch once every source
-reached the tolerance in FP64.  A system with lmax > 4 that is to run
+reached the tolerance in FP64.  A system with lmax > 5 that is to run
 matrix-free (matrix_free=True, or "auto" with matrices above a quarter of
 the budget) has no complex64 operator: it joins a second set, solved by
 FP64 BiCGSTAB on the FP64 matrix-free operator (`mpa_systems_advance`),
@@ -205,8 +205,8 @@ class ServingEngine:
         if self.precision == "mixed":
             if self.matrix_free is False:
                 return False, True
-            if w.lmax > 4:
-                # no complex64 matrix-free operator above lmax 4: stored matrices, or FP64 matrix-free
+            if w.lmax > 5:
+                # no complex64 matrix-free operator above lmax 5: stored matrices, or FP64 matrix-free
                 mf = self.matrix_free is True or 24.0 * w.n * w.n > 0.25 * self.budget
                 return mf, not mf
             return self.matrix_free is True or w.n >= 6000 or 24.0 * w.n * w.n > 0.25 * self.budget, True

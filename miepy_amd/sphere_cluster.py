@@ -12,7 +12,7 @@ the solve of :664-717 running as HIP kernels on the device:
           refinement ('mixed'); or, with no stored matrix, BiCGSTAB on the
           FP64 matrix-free operator ('matfree') or complex64 corrections on
           the FP16-split matrix-free operator + FP64 matrix-free refinement
-          ('mixed_matfree', lmax <= 4) -> p_scat / p_int, cross-sections,
+          ('mixed_matfree', lmax <= 5) -> p_scat / p_int, cross-sections,
           force, torque in one fused kernel.
 
 Additional keyword arguments (the reference hard-codes these,
@@ -78,8 +78,8 @@ class sphere_cluster:
         self.refine_tol = refine_tol
         if precision not in ("fp64", "mixed", "matfree", "mixed_matfree"):
             raise ValueError("precision must be 'fp64', 'mixed', 'matfree' or 'mixed_matfree'")
-        if precision == "mixed_matfree" and lmax > 4:
-            raise ValueError("precision 'mixed_matfree' covers lmax <= 4")
+        if precision == "mixed_matfree" and lmax > 5:
+            raise ValueError("precision 'mixed_matfree' covers lmax <= 5")
 
         self.auto_origin = False
         if origin is None:

@@ -158,15 +158,49 @@ def test_mixed_precision_serving(serve):
                 assert np.max(np.abs(fm - f64)) <= 1e-8 * np.abs(f64).max()
 
 
-def test_mixed_engine_runs_large_lmax5_matrix_free_in_fp64(serve):
+def test_mixed_engine_runs_large_lmax5_with_the_complex64_operator(serve):
     """precision='mixed' with a budget below an lmax-5 system's two matrices
-    (24 n^2 = 0.42 GB for n = 4200): the system has no complex64 matrix-free
+    (24 n^2 = 0.42 GB for n = 4200): the system runs matrix-free and refined
+    (complex64 corrections on the K-sliced FP16-split operator, FP64
+    residuals on the FP64 matrix-free operator) beside refined small systems;
+    its cross-sections and forces match the FP64 engine's matrix-free solve
+    to 1e-8 (both solved to 1e-10)."""
+    from miepy_amd import workloads as W
+    big = W.scaled_random(60, 5, seed=3)
+    big.tol = 1e-10
+    small = [ws[0] for _, ws in _requests(17, 2)]
+    budget = 1.2e9
+    eng = serve.ServingEngine(hbm_budget_bytes=budget, iterations_per_step=3, precision="mixed")
+    assert eng._route(big) == (True, True) and 24 * big.n ** 2 > 0.25 * eng.budget
+    rb = serve.Request(0, [big], 0.0)
+    eng.submit(rb)
+    rs = [serve.Request(i + 1, [w], 0.0) for i, w in enumerate(small)]
+    for r in rs:
+        eng.submit(r)
+    eng.drain()
+    assert rb.status == "done" and all(r.status == "done" for r in rs)
+    assert rb.results[0]["matrix_free"] and rb.results[0]["refined"]
+    e64 = serve.ServingEngine(hbm_budget_bytes=budget, iterations_per_step=3, matrix_free=True)
+    r64 = serve.Request(0, [big], 0.0)
+    e64.submit(r64)
+    e64.drain()
+    c, c64 = rb.results[0]["cross_sections"], r64.results[0]["cross_sections"]
+    f, f64 = rb.results[0]["force"], r64.results[0]["force"]
+    print(f"[serving] lmax 5 refined vs FP64: cross-sections {np.max(np.abs(c - c64)) / np.abs(c64).max():.2e}, "
+          f"forces {np.max(np.abs(f - f64)) / np.abs(f64).max():.2e}", flush=True)
+    np.testing.assert_allclose(c, c64, rtol=1e-8, atol=1e-8 * np.abs(c64).max())
+    assert np.max(np.abs(f - f64)) <= 1e-7 * np.abs(f64).max()
+
+
+def test_mixed_engine_runs_large_lmax6_matrix_free_in_fp64(serve):
+    """precision='mixed' with a budget below an lmax-6 system's two matrices
+    (24 n^2 = 0.43 GB for n = 4224): the system has no complex64 matrix-free
     operator, so it runs FP64 BiCGSTAB on the FP64 matrix-free operator in
     the same engine steps as refined small systems; its result equals the
     FP64 engine's matrix-free solve bit for bit (the same kernels and
     iterations) and the small systems' results equal theirs alone."""
     from miepy_amd import workloads as W
-    big = W.scaled_random(60, 5, seed=3)
+    big = W.scaled_random(44, 6, seed=3)
     big.tol = 1e-9
     small = [ws[0] for _, ws in _requests(17, 3)]
     for w in small:

@@ -4,7 +4,7 @@ helpers, on the CPU (no device calls).
 - Routing (`ServingEngine._route`): FP64 engine: matrix-free for lmax <= 3 and
   n >= 6000 or matrices above a quarter of the budget ("auto"), as forced
   otherwise; mixed engine: refined systems on stored matrices or, for
-  lmax <= 4, the complex64 matrix-free operator; lmax 5 systems that are to
+  lmax <= 5, the complex64 matrix-free operator; lmax 6 systems that are to
   run matrix-free (forced, or matrices above a quarter of the budget) leave
   the refined set for the FP64 matrix-free set.
 - tmatrix_reduce_lmax (src/miepy/tmatrix/functions.py:6-24): the leading
@@ -41,11 +41,15 @@ def test_serving_routes():
     e = _engine("mixed", "auto", big)
     assert route(e, _w(3, 100)) == (False, True)
     assert route(e, _w(4, 300)) == (True, True)          # lmax 4, n = 14400: complex64 matrix-free
-    assert route(e, _w(5, 100)) == (False, True)         # lmax 5 stored while it fits
+    assert route(e, _w(5, 80)) == (False, True)          # lmax 5, n = 5600: stored
+    assert route(e, _w(5, 100)) == (True, True)          # lmax 5, n = 7000: complex64 matrix-free
     small = _engine("mixed", "auto", 1.2e9)
-    assert route(small, _w(5, 60)) == (True, False)      # lmax 5, 24 n^2 = 0.42 GB > budget / 4: FP64 matrix-free
+    assert route(small, _w(5, 60)) == (True, True)       # lmax 5, 24 n^2 = 0.42 GB > budget / 4: matrix-free
+    assert route(small, _w(6, 44)) == (True, False)      # lmax 6, 0.43 GB: FP64 matrix-free
     assert route(small, _w(5, 29)) == (False, True)      # 0.1 GB: stored, refined
-    assert route(_engine("mixed", True, big), _w(5, 10)) == (True, False)
+    assert route(small, _w(6, 20)) == (False, True)
+    assert route(_engine("mixed", True, big), _w(6, 10)) == (True, False)
+    assert route(_engine("mixed", True, big), _w(5, 10)) == (True, True)
     assert route(_engine("mixed", True, big), _w(4, 10)) == (True, True)
     assert route(_engine("mixed", False, big), _w(5, 2000)) == (False, True)
 
