@@ -61,6 +61,8 @@ def parse():
                          "a short command to run under rocprofv3)")
     ap.add_argument("--matfree-variant", type=int, default=None,
                     help="complex64 matrix-free kernel variant (mpa_matfree_variant); default: the library's")
+    ap.add_argument("--overlap", type=int, choices=[0, 1], default=None,
+                    help="MPA_OPT_OVERLAP: stored-matrix GEMM beside the matrix-free products (library default 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tiles (no autotuning in warmup)")
     ap.add_argument("--cpu-gemvs", type=int, default=3, help="timed oracle GEMVs in the CPU sample")
@@ -727,6 +729,9 @@ def main():
         from miepy_amd import _lib as L
         if L.load().mpa_matfree_variant(L.MPA_C64, args.matfree_variant, None) != args.matfree_variant:
             raise SystemExit(f"no complex64 matrix-free variant {args.matfree_variant}")
+    if args.overlap is not None:
+        from miepy_amd import _lib as L
+        L.check(L.load().mpa_set_option(L.MPA_OPT_OVERLAP, args.overlap), "set_option")
     if args.workload == "batch256":
         return bench_batch(args)
     if args.workload == "serving":

@@ -54,7 +54,15 @@ extern "C" {
                                         matrix cores on hi/lo split operands (three products, FP32
                                         accumulation, per-pair and per-column scaling; relative error
                                         per term about 2^-21), 1 FP32 matrix cores. */
-#define MPA_OPT_COUNT 3
+#define MPA_OPT_OVERLAP 3 /* 1 (default): in a product phase of the batched solvers that has both
+                             stored-matrix systems and matrix-free systems, the batched GEMM over the
+                             stored matrices runs on a second stream of the device beside the
+                             matrix-free products (HBM-bound and matrix-core-bound work side by
+                             side), joined before the phase ends; 0: one after the other on the
+                             caller's stream.  Results are identical either way.  Off while launch
+                             timing is on (mpa_bicgstab_noalloc solves one stored system, so it never
+                             forks). */
+#define MPA_OPT_COUNT 4
 
 const char* mpa_version(void);
 const char* mpa_last_error(void);

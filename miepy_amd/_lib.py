@@ -23,6 +23,7 @@ MPA_C64 = 1
 MPA_OPT_GRAPHS = 0
 MPA_OPT_MATFREE_FP64_KERNEL = 1
 MPA_OPT_MATFREE_C64_KERNEL = 2
+MPA_OPT_OVERLAP = 3
 MPA_SOLVE_REF_LOOP = 1
 
 STATUS_NAMES = {

@@ -473,11 +473,31 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
 }
 
 // All matvec jobs of one phase: the batched GEMV over stored matrices and
-// the matrix-free systems.
+// the matrix-free systems.  With both kinds present (and MPA_OPT_OVERLAP),
+// the GEMV, bound by HBM, runs on the device's side stream beside the
+// matrix-free products, bound by the matrix cores: a fork event recorded on
+// `st` starts it, a join event recorded after it is waited on by `st`, so
+// everything after the phase sees both (in a graph capture the fork and
+// join make the side stream's launch a parallel branch of the graph).
 template <typename T>
-int matvec_phase(const Plan& P, const MvDev* d_jobs, const std::vector<MfJob>& mf, hipStream_t st) {
+int matvec_phase(const Plan& P, const MvDev* d_jobs, const std::vector<MfJob>& mf, hipStream_t st,
+                 bool overlap = true) {
     bool any = false;
     for (int c = 0; c < kTileClasses; c++) any |= P.n_mv_work[c] > 0;
+    if (any && !mf.empty() && overlap && mpa_option(MPA_OPT_OVERLAP) != 0 && !mpa_profile_active()) {
+        SideLease lease;
+        const SideStream* side = lease.get();
+        if (side) {
+            MPA_HIP_CHECK(hipEventRecord(side->fork, st));
+            MPA_HIP_CHECK(hipStreamWaitEvent(side->stream, side->fork, 0));
+            const int rc1 = gemv_launch(sizeof(T), d_jobs, P.d_sys, P.d_mv_work, P.n_mv_work, side->stream);
+            const int rc2 = matfree_launch_many(mf, P.d_sys, st);
+            // join whatever was launched, also after an error, so no branch is left open
+            MPA_HIP_CHECK(hipEventRecord(side->join, side->stream));
+            MPA_HIP_CHECK(hipStreamWaitEvent(st, side->join, 0));
+            return rc1 ? rc1 : rc2;
+        }
+    }
     if (any) {
         const int rc = gemv_launch(sizeof(T), d_jobs, P.d_sys, P.d_mv_work, P.n_mv_work, st);
         if (rc) return rc;

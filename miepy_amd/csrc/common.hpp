@@ -138,6 +138,25 @@ int mpa_option(int key);
 bool mpa_profile_active();
 // Non-blocking stream of the current device on which iterations are captured.
 hipStream_t mpa_capture_stream();
+// A second non-blocking stream of the current device (default priority) with
+// a fork and a join event, for work that runs beside the caller's stream.
+struct SideStream {
+    hipStream_t stream = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+// Exclusive use of the device's side stream and its events for the lifetime
+// of the lease (the events are re-recorded by every user: the record and the
+// waits on it must not interleave with another caller's); get() is null if
+// they could not be created.
+class SideLease {
+   public:
+    SideLease();
+    const SideStream* get() const { return side_; }
+
+   private:
+    std::lock_guard<std::mutex> lock_;
+    SideStream* side_;
+};
 // Exclusive use of the process's pinned host staging buffer (>= bytes) for
 // the lifetime of the lease; ptr() is null if the allocation failed.
 class PinnedLease {

@@ -12,8 +12,11 @@
 namespace {
 
 std::mutex g_mu;
-int g_options[MPA_OPT_COUNT] = {1, 0, 0};  // graphs on; matrix-free kernels: FP64 and FP16-split matrix cores
+// graphs on; matrix-free kernels: FP64 and FP16-split matrix cores; overlapped product phases
+int g_options[MPA_OPT_COUNT] = {1, 0, 0, 1};
 std::map<int, hipStream_t> g_capture;  // per device
+std::map<int, SideStream> g_side;      // per device
+std::mutex g_side_mu;                  // held by a SideLease (apart from g_mu: its holder reads options)
 void* g_pinned = nullptr;
 size_t g_pinned_bytes = 0;
 
@@ -34,6 +37,24 @@ hipStream_t mpa_capture_stream() {
     if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return nullptr;
     g_capture[dev] = s;
     return s;
+}
+
+SideLease::SideLease() : lock_(g_side_mu), side_(nullptr) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    auto it = g_side.find(dev);
+    if (it == g_side.end()) {
+        SideStream s{};
+        if (hipStreamCreateWithFlags(&s.stream, hipStreamNonBlocking) != hipSuccess) return;
+        if (hipEventCreateWithFlags(&s.fork, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&s.join, hipEventDisableTiming) != hipSuccess) {
+            if (s.fork) (void)hipEventDestroy(s.fork);
+            (void)hipStreamDestroy(s.stream);
+            return;
+        }
+        it = g_side.emplace(dev, s).first;
+    }
+    side_ = &it->second;
 }
 
 PinnedLease::PinnedLease(size_t bytes) : lock_(g_mu), ptr_(nullptr) {

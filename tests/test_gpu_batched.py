@@ -183,3 +183,39 @@ def test_observables_batched_equals_single(dev):
         one = dev.observables(*c)
         for key in ("p_scat", "p_int", "C_part", "C_total", "F", "T"):
             assert np.array_equal(g[key].cpu().numpy(), one[key].cpu().numpy()), (q, key)
+
+
+def test_overlapped_product_phases_equal_sequential(dev):
+    """MPA_OPT_OVERLAP: the batched GEMM over the stored matrices on the side
+    stream beside the matrix-free products gives the same solve, bit for bit,
+    as the two one after the other, with the iterations replayed from HIP
+    graphs (the fork and join captured as parallel branches) and eagerly."""
+    import miepy_amd._lib as L
+    from miepy_amd.batch import ClusterBatch
+    from miepy_amd.workloads import batch256
+    lib = L.load()
+    ws = batch256(seed=9, count=6)
+    for w in ws:
+        w.position = w.position[: min(w.N, 80)]
+        w.N = w.position.shape[0]
+        w.radius, w.eps = w.radius[: w.N], w.eps[: w.N]
+    saved = (lib.mpa_get_option(L.MPA_OPT_OVERLAP), lib.mpa_get_option(L.MPA_OPT_GRAPHS))
+    out = {}
+    try:
+        for overlap, graphs in ((0, 1), (1, 1), (1, 0)):
+            lib.mpa_set_option(L.MPA_OPT_OVERLAP, overlap)
+            lib.mpa_set_option(L.MPA_OPT_GRAPHS, graphs)
+            B = ClusterBatch(ws, precision="mixed_matfree", matfree_min_n=1500)
+            assert any(c.op is not None for c in B.clusters) and any(c.op is None for c in B.clusters)
+            for _ in range(2):  # a second solve replays what the first recorded
+                B.solve()
+            assert B.converged()
+            out[(overlap, graphs)] = B.host_results()
+    finally:
+        lib.mpa_set_option(L.MPA_OPT_OVERLAP, saved[0])
+        lib.mpa_set_option(L.MPA_OPT_GRAPHS, saved[1])
+    ref = out[(0, 1)]
+    for key in ((1, 1), (1, 0)):
+        for a, b in zip(out[key], ref):
+            for k in ("p_inc", "C_total", "F"):
+                assert np.array_equal(a[k], b[k]), (key, k)
