@@ -477,6 +477,28 @@ int mpa_cluster_observables(int N, int lmax, int nsrc, const double* p_inc, cons
                             const double* mie_scat, const double* mie_int, double k, double eps_b, double* p_scat,
                             double* p_int, double* C_part, double* C_total, double* F, double* T, void* stream);
 
+/* One cluster of mpa_sphere_aggregate_tmatrix_batched: the arguments of
+ * mpa_sphere_aggregate_tmatrix (dtype = its out_dtype), or, with out32 set
+ * (dtype MPA_C128), of mpa_sphere_aggregate_tmatrix_dual (out = the c128
+ * matrix, out32 = its complex64 rounding). */
+typedef struct {
+    int N, lmax, dtype, add_identity;
+    const double* positions;
+    const double* mie;
+    double k;
+    void* out;
+    void* out32;
+} mpa_asm_cluster;
+
+/* Sphere-cluster aggregate T-matrices of `count` clusters of any sizes and
+ * lmax in a few launches (descriptors in the kernel arguments, 32 per launch
+ * and output kind); each matrix equals its own single-cluster call's bit for
+ * bit.  The reference assembles each cluster on its own
+ * (cpp/src/interactions.cpp:370-415 through sphere_aggregate_tmatrix,
+ * cpp/src/interactions_pyb.cpp); this batched form is for many independent
+ * clusters solved together. */
+int mpa_sphere_aggregate_tmatrix_batched(int count, const mpa_asm_cluster* clusters, void* stream);
+
 /* One cluster of mpa_cluster_observables_batched: the arguments of
  * mpa_cluster_observables for that cluster (same shapes and meanings; p_int,
  * mie_int, C_total and T may be NULL as there). */

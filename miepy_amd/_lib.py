@@ -118,6 +118,13 @@ class MpaRefineSystem(ctypes.Structure):
                 ("maxiter", ctypes.c_int), ("max_refine", ctypes.c_int), ("op", ctypes.c_void_p)]
 
 
+class MpaAsmCluster(ctypes.Structure):
+    """mpa_asm_cluster (include/miepy_amd.h)"""
+    _fields_ = [("N", ctypes.c_int), ("lmax", ctypes.c_int), ("dtype", ctypes.c_int), ("add_identity", ctypes.c_int),
+                ("positions", ctypes.c_void_p), ("mie", ctypes.c_void_p), ("k", ctypes.c_double),
+                ("out", ctypes.c_void_p), ("out32", ctypes.c_void_p)]
+
+
 class MpaObsCluster(ctypes.Structure):
     """mpa_obs_cluster (include/miepy_amd.h)"""
     _fields_ = [("N", ctypes.c_int), ("lmax", ctypes.c_int), ("nsrc", ctypes.c_int), ("pad", ctypes.c_int),
@@ -131,6 +138,7 @@ _ps = ctypes.POINTER(MpaSystem)
 _pop = ctypes.POINTER(MpaClusterOperator)
 _prs = ctypes.POINTER(MpaRefineSystem)
 _pobs = ctypes.POINTER(MpaObsCluster)
+_pasm = ctypes.POINTER(MpaAsmCluster)
 SIGNATURES.update({
     "mpa_particle_aggregate_tmatrix": (_i, [_i, _i, _v, _v, _d, _i, _i, _v, _v]),
     "mpa_tmatrix_apply": (_i, [_i, _i, _i, _v, _v, _v, _v]),
@@ -152,6 +160,7 @@ SIGNATURES.update({
     "mpa_refine_scratch_bytes": (_sz, [_i, _prs]),
     "mpa_solve_refined_batched": (_i, [_i, _prs, _v, _pi, _pi, _pd, _pi, _v]),
     "mpa_cluster_observables_batched": (_i, [_i, _pobs, _v]),
+    "mpa_sphere_aggregate_tmatrix_batched": (_i, [_i, _pasm, _v]),
 })
 
 _lib = None

@@ -89,13 +89,9 @@ class ClusterBatch:
             xs, self.reports = D.bicgstab_batched([c.op for c in cs], rhs, tol=tols, maxiter=self.maxiter,
                                                   add_identity=True)
             return self._reduce(xs)
-        for c in cs:
-            if c.op is not None:
-                continue
-            if c.A32 is not None:
-                D.assemble_dual(c.pos, c.mie_scat, c.k, add_identity=True, out64=c.A64, out32=c.A32)
-            else:
-                D.assemble(c.pos, c.mie_scat, c.k, dtype=D.C128, add_identity=True, out=c.A64)
+        stored = [c for c in cs if c.op is None]
+        if stored:  # every stored matrix (and its complex64 rounding) in a few batched launches
+            D.assemble_batched([(c.pos, c.mie_scat, c.k, c.A64, c.A32) for c in stored], add_identity=True)
         if self.precision == "fp64":
             xs, self.reports = D.bicgstab_batched([c.A64 for c in cs], rhs, tol=tols, maxiter=self.maxiter)
         else:

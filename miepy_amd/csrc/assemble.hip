@@ -64,13 +64,12 @@ size_t lds_bytes(int lmax, int J, int nterms, bool tab_lds) {
 // every entry to `out32` in the same pass (the mixed-precision solve needs
 // both matrices; the FP64 arithmetic is done once); TAB_LDS: stage the
 // coefficient table in LDS.
+// The work of workgroup `blk` of one cluster's assembly grid (N x ntiles).
 template <typename OutT, bool DUAL, bool TAB_LDS>
-__global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int J, int ntiles,
-                                                             const double* __restrict__ pos,
-                                                             const double2* __restrict__ mie, double k, TabDev gtab,
-                                                             int nterms, int add_identity, OutT* __restrict__ out,
-                                                             float2* __restrict__ out32) {
-    extern __shared__ double2 lds[];
+__device__ __forceinline__ void assemble_body(int blk, double2* lds, int N, int lmax, int J, int ntiles,
+                                              const double* __restrict__ pos, const double2* __restrict__ mie,
+                                              double k, TabDev gtab, int nterms, int add_identity,
+                                              OutT* __restrict__ out, float2* __restrict__ out32) {
     const int rmax = lmax_to_rmax(lmax);
     const int bs = 2 * rmax;
     const size_t n = (size_t)bs * N;
@@ -81,8 +80,8 @@ __global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int
     double2* eph = HL + J * Lc;
     double2* mieJ = eph + J * nmu;
 
-    const int i = blockIdx.x / ntiles;
-    const int j0 = (blockIdx.x % ntiles) * J;
+    const int i = blk / ntiles;
+    const int j0 = (blk % ntiles) * J;
     const int jcount = min(J, N - j0);
 
     TabDev tab = gtab;
@@ -132,6 +131,55 @@ __global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int
             put<OutT, DUAL>(out, out32, base + (size_t)(rmax + r) * n + rmax, cmul(xa, m1));
         }
     }
+}
+
+template <typename OutT, bool DUAL, bool TAB_LDS>
+__global__ __launch_bounds__(kThreads) void assemble_kernel(int N, int lmax, int J, int ntiles,
+                                                             const double* __restrict__ pos,
+                                                             const double2* __restrict__ mie, double k, TabDev gtab,
+                                                             int nterms, int add_identity, OutT* __restrict__ out,
+                                                             float2* __restrict__ out32) {
+    extern __shared__ double2 lds[];
+    assemble_body<OutT, DUAL, TAB_LDS>(blockIdx.x, lds, N, lmax, J, ntiles, pos, mie, k, gtab, nterms, add_identity,
+                                       out, out32);
+}
+
+// ---- many clusters per launch: descriptors by value in the kernel arguments
+// (kAsmBatchMax per launch), workgroups dealt to the clusters in order; each
+// cluster keeps the tile variant of its own launch, and every entry is
+// formed by the same arithmetic whatever the tiling, so a batched matrix
+// equals the single-cluster one bit for bit.
+constexpr int kAsmBatchMax = 32;
+
+struct AsmJob {
+    int N, lmax, J, ntiles, nterms, add_identity, tab_lds, pad;
+    const double* pos;
+    const double2* mie;
+    double k;
+    TabDev tab;
+    void* out;
+    float2* out32;
+};
+
+struct AsmBatch {
+    int count;
+    int first[kAsmBatchMax + 1];  // prefix of each cluster's workgroups
+    AsmJob job[kAsmBatchMax];
+};
+
+template <typename OutT, bool DUAL>
+__global__ __launch_bounds__(kThreads) void assemble_kernel_b(AsmBatch B) {
+    extern __shared__ double2 lds[];
+    int q = 0;
+    while (q + 1 < B.count && (int)blockIdx.x >= B.first[q + 1]) q++;
+    const AsmJob& A = B.job[q];
+    const int blk = blockIdx.x - B.first[q];
+    if (A.tab_lds)
+        assemble_body<OutT, DUAL, true>(blk, lds, A.N, A.lmax, A.J, A.ntiles, A.pos, A.mie, A.k, A.tab, A.nterms,
+                                        A.add_identity, (OutT*)A.out, A.out32);
+    else
+        assemble_body<OutT, DUAL, false>(blk, lds, A.N, A.lmax, A.J, A.ntiles, A.pos, A.mie, A.k, A.tab, A.nterms,
+                                         A.add_identity, (OutT*)A.out, A.out32);
 }
 
 // Column blocks per tile: the J <= 32 that keeps the most lanes busy in
@@ -397,4 +445,75 @@ extern "C" int mpa_sphere_aggregate_tmatrix_dual(int N, int lmax, const double* 
     ProfScope prof(2, s);
     return launch_assembly<double2, true>(N, lmax, positions, mie, k, add_identity, (double2*)out64, (float2*)out32,
                                           s, "sphere_aggregate_tmatrix_dual");
+}
+
+// ---- many clusters --------------------------------------------------------------
+
+namespace {
+
+template <typename OutT, bool DUAL>
+int launch_assembly_batched(const std::vector<const mpa_asm_cluster*>& cs, hipStream_t s) {
+    for (size_t b = 0; b < cs.size(); b += kAsmBatchMax) {
+        const int cnt = (int)std::min<size_t>(kAsmBatchMax, cs.size() - b);
+        AsmBatch B{};
+        B.count = cnt;
+        size_t lds = 0;
+        for (int q = 0; q < cnt; q++) {
+            const mpa_asm_cluster& C = *cs[b + q];
+            const TransTable* T = mpa_get_table(C.lmax);
+            if (!T) return MPA_ERR_ARG;
+            const std::vector<AsmVariant> vars = asm_variants(C.lmax, T->nterms);
+            const AsmVariant var = vars[std::min<size_t>(g_asm_variant[C.lmax], vars.size() - 1)];
+            AsmJob& A = B.job[q];
+            A.N = C.N;
+            A.lmax = C.lmax;
+            A.J = var.J;
+            A.ntiles = (C.N + var.J - 1) / var.J;
+            A.nterms = T->nterms;
+            A.add_identity = C.add_identity;
+            A.tab_lds = var.tab_lds ? 1 : 0;
+            A.pos = C.positions;
+            A.mie = (const double2*)C.mie;
+            A.k = C.k;
+            A.tab = TabDev{T->cmeta, T->ccoef, T->clidx};
+            A.out = C.out;
+            A.out32 = (float2*)C.out32;
+            const long long wg = (long long)B.first[q] + (long long)C.N * A.ntiles;
+            if (wg > 0x7fffffffLL) {
+                mpa_set_error("sphere_aggregate_tmatrix_batched: batch too large for one launch");
+                return MPA_ERR_ARG;
+            }
+            B.first[q + 1] = (int)wg;
+            lds = std::max(lds, lds_bytes(C.lmax, var.J, T->nterms, var.tab_lds));
+        }
+        hipLaunchKernelGGL((assemble_kernel_b<OutT, DUAL>), dim3((unsigned)B.first[cnt]), dim3(kThreads), lds, s, B);
+        MPA_LAUNCH_CHECK();
+    }
+    return MPA_OK;
+}
+
+}  // namespace
+
+extern "C" int mpa_sphere_aggregate_tmatrix_batched(int count, const mpa_asm_cluster* clusters, void* stream) {
+    if (count < 0 || (count > 0 && !clusters)) {
+        mpa_set_error("sphere_aggregate_tmatrix_batched: bad argument");
+        return MPA_ERR_ARG;
+    }
+    std::vector<const mpa_asm_cluster*> c128, c64, dual;
+    for (int q = 0; q < count; q++) {
+        const mpa_asm_cluster& C = clusters[q];
+        if (C.N < 1 || C.lmax < 1 || C.lmax > MPA_LMAX_MAX || !C.positions || !C.mie || !C.out || !(C.k > 0) ||
+            (C.dtype != MPA_C128 && C.dtype != MPA_C64) || (C.out32 && C.dtype != MPA_C128)) {
+            mpa_set_error("sphere_aggregate_tmatrix_batched: bad cluster %d", q);
+            return MPA_ERR_ARG;
+        }
+        (C.out32 ? dual : C.dtype == MPA_C128 ? c128 : c64).push_back(&C);
+    }
+    hipStream_t s = (hipStream_t)stream;
+    ProfScope prof(2, s);
+    int rc = MPA_OK;
+    if (!dual.empty()) rc = launch_assembly_batched<double2, true>(dual, s);
+    if (rc == MPA_OK && !c128.empty()) rc = launch_assembly_batched<double2, false>(c128, s);
+    if (rc == MPA_OK && !c64.empty()) rc = launch_assembly_batched<float2, false>(c64, s);
+    return rc;
 }

@@ -125,6 +125,33 @@ def assemble_dual(positions, mie, k, add_identity=False, out64=None, out32=None)
     return out64, out32
 
 
+def assemble_batched(clusters, add_identity=True):
+    """Aggregate T-matrices of many clusters in a few launches
+    (mpa_sphere_aggregate_tmatrix_batched); each equals its own assemble() /
+    assemble_dual() call bit for bit.  clusters: [(positions [N, 3] f64
+    device, mie [N, 2, lmax] c128 device, k, out, out32 or None)] with out
+    [n, n] complex128 or complex64 (out32: the complex64 rounding of a
+    complex128 out, written in the same pass)."""
+    lib = L.load()
+    descs = (L.MpaAsmCluster * len(clusters))()
+    for q, (pos, mie, k, out, out32) in enumerate(clusters):
+        N = pos.shape[0]
+        lmax = mie.shape[-1]
+        n = 2 * lmax_to_rmax(lmax) * N
+        if (tuple(out.shape) != (n, n) or out.dtype not in (C128, C64) or not out.is_contiguous()
+                or (out32 is not None and (tuple(out32.shape) != (n, n) or out32.dtype != C64 or out.dtype != C128))):
+            raise ValueError("out must be a contiguous [n, n] complex128/complex64 tensor (out32: complex64 beside "
+                             "a complex128 out)")
+        if pos.dtype != torch.float64 or mie.dtype != C128 or not pos.is_contiguous() or not mie.is_contiguous():
+            raise ValueError("positions must be contiguous float64 and mie contiguous complex128 device tensors")
+        d = descs[q]
+        d.N, d.lmax, d.dtype, d.add_identity = N, lmax, DTYPE_CODE[out.dtype], 1 if add_identity else 0
+        d.positions, d.mie, d.k = pos.data_ptr(), mie.data_ptr(), float(k)
+        d.out = out.data_ptr()
+        d.out32 = out32.data_ptr() if out32 is not None else None
+    L.check(lib.mpa_sphere_aggregate_tmatrix_batched(len(clusters), descs, _stream()), "sphere_aggregate_tmatrix_batched")
+
+
 def matvec(A, x, add_identity=False, out=None):
     """y = (I +) A x; x is [n] or [S, n] on the device, same dtype as A."""
     lib = L.load()

@@ -219,3 +219,42 @@ def test_overlapped_product_phases_equal_sequential(dev):
         for a, b in zip(out[key], ref):
             for k in ("p_inc", "C_total", "F"):
                 assert np.array_equal(a[k], b[k]), (key, k)
+
+
+def test_batched_assembly_equals_single(dev):
+    """mpa_sphere_aggregate_tmatrix_batched over clusters of mixed N and lmax
+    (1-6) and all three output kinds (c128, c64, c128 + c64 in one pass),
+    more than one launch's worth of each kind: every matrix equals its own
+    single-cluster assembly bit for bit."""
+    import torch
+    from miepy_amd.workloads import random_cluster
+    from miepy_amd.mie import mie_coefficients
+    rng = np.random.default_rng(12)
+    items, refs = [], []
+    for q in range(70):
+        N = int(rng.integers(1, 24))
+        lmax = int(rng.integers(1, 7))
+        k = 2 * np.pi / (rng.uniform(450, 900) * nm)
+        pos = random_cluster(N, 500 * nm * N ** (1 / 3) + 200 * nm, 60 * nm, rng)
+        ms, _ = mie_coefficients(np.full(N, 60 * nm), np.full(N, 3.0 + 0.1j), np.ones(N), 1.0, 1.0, k, lmax)
+        pos_d, ms_d = dev.to_device(pos, torch.float64), dev.to_device(ms, dev.C128)
+        n = 2 * lmax * (lmax + 2) * N
+        kind = q % 3
+        if kind == 0:
+            out, out32 = torch.empty((n, n), dtype=dev.C128, device="cuda"), None
+            ref = (dev.assemble(pos_d, ms_d, k, dtype=dev.C128, add_identity=True),)
+        elif kind == 1:
+            out, out32 = torch.empty((n, n), dtype=dev.C64, device="cuda"), None
+            ref = (dev.assemble(pos_d, ms_d, k, dtype=dev.C64, add_identity=True),)
+        else:
+            out = torch.empty((n, n), dtype=dev.C128, device="cuda")
+            out32 = torch.empty((n, n), dtype=dev.C64, device="cuda")
+            ref = dev.assemble_dual(pos_d, ms_d, k, add_identity=True)
+        items.append((pos_d, ms_d, k, out, out32))
+        refs.append(ref)
+    dev.assemble_batched(items, add_identity=True)
+    torch.cuda.synchronize()
+    for q, ((_, _, _, out, out32), ref) in enumerate(zip(items, refs)):
+        assert np.array_equal(out.cpu().numpy(), ref[0].cpu().numpy()), q
+        if out32 is not None:
+            assert np.array_equal(out32.cpu().numpy(), ref[1].cpu().numpy()), q
