@@ -183,6 +183,34 @@ bool valid(int nsys, const mpa_refine_system* s) {
     return true;
 }
 
+// The FP64 residual products of one refinement step: the batched GEMV over
+// the stored complex128 matrices and the FP64 matrix-free products; with
+// both present (and MPA_OPT_OVERLAP) the GEMV runs on the device's side
+// stream beside the matrix-free products, joined before the caller's next
+// launch, as in the solver's product phases.
+int residual_products(const MvDev* d_mv, int2* const* d_mw, const int* n_mw, const std::vector<MfJob>& mf,
+                      hipStream_t st) {
+    bool any_mv = false;
+    for (int c = 0; c < kTileClasses; c++) any_mv |= n_mw[c] > 0;
+    if (any_mv && !mf.empty() && mpa_option(MPA_OPT_OVERLAP) != 0 && !mpa_profile_active()) {
+        SideLease lease;
+        const SideStream* side = lease.get();
+        if (side) {
+            MPA_HIP_CHECK(hipEventRecord(side->fork, st));
+            MPA_HIP_CHECK(hipStreamWaitEvent(side->stream, side->fork, 0));
+            const int rc1 = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, side->stream);
+            const int rc2 = matfree_launch_many(mf, nullptr, st);
+            MPA_HIP_CHECK(hipEventRecord(side->join, side->stream));
+            MPA_HIP_CHECK(hipStreamWaitEvent(st, side->join, 0));
+            return rc1 ? rc1 : rc2;
+        }
+    }
+    int rc = MPA_OK;
+    if (any_mv && (rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st))) return rc;
+    if (!mf.empty() && (rc = matfree_launch_many(mf, nullptr, st))) return rc;
+    return MPA_OK;
+}
+
 int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer_out, int* inner_out,
                  double* resid_out, int* status_out, hipStream_t st) {
     // descriptors
@@ -349,13 +377,7 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
             }
         hipLaunchKernelGGL(k_ref_accum, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         MPA_LAUNCH_CHECK();
-        bool any_mv = false;
-        for (int c = 0; c < kTileClasses; c++) any_mv |= n_mw[c] > 0;
-        if (any_mv) {
-            rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st);
-            if (rc) return rc;
-        }
-        if (!mf.empty() && (rc = matfree_launch_many(mf, nullptr, st))) return rc;
+        if ((rc = residual_products(d_mv, d_mw, n_mw, mf, st))) return rc;
         hipLaunchKernelGGL(k_ref_residual, dim3(nv), dim3(kThreads), 0, st, d_ref, d_vw);
         hipLaunchKernelGGL(k_ref_norms, dim3(nc), dim3(kThreads), 0, st, d_ref, d_cw);
         MPA_LAUNCH_CHECK();
@@ -634,10 +656,7 @@ int refine_iterate(int nsys, const mpa_refine_system* R, void* scratch, bool sta
         if ((rc = upload(L, d_mw, n_mw))) return rc;
         hipLaunchKernelGGL(k_ref_accum, dim3((unsigned)L.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
         MPA_LAUNCH_CHECK();
-        bool any_mv = false;
-        for (int c = 0; c < kTileClasses; c++) any_mv |= n_mw[c] > 0;
-        if (any_mv && (rc = gemv_launch(8, d_mv, nullptr, d_mw, n_mw, st))) return rc;
-        if (!L.mf.empty() && (rc = matfree_launch_many(L.mf, nullptr, st))) return rc;
+        if ((rc = residual_products(d_mv, d_mw, n_mw, L.mf, st))) return rc;
         hipLaunchKernelGGL(k_ref_residual, dim3((unsigned)L.vw.size()), dim3(kThreads), 0, st, d_ref, d_vw);
         hipLaunchKernelGGL(k_ref_norms, dim3((unsigned)L.cw.size()), dim3(kThreads), 0, st, d_ref, d_cw);
         MPA_LAUNCH_CHECK();
