@@ -426,7 +426,7 @@ class ServingEngine:
             "cross_sections": obs["C_total"].cpu().numpy(),  # [S, 3]
             "force": obs["F"].cpu().numpy(),                  # [S, N, 3]
             "iterations": iters, "residual": resid, "status": status, "matrix_free": s.matrix_free,
-            "refined": s.refine, "fp64_fallback": fallback, "converged": ok,
+            "refined": s.refine, "fp64_fallback": fallback, "converged": ok, "done_at": self.clock(),
         }
         if ok:
             self.solves_done += 1
