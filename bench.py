@@ -63,6 +63,8 @@ def parse():
                     help="complex64 matrix-free kernel variant (mpa_matfree_variant); default: the library's")
     ap.add_argument("--overlap", type=int, choices=[0, 1], default=None,
                     help="MPA_OPT_OVERLAP: stored-matrix GEMM beside the matrix-free products (library default 1)")
+    ap.add_argument("--graphs", type=int, choices=[0, 1], default=None,
+                    help="MPA_OPT_GRAPHS: solver iterations replayed from HIP graphs (library default 1)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tiles (no autotuning in warmup)")
     ap.add_argument("--cpu-gemvs", type=int, default=3, help="timed oracle GEMVs in the CPU sample")
@@ -745,6 +747,9 @@ def main():
     if args.overlap is not None:
         from miepy_amd import _lib as L
         L.check(L.load().mpa_set_option(L.MPA_OPT_OVERLAP, args.overlap), "set_option")
+    if args.graphs is not None:
+        from miepy_amd import _lib as L
+        L.check(L.load().mpa_set_option(L.MPA_OPT_GRAPHS, args.graphs), "set_option")
     if args.workload == "batch256":
         return bench_batch(args)
     if args.workload == "serving":
