@@ -1844,23 +1844,26 @@ struct SliceLoop<KP, KP> {
 
 // ---- lmax 5: the operand block of a column (174 KB) exceeds the LDS
 //
-// k_matfree_mf16k is the contraction of k_matfree_mf16s (one source per
-// workgroup, the same per-pair power-of-two scales and running total) with
-// the column's operand block staged in KP slices of K blocks (lmax 5: 6, 6
-// and 5 of the 17) through two LDS buffers: the next slice (of this column or
-// of the next one) is loaded into registers while the MFMAs of the current
-// slice run and written to the other buffer after them, one barrier per
-// slice; the B operands are formed slice by slice (hle_part), so only one
-// slice's stay live.  One workgroup per CU (two 61 KB buffers): each wave has
-// its SIMD's 512 registers for the accumulators and running totals of the NT
-// tiles, the slice in flight and the pair's special functions.
+// k_matfree_mf16k2 is the contraction of k_matfree_mf16s (one source per
+// workgroup, the same per-pair power-of-two scales and running totals) with
+// the column's operand block taken in KP slices of K blocks (lmax 5, KP 6:
+// 3, 3, 3, 3, 3 and 2 of the 17) through two 30 KB LDS buffers, each filled
+// by DMA straight from global memory, and the B operands formed slice by
+// slice (hle_geom, then hle_part resuming the recurrences), so that a wave
+// fits in the 256 registers of two waves per SIMD and two workgroups share
+// a CU.  Slice q + 1's DMA is issued at
+// the start of slice q, ahead of slice q's B operands (VALU); each wave waits
+// for its own share before reading slice q's operands, so by the barrier that
+// ends slice q the whole of slice q + 1 has landed.  The other workgroup on
+// the CU covers these waits.  (The earlier form, one workgroup per CU with
+// register-staged 61 KB slices, took 7.4 ms at N = 3000 with two sources
+// against 6.0 ms here.)
 template <int L, int KP>
-__global__ __launch_bounds__(kThreads, 1) void k_matfree_mf16k(Mf16Batch B, const SysDev* __restrict__ sys) {
+__global__ __launch_bounds__(kThreads, 2) void k_matfree_mf16k2(Mf16Batch B, const SysDev* __restrict__ sys) {
     using Sh = F16Shape<L>;
     constexpr int NC = Sh::NC, NT = Sh::NT, NKB = Sh::NKB, PER = Sh::PER;
-    constexpr int KBP = (NKB + KP - 1) / KP;             // K blocks per slice
-    constexpr int SL = 2 * NT * KBP * 64;                // f4v entries of a slice buffer: [hl][tile][kbi][lane]
-    constexpr int LDP = (SL + kThreads - 1) / kThreads;  // entries per thread and slice
+    constexpr int KBP = (NKB + KP - 1) / KP;  // K blocks per slice
+    constexpr int SL = 2 * NT * KBP * 64;     // f4v entries of a slice buffer: [hl][tile][kbi][lane]
     __shared__ f4v Ws[2][SL];
     const int v0 = xcd_swizzle(blockIdx.x, gridDim.x);
     const int q = batch_job(B, v0);
@@ -1891,28 +1894,19 @@ __global__ __launch_bounds__(kThreads, 1) void k_matfree_mf16k(Mf16Batch B, cons
 #pragma unroll
         for (int e = 0; e < 16; e++) R[t][e] = 0.f;
     int cur = 0;
-    f4v pf[LDP];
-    // slice p of column j: buffer entry e = ((hl NT + t) KBP + kbi) 64 + lane <- the column's block entry
-    // ((hl NT + t) NKB + p KBP + kbi) 64 + lane, zero for kbi past the slice's K blocks
-    auto load = [&](int j, int p) {
+    // slice p of column j by DMA: runs of 64 entries (hl, tile, kbi) dealt to the 4 waves; buffer entry
+    // ((hl NT + t) KBP + kbi) 64 + lane <- the column's block entry ((hl NT + t) NKB + p KBP + kbi) 64 + lane
+    auto dma = [&](int buf, int j, int p) {
         const int kb0 = p * KBP, nk = min(KBP, NKB - kb0);
         const f4v* src = Wsrc + (size_t)j * PER;
-#pragma unroll
-        for (int r = 0; r < LDP; r++) {
-            const int e = r * kThreads + tid;
-            const int run = e >> 6, ht = run / KBP, kbi = run - ht * KBP;
-            pf[r] = (e < SL && kbi < nk) ? src[((size_t)ht * NKB + kb0 + kbi) * 64 + (e & 63)] : f4v{0, 0, 0, 0};
+        for (int r = w; r < 2 * NT * nk; r += 4) {  // wave-uniform
+            const int ht = r / nk, kbi = r - ht * nk;
+            __builtin_amdgcn_global_load_lds((const void*)(src + ((size_t)ht * NKB + kb0 + kbi) * 64 + lane),
+                                             (lds_ptr_t)(&Ws[buf][(ht * KBP + kbi) * 64]), 16, 0, 0);
         }
     };
-    auto put = [&](int buf) {
-#pragma unroll
-        for (int r = 0; r < LDP; r++) {
-            const int e = r * kThreads + tid;
-            if (e < SL) Ws[buf][e] = pf[r];
-        }
-    };
-    load(jbeg, 0);
-    put(0);
+    dma(0, jbeg, 0);
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     __syncthreads();
     int buf = 0;
     for (int j = jbeg; j < jend; j++) {
@@ -1931,14 +1925,13 @@ __global__ __launch_bounds__(kThreads, 1) void k_matfree_mf16k(Mf16Batch B, cons
         for (int t = 0; t < NT; t++)
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[t][e] = 0.f;
-        // slice p: load the next slice, form this slice's B operands, its MFMAs, store the next slice
         auto step = [&](auto pc) {
             constexpr int p = decltype(pc)::value;
             constexpr int U0 = 4 * KBP * p;
             constexpr int U1 = (4 * KBP * (p + 1) < Sh::NCOMBO) ? 4 * KBP * (p + 1) : Sh::NCOMBO;
             constexpr int NKS = (U1 - U0 + 3) / 4;  // K blocks of this slice
-            const bool more = p + 1 < KP || j + 1 < jend;  // uniform
-            if (more) load(p + 1 < KP ? j : j + 1, p + 1 < KP ? p + 1 : 0);
+            if (p + 1 < KP) dma(buf ^ 1, j, p + 1);
+            else if (j + 1 < jend) dma(buf ^ 1, j + 1, 0);
             if (wlive) {
                 h8v bh[NKS], bl[NKS];
                 hle_part<L, U0, U1>(hx, hy, G, Rc, bh, bl);
@@ -1954,7 +1947,8 @@ __global__ __launch_bounds__(kThreads, 1) void k_matfree_mf16k(Mf16Batch B, cons
                         a = __builtin_amdgcn_mfma_f32_32x32x16_f16(al, bh[kbi], a, 0, 0, 0);
                     }
             }
-            if (more) put(buf ^ 1);
+            // this wave's share of the next slice, then the barrier: the next slice is whole, this one read
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             __syncthreads();
             buf ^= 1;
         };
@@ -2492,7 +2486,7 @@ size_t matfree_workspace_bytes(int N, int lmax, int S) {
 }
 
 // lmax <= 3: FP16-split or FP32 matrix-core kernel; lmax 4: FP16-split kernel (whatever the option says);
-// lmax 5: the FP16-split kernel with the operand block staged in K slices (k_matfree_mf16k)
+// lmax 5: the FP16-split kernel with the operand block staged in K slices (k_matfree_mf16k2)
 constexpr int kC64MaxLmax = 5;
 bool matfree_c64_supported(int N, int lmax, bool tmatrix) { return lmax <= kC64MaxLmax && N >= 1 && !tmatrix; }
 
@@ -2544,7 +2538,7 @@ int launch_mf16s_pack(const MfJob* const* jobs, int count, const SysDev* sys, hi
     hipLaunchKernelGGL((k_matfree_wh_b<L>), dim3((unsigned)Bw.first[count]), dim3(WhShape<L>::TH), 0, st, Bw, sys,
                        T->nhterms, T->hlist, T->hterm_src, T->hterm_coef);
     if constexpr (L == 5)
-        hipLaunchKernelGGL((k_matfree_mf16k<5, 3>), dim3((unsigned)Bc.first[count]), block, 0, st, Bc, sys);
+        hipLaunchKernelGGL((k_matfree_mf16k2<5, 6>), dim3((unsigned)Bc.first[count]), block, 0, st, Bc, sys);
     else
         hipLaunchKernelGGL((k_matfree_mf16s<L, SG>), dim3((unsigned)Bc.first[count]), block, 0, st, Bc, sys);
     if (Bs.count)

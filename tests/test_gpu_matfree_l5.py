@@ -1,6 +1,7 @@
-"""The complex64 matrix-free operator at lmax 5 (k_matfree_mf16k: the
+"""The complex64 matrix-free operator at lmax 5 (k_matfree_mf16k2: the
 FP16-split contraction with the 174 KB operand block of a column staged in K
-slices through two LDS buffers, B operands formed slice by slice).
+slices through two LDS buffers by DMA, B operands formed slice by slice, two
+workgroups per CU).
 
 - against the FP64 matrix-free operator (which the oracle pins at lmax 5 in
   test_gpu_matfree.py) for 1, 3 and 8 sources: single-chunk (direct-write)
