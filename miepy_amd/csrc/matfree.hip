@@ -2321,13 +2321,24 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64s(MfJob J, const SysDe
                 }
             }
         }
-#pragma unroll
-        for (int t = 0; t < KT; t++) {
+        // the B values of K step t + 2 are read from LDS while the MFMAs of step t run (one wave per
+        // SIMD: nothing else covers the LDS latency)
+        double bq[3][NT];
+        auto ldb = [&](int t, double (&b)[NT]) {
             const int h = hi ? slots.hb[t] : slots.ha[t];
 #pragma unroll
+            for (int tile = 0; tile < NT; tile++) b[tile] = ((const double*)&Ws[buf][h][tile * 8 + (col >> 1)])[comp];
+        };
+        ldb(0, bq[0]);
+        if (KT > 1) ldb(1, bq[1]);
+#pragma unroll
+        for (int t = 0; t < KT; t++) {
+            if (t + 2 < KT) ldb(t + 2, bq[(t + 2) % 3]);
+            __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of this step's MFMAs
+#pragma unroll
             for (int tile = 0; tile < NT; tile++) {
-                const double wv = ((const double*)&Ws[buf][h][tile * 8 + (col >> 1)])[comp];
-                const double bv = __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, wv) ^ bsign);
+                const double bv =
+                    __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, bq[t % 3][tile]) ^ bsign);
                 acc[tile][t % KS] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], bv, acc[tile][t % KS], 0, 0, 0);
             }
         }
