@@ -485,7 +485,8 @@ __global__ __launch_bounds__(kThreads) void k_matfree_fac(MfJob J, const SysDev*
 inline bool fac_path(int N, int lmax) {
     return lmax <= kFacMaxLmax && N >= kFacMinN && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 2;
 }
-constexpr int kMf64MaxLmax = 4;  // FP64 matrix-core kernel: lmax <= 4 (128 KB of LDS at lmax 4)
+constexpr int kMf64MaxLmax = 5;  // FP64 matrix-core kernels: lmax <= 4 two LDS buffers (128 KB at lmax 4),
+                                 // lmax 5 one (136 KB)
 inline bool mf64_path(int lmax) { return lmax <= kMf64MaxLmax && mpa_option(MPA_OPT_MATFREE_FP64_KERNEL) == 0; }
 inline bool fac_possible(int N, int lmax) { return lmax <= kFacMaxLmax && N >= kFacMinN; }
 
@@ -2192,6 +2193,74 @@ struct Mf64Slots {
     }
 };
 
+// The pair special functions of the FP64 matrix-core kernels for one lane: the
+// A value of every K step (K order of Mf64Slots<L>) for the pair (this
+// lane's row particle at (xi, yi, zi), column particle j); zero off the pairs
+// (on = false).  odd, hi: the lane group's Re/Im and order choices, sgn the
+// sign of Im e it sees.
+template <int L>
+__device__ __forceinline__ void mf64_pair_values(double xi, double yi, double zi, const double* __restrict__ pos,
+                                                 int j, double k, bool on, int odd, int hi, double sgn,
+                                                 double (&af)[Mf64Slots<L>::KT]) {
+    constexpr int PM = 2 * L, NM0 = Mf64Slots<L>::NM0;
+    double dx = xi - pos[3 * j], dy = yi - pos[3 * j + 1], dz = zi - pos[3 * j + 2];
+    dx = on ? dx : 1.0;  // a harmless geometry for the zero entries
+    dy = on ? dy : 0.0;
+    dz = on ? dz : 0.0;
+    const double rad = sqrt(dx * dx + dy * dy + dz * dz);
+    const double ct = dz / rad, st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
+    const double zz = k * rad;
+    const double rz = on ? 1.0 / zz : 0.0;  // zero Hankel values off the pairs
+    double sz, cz;
+    sincos(zz, &sz, &cz);
+    const double rho = sqrt(dx * dx + dy * dy);
+    const double2 e1 = rho > 0 ? make_double2(dx / rho, dy / rho) : make_double2(1.0, 0.0);
+    double2 hh[PM + 1];
+    hh[0] = make_double2(sz * rz, -cz * rz);
+    hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
+#pragma unroll
+    for (int l = 2; l <= PM; l++)
+        hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
+                             (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
+    // m = 0: P_l^0 by the recurrence in l; steps of two orders
+    double P0[PM + 1];
+    P0[0] = 1.0;
+    P0[1] = ct;
+#pragma unroll
+    for (int l = 2; l <= PM; l++) P0[l] = ((2 * l - 1) * ct * P0[l - 1] - (l - 1) * P0[l - 2]) * (1.0 / l);
+    // (a0, b0) = (1, 0) or (0, 1) picks Re or Im exactly; fh = 0 or 1
+    // picks the step's first or second order exactly
+    const double a0 = odd ? 0.0 : 1.0, b0 = odd ? 1.0 : 0.0, fh = hi ? 1.0 : 0.0;
+#pragma unroll
+    for (int q = 0; q < NM0; q++) {
+        const int l1 = 2 * q, l2 = 2 * q + 1;
+        const double v1 = P0[l1] * fma(hh[l1].x, a0, hh[l1].y * b0);
+        const double v2 = l2 <= PM ? P0[l2] * fma(hh[l2].x, a0, hh[l2].y * b0) : 0.0;
+        af[q] = fma(v2, fh, v1 * (1.0 - fh));
+    }
+    // m > 0: one step per (l, m), P_l^m by the recurrence in l
+    double2 e = make_double2(1.0, 0.0);
+    double pmm = 1.0;
+    int t = NM0;
+#pragma unroll
+    for (int m = 1; m <= PM; m++) {
+        e = cmul(e, e1);
+        pmm *= (2 * m - 1) * st;
+        const double sy = sgn * e.y;
+        const double a = odd ? -sy : e.x, b = odd ? e.x : sy;
+        double p2 = 0.0, p1 = 0.0;
+#pragma unroll
+        for (int l = m; l <= PM; l++) {
+            const double P = (l == m)       ? pmm
+                             : (l == m + 1) ? ct * (2 * m + 1) * pmm
+                                            : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.0 / (l - m));
+            p2 = p1;
+            p1 = P;
+            af[t++] = P * fma(hh[l].x, a, hh[l].y * b);
+        }
+    }
+}
+
 template <int L, int KS>
 __global__ __launch_bounds__(kThreads) void k_matfree_mf64s(MfJob J, const SysDev* __restrict__ sys, int nchunks,
                                                             int chunk, const double2* __restrict__ Wm) {
@@ -2262,65 +2331,7 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64s(MfJob J, const SysDe
         const bool more = j + 1 < jend;
         if (more) fetch(j + 1);
         double af[KT];
-        {
-            const bool on = live && j != ii;
-            double dx = xi - J.pos[3 * j], dy = yi - J.pos[3 * j + 1], dz = zi - J.pos[3 * j + 2];
-            dx = on ? dx : 1.0;  // a harmless geometry for the zero entries
-            dy = on ? dy : 0.0;
-            dz = on ? dz : 0.0;
-            const double rad = sqrt(dx * dx + dy * dy + dz * dz);
-            const double ct = dz / rad, st = sqrt(fmax(0.0, (1.0 - ct) * (1.0 + ct)));
-            const double zz = J.k * rad;
-            const double rz = on ? 1.0 / zz : 0.0;  // zero Hankel values off the pairs
-            double sz, cz;
-            sincos(zz, &sz, &cz);
-            const double rho = sqrt(dx * dx + dy * dy);
-            const double2 e1 = rho > 0 ? make_double2(dx / rho, dy / rho) : make_double2(1.0, 0.0);
-            double2 hh[PM + 1];
-            hh[0] = make_double2(sz * rz, -cz * rz);
-            hh[1] = make_double2(hh[0].x * rz + hh[0].y, hh[0].y * rz - hh[0].x);
-#pragma unroll
-            for (int l = 2; l <= PM; l++)
-                hh[l] = make_double2((2 * l - 1) * rz * hh[l - 1].x - hh[l - 2].x,
-                                     (2 * l - 1) * rz * hh[l - 1].y - hh[l - 2].y);
-            // m = 0: P_l^0 by the recurrence in l; steps of two orders
-            double P0[PM + 1];
-            P0[0] = 1.0;
-            P0[1] = ct;
-#pragma unroll
-            for (int l = 2; l <= PM; l++) P0[l] = ((2 * l - 1) * ct * P0[l - 1] - (l - 1) * P0[l - 2]) * (1.0 / l);
-            // (a0, b0) = (1, 0) or (0, 1) picks Re or Im exactly; fh = 0 or 1
-            // picks the step's first or second order exactly
-            const double a0 = odd ? 0.0 : 1.0, b0 = odd ? 1.0 : 0.0, fh = hi ? 1.0 : 0.0;
-#pragma unroll
-            for (int q = 0; q < NM0; q++) {
-                const int l1 = 2 * q, l2 = 2 * q + 1;
-                const double v1 = P0[l1] * fma(hh[l1].x, a0, hh[l1].y * b0);
-                const double v2 = l2 <= PM ? P0[l2] * fma(hh[l2].x, a0, hh[l2].y * b0) : 0.0;
-                af[q] = fma(v2, fh, v1 * (1.0 - fh));
-            }
-            // m > 0: one step per (l, m), P_l^m by the recurrence in l
-            double2 e = make_double2(1.0, 0.0);
-            double pmm = 1.0;
-            int t = NM0;
-#pragma unroll
-            for (int m = 1; m <= PM; m++) {
-                e = cmul(e, e1);
-                pmm *= (2 * m - 1) * st;
-                const double sy = sgn * e.y;
-                const double a = odd ? -sy : e.x, b = odd ? e.x : sy;
-                double p2 = 0.0, p1 = 0.0;
-#pragma unroll
-                for (int l = m; l <= PM; l++) {
-                    const double P = (l == m)       ? pmm
-                                     : (l == m + 1) ? ct * (2 * m + 1) * pmm
-                                                    : ((2 * l - 1) * ct * p1 - (l + m - 1) * p2) * (1.0 / (l - m));
-                    p2 = p1;
-                    p1 = P;
-                    af[t++] = P * fma(hh[l].x, a, hh[l].y * b);
-                }
-            }
-        }
+        mf64_pair_values<L>(xi, yi, zi, J.pos, j, J.k, live && j != ii, odd, hi, sgn, af);
         // the B values of K step t + 2 are read from LDS while the MFMAs of step t run (one wave per
         // SIMD: nothing else covers the LDS latency)
         double bq[3][NT];
@@ -2358,6 +2369,119 @@ __global__ __launch_bounds__(kThreads) void k_matfree_mf64s(MfJob J, const SysDe
                 for (int k = 1; k < KS; k++) v += acc[tile][k][q];
                 ((double*)J.part)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = v;
             }
+        }
+}
+
+// k_matfree_mf64k: the FP64 matrix-core contraction for lmax 5, where one
+// column's W block (121 x 70 complex, 136 KB) fills the LDS: a single buffer,
+// filled by DMA straight from Wm (the same layout), while the waves form the
+// next column's pair special functions; one accumulator chain per tile
+// (nine tiles interleave), the B values read two K steps ahead.
+template <int L>
+__global__ __launch_bounds__(kThreads) void k_matfree_mf64k(MfJob J, const SysDev* __restrict__ sys, int nchunks,
+                                                            int chunk, const double2* __restrict__ Wm) {
+    constexpr int R = L * (L + 2);
+    constexpr int NC = 2 * R;
+    constexpr int NT = (NC + 7) / 8;
+    constexpr int PM = 2 * L;
+    constexpr int NH = (PM + 1) * (PM + 1);
+    using SM = Mf64Slots<L>;
+    constexpr int KT = SM::KT;
+    static_assert(KT == (2 * NH + 3) / 4, "one K step per combination");
+    constexpr SM slots{};
+    constexpr int WB = NH * NC;  // double2 entries of a column's block
+    __shared__ double2 Ws[WB];
+    if (J.skip >= 0 && job_done(sys[J.skip])) return;
+    const int N = J.N;
+    const int s = blockIdx.y;
+    const int groups = (N + 63) / 64;
+    const int v = xcd_swizzle(blockIdx.x, gridDim.x);
+    const int c = v / groups;
+    const int group = v - c * groups;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int g = lane >> 4, col = lane & 15, part = col & 1;
+    const int odd = g & 1, hi = g >> 1;
+    const int comp = odd ^ part;  // B: Re W (0) or Im W (1)
+    const unsigned long long bsign = (odd && !part) ? 0x8000000000000000ull : 0ull;
+    const double sgn = hi ? 1.0 : -1.0;  // sign of Im e seen by this lane group
+    const int i = group * 64 + w * 16 + col;
+    const bool live = i < N;
+    const int ii = live ? i : N - 1;
+    const double xi = J.pos[3 * ii], yi = J.pos[3 * ii + 1], zi = J.pos[3 * ii + 2];
+    const double2* Wall = Wm + (size_t)s * N * WB;
+    const int jbeg = c * chunk, jend = min(N, jbeg + chunk);  // an empty chunk still writes its (zero) rows
+    d4v acc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; t++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[t][q] = 0.0;
+    // column j's block into LDS: 16-byte entries dealt in rounds of 64 per wave
+    auto dma = [&](int j) {
+        const double2* src = Wall + (size_t)j * WB;
+#pragma unroll
+        for (int r = 0; r < (WB + kThreads - 1) / kThreads; r++) {
+            const int e = r * kThreads + w * 64 + lane;
+            if (e < WB)
+                __builtin_amdgcn_global_load_lds((const void*)(src + e), (lds_ptr_t)(Ws + r * kThreads + w * 64), 16,
+                                                 0, 0);
+        }
+    };
+    double af[KT];
+    if (jbeg < jend) {  // uniform
+        dma(jbeg);
+        mf64_pair_values<L>(xi, yi, zi, J.pos, jbeg, J.k, live && jbeg != ii, odd, hi, sgn, af);
+    }
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    constexpr int LAST = NC - (NT - 1) * 8;  // columns of the last tile inside NC
+    for (int j = jbeg; j < jend; j++) {
+        // the lane's LDS offsets, re-derived every column (an empty asm hides that they do not change), so
+        // the compiler forms each step's addresses in the step instead of keeping all of them in registers
+        int lc = col >> 1;
+        asm volatile("" : "+v"(lc));
+        const bool in = lc < LAST;
+        const double* pb = (const double*)Ws + comp + 2 * lc;                // tiles 0 .. NT - 2
+        const double* pl = (const double*)Ws + comp + 2 * (in ? lc : 0);     // the last tile
+        double bq[3][NT];
+        auto ldb = [&](int t, double (&b)[NT]) {
+            const int h = hi ? slots.hb[t] : slots.ha[t];
+            const double* ph = pb + 2 * NC * h;
+#pragma unroll
+            for (int tile = 0; tile < NT - 1; tile++) b[tile] = ph[16 * tile];
+            // the last tile's columns past NC read zero (and no address past the block)
+            const double wv = pl[2 * NC * h + 16 * (NT - 1)];
+            b[NT - 1] = in ? wv : 0.0;
+        };
+        ldb(0, bq[0]);
+        ldb(1, bq[1]);
+#pragma unroll
+        for (int t = 0; t < KT; t++) {
+            if (t + 2 < KT) ldb(t + 2, bq[(t + 2) % 3]);
+            __builtin_amdgcn_sched_barrier(0);  // keep the reads ahead of this step's MFMAs
+#pragma unroll
+            for (int tile = 0; tile < NT; tile++) {
+                const double bv =
+                    __builtin_bit_cast(double, __builtin_bit_cast(unsigned long long, bq[t % 3][tile]) ^ bsign);
+                acc[tile] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[t], bv, acc[tile], 0, 0, 0);
+            }
+        }
+        __syncthreads();  // every wave has read this column's block
+        if (j + 1 < jend) {  // uniform
+            dma(j + 1);
+            mf64_pair_values<L>(xi, yi, zi, J.pos, j + 1, J.k, live && j + 1 != ii, odd, hi, sgn, af);
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+            __syncthreads();
+        }
+    }
+    const size_t n = (size_t)NC * N;
+#pragma unroll
+    for (int tile = 0; tile < NT; tile++)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int ip = group * 64 + w * 16 + 4 * q + g;
+            const int cc = tile * 8 + (col >> 1);
+            if (ip < N && cc < NC)
+                ((double*)J.part)[2 * (((size_t)c * J.S + s) * n + (size_t)ip * NC + cc) + part] = acc[tile][q];
         }
 }
 
@@ -2733,7 +2857,8 @@ int matfree_launch(const MfJob& J0, const SysDev* sys, hipStream_t st) {
         if (J.lmax == 1) MPA_MF64(1);
         else if (J.lmax == 2) MPA_MF64(2);
         else if (J.lmax == 3) MPA_MF64(3);
-        else MPA_MF64(4);
+        else if (J.lmax == 4) MPA_MF64(4);
+        else hipLaunchKernelGGL((k_matfree_mf64k<5>), grid, block, 0, st, J, sys, nch, chunk, Wm);
 #undef MPA_MF64
         hipLaunchKernelGGL(k_matfree_sum, dim3(sum_blocks), dim3(kThreads), 0, st, J, sys, nch);
         MPA_LAUNCH_CHECK();

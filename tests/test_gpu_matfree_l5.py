@@ -106,3 +106,34 @@ def test_lmax5_mixed_matfree_solve(dev):
     e_p = float(np.linalg.norm(got.p_scat - ref.p_scat) / np.linalg.norm(ref.p_scat))
     print(f"[l5] mixed_matfree vs FP64: cross-sections {e_c:.2e}, p_scat {e_p:.2e}", flush=True)
     assert e_c <= 1e-8 and e_p <= 1e-8
+
+
+@pytest.mark.parametrize("N,S", [(3, 1), (70, 2), (130, 3), (200, 1)])
+def test_lmax5_fp64_matrix_core_operator_matches_dense(dev, N, S):
+    """The FP64 matrix-free product at lmax 5 (k_matfree_mf64k: one column's
+    W block in a single LDS buffer filled by DMA) against the assembled matrix
+    times x, and against the FP64 LDS kernel (MPA_OPT_MATFREE_FP64_KERNEL = 1,
+    checked against the oracle's matrix in test_gpu_matfree.py), 1e-12 of the
+    largest entry: one and several column chunks (some of them empty), rows
+    past N in the last 64-row group, one to three sources, after a complex64
+    product has left other data in the workspace."""
+    import miepy_amd._lib as L
+    lib = L.load()
+    pos, mie = _cluster(N, 5, seed=900 + N)
+    k = 2 * np.pi / (600 * nm)
+    rng = np.random.default_rng(N)
+    op = dev.ClusterOperator(pos, mie, k, max_rhs=S)
+    X = dev.to_device(rng.normal(size=(S, op.n)) + 1j * rng.normal(size=(S, op.n)), dev.C128)
+    dev.matfree_matvec(op, X.to(dev.C64))  # leaves other data in the shared workspace (partial rows included)
+    got = dev.matfree_matvec(op, X, add_identity=True).cpu().numpy()
+    A = dev.assemble(pos, mie, k, add_identity=True)
+    dense = dev.matvec(A, X).cpu().numpy()
+    del A
+    assert np.max(np.abs(got - dense)) <= 1e-12 * np.max(np.abs(dense))
+    saved = lib.mpa_get_option(L.MPA_OPT_MATFREE_FP64_KERNEL)
+    try:
+        lib.mpa_set_option(L.MPA_OPT_MATFREE_FP64_KERNEL, 1)
+        lds = dev.matfree_matvec(op, X, add_identity=True).cpu().numpy()
+    finally:
+        lib.mpa_set_option(L.MPA_OPT_MATFREE_FP64_KERNEL, saved)
+    assert np.max(np.abs(got - lds)) <= 1e-12 * np.max(np.abs(lds))
