@@ -46,10 +46,10 @@ extern "C" {
                             HIP graph (one launch per iteration instead of one per kernel); 0: eager
                             launches.  Results are identical either way.  Graphs are not used while
                             launch timing (mpa_profile_enable) is on. */
-#define MPA_OPT_MATFREE_FP64_KERNEL 1 /* kernel of the FP64 matrix-free product for lmax <= 3: 0 (default)
-                                         the FP64 matrix-core kernel, 1 the LDS kernel (always used for
-                                         lmax > 3), 2 the factorized register-resident kernel (N >= 128).
-                                         Results agree to rounding. */
+#define MPA_OPT_MATFREE_FP64_KERNEL 1 /* kernel of the FP64 matrix-free product: 0 (default) the FP64
+                                         matrix-core kernels (lmax <= 5), 1 the LDS kernel (always used
+                                         for lmax > 5), 2 the factorized register-resident kernel
+                                         (lmax <= 3, N >= 128).  Results agree to rounding. */
 #define MPA_OPT_MATFREE_C64_KERNEL 2 /* kernel of the complex64 matrix-free product: 0 (default) FP16
                                         matrix cores on hi/lo split operands (three products, FP32
                                         accumulation, per-pair and per-column scaling; relative error
