@@ -1026,8 +1026,9 @@ __device__ __forceinline__ void wh_body(const MfJob& J, int j0, int s0, int s1, 
                 lv[2 * e2 + 1] = lo.y;
             }
             h8v* dst = Wh + ((size_t)s * J.N + j) * Sh::PER;
-            dst[tk * 64 + lane] = hv;
-            dst[(NT * NKB + tk) * 64 + lane] = lv;
+            // streaming stores: the blocks are read back by the contraction, not by this kernel
+            __builtin_nontemporal_store(hv, &dst[tk * 64 + lane]);
+            __builtin_nontemporal_store(lv, &dst[(NT * NKB + tk) * 64 + lane]);
         }
         __syncthreads();  // Wn, zs and red are rewritten for the next source
     }
