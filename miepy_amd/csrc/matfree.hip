@@ -2633,13 +2633,17 @@ constexpr int mf16s_sg(int L) { return L <= 2 ? 4 : (L == 3 ? 2 : 1); }
 // Column chunks of one job in k_matfree_mf16s: about 2048 workgroups for
 // the job alone (128-row groups x source groups x chunks: four rounds of two
 // workgroups per CU, so the last, partly filled round costs little), but no
-// chunk under 32 columns (a workgroup's set-up and partial rows stay small
-// against its columns), and never more chunks than the workspace holds
-// partial rows for.
+// chunk under 16 columns (8 at lmax <= 2, whose columns are cheap), so a
+// workgroup's set-up and partial rows stay small against its columns, and
+// never more chunks than the workspace holds partial rows for.  (Measured on
+// batch256's 33 matrix-free clusters: 16 against 32 columns took the batched
+// product from 3.26 to 3.10 ms; 8 columns were slower above lmax 2 and
+// faster for its one lmax-2 cluster, N = 376: 282 workgroups, not 66.)
 inline int mf16s_chunks(int N, int S, int L) {
     const int groups = (N + 127) / 128, nsg = (S + mf16s_sg(L) - 1) / mf16s_sg(L);
     int c = (2048 + groups * nsg - 1) / (groups * nsg);
-    c = std::min(c, std::max(1, N / 32));
+    const int minc = L <= 2 ? 8 : 16;
+    c = std::min(c, std::max(1, N / minc));
     c = std::min(c, mf32_chunks(N));
     return c < 1 ? 1 : c;
 }
