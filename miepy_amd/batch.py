@@ -33,6 +33,7 @@ class PreparedCluster:
         self.p_src = D.plane_wave_structure(list(w.sources), self.pos, k, w.lmax)
         self.S = self.p_src.shape[0]
         self.n = w.n
+        self.rhs = self.p_src.reshape(self.S, self.n)  # one right-hand side per source
         self.k = k
         self.eps_b = float(np.real(w.medium_eps))
         self.A64 = None
@@ -67,6 +68,8 @@ class ClusterBatch:
                 c.A32 = torch.empty((c.n, c.n), dtype=D.C64, device=c.pos.device)
         self.reports = None
         self.results = None
+        self._obs_cache = {}
+        self._asm_cache = {}
 
     def autotune(self, reps=3):
         """Pick the matvec tile of every (dtype, RHS-count class) present on
@@ -84,14 +87,15 @@ class ClusterBatch:
         """Assemble, solve and reduce every cluster; results stay on the device."""
         cs = self.clusters
         tols = [c.w.tol * self.tol_scale for c in cs]
-        rhs = [c.p_src.reshape(c.S, c.n) for c in cs]
+        rhs = [c.rhs for c in cs]
         if self.precision == "matfree":
             xs, self.reports = D.bicgstab_batched([c.op for c in cs], rhs, tol=tols, maxiter=self.maxiter,
                                                   add_identity=True)
             return self._reduce(xs)
         stored = [c for c in cs if c.op is None]
         if stored:  # every stored matrix (and its complex64 rounding) in a few batched launches
-            D.assemble_batched([(c.pos, c.mie_scat, c.k, c.A64, c.A32) for c in stored], add_identity=True)
+            D.assemble_batched([(c.pos, c.mie_scat, c.k, c.A64, c.A32) for c in stored], add_identity=True,
+                               cache=self._asm_cache)
         if self.precision == "fp64":
             xs, self.reports = D.bicgstab_batched([c.A64 for c in cs], rhs, tol=tols, maxiter=self.maxiter)
         else:
@@ -102,13 +106,13 @@ class ClusterBatch:
         return self._reduce(xs)
 
     def _reduce(self, xs):
-        """Observables of every cluster in a few batched launches."""
+        """Observables of every cluster in a few batched launches (the
+        clusters' static descriptor columns are kept between solves)."""
         cs = self.clusters
-        p_incs = [x.reshape(c.S, c.w.N, 2, c.w.rmax) for c, x in zip(cs, xs)]
-        self.results = D.observables_batched([(p, c.p_src, c.mie_scat, c.mie_int, c.k, c.eps_b)
-                                              for c, p in zip(cs, p_incs)])
-        for obs, p in zip(self.results, p_incs):
-            obs["p_inc"] = p
+        self.results = D.observables_batched([(x, c.p_src, c.mie_scat, c.mie_int, c.k, c.eps_b)
+                                              for c, x in zip(cs, xs)], cache=self._obs_cache)
+        for obs, c, x in zip(self.results, cs, xs):
+            obs.defer("p_inc", x, (c.S, c.w.N, 2, c.w.rmax))
         return self.results
 
     def converged(self):

@@ -6,6 +6,7 @@ path: without a visible GPU or without the built library these raise.
 """
 
 import ctypes
+from collections.abc import MutableMapping
 
 import numpy as np
 import torch
@@ -125,14 +126,21 @@ def assemble_dual(positions, mie, k, add_identity=False, out64=None, out32=None)
     return out64, out32
 
 
-def assemble_batched(clusters, add_identity=True):
+def assemble_batched(clusters, add_identity=True, cache=None):
     """Aggregate T-matrices of many clusters in a few launches
     (mpa_sphere_aggregate_tmatrix_batched); each equals its own assemble() /
     assemble_dual() call bit for bit.  clusters: [(positions [N, 3] f64
     device, mie [N, 2, lmax] c128 device, k, out, out32 or None)] with out
     [n, n] complex128 or complex64 (out32: the complex64 rounding of a
-    complex128 out, written in the same pass)."""
+    complex128 out, written in the same pass).  cache: a dict kept by a
+    caller that assembles the same tensors again (the same clusters, inputs
+    and outputs): their checked descriptors are reused."""
     lib = L.load()
+    key = (len(clusters), bool(add_identity))
+    if cache is not None and cache.get("key") == key:
+        L.check(lib.mpa_sphere_aggregate_tmatrix_batched(len(clusters), cache["descs"], _stream()),
+                "sphere_aggregate_tmatrix_batched")
+        return
     descs = (L.MpaAsmCluster * len(clusters))()
     for q, (pos, mie, k, out, out32) in enumerate(clusters):
         N = pos.shape[0]
@@ -150,6 +158,8 @@ def assemble_batched(clusters, add_identity=True):
         d.out = out.data_ptr()
         d.out32 = out32.data_ptr() if out32 is not None else None
     L.check(lib.mpa_sphere_aggregate_tmatrix_batched(len(clusters), descs, _stream()), "sphere_aggregate_tmatrix_batched")
+    if cache is not None:
+        cache["key"], cache["descs"] = key, descs
 
 
 def matvec(A, x, add_identity=False, out=None):
@@ -485,49 +495,143 @@ def observables(p_inc, p_src, mie_scat, mie_int, k, eps_b, want_int=True, want_t
     return dict(p_scat=p_scat, p_int=p_int, C_part=C_part, C_total=C_total, F=F, T=T)
 
 
-def observables_batched(clusters, want_int=True, want_torque=True):
+# mpa_obs_cluster as a numpy record (pointers as 64-bit integers), so that the
+# descriptors of many clusters are filled column by column
+_OBS_DTYPE = np.dtype([("N", "<i4"), ("lmax", "<i4"), ("nsrc", "<i4"), ("pad", "<i4"), ("p_inc", "<u8"),
+                       ("p_src", "<u8"), ("mie_scat", "<u8"), ("mie_int", "<u8"), ("k", "<f8"), ("eps_b", "<f8"),
+                       ("p_scat", "<u8"), ("p_int", "<u8"), ("C_part", "<u8"), ("C_total", "<u8"), ("F", "<u8"),
+                       ("T", "<u8")])
+assert _OBS_DTYPE.itemsize == ctypes.sizeof(L.MpaObsCluster)
+
+
+class _ObsOutputs:
+    """The flat output arrays of one batched observables call and, per output
+    kind, each cluster's offset and shape in them."""
+
+    def __init__(self, flats, offsets, shapes):
+        self.flats, self.offsets, self.shapes = flats, offsets, shapes
+
+    def view(self, key, q):
+        flat = self.flats[key]
+        if flat is None:
+            return None
+        shape = self.shapes[key][q]
+        off = int(self.offsets[key][q])
+        return flat[off:off + int(np.prod(shape))].view(shape)
+
+
+class ObsResult(MutableMapping):
+    """One cluster's observables (p_scat, p_int, C_part, C_total, F, T): views
+    into the batch's flat outputs, each made on its first access; other
+    entries can be set as in a dict."""
+
+    __slots__ = ("_out", "_q", "_d", "_lazy")
+    KEYS = ("p_scat", "p_int", "C_part", "C_total", "F", "T")
+
+    def __init__(self, out, q):
+        self._out, self._q, self._d, self._lazy = out, q, {}, {}
+
+    def defer(self, key, tensor, shape):
+        """An entry that is tensor.view(shape), made on its first access."""
+        self._lazy[key] = (tensor, shape)
+
+    def __getitem__(self, key):
+        if key in self._d:
+            return self._d[key]
+        if key in self._lazy:
+            t, shape = self._lazy.pop(key)
+            v = self._d[key] = t.view(shape)
+            return v
+        if key not in self.KEYS:
+            raise KeyError(key)
+        v = self._d[key] = self._out.view(key, self._q)
+        return v
+
+    def __setitem__(self, key, value):
+        self._lazy.pop(key, None)
+        self._d[key] = value
+
+    def __delitem__(self, key):
+        raise TypeError("observables results are not deleted")
+
+    def _extra(self):
+        return [k for k in list(self._d) + list(self._lazy) if k not in self.KEYS]
+
+    def __iter__(self):
+        yield from self.KEYS
+        yield from dict.fromkeys(self._extra())
+
+    def __len__(self):
+        return len(self.KEYS) + len(dict.fromkeys(self._extra()))
+
+
+def observables_batched(clusters, want_int=True, want_torque=True, cache=None):
     """observables() for many clusters in a few launches
     (mpa_cluster_observables_batched); each cluster's results equal its own
     observables() call bit for bit.  clusters: [(p_inc, p_src, mie_scat,
-    mie_int, k, eps_b)] with p_inc, p_src [S, N, 2, rmax] c128.  Each output
-    kind is one flat allocation; returns one dict of views per cluster."""
+    mie_int, k, eps_b)] with p_src [S, N, 2, rmax] c128 and p_inc the same
+    S x N x 2 x rmax values (any shape, contiguous).  Each output kind is one
+    flat allocation; returns one mapping per cluster whose entries are views
+    into them (the descriptors are filled column by column, so the host work
+    per cluster is a few pointer reads).  cache: a dict kept by a caller that
+    passes the same clusters' inputs (p_src, Mie coefficients, k, eps_b) again
+    with new p_inc: their descriptor columns are reused."""
     lib = L.load()
     if not clusters:
         return []
     dev = clusters[0][0].device
-    shapes = [tuple(c[0].shape) for c in clusters]
-    lm = [c[2].shape[-1] for c in clusters]
-    sizes = {"p": sum(S * N * 2 * r for S, N, _, r in shapes), "C_part": sum(S * N * 6 * l for (S, N, _, _), l in
-                                                                             zip(shapes, lm)),
-             "C_total": sum(S * 3 for S, *_ in shapes), "F": sum(S * N * 3 for S, N, *_ in shapes)}
-    p_scat_all = torch.empty(sizes["p"], dtype=C128, device=dev)
-    p_int_all = torch.empty(sizes["p"], dtype=C128, device=dev) if want_int else None
-    C_part_all = torch.empty(sizes["C_part"], dtype=torch.float64, device=dev)
-    C_total_all = torch.empty(sizes["C_total"], dtype=torch.float64, device=dev)
-    F_all = torch.empty(sizes["F"], dtype=torch.float64, device=dev)
-    T_all = torch.empty(sizes["F"], dtype=torch.float64, device=dev) if want_torque else None
-    descs = (L.MpaObsCluster * len(clusters))()
-    out = []
-    op = oc = ot = of = 0
-    for q, ((p_inc, p_src, mie_scat, mie_int, k, eps_b), (S, N, _, rmax), lmax) in enumerate(zip(clusters, shapes, lm)):
-        np_, nc, nf = S * N * 2 * rmax, S * N * 6 * lmax, S * N * 3
-        r = dict(p_scat=p_scat_all[op:op + np_].view(S, N, 2, rmax),
-                 p_int=p_int_all[op:op + np_].view(S, N, 2, rmax) if want_int else None,
-                 C_part=C_part_all[oc:oc + nc].view(S, N, 3, 2, lmax), C_total=C_total_all[ot:ot + 3 * S].view(S, 3),
-                 F=F_all[of:of + nf].view(S, N, 3), T=T_all[of:of + nf].view(S, N, 3) if want_torque else None)
-        op, oc, ot, of = op + np_, oc + nc, ot + 3 * S, of + nf
-        d = descs[q]
-        d.N, d.lmax, d.nsrc = N, lmax, S
-        d.p_inc, d.p_src, d.mie_scat = p_inc.data_ptr(), p_src.data_ptr(), mie_scat.data_ptr()
-        d.mie_int = mie_int.data_ptr() if want_int else None
-        d.k, d.eps_b = float(k), float(np.real(eps_b))
-        d.p_scat, d.C_part, d.C_total, d.F = (r["p_scat"].data_ptr(), r["C_part"].data_ptr(),
-                                              r["C_total"].data_ptr(), r["F"].data_ptr())
-        d.p_int = r["p_int"].data_ptr() if want_int else None
-        d.T = r["T"].data_ptr() if want_torque else None
-        out.append(r)
-    L.check(lib.mpa_cluster_observables_batched(len(clusters), descs, _stream()), "cluster_observables_batched")
-    return out
+    q = len(clusters)
+    st = cache.get("static") if cache is not None else None
+    if st is None or st["q"] != q or st["want"] != (want_int, want_torque):
+        shp = [c[1].shape for c in clusters]
+        S = np.array([t[0] for t in shp], dtype=np.int64)
+        N = np.array([t[1] for t in shp], dtype=np.int64)
+        R = np.array([t[3] for t in shp], dtype=np.int64)
+        Lm = np.array([c[2].shape[-1] for c in clusters], dtype=np.int64)
+        counts = {"p": S * N * 2 * R, "C_part": S * N * 6 * Lm, "C_total": 3 * S, "F": S * N * 3}
+        offs = {k: np.concatenate(([0], np.cumsum(v)[:-1])) for k, v in counts.items()}
+        d0 = np.zeros(q, dtype=_OBS_DTYPE)
+        d0["N"], d0["lmax"], d0["nsrc"] = N, Lm, S
+        d0["p_src"] = [c[1].data_ptr() for c in clusters]
+        d0["mie_scat"] = [c[2].data_ptr() for c in clusters]
+        if want_int:
+            d0["mie_int"] = [c[3].data_ptr() for c in clusters]
+        d0["k"] = [float(c[4]) for c in clusters]
+        d0["eps_b"] = [float(np.real(c[5])) for c in clusters]
+        S_, N_, R_, L_ = S.tolist(), N.tolist(), R.tolist(), Lm.tolist()
+        shapes = {"p": [(a, b, 2, c) for a, b, c in zip(S_, N_, R_)],
+                  "C_part": [(a, b, 3, 2, c) for a, b, c in zip(S_, N_, L_)],
+                  "C_total": [(a, 3) for a in S_], "F": [(a, b, 3) for a, b in zip(S_, N_)]}
+        st = {"q": q, "want": (want_int, want_torque), "counts": {k: int(v.sum()) for k, v in counts.items()},
+              "offs": offs, "d0": d0, "shapes": shapes}
+        if cache is not None:
+            cache["static"] = st
+    counts, offs, shapes = st["counts"], st["offs"], st["shapes"]
+    p_scat_all = torch.empty(counts["p"], dtype=C128, device=dev)
+    p_int_all = torch.empty(counts["p"], dtype=C128, device=dev) if want_int else None
+    C_part_all = torch.empty(counts["C_part"], dtype=torch.float64, device=dev)
+    C_total_all = torch.empty(counts["C_total"], dtype=torch.float64, device=dev)
+    F_all = torch.empty(counts["F"], dtype=torch.float64, device=dev)
+    T_all = torch.empty(counts["F"], dtype=torch.float64, device=dev) if want_torque else None
+    d = st["d0"].copy()
+    d["p_inc"] = [c[0].data_ptr() for c in clusters]
+    d["p_scat"] = p_scat_all.data_ptr() + 16 * offs["p"]
+    if want_int:
+        d["p_int"] = p_int_all.data_ptr() + 16 * offs["p"]
+    d["C_part"] = C_part_all.data_ptr() + 8 * offs["C_part"]
+    d["C_total"] = C_total_all.data_ptr() + 8 * offs["C_total"]
+    d["F"] = F_all.data_ptr() + 8 * offs["F"]
+    if want_torque:
+        d["T"] = T_all.data_ptr() + 8 * offs["F"]
+    L.check(lib.mpa_cluster_observables_batched(q, d.ctypes.data_as(ctypes.POINTER(L.MpaObsCluster)), _stream()),
+            "cluster_observables_batched")
+    out = _ObsOutputs({"p_scat": p_scat_all, "p_int": p_int_all, "C_part": C_part_all, "C_total": C_total_all,
+                       "F": F_all, "T": T_all},
+                      {"p_scat": offs["p"], "p_int": offs["p"], "C_part": offs["C_part"],
+                       "C_total": offs["C_total"], "F": offs["F"], "T": offs["F"]},
+                      {"p_scat": shapes["p"], "p_int": shapes["p"], "C_part": shapes["C_part"],
+                       "C_total": shapes["C_total"], "F": shapes["F"], "T": shapes["F"]})
+    return [ObsResult(out, i) for i in range(q)]
 
 
 def translation_table(lmax):
@@ -822,10 +926,11 @@ def bicgstab_batched(matrices, rhs, tol=1e-5, maxiter=1000, add_identity=False, 
     rc = lib.mpa_bicgstab_batched(k, structs, code, _ptr(scratch), it, res, stt, int(sync_every), _stream())
     if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED, L.MPA_NONFINITE):
         L.check(rc, "bicgstab_batched")
+    it_l, res_l, stt_l = list(it), list(res), list(stt)  # once, then a slice per system
     reports, o = [], 0
     for x in xs:
         S = x.shape[0]
-        reports.append(SolveReport(list(it[o:o + S]), list(res[o:o + S]), list(stt[o:o + S])))
+        reports.append(SolveReport(it_l[o:o + S], res_l[o:o + S], stt_l[o:o + S]))
         o += S
     return xs, reports
 
@@ -878,11 +983,14 @@ def solve_refined_batched(A32s, A64s, rhs, tol=1e-10, inner_tol=1e-5, maxiter=10
     rc = lib.mpa_solve_refined_batched(k, structs, _ptr(scratch), outer, inner, res, stt, _stream())
     if rc not in (L.MPA_OK, L.MPA_NOT_CONVERGED, L.MPA_NONFINITE):
         L.check(rc, "solve_refined_batched")
+    # the per-RHS records as Python lists once, then a slice per system
+    outer_l, inner_l, res_l, stt_l = list(outer), list(inner), list(res), list(stt)
+    any_bad = fallback and any(v != L.MPA_OK for v in stt_l)
     reports, o = [], 0
     for i, x in enumerate(xs):
         S = x.shape[0]
-        rep = RefineReport(list(outer[o:o + S]), list(inner[o:o + S]), list(res[o:o + S]), list(stt[o:o + S]))
-        if fallback and any(v != L.MPA_OK for v in rep.status):
+        rep = RefineReport(outer_l[o:o + S], inner_l[o:o + S], res_l[o:o + S], stt_l[o:o + S])
+        if any_bad and any(v != L.MPA_OK for v in rep.status):
             # the operator that stands for the FP64 system: the cluster operator (always I + T) or A64
             A = A32s[i] if isinstance(A32s[i], ClusterOperator) else A64s[i]
             ai = True if isinstance(A32s[i], ClusterOperator) else add_identity
