@@ -2633,16 +2633,18 @@ constexpr int mf16s_sg(int L) { return L <= 2 ? 4 : (L == 3 ? 2 : 1); }
 // Column chunks of one job in k_matfree_mf16s: about 2048 workgroups for
 // the job alone (128-row groups x source groups x chunks: four rounds of two
 // workgroups per CU, so the last, partly filled round costs little), but no
-// chunk under 16 columns (8 at lmax <= 2, whose columns are cheap), so a
-// workgroup's set-up and partial rows stay small against its columns, and
-// never more chunks than the workspace holds partial rows for.  (Measured on
-// batch256's 33 matrix-free clusters: 16 against 32 columns took the batched
-// product from 3.26 to 3.10 ms; 8 columns were slower above lmax 2 and
-// faster for its one lmax-2 cluster, N = 376: 282 workgroups, not 66.)
+// chunk under 32 columns when that still gives the job about one round (460
+// workgroups), else none under 16 (8 at lmax <= 2, whose columns are cheap),
+// so a workgroup's set-up and partial rows stay small against its columns;
+// never more chunks than the workspace holds partial rows for.  (Measured:
+// batch256's batched product 3.26 -> 3.10 ms with 16 against 32 columns for
+// its clusters of 125-377 spheres; single systems of 150-400 spheres at
+// lmax 4 and 5 1.3-1.9x faster, of 1000 and more 0-5 % slower, hence the
+// 32-column floor where the job fills a round on its own.)
 inline int mf16s_chunks(int N, int S, int L) {
-    const int groups = (N + 127) / 128, nsg = (S + mf16s_sg(L) - 1) / mf16s_sg(L);
-    int c = (2048 + groups * nsg - 1) / (groups * nsg);
-    const int minc = L <= 2 ? 8 : 16;
+    const int groups = (N + 127) / 128, nsg = (S + mf16s_sg(L) - 1) / mf16s_sg(L), gs = groups * nsg;
+    int c = (2048 + gs - 1) / gs;
+    const int minc = gs * std::max(1, N / 32) >= 460 ? 32 : (L <= 2 ? 8 : 16);
     c = std::min(c, std::max(1, N / minc));
     c = std::min(c, mf32_chunks(N));
     return c < 1 ? 1 : c;
