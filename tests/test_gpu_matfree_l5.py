@@ -12,7 +12,10 @@ workgroups per CU).
 - lmax-5 jobs batched with lmax-3 and lmax-4 jobs in one batched product
   equal their products alone bit for bit;
 - a complex64 + FP64-refinement solve through it (`mixed_matfree`) matches
-  the FP64 stored solve: cross-sections and p_scat within 1e-8.
+  the FP64 stored solve: cross-sections and p_scat within 1e-8;
+- the FP64 matrix-free product at lmax 5 (k_matfree_mf64k, the refinement's
+  residuals) equals the assembled matrix times x and the FP64 LDS kernel to
+  1e-12, with other data left in the workspace by a complex64 product.
 """
 
 import numpy as np
