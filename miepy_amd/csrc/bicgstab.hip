@@ -440,6 +440,7 @@ int build_plan(Plan& P, int nsys, const mpa_system* S_, int dtype, void* scratch
         std::vector<int2>& mw = P.mv_work[gemv_class(S)];
         for (int b = 0; b < rb; b++) mw.push_back(make_int2(i, b));
     }
+    for (auto& w : P.mv_work) order_work_longest_first(w, [&](int i) { return S_[i].n; });
     char* p = (char*)scratch;
     P.d_sys = (SysDev*)p; p += align256(sizeof(SysDev) * nsys);
     P.d_vec = (int3*)p; p += align256(sizeof(int3) * P.vec_work.size());

@@ -714,6 +714,7 @@ extern "C" int mpa_autotune_matvec_batched(int count, const int* n, const int* n
         work.clear();
         for (int j = 0; j < (int)jobs.size(); j++)
             for (int b = 0; b < (jobs[j].n + rows - 1) / rows; b++) work.push_back(make_int2(j, b));
+        order_work_longest_first(work, [&](int j) { return jobs[j].n; });  // as the solvers order it
         if (hipMemcpyAsync(d_work, work.data(), sizeof(int2) * work.size(), hipMemcpyHostToDevice, st) != hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess) {
             rc = MPA_ERR_HIP;

@@ -268,6 +268,7 @@ int refine_batch(int nsys, const mpa_refine_system* R, void* scratch, int* outer
         }
         total += S;
     }
+    for (auto& w : mw) order_work_longest_first(w, [&](int i) { return R[i].n; });
     char* p = (char*)scratch;
     RefDev* d_ref = (RefDev*)p; p += align256(sizeof(RefDev) * nsys);
     MvDev* d_mv = (MvDev*)p; p += align256(sizeof(MvDev) * nsys);
@@ -495,6 +496,7 @@ SubLists sub_lists(const std::vector<int>& idx, const mpa_refine_system* R, cons
             for (int b = 0; b < rb; b++) L.mw[gemv_class(q.nrhs)].push_back(make_int2(i, b));
         }
     }
+    for (auto& w : L.mw) order_work_longest_first(w, [&](int i) { return R[i].n; });
     return L;
 }
 
