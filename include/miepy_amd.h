@@ -417,7 +417,7 @@ typedef struct {
     int maxiter, max_refine;
     const mpa_cluster_operator* op; /* host pointer or NULL: with op set, A32 and A64 are NULL, the
                                        corrections use the complex64 matrix-core operator and the
-                                       residual the FP64 one (no matrix is stored; spheres, lmax <= 3) */
+                                       residual the FP64 one (no matrix is stored; spheres, lmax <= 5) */
 } mpa_refine_system;
 
 size_t mpa_refine_state_bytes(int n, int nrhs);
